@@ -1,0 +1,111 @@
+"""ctypes binding of the engine's C-ABI (include/lblg.h).
+
+The shared library is nasoq_amd/liblblg.so, built in-tree by
+nasoq_amd/build.py.  Loading fails loudly when it is missing: there is no
+fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblblg.so")
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class Settings(ctypes.Structure):
+    """Mirror of lblg_settings (field order must match include/lblg.h)."""
+    _fields_ = [
+        ("ldl_variant", ctypes.c_int),
+        ("req_ref_iter", ctypes.c_int),
+        ("max_inner_iter", ctypes.c_int),
+        ("tol_abs", ctypes.c_double),
+        ("tol_rel", ctypes.c_double),
+        ("reg_diag", ctypes.c_double),
+        ("regularization", ctypes.c_int),
+        ("cost_param", ctypes.c_int),
+        ("level_param", ctypes.c_int),
+        ("final_seq_node", ctypes.c_int),
+        ("n_relax", ctypes.c_int * 3),
+        ("z_relax", ctypes.c_double * 3),
+        ("ordering", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("target_bwd_err", ctypes.c_double),
+        ("max_refine_rounds", ctypes.c_int),
+    ]
+
+
+# (name, restype, argtypes) for every exported entry point of include/lblg.h
+SIGNATURES = [
+    ("lblg_default_settings", None, [ctypes.POINTER(Settings)]),
+    ("lblg_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _i32p, _i32p, _f64p]),
+    ("lblg_destroy", ctypes.c_int, [_vp]),
+    ("lblg_analyze", ctypes.c_int, [_vp, ctypes.POINTER(Settings), _i32p, _i32p]),
+    ("lblg_set_settings", ctypes.c_int, [_vp, ctypes.POINTER(Settings)]),
+    ("lblg_order_nd", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _i32p]),
+    ("lblg_symbolic_scalar", ctypes.c_int64, [_vp, ctypes.c_int]),
+    ("lblg_symbolic_ints", ctypes.c_int64, [_vp, ctypes.c_int, _i32p]),
+    ("lblg_symbolic_longs", ctypes.c_int64, [_vp, ctypes.c_int, _i64p]),
+    ("lblg_symbolic_flops", ctypes.c_int, [_vp, _f64p, _f64p]),
+    ("lblg_device_count", ctypes.c_int, [_i32p]),
+    ("lblg_last_error", ctypes.c_char_p, []),
+    ("lblg_factor", ctypes.c_int, [_vp]),
+    ("lblg_refactor_resident", ctypes.c_int, [_vp]),
+    ("lblg_set_values", ctypes.c_int, [_vp, _f64p]),
+    ("lblg_solve", ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int64, _f64p, ctypes.c_int64, _f64p]),
+    ("lblg_solve_device", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _f64p]),
+    ("lblg_num_ref_iter", ctypes.c_int, [_vp]),
+    ("lblg_synchronize", ctypes.c_int, [_vp]),
+    ("lblg_stream", _vp, [_vp]),
+    ("lblg_get_factor", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, _i32p]),
+    ("lblg_inertia", ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
+    ("lblg_factor_info", ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
+    ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p]),
+]
+
+_lib = None
+
+
+def load():
+    """Load liblblg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "nasoq_amd/liblblg.so is missing: build it with `python nasoq_amd/build.py` "
+            "(there is no non-native fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def iptr(a):
+    return a.ctypes.data_as(_i32p) if a is not None else None
+
+
+def lptr(a):
+    return a.ctypes.data_as(_i64p) if a is not None else None
+
+
+def dptr(a):
+    return a.ctypes.data_as(_f64p) if a is not None else None
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = ""
+        if _lib is not None:
+            m = _lib.lblg_last_error()
+            msg = m.decode() if m else ""
+        raise RuntimeError(f"{what} failed with status {rc} {msg}")
+    return rc

@@ -1,0 +1,202 @@
+// C-ABI entry points of the numeric phase (device factorization, solves,
+// refinement, inertia and residuals).  See include/lblg.h for the reference
+// interface each function replaces.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "engine/device.hpp"
+#include "engine/engine.hpp"
+#include "engine/handle.hpp"
+#include "engine/kernels.hpp"
+#include "lblg.h"
+
+using lblg::Handle;
+
+namespace {
+thread_local char g_last_error[512] = {0};
+void set_error(const char* what) {
+  std::strncpy(g_last_error, what, sizeof(g_last_error) - 1);
+  g_last_error[sizeof(g_last_error) - 1] = 0;
+}
+template <typename F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    set_error("allocation failed");
+    return LBLG_ERR_NOMEM;
+  } catch (const std::invalid_argument& e) {
+    set_error(e.what());
+    return LBLG_ERR_ARG;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return LBLG_ERR_DEVICE;
+  }
+}
+}  // namespace
+
+extern "C" const char* lblg_last_error(void) { return g_last_error; }
+
+extern "C" int lblg_device_count(int* count) {
+  if (!count) return LBLG_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return LBLG_OK;
+}
+
+extern "C" int lblg_factor(lblg_handle* hh) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (!h->analyzed) return LBLG_ERR_STATE;
+  if (h->set.precision != LBLG_F64) return LBLG_ERR_ARG;
+  return guarded([&] {
+    auto t0 = std::chrono::steady_clock::now();
+    lblg::factor(*h);
+    h->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_refactor_resident(lblg_handle* hh) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (!h->analyzed || !h->dev) return LBLG_ERR_STATE;
+  return guarded([&] {
+    lblg::factor_device(*h);
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_set_values(lblg_handle* hh, const double* Ax) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !Ax) return LBLG_ERR_ARG;
+  std::memcpy(h->Ax.data(), Ax, sizeof(double) * h->Ax.size());
+  h->factored = false;
+  return LBLG_OK;
+}
+
+static int do_solve(lblg_handle* hh, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx, bool dev,
+                    double* bwd) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !B || !X || nrhs < 1 || nrhs > lblg::MAX_RHS || ldb < h->n || ldx < h->n) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  if (h->set.max_inner_iter < 0 || h->set.req_ref_iter < 0) return LBLG_ERR_ARG;
+  return guarded([&] {
+    lblg::solve(*h, nrhs, B, ldb, X, ldx, dev, bwd);
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_solve(lblg_handle* h, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx,
+                          double* bwd_err) {
+  return do_solve(h, nrhs, B, ldb, X, ldx, false, bwd_err);
+}
+
+extern "C" int lblg_solve_device(lblg_handle* h, int nrhs, const double* dB, int64_t ldb, double* dX,
+                                 int64_t ldx, double* bwd_err) {
+  return do_solve(h, nrhs, dB, ldb, dX, ldx, true, bwd_err);
+}
+
+extern "C" int lblg_num_ref_iter(const lblg_handle* hh) {
+  const Handle* h = reinterpret_cast<const Handle*>(hh);
+  return h ? h->num_ref_iter : LBLG_ERR_ARG;
+}
+
+extern "C" int lblg_synchronize(lblg_handle* hh) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !h->dev) return LBLG_ERR_STATE;
+  return hipStreamSynchronize(h->dev->stream) == hipSuccess ? LBLG_OK : LBLG_ERR_DEVICE;
+}
+
+extern "C" void* lblg_stream(lblg_handle* hh) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  return (h && h->dev) ? (void*)h->dev->stream : nullptr;
+}
+
+extern "C" int lblg_get_factor(lblg_handle* hh, double* valL, double* d_val, int* rows, int* perm_final) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  return guarded([&] {
+    lblg::DeviceFactor& D = *h->dev;
+    hipStream_t st = D.stream;
+    const lblg::Symbolic& S = h->sym;
+    if (valL) hipMemcpyAsync(valL, D.valL, sizeof(double) * S.p[h->n], hipMemcpyDeviceToHost, st);
+    if (d_val) hipMemcpyAsync(d_val, D.dval, sizeof(double) * 2 * h->n, hipMemcpyDeviceToHost, st);
+    if (rows) hipMemcpyAsync(rows, D.rows, sizeof(int) * S.pi[S.nsuper], hipMemcpyDeviceToHost, st);
+    if (perm_final) hipMemcpyAsync(perm_final, D.perm_f, sizeof(int) * h->n, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("copy failed");
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_inertia(lblg_handle* hh, int64_t* pos, int64_t* neg, int64_t* zero) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !pos || !neg || !zero) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  std::vector<double> d(2 * (size_t)h->n);
+  int rc = lblg_get_factor(hh, nullptr, d.data(), nullptr, nullptr);
+  if (rc) return rc;
+  const int n = h->n;
+  int64_t a = 0, b = 0, z = 0;
+  for (int i = 0; i < n; ++i) {
+    if (d[n + i] == 0.0) {
+      if (d[i] > 0) ++a;
+      else if (d[i] < 0) ++b;
+      else ++z;
+    } else {
+      const double det = d[i] * d[i + 1] - d[n + i] * d[n + i];
+      const double tr = d[i] + d[i + 1];
+      if (det < 0) {
+        ++a;
+        ++b;
+      } else if (det > 0) {
+        if (tr > 0) a += 2; else b += 2;
+      } else {
+        ++z;
+        if (tr > 0) ++a; else if (tr < 0) ++b; else ++z;
+      }
+      ++i;
+    }
+  }
+  *pos = a;
+  *neg = b;
+  *zero = z;
+  return LBLG_OK;
+}
+
+extern "C" int lblg_factor_info(const lblg_handle* hh, int* zero_pivot, double* k0_norm_inf,
+                                double* factor_seconds) {
+  const Handle* h = reinterpret_cast<const Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  if (zero_pivot) *zero_pivot = h->zero_pivot;
+  if (k0_norm_inf) *k0_norm_inf = h->k0_norm_inf;
+  if (factor_seconds) *factor_seconds = h->factor_seconds;
+  return LBLG_OK;
+}
+
+extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C) {
+  if (!A || !B || !C) return LBLG_ERR_ARG;
+  return guarded([&] {
+    double *dA, *dB, *dC;
+    if (hipMalloc(&dA, 64 * 8) != hipSuccess || hipMalloc(&dB, 64 * 8) != hipSuccess ||
+        hipMalloc(&dC, 256 * 8) != hipSuccess)
+      throw std::runtime_error("alloc");
+    hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice);
+    hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice);
+    lblg::launch_mfma_selftest(dA, dB, dC, nullptr);
+    hipMemcpy(C, dC, 256 * 8, hipMemcpyDeviceToHost);
+    hipFree(dA);
+    hipFree(dB);
+    hipFree(dC);
+    return LBLG_OK;
+  });
+}

@@ -1,0 +1,118 @@
+// Device-resident LBL factor and the host-side schedules that drive it.
+//
+// HBM layout (one handle):
+//   valL     supernode panels, column-major r_s x w_s, at px[s] (the
+//            reference's BCSC value layout, include/nasoq/common/def.h:131-220)
+//   dval     2n: diagonal of D in [0,n), 2x2 sub-diagonals in [n,2n)
+//   rows     row labels of every supernode (BCSC s[]), relabelled after
+//            pivoting so no panel row is ever moved
+//   perm_piv local pivot permutation (pivoted position -> pre-pivot position)
+//   amap     for every stored entry of K: its slot in valL
+//   K (CSR)  full symmetric K for the SpMV of the refinement
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace lblg {
+
+// list-driven launches: a launch processes the supernodes in a device list
+struct SnList {
+  int* d = nullptr;  // device array of supernode ids
+  int count = 0;
+};
+
+// one GPU level of the factorization
+struct FactorLevel {
+  SnList small;                    // w <= SMALL_W: LDS Bunch-Kaufman
+  std::vector<int> medium;         // SMALL_W < w <= GROUP_W: one workgroup each
+  std::vector<int> large;          // w > GROUP_W: a cooperative group each
+  SnList offdiag;                  // supernodes with rows below the diagonal block
+  int offdiag_blocks = 0;          // total 64-row blocks of those
+  int* d_offdiag_blk = nullptr;    // prefix of 64-row blocks per offdiag supernode
+  SnList schur;                    // supernodes with an update (m > 0)
+  int64_t schur_tiles = 0;
+  int64_t* d_schur_tile = nullptr; // prefix of 64x64 lower tiles per schur supernode
+  int max_w = 0;
+  std::vector<int> medium_dummy;
+  int* d_medium = nullptr;
+  int* d_large = nullptr;
+};
+
+// solve pieces: supernodes split into column blocks of at most SOLVE_W
+struct SolveLevel {
+  int npieces = 0;
+  int nblocks = 0;       // total row blocks (256 rows) of all pieces
+  int* d_piece = nullptr;   // piece ids
+  int* d_blk = nullptr;     // prefix of row blocks per piece
+};
+
+struct DeviceFactor {
+  int n = 0, nsuper = 0, nrhs_cap = 0;
+  hipStream_t stream = nullptr;
+  // symbolic (device)
+  int* super = nullptr;
+  int64_t* pi = nullptr;
+  int64_t* px = nullptr;   // value offset of supernode s
+  int* rows = nullptr;     // row labels (ssize), relabelled after factor
+  int* rows0 = nullptr;    // pre-pivot row labels (kept for re-factorization)
+  int* col2sup = nullptr;
+  // numeric
+  double* valL = nullptr;
+  double* dval = nullptr;
+  int* perm_piv = nullptr;   // local: new position -> old position (global indices)
+  int* newpos = nullptr;     // inverse of perm_piv
+  int* perm_f = nullptr;     // final ordering: ordered index -> original index
+  int* iperm_f = nullptr;
+  int* perm0 = nullptr;      // symbolic ordering before pivoting
+  int64_t* amap = nullptr;   // entry -> valL slot
+  double* Kx = nullptr;      // values of K (lower CSC, as given)
+  int* Kdiag = nullptr;      // position of each diagonal entry in Kx
+  unsigned char* pert = nullptr;  // perturbed-diagonal flags
+  int nnzK = 0;
+  // full CSR of K0 (unperturbed; perturbed diagonals zeroed) in the original ordering
+  int* Krp = nullptr;
+  int* Kci = nullptr;
+  double* Kv = nullptr;
+  int64_t* Ksrc = nullptr;   // CSR entry -> lower CSC entry (to refresh values)
+  // workspace
+  double* W = nullptr;          // BK panel workspace (max over concurrent supernodes)
+  int64_t W_elems = 0;
+  int64_t* woff = nullptr;      // per-supernode offset into W
+  int* kpos = nullptr;          // next column to factor (wide supernodes)
+  int* kstart = nullptr;        // first column of the last panel
+  unsigned char* isdiag = nullptr;
+  int64_t nnzf = 0;             // entries of the full CSR
+  double* k0norm = nullptr;     // ||K0||_inf on the device
+  int* sync = nullptr;          // group-barrier counters
+  double* red = nullptr;        // reduction scratch
+  int* status = nullptr;        // device flags (zero pivots etc.)
+  // solve
+  double* X = nullptr;          // n x nrhs working vectors (row-interleaved)
+  double* Y = nullptr;
+  double* Z = nullptr;
+  double* acc = nullptr;
+  double* acc2 = nullptr;       // ordered solution block
+  double* T = nullptr;
+  double* Xs = nullptr;
+  double* V = nullptr;          // GMRES basis (mr+1) x n x nrhs
+  double* R = nullptr;
+  double* B = nullptr;
+  double* gm = nullptr;         // GMRES small state
+  double* part = nullptr;       // reduction partials
+  double* dots = nullptr;       // reduction results
+  const double** ptrs = nullptr;
+  int nblk_red = 512;
+  int mr_cap = 0;
+  // schedules
+  std::vector<FactorLevel> flevels;
+  std::vector<SolveLevel> slevels;
+  int* piece_sn = nullptr;      // piece -> supernode
+  int* piece_c0 = nullptr;      // piece -> first local column within the supernode
+  int npieces = 0;
+  std::vector<void*> owned;     // device allocations freed at destruction
+  ~DeviceFactor();
+};
+
+}  // namespace lblg

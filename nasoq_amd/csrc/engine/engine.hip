@@ -1,0 +1,546 @@
+// Host orchestration of the device LBL engine: uploads the symbolic result,
+// builds the level schedules, runs the level-scheduled supernodal
+// factorization, the blocked sweeps and the device GMRES refinement.
+//
+// Reference call flow replaced here (src/QP/linear_solver_wrapper.cpp):
+//   numerical_factorization (:602-768) -> ldl_left_sn_parallel_02 + reorder_matrix
+//   solve_only (:1305-1404) -> solve_phase_ldl_blocked_parallel_permuted +
+//                              pmgmres_ldlt_auto (refinement on the unperturbed K)
+#include "engine/engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "engine/device.hpp"
+#include "engine/handle.hpp"
+#include "engine/kernels.hpp"
+
+namespace lblg {
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+DeviceFactor::~DeviceFactor() {
+  for (void* p : owned) hipFree(p);
+  if (stream) hipStreamDestroy(stream);
+}
+
+template <typename T>
+static T* dalloc(DeviceFactor& D, int64_t count) {
+  if (count <= 0) count = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(T) * (size_t)count);
+  if (e != hipSuccess) throw std::bad_alloc();
+  D.owned.push_back(p);
+  return static_cast<T*>(p);
+}
+template <typename T>
+static T* dupload(DeviceFactor& D, const std::vector<T>& v) {
+  T* p = dalloc<T>(D, (int64_t)v.size());
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, D.stream));
+  return p;
+}
+
+static SnArgs snargs(const DeviceFactor& D) {
+  return SnArgs{D.super, D.pi, D.px, D.rows, D.col2sup, D.n};
+}
+
+static int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
+
+// ---------------------------------------------------------------------------
+// one-time device setup for an analysed handle
+// ---------------------------------------------------------------------------
+static void setup(Handle& h) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device");
+  h.dev.reset(new DeviceFactor);
+  DeviceFactor& D = *h.dev;
+  const Symbolic& S = h.sym;
+  const int n = h.n, ns = S.nsuper;
+  D.n = n;
+  D.nsuper = ns;
+  HIPCHK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
+
+  std::vector<int64_t> px(ns + 1);
+  for (int s = 0; s < ns; ++s) px[s] = S.p[S.super[s]];
+  px[ns] = S.p[n];
+  D.super = dupload(D, S.super);
+  D.pi = dupload(D, S.pi);
+  D.px = dupload(D, px);
+  std::vector<int> rows0(S.s.begin(), S.s.begin() + S.pi[ns]);
+  D.rows0 = dupload(D, rows0);
+  D.rows = dalloc<int>(D, (int64_t)rows0.size());
+  D.col2sup = dupload(D, S.col2sup);
+  D.perm0 = dupload(D, S.perm);
+  D.valL = dalloc<double>(D, S.p[n]);
+  D.dval = dalloc<double>(D, 2 * (int64_t)n);
+  D.perm_piv = dalloc<int>(D, n);
+  D.newpos = dalloc<int>(D, n);
+  D.perm_f = dalloc<int>(D, n);
+  D.iperm_f = dalloc<int>(D, n);
+  D.status = dalloc<int>(D, 16);
+
+  // entry -> panel slot map (the destination of each entry of P K P^T)
+  const int nnz = h.Ap[n];
+  D.nnzK = nnz;
+  std::vector<int64_t> amap(nnz);
+  std::vector<unsigned char> isdiag(nnz, 0);
+  for (int j = 0; j < n; ++j) {
+    for (int q = h.Ap[j]; q < h.Ap[j + 1]; ++q) {
+      const int i = h.Ai[q];
+      isdiag[q] = (i == j);
+      const int a = S.iperm[i], b = S.iperm[j];
+      const int c = std::min(a, b), rw = std::max(a, b);
+      const int t = S.col2sup[c];
+      const int* rb = S.s.data() + S.pi[t];
+      const int* re = S.s.data() + S.pi[t + 1];
+      const int64_t pos = std::lower_bound(rb, re, rw) - rb;
+      const int64_t rt = S.pi[t + 1] - S.pi[t];
+      amap[q] = px[t] + (int64_t)(c - S.super[t]) * rt + pos;
+    }
+  }
+  D.amap = dupload(D, amap);
+  D.Kx = dalloc<double>(D, nnz);
+  D.pert = dalloc<unsigned char>(D, nnz);
+  D.isdiag = dupload(D, isdiag);
+  D.red = dalloc<double>(D, 1024);
+  D.k0norm = dalloc<double>(D, 1);
+
+  // full CSR of K in the original ordering (structure) and its source entries
+  {
+    std::vector<int> cnt(n + 1, 0);
+    for (int j = 0; j < n; ++j)
+      for (int q = h.Ap[j]; q < h.Ap[j + 1]; ++q) {
+        const int i = h.Ai[q];
+        cnt[i + 1]++;
+        if (i != j) cnt[j + 1]++;
+      }
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> ci(cnt[n]);
+    std::vector<int64_t> src(cnt[n]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int j = 0; j < n; ++j)
+      for (int q = h.Ap[j]; q < h.Ap[j + 1]; ++q) {
+        const int i = h.Ai[q];
+        int d = fill[i]++;
+        ci[d] = j;
+        src[d] = q;
+        if (i != j) {
+          d = fill[j]++;
+          ci[d] = i;
+          src[d] = q;
+        }
+      }
+    D.Krp = dupload(D, cnt);
+    D.Kci = dupload(D, ci);
+    D.Ksrc = dupload(D, src);
+    D.Kv = dalloc<double>(D, cnt[n]);
+    D.nnzf = cnt[n];
+    h.csr_src = std::move(src);
+    h.csr_rp = std::move(cnt);
+  }
+
+  // ---- factor levels: height of each supernode above the leaves
+  std::vector<int> lev(ns, 0);
+  int nlev = 0;
+  for (int s = 0; s < ns; ++s) {
+    nlev = std::max(nlev, lev[s] + 1);
+    if (S.sparent[s] >= 0) lev[S.sparent[s]] = std::max(lev[S.sparent[s]], lev[s] + 1);
+  }
+  std::vector<std::vector<int>> bylev(nlev);
+  for (int s = 0; s < ns; ++s) bylev[lev[s]].push_back(s);
+  const int group_w = env_int("LBLG_GROUP_W", 2048);
+  int64_t wmax_total = 0;
+  std::vector<int64_t> woff(ns, 0);
+  D.flevels.resize(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    FactorLevel& F = D.flevels[l];
+    std::vector<int> small, off, sch, offblk, big;
+    std::vector<int64_t> tiles;
+    int blk = 0;
+    int64_t tl = 0, wsum = 0;
+    for (int s : bylev[l]) {
+      const int w = S.super[s + 1] - S.super[s];
+      const int64_t r = S.pi[s + 1] - S.pi[s];
+      F.max_w = std::max(F.max_w, w);
+      if (w <= SMALL_W) {
+        small.push_back(s);
+      } else {
+        if (w <= group_w) F.medium.push_back(s); else F.large.push_back(s);
+        const int G = w <= group_w ? 1 : env_int("LBLG_GROUPS", 32);
+        woff[s] = wsum;
+        wsum += (int64_t)w * PANEL_NB + 2 * (int64_t)w + 4 * G + 8;
+      }
+      if (r > w) {
+        off.push_back(s);
+        offblk.push_back(blk);
+        blk += (int)((r - w + 63) / 64);
+        sch.push_back(s);
+        tiles.push_back(tl);
+        const int64_t T = (r - w + 63) / 64;
+        tl += T * (T + 1) / 2;
+      }
+    }
+    wmax_total = std::max(wmax_total, wsum);
+    F.small.count = (int)small.size();
+    F.small.d = dupload(D, small);
+    F.offdiag.count = (int)off.size();
+    F.offdiag.d = dupload(D, off);
+    F.offdiag_blocks = blk;
+    F.d_offdiag_blk = dupload(D, offblk);
+    F.schur.count = (int)sch.size();
+    F.schur.d = dupload(D, sch);
+    F.schur_tiles = tl;
+    F.d_schur_tile = dupload(D, tiles);
+    F.d_medium = dupload(D, F.medium);
+    F.d_large = dupload(D, F.large);
+  }
+  D.W = dalloc<double>(D, wmax_total);
+  D.W_elems = wmax_total;
+  D.woff = dupload(D, woff);
+  D.kpos = dalloc<int>(D, ns);
+  D.kstart = dalloc<int>(D, ns);
+  D.sync = dalloc<int>(D, std::max<int64_t>(ns, 1));
+
+  // ---- solve pieces and their levels
+  std::vector<int> psn, pc0, first_piece(ns), piece_par;
+  for (int s = 0; s < ns; ++s) {
+    const int w = S.super[s + 1] - S.super[s];
+    first_piece[s] = (int)psn.size();
+    for (int c = 0; c < w; c += SOLVE_W) {
+      psn.push_back(s);
+      pc0.push_back(c);
+    }
+  }
+  const int np = (int)psn.size();
+  piece_par.assign(np, -1);
+  for (int p = 0; p < np; ++p) {
+    const int s = psn[p];
+    const int w = S.super[s + 1] - S.super[s];
+    if (pc0[p] + SOLVE_W < w) piece_par[p] = p + 1;
+    else if (S.sparent[s] >= 0) piece_par[p] = first_piece[S.sparent[s]];
+  }
+  std::vector<int> plev(np, 0);
+  int nplev = 0;
+  for (int p = 0; p < np; ++p) {  // parents come later in this numbering
+    nplev = std::max(nplev, plev[p] + 1);
+    if (piece_par[p] >= 0) plev[piece_par[p]] = std::max(plev[piece_par[p]], plev[p] + 1);
+  }
+  std::vector<std::vector<int>> pbylev(nplev);
+  for (int p = 0; p < np; ++p) pbylev[plev[p]].push_back(p);
+  D.slevels.resize(nplev);
+  for (int l = 0; l < nplev; ++l) {
+    SolveLevel& L = D.slevels[l];
+    std::vector<int> pre;
+    int nb = 0;
+    for (int p : pbylev[l]) {
+      const int s = psn[p];
+      const int w = S.super[s + 1] - S.super[s];
+      const int64_t r = S.pi[s + 1] - S.pi[s];
+      const int pw = std::min(SOLVE_W, w - pc0[p]);
+      pre.push_back(nb);
+      nb += (int)((r - pc0[p] - pw + 255) / 256);
+    }
+    L.npieces = (int)pbylev[l].size();
+    L.nblocks = nb;
+    L.d_piece = dupload(D, pbylev[l]);
+    L.d_blk = dupload(D, pre);
+  }
+  D.piece_sn = dupload(D, psn);
+  D.piece_c0 = dupload(D, pc0);
+  D.npieces = np;
+  HIPCHK(hipStreamSynchronize(D.stream));
+}
+
+// ---------------------------------------------------------------------------
+// numeric factorization
+// ---------------------------------------------------------------------------
+void factor(Handle& h) {
+  if (!h.dev) setup(h);
+  DeviceFactor& D = *h.dev;
+  HIPCHK(hipMemcpyAsync(D.Kx, h.Ax.data(), sizeof(double) * D.nnzK, hipMemcpyHostToDevice, D.stream));
+  factor_device(h);
+}
+
+void factor_device(Handle& h) {
+  DeviceFactor& D = *h.dev;
+  const Symbolic& S = h.sym;
+  const int n = h.n;
+  hipStream_t st = D.stream;
+  const double reg = h.set.reg_diag;
+  SnArgs a = snargs(D);
+  a.rows = D.rows0;  // pre-pivot labels during the factorization
+  launch_fill(D.valL, S.p[n], 0.0, st);
+  launch_fill(D.dval, 2 * (int64_t)n, 0.0, st);
+  launch_iota(D.perm_piv, n, st);
+  launch_fill_int(D.status, 16, 0, st);
+  launch_fill_int(D.kpos, D.nsuper, 0, st);
+  launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, h.set.regularization, D.pert, st);
+  launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg, D.valL, st);
+  const int groups = env_int("LBLG_GROUPS", 32);
+  for (size_t l = 0; l < D.flevels.size(); ++l) {
+    FactorLevel& F = D.flevels[l];
+    launch_bk_small(F.small.d, F.small.count, a, D.valL, D.dval, D.perm_piv, D.status, st);
+    // medium supernodes: one workgroup each, panel by panel
+    if (!F.medium.empty()) {
+      int wmax = 0;
+      for (int s : F.medium) wmax = std::max(wmax, S.super[s + 1] - S.super[s]);
+      const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
+      for (int p = 0; p < rounds; ++p) {
+        launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, D.valL, D.dval, D.perm_piv, D.W, D.woff,
+                        D.kpos, D.kstart, D.sync, D.status, false, st);
+        const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
+        const int T = (wmax - lo + 63) / 64;
+        if (T > 0)
+          launch_trailing(F.d_medium, (int)F.medium.size(), T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff,
+                          D.kpos, D.kstart, st);
+      }
+    }
+    // large supernodes: a cooperative group of workgroups each, one at a time
+    for (size_t q = 0; q < F.large.size(); ++q) {
+      const int s = F.large[q];
+      const int w = S.super[s + 1] - S.super[s];
+      const int rounds = (w + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
+      for (int p = 0; p < rounds; ++p) {
+        launch_fill_int(D.sync, 1, 0, st);
+        launch_bk_panel(F.d_large + q, 1, groups, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
+                        D.kstart, D.sync, D.status, true, st);
+        const int lo = std::min(w, (p + 1) * (PANEL_NB - 1));
+        const int T = (w - lo + 63) / 64;
+        if (T > 0)
+          launch_trailing(F.d_large + q, 1, T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff, D.kpos, D.kstart,
+                          st);
+      }
+    }
+    launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, D.valL, D.dval,
+                   D.perm_piv, st);
+    launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, D.valL, D.dval, st);
+  }
+  launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
+  // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
+  launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
+  int stat[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  h.zero_pivot = stat[0] & 1;
+  if (stat[0] & 4) throw std::runtime_error("group barrier timed out");
+  h.factored = true;
+}
+
+// ---------------------------------------------------------------------------
+// M^{-1}: P L^{-T} D^{-1} L^{-1} P^T applied to an ordered block (in -> out)
+// ---------------------------------------------------------------------------
+static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
+  if (D.nrhs_cap >= nrhs && D.mr_cap >= mr && D.X) return;
+  const int64_t nn = (int64_t)D.n * std::max(nrhs, D.nrhs_cap);
+  const int cap = std::max(nrhs, D.nrhs_cap);
+  const int mrc = std::max(mr, std::max(D.mr_cap, 1));
+  D.X = dalloc<double>(D, nn);
+  D.Y = dalloc<double>(D, nn);
+  D.Z = dalloc<double>(D, nn);
+  D.acc = dalloc<double>(D, nn);
+  D.R = dalloc<double>(D, nn);
+  D.B = dalloc<double>(D, nn);
+  D.T = dalloc<double>(D, nn);
+  D.Xs = dalloc<double>(D, nn);
+  D.V = dalloc<double>(D, nn * (mrc + 1));
+  D.gm = dalloc<double>(D, gmres_state_size(mrc) * cap);
+  D.nblk_red = 512;
+  D.part = dalloc<double>(D, (int64_t)D.nblk_red * 3 * cap * (mrc + 2));
+  D.dots = dalloc<double>(D, 3 * cap * (mrc + 2));
+  D.ptrs = dalloc<const double*>(D, 2 * (mrc + 2));
+  D.nrhs_cap = cap;
+  D.mr_cap = mrc;
+}
+
+void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
+  hipStream_t st = D.stream;
+  SnArgs a = snargs(D);
+  const int64_t nn = (int64_t)D.n * nrhs;
+  launch_copy(D.X, in, nn, st);
+  for (size_t l = 0; l < D.slevels.size(); ++l) {
+    const SolveLevel& L = D.slevels[l];
+    launch_fwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.X, D.Y, nrhs, st);
+    launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, D.Y, D.X,
+                    nrhs, st);
+  }
+  launch_dsolve(D.n, D.dval, D.Y, D.Z, nrhs, st);
+  launch_fill(D.acc, nn, 0.0, st);
+  for (size_t l = D.slevels.size(); l-- > 0;) {
+    const SolveLevel& L = D.slevels[l];
+    launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, out, D.acc,
+                    nrhs, st);
+    launch_bwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.Z, D.acc, out, nrhs, st);
+  }
+}
+
+static void spmv_ordered(DeviceFactor& D, const double* x, double* y, int nrhs) {
+  launch_spmv(D.n, D.Krp, D.Kci, D.Kv, D.perm_f, D.iperm_f, x, y, nrhs, D.stream);
+}
+
+static void dots(DeviceFactor& D, int nrhs, const std::vector<std::pair<const double*, const double*>>& pairs,
+                 double* out) {
+  std::vector<const double*> ptr(2 * pairs.size());
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    ptr[i] = pairs[i].first;
+    ptr[pairs.size() + i] = pairs[i].second;
+  }
+  HIPCHK(hipMemcpyAsync(D.ptrs, ptr.data(), sizeof(double*) * ptr.size(), hipMemcpyHostToDevice, D.stream));
+  launch_dots(D.n, nrhs, D.ptrs, D.ptrs + pairs.size(), (int)pairs.size(), D.part, D.nblk_red, out, D.stream);
+}
+
+// one GMRES(mr) cycle per column, pmgmres_ldlt_auto (mgmres.cpp:1015-1211)
+static void gmres_cycle(DeviceFactor& D, const double* b, double* x, int nrhs, int mr, int itr,
+                        double tol_abs, double tol_rel) {
+  hipStream_t st = D.stream;
+  const int n = D.n;
+  const int64_t nn = (int64_t)n * nrhs;
+  const int MR = D.mr_cap;
+  const int64_t SS = gmres_state_size(MR);
+  double* mask = D.gm + gmres_off_S(MR) + 7;
+  double* coef0 = D.gm + gmres_off_D(MR);
+  double* reo = D.gm + gmres_off_D(MR) + 1;
+  auto Vk = [&](int k) { return D.V + (int64_t)k * nn; };
+  spmv_ordered(D, x, D.T, nrhs);
+  launch_sub(nn, b, D.T, D.R, st);
+  apply_solve(D, D.R, D.Xs, nrhs);
+  dots(D, nrhs, {{D.Xs, D.Xs}}, D.dots);
+  launch_gmres_scalar(0, nrhs, MR, D.gm, D.dots, 0, 0, itr, tol_abs, tol_rel, st);
+  launch_fill(D.V, nn * (MR + 1), 0.0, st);
+  launch_axpby(n, nrhs, coef0, (int)SS, D.Xs, Vk(0), mask, (int)SS, 0, 1, st);
+  for (int k = 0; k < mr; ++k) {
+    spmv_ordered(D, Vk(k), D.T, nrhs);
+    apply_solve(D, D.T, Vk(k + 1), nrhs);
+    dots(D, nrhs, {{Vk(k + 1), Vk(k + 1)}}, D.dots);
+    launch_gmres_scalar(1, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+    for (int j = 0; j <= k; ++j) {
+      dots(D, nrhs, {{Vk(k + 1), Vk(j)}}, D.dots);
+      launch_gmres_scalar(2, nrhs, MR, D.gm, D.dots, j, k, itr, tol_abs, tol_rel, st);
+      launch_axpby(n, nrhs, coef0, (int)SS, Vk(j), Vk(k + 1), mask, (int)SS, 0, 0, st);
+    }
+    dots(D, nrhs, {{Vk(k + 1), Vk(k + 1)}}, D.dots);
+    launch_gmres_scalar(3, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+    for (int j = 0; j <= k; ++j) {  // conditional re-orthogonalisation (flag on device)
+      dots(D, nrhs, {{Vk(k + 1), Vk(j)}}, D.dots);
+      launch_gmres_scalar(4, nrhs, MR, D.gm, D.dots, j, k, itr, tol_abs, tol_rel, st);
+      launch_axpby(n, nrhs, coef0, (int)SS, Vk(j), Vk(k + 1), reo, (int)SS, 0, 0, st);
+    }
+    dots(D, nrhs, {{Vk(k + 1), Vk(k + 1)}}, D.dots);
+    launch_gmres_scalar(5, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+    launch_axpby(n, nrhs, coef0, (int)SS, Vk(k + 1), Vk(k + 1), mask, (int)SS, 0, 1, st);
+    launch_gmres_scalar(6, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+  }
+  launch_gmres_scalar(7, nrhs, MR, D.gm, D.dots, 0, 0, itr, tol_abs, tol_rel, st);
+  launch_gmres_update(n, nrhs, MR, D.V, D.gm, x, st);
+}
+
+// backward error per column in the ordered space (permutation invariant):
+// ||b - K0 x||_1 / (||K0||_inf ||x||_1 + ||b||_1)
+static void backward_errors(Handle& h, const double* b, const double* x, int nrhs, double* err) {
+  DeviceFactor& D = *h.dev;
+  const int64_t nn = (int64_t)D.n * nrhs;
+  spmv_ordered(D, x, D.T, nrhs);
+  launch_sub(nn, b, D.T, D.R, D.stream);
+  launch_abs_sums(D.n, nrhs, D.R, x, b, D.part, D.nblk_red, D.dots, D.stream);
+  std::vector<double> sums(3 * nrhs);
+  HIPCHK(hipMemcpyAsync(sums.data(), D.dots, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, D.stream));
+  HIPCHK(hipStreamSynchronize(D.stream));
+  const double an = h.k0_norm_inf;
+  for (int c = 0; c < nrhs; ++c) {
+    const double den = an * sums[nrhs + c] + sums[2 * nrhs + c];
+    err[c] = den > 0 ? sums[c] / den : 0.0;
+  }
+}
+
+// solve for an ordered-interleaved right-hand side block already on the
+// device (D.B), result in D.Xs-independent buffer xo
+void solve_ordered(Handle& h, double* xo, int nrhs, double* bwd_out) {
+  DeviceFactor& D = *h.dev;
+  const lblg_settings& s = h.set;
+  const int mr = std::max(s.max_inner_iter, 1);
+  hipStream_t st = D.stream;
+  apply_solve(D, D.B, xo, nrhs);
+  h.num_ref_iter = 0;
+  if (s.req_ref_iter > 0 || s.target_bwd_err > 0) {
+    launch_fill(D.gm, gmres_state_size(D.mr_cap) * nrhs, 0.0, st);
+    // mark every column running
+    std::vector<double> on(nrhs, 1.0);
+    for (int c = 0; c < nrhs; ++c)
+      HIPCHK(hipMemcpyAsync(D.gm + c * gmres_state_size(D.mr_cap) + gmres_off_S(D.mr_cap) + 3, on.data(),
+                            sizeof(double), hipMemcpyHostToDevice, st));
+    if (s.target_bwd_err > 0) {
+      std::vector<double> err(nrhs);
+      backward_errors(h, D.B, xo, nrhs, err.data());
+      int rounds = 0;
+      while (*std::max_element(err.begin(), err.end()) > s.target_bwd_err && rounds < s.max_refine_rounds) {
+        // each round: one reference GMRES cycle (itr_max = 1 with mr inner steps)
+        launch_fill(D.gm, gmres_state_size(D.mr_cap) * nrhs, 0.0, st);
+        for (int c = 0; c < nrhs; ++c)
+          HIPCHK(hipMemcpyAsync(D.gm + c * gmres_state_size(D.mr_cap) + gmres_off_S(D.mr_cap) + 3, on.data(),
+                                sizeof(double), hipMemcpyHostToDevice, st));
+        gmres_cycle(D, D.B, xo, nrhs, mr, 0, 0.0, 0.0);
+        ++rounds;
+        backward_errors(h, D.B, xo, nrhs, err.data());
+      }
+      h.num_ref_iter = rounds;
+      if (bwd_out) std::copy(err.begin(), err.end(), bwd_out);
+      return;
+    }
+    for (int itr = 0; itr < s.req_ref_iter; ++itr)
+      gmres_cycle(D, D.B, xo, nrhs, mr, itr, s.tol_abs, s.tol_rel);
+    h.num_ref_iter = s.req_ref_iter;
+  }
+  if (bwd_out) backward_errors(h, D.B, xo, nrhs, bwd_out);
+}
+
+void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx, bool device_ptrs,
+           double* bwd_out) {
+  DeviceFactor& D = *h.dev;
+  const int mr = std::max(h.set.max_inner_iter, 1);
+  ensure_solve_buffers(D, nrhs, mr);
+  hipStream_t st = D.stream;
+  const double* Bd = B;
+  double* Xd = X;
+  double* tmpB = nullptr;
+  double* tmpX = nullptr;
+  if (!device_ptrs) {
+    tmpB = dalloc<double>(D, (int64_t)D.n * nrhs);
+    tmpX = dalloc<double>(D, (int64_t)D.n * nrhs);
+    for (int c = 0; c < nrhs; ++c)
+      HIPCHK(hipMemcpyAsync(tmpB + (int64_t)c * D.n, B + (int64_t)c * ldb, sizeof(double) * D.n,
+                            hipMemcpyHostToDevice, st));
+    Bd = tmpB;
+    Xd = tmpX;
+  }
+  launch_gather_rhs(D.n, nrhs, Bd, device_ptrs ? ldb : D.n, D.perm_f, D.B, st);
+  double* xo = D.acc2 ? D.acc2 : (D.acc2 = dalloc<double>(D, (int64_t)D.n * D.nrhs_cap));
+  solve_ordered(h, xo, nrhs, bwd_out);
+  launch_scatter_sol(D.n, nrhs, xo, D.perm_f, Xd, device_ptrs ? ldx : D.n, st);
+  if (!device_ptrs) {
+    for (int c = 0; c < nrhs; ++c)
+      HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, tmpX + (int64_t)c * D.n, sizeof(double) * D.n,
+                            hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // release the staging buffers
+    for (double* p : {tmpB, tmpX}) {
+      auto it = std::find(D.owned.begin(), D.owned.end(), (void*)p);
+      if (it != D.owned.end()) D.owned.erase(it);
+      hipFree(p);
+    }
+  }
+}
+
+}  // namespace lblg

@@ -1,0 +1,17 @@
+// Host-side driver of the device engine (engine.hip).
+#pragma once
+#include <cstdint>
+
+#include "engine/handle.hpp"
+
+namespace lblg {
+struct DeviceFactor;
+// upload values from the handle's host copy and factor
+void factor(Handle& h);
+// re-factor from the values already resident on the device
+void factor_device(Handle& h);
+// solve nrhs right-hand sides (column-major, leading dimensions ldb/ldx) with
+// the refinement the settings ask for; pointers are host or device memory
+void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx, bool device_ptrs,
+           double* bwd_out);
+}  // namespace lblg

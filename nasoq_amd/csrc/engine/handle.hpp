@@ -1,0 +1,35 @@
+// Engine handle: host copy of the KKT matrix, settings, symbolic analysis and
+// (once factored) the device-resident factor.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "host/symbolic.hpp"
+#include "lblg.h"
+
+namespace lblg {
+
+struct DeviceFactor;  // engine/device.hpp
+
+struct Handle {
+  int n = 0;
+  std::vector<int> Ap, Ai;
+  std::vector<double> Ax;          // values as given (unperturbed)
+  lblg_settings set{};
+  Symbolic sym;
+  bool analyzed = false;
+  bool factored = false;
+  std::vector<int> perturbed;      // columns whose diagonal was replaced by reg_diag
+  std::vector<unsigned char> isdiag;
+  std::vector<int64_t> csr_src;
+  std::vector<int> csr_rp;
+  double k0_norm_inf = 0.0;
+  int num_ref_iter = 0;
+  int zero_pivot = 0;
+  double factor_seconds = 0.0;
+  std::unique_ptr<DeviceFactor> dev;
+  Handle();
+  ~Handle();
+};
+
+}  // namespace lblg

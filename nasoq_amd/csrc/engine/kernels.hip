@@ -1,0 +1,1485 @@
+// gfx950 kernels of the LBL engine: assembly, Bunch-Kaufman factorization of
+// diagonal blocks (LDS-resident for narrow supernodes, panel-blocked with a
+// workgroup-group barrier for wide ones), FP64 MFMA trailing and
+// Schur-complement updates with atomic scatter into ancestor panels,
+// off-diagonal TRSM, the blocked L / D / L^T sweeps, the symmetric SpMV and
+// the GMRES vector kernels.
+//
+// Reference counterparts: ldl_left_sn_parallel_02
+// (src/ldl/parallel_blocked_ldlt_02.cpp:17-366), LAPACK dsytf2/dlasyf as
+// reached through LAPACKE_dsytrf at :155, reorder_after_sytrf,
+// blocked_2by2_solver, blocked_2by2_mult (src/common/Sym_BLAS.cpp:128-196,
+// 492-529, 573-599), H2LeveledBlockedLsolve / H2LeveledBlockedLTsolve
+// (src/triangularSolve/Triangular_BCSC.cpp:378-446, 534-600),
+// spmv_csc_sym_one_int (src/matrixVector/spmv_CSC.cpp:26-103) and the vector
+// steps of pmgmres_ldlt_auto (src/gmres/mgmres.cpp:1015-1211).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "engine/kernels.hpp"
+
+namespace lblg {
+
+#define HIP_LAUNCH_CHECK(what)                                                                     \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+#define BK_ALPHA 0.6403882032022076  // (1 + sqrt(17)) / 8
+#define SFMIN 2.2250738585072014e-308
+
+static int grid_for(int64_t count, int block) {
+  int64_t g = (count + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// --------------------------------------------------------------------------
+// small helpers
+// --------------------------------------------------------------------------
+__global__ void k_fill(double* p, int64_t count, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_int(int* p, int64_t count, int v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_iota(int* p, int count) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = i;
+}
+__global__ void k_copy(double* dst, const double* src, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_fill(double* p, int64_t count, double v, hipStream_t st) {
+  if (count <= 0) return;
+  k_fill<<<grid_for(count, 256), 256, 0, st>>>(p, count, v);
+  HIP_LAUNCH_CHECK("fill");
+}
+void launch_fill_int(int* p, int64_t count, int v, hipStream_t st) {
+  if (count <= 0) return;
+  k_fill_int<<<grid_for(count, 256), 256, 0, st>>>(p, count, v);
+  HIP_LAUNCH_CHECK("fill_int");
+}
+void launch_iota(int* p, int count, hipStream_t st) {
+  if (count <= 0) return;
+  k_iota<<<grid_for(count, 256), 256, 0, st>>>(p, count);
+  HIP_LAUNCH_CHECK("iota");
+}
+void launch_copy(double* dst, const double* src, int64_t count, hipStream_t st) {
+  if (count <= 0) return;
+  k_copy<<<grid_for(count, 256), 256, 0, st>>>(dst, src, count);
+  HIP_LAUNCH_CHECK("copy");
+}
+
+// --------------------------------------------------------------------------
+// assembly (parallel_blocked_ldlt_02.cpp:81-91) with the diagonal
+// perturbation of find_perturbation (linear_solver_wrapper.cpp:352-363)
+// --------------------------------------------------------------------------
+__global__ void k_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz,
+                           double reg, double* valL) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+    const double v = pert[q] ? reg : Kx[q];
+    valL[amap[q]] = v;
+  }
+}
+void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
+                     double* valL, hipStream_t st) {
+  if (nnz <= 0) return;
+  k_assemble<<<grid_for(nnz, 256), 256, 0, st>>>(amap, Kx, pert, nnz, reg, valL);
+  HIP_LAUNCH_CHECK("assemble");
+}
+
+// --------------------------------------------------------------------------
+// reductions with LAPACK idamax tie-breaking (first index of the maximum)
+// --------------------------------------------------------------------------
+__device__ inline void better(double& v, int& i, double ov, int oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+__device__ inline void wave_argmax(double& v, int& i) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_xor(v, off, 64);
+    int oi = __shfl_xor(i, off, 64);
+    better(v, i, ov, oi);
+  }
+}
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// workgroup argmax over 256 threads; result valid in all threads
+__device__ inline void block_argmax(double& v, int& i, double* sv, int* si) {
+  wave_argmax(v, i);
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  __syncthreads();
+  if (ln == 0) {
+    sv[wv] = v;
+    si[wv] = i;
+  }
+  __syncthreads();
+  v = sv[0];
+  i = si[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) better(v, i, sv[q], si[q]);
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------
+// Bunch-Kaufman of a narrow diagonal block (w <= 64) held in LDS.
+// One workgroup (256 threads) per supernode.  Pivot decisions follow LAPACK
+// dsytf2 (uplo = 'L').  Each interchange is also applied to the earlier
+// columns of L, giving the standard form that reorder_after_sytrf
+// (Sym_BLAS.cpp:128-196) produces; the 2x2 sub-diagonals go to
+// dval[n + c] and the diagonal of the block becomes 1.
+// --------------------------------------------------------------------------
+#define LDA_S 65
+__global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* super, const int64_t* pi,
+                                                  const int64_t* px, double* valL, double* dval,
+                                                  int* perm_piv, int n, int* status) {
+  __shared__ double a[64 * LDA_S];
+  __shared__ double wk[64], wk1[64];
+  __shared__ int lperm[64], kind[64];
+  __shared__ int sh_kp, sh_kstep, sh_zero;
+  const int s = list[blockIdx.x];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  double* P = valL + px[s];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < w * w; idx += 256) {
+    const int i = idx % w, j = idx / w;
+    a[i + j * LDA_S] = (i >= j) ? P[i + j * r] : 0.0;
+  }
+  if (t < 64) {
+    lperm[t] = t;
+    kind[t] = 1;
+  }
+  __syncthreads();
+  int k = 0;
+  while (k < w) {
+    if (t < 64) {  // wave 0 takes the pivot decision
+      const double absakk = fabs(a[k + k * LDA_S]);
+      const int i = k + 1 + t;
+      double v = (i < w) ? fabs(a[i + k * LDA_S]) : -1.0;
+      if (!(v == v)) v = -1.0;
+      int iv = (i < w) ? i : 0x7fffffff;
+      wave_argmax(v, iv);
+      const double colmax = (k < w - 1) ? v : 0.0;
+      const int imax = (k < w - 1) ? iv : k;
+      int kp = k, kstep = 1, zero = 0;
+      if (fmax(absakk, colmax) == 0.0 || !(absakk == absakk)) {
+        zero = 1;
+      } else if (!(absakk >= BK_ALPHA * colmax)) {
+        const int j = k + t;
+        double rv = -1.0;
+        if (j < imax) rv = fabs(a[imax + j * LDA_S]);
+        else if (j > imax && j < w) rv = fabs(a[j + imax * LDA_S]);
+        int dummy = 0;
+        wave_argmax(rv, dummy);
+        const double rowmax = rv;
+        if (absakk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+          kp = k;
+        } else if (fabs(a[imax + imax * LDA_S]) >= BK_ALPHA * rowmax) {
+          kp = imax;
+        } else {
+          kp = imax;
+          kstep = 2;
+        }
+      }
+      if (t == 0) {
+        sh_kp = kp;
+        sh_kstep = kstep;
+        sh_zero = zero;
+      }
+    }
+    __syncthreads();
+    const int kp = sh_kp, kstep = sh_kstep, kk = k + kstep - 1;
+    const int zero = sh_zero;
+    if (zero && t == 0) atomicOr(status, 1);
+    if (kp != kk) {
+      for (int q = t; q < 64 * 3; q += 256) {
+        const int part = q / 64, x = q % 64;
+        if (part == 0) {
+          const int i = kp + 1 + x;
+          if (i < w) {
+            const double tmp = a[i + kk * LDA_S];
+            a[i + kk * LDA_S] = a[i + kp * LDA_S];
+            a[i + kp * LDA_S] = tmp;
+          }
+        } else if (part == 1) {
+          const int j = kk + 1 + x;
+          if (j < kp) {
+            const double tmp = a[j + kk * LDA_S];
+            a[j + kk * LDA_S] = a[kp + j * LDA_S];
+            a[kp + j * LDA_S] = tmp;
+          }
+        } else {
+          const int j = x;
+          if (j < k) {
+            const double tmp = a[kk + j * LDA_S];
+            a[kk + j * LDA_S] = a[kp + j * LDA_S];
+            a[kp + j * LDA_S] = tmp;
+          }
+        }
+      }
+      if (t == 0) {
+        double tmp = a[kk + kk * LDA_S];
+        a[kk + kk * LDA_S] = a[kp + kp * LDA_S];
+        a[kp + kp * LDA_S] = tmp;
+        if (kstep == 2) {
+          tmp = a[k + 1 + k * LDA_S];
+          a[k + 1 + k * LDA_S] = a[kp + k * LDA_S];
+          a[kp + k * LDA_S] = tmp;
+        }
+        const int tp = lperm[kk];
+        lperm[kk] = lperm[kp];
+        lperm[kp] = tp;
+      }
+      __syncthreads();
+    }
+    if (t == 0 && kstep == 2) {
+      kind[k] = 2;
+      kind[k + 1] = 0;
+    }
+    if (!zero) {
+      if (kstep == 1) {
+        if (k < w - 1) {
+          const double akk = a[k + k * LDA_S];
+          const bool big = fabs(akk) >= SFMIN;
+          const double d11 = 1.0 / akk;
+          const int m = w - k - 1;
+          if (!big) {
+            for (int i = k + 1 + t; i < w; i += 256) a[i + k * LDA_S] /= akk;
+            __syncthreads();
+          }
+          for (int idx = t; idx < m * m; idx += 256) {
+            const int i = k + 1 + idx % m, j = k + 1 + idx / m;
+            if (i >= j) {
+              const double f = big ? -d11 * a[j + k * LDA_S] : -akk * a[j + k * LDA_S];
+              a[i + j * LDA_S] += a[i + k * LDA_S] * f;
+            }
+          }
+          __syncthreads();
+          if (big)
+            for (int i = k + 1 + t; i < w; i += 256) a[i + k * LDA_S] *= d11;
+        }
+      } else if (k < w - 2) {
+        double d21 = a[k + 1 + k * LDA_S];
+        const double d11 = a[k + 1 + (k + 1) * LDA_S] / d21;
+        const double d22 = a[k + k * LDA_S] / d21;
+        const double tt = 1.0 / (d11 * d22 - 1.0);
+        d21 = tt / d21;
+        for (int j = k + 2 + t; j < w; j += 256) {
+          wk[j] = d21 * (d11 * a[j + k * LDA_S] - a[j + (k + 1) * LDA_S]);
+          wk1[j] = d21 * (d22 * a[j + (k + 1) * LDA_S] - a[j + k * LDA_S]);
+        }
+        __syncthreads();
+        const int m = w - k - 2;
+        for (int idx = t; idx < m * m; idx += 256) {
+          const int i = k + 2 + idx % m, j = k + 2 + idx / m;
+          if (i >= j) a[i + j * LDA_S] -= a[i + k * LDA_S] * wk[j] + a[i + (k + 1) * LDA_S] * wk1[j];
+        }
+        __syncthreads();
+        for (int j = k + 2 + t; j < w; j += 256) {
+          a[j + k * LDA_S] = wk[j];
+          a[j + (k + 1) * LDA_S] = wk1[j];
+        }
+      }
+    }
+    __syncthreads();
+    k += kstep;
+  }
+  // D extraction and unit diagonal
+  if (t < w) {
+    const int l = t;
+    dval[c0 + l] = a[l + l * LDA_S];
+    dval[n + c0 + l] = 0.0;
+    if (kind[l] == 2) dval[n + c0 + l] = a[l + 1 + l * LDA_S];
+    perm_piv[c0 + l] = c0 + lperm[l];
+  }
+  __syncthreads();
+  if (t < w) {
+    const int l = t;
+    if (kind[l] == 2) a[l + 1 + l * LDA_S] = 0.0;
+    a[l + l * LDA_S] = 1.0;
+  }
+  __syncthreads();
+  for (int idx = t; idx < w * w; idx += 256) {
+    const int i = idx % w, j = idx / w;
+    if (i >= j) P[i + j * r] = a[i + j * LDA_S];
+  }
+}
+void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, double* dval,
+                     int* perm_piv, int* status, hipStream_t st) {
+  if (count <= 0) return;
+  k_bk_small<<<count, 256, 0, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n, status);
+  HIP_LAUNCH_CHECK("bk_small");
+}
+
+// --------------------------------------------------------------------------
+// group barrier among the G workgroups that share one supernode.  The wait
+// is bounded: after the spin limit the barrier flags status bit 2 and
+// proceeds, so no wave can wait forever.
+// --------------------------------------------------------------------------
+__device__ inline void group_barrier(int* ctr, int G, int& epoch, int* status) {
+  __threadfence();
+  __syncthreads();
+  ++epoch;
+  if (G > 1) {
+    if (threadIdx.x == 0) {
+      atomicAdd(ctr, 1);
+      const int target = epoch * G;
+      long spins = 0;
+      while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1L << 26)) {
+          atomicOr(status, 4);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    __threadfence();
+  }
+}
+
+// --------------------------------------------------------------------------
+// Panel Bunch-Kaufman for a wide diagonal block (w > SMALL_W), LAPACK dlasyf
+// style: columns of the current panel are formed left-looking from the
+// panel's L and W = L*D, pivots are chosen by the dsytf2 rules on the fully
+// updated columns, and the trailing block is updated afterwards
+// (k_trailing).  Interchanges are applied to all earlier columns of L and to
+// the columns of the rows below the diagonal block (the column permutation
+// of LAPACKE_dlapmt at parallel_blocked_ldlt_02.cpp:164-165), so no separate
+// permutation pass is needed.  G workgroups cooperate on one supernode.
+//
+// Per-supernode workspace at W + woff[s]:  Wp (w x NB, ld w) | t0 (w) | t1 (w)
+//                                         | part (2 G doubles) | parti (2 G ints)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const int* super,
+                                                  const int64_t* pi, const int64_t* px, double* valL,
+                                                  double* dval, int* perm_piv, double* Wbase,
+                                                  const int64_t* woff, int* kpos, int* kstart,
+                                                  int* sync, int n, int* status) {
+  __shared__ double sv[8];
+  __shared__ int si[8];
+  const int sidx = blockIdx.x / G, g = blockIdx.x % G;
+  const int s = list[sidx];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  double* A = valL + px[s];
+  double* Wp = Wbase + woff[s];
+  double* t0 = Wp + (int64_t)w * PANEL_NB;
+  double* t1 = t0 + w;
+  double* part = t1 + w;
+  int* parti = reinterpret_cast<int*>(part + 2 * G);
+  int* ctr = sync + sidx;
+  const int gt = g * 256 + threadIdx.x, GT = G * 256;
+  const int k0 = kpos[s];
+  if (k0 >= w) return;  // whole group returns together (k0 is uniform)
+  const bool last = (w - k0) <= PANEL_NB;
+  int epoch = 0;
+  int k = k0;
+#define AA(i, j) A[(int64_t)(j) * r + (i)]
+#define WW(i, j) Wp[(int64_t)((j) - k0) * w + (i)]
+  while (k < w && (last || k - k0 < PANEL_NB - 1)) {
+    // ---- phase A: updated column k for own rows, local argmax
+    double bv = -1.0;
+    int bi = 0x7fffffff;
+    for (int i = k + gt; i < w; i += GT) {
+      double v = AA(i, k);
+      for (int j = k0; j < k; ++j) v -= AA(i, j) * WW(k, j);
+      t0[i] = v;
+      if (i > k) {
+        double av = fabs(v);
+        if (!(av == av)) av = -1.0;
+        better(bv, bi, av, i);
+      }
+    }
+    block_argmax(bv, bi, sv, si);
+    if (threadIdx.x == 0) {
+      part[g] = bv;
+      parti[g] = bi;
+    }
+    group_barrier(ctr, G, epoch, status);
+    double colmax = -1.0;
+    int imax = 0x7fffffff;
+    for (int q = 0; q < G; ++q) better(colmax, imax, part[q], parti[q]);
+    if (k == w - 1 || colmax < 0) {
+      colmax = 0.0;
+      imax = k;
+    }
+    const double akk_v = t0[k];
+    const double absakk = fabs(akk_v);
+    int kp = k, kstep = 1, zero = 0;
+    bool use_t1_as_k = false;
+    if (fmax(absakk, colmax) == 0.0 || !(absakk == absakk)) {
+      zero = 1;
+    } else if (!(absakk >= BK_ALPHA * colmax)) {
+      // ---- phase B: updated column imax, row maximum
+      double rv = -1.0;
+      int ri = 0x7fffffff;
+      for (int i = k + gt; i < w; i += GT) {
+        double v = (i < imax) ? AA(imax, i) : AA(i, imax);
+        for (int j = k0; j < k; ++j) v -= AA(i, j) * WW(imax, j);
+        t1[i] = v;
+        if (i != imax) {
+          double av = fabs(v);
+          if (!(av == av)) av = -1.0;
+          better(rv, ri, av, i);
+        }
+      }
+      block_argmax(rv, ri, sv, si);
+      if (threadIdx.x == 0) {
+        part[G + g] = rv;
+        parti[G + g] = ri;
+      }
+      group_barrier(ctr, G, epoch, status);
+      double rowmax = -1.0;
+      int rj = 0x7fffffff;
+      for (int q = 0; q < G; ++q) better(rowmax, rj, part[G + q], parti[G + q]);
+      if (rowmax < 0) rowmax = 0.0;
+      if (absakk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+        kp = k;
+      } else if (fabs(t1[imax]) >= BK_ALPHA * rowmax) {
+        kp = imax;
+        use_t1_as_k = true;
+      } else {
+        kp = imax;
+        kstep = 2;
+      }
+    }
+    const int kk = k + kstep - 1;
+    const bool swap = kp != kk;
+    // ---- phase C: interchange, W and L columns
+    if (zero && gt == 0) atomicOr(status, 1);
+    if (swap) {
+      // rows kk/kp of the earlier L columns and of the panel's W columns
+      for (int j = gt; j < k; j += GT) {
+        const double x = AA(kk, j);
+        AA(kk, j) = AA(kp, j);
+        AA(kp, j) = x;
+      }
+      for (int j = k0 + gt; j < k; j += GT) {
+        const double x = WW(kk, j);
+        WW(kk, j) = WW(kp, j);
+        WW(kp, j) = x;
+      }
+      // columns kk/kp of the rows below the diagonal block
+      for (int64_t q = w + gt; q < r; q += GT) {
+        const double x = AA(q, kk);
+        AA(q, kk) = AA(q, kp);
+        AA(q, kp) = x;
+      }
+      if (gt == 0) {
+        const int tp = perm_piv[c0 + kk];
+        perm_piv[c0 + kk] = perm_piv[c0 + kp];
+        perm_piv[c0 + kp] = tp;
+      }
+    }
+    // own rows: non-updated copy of column kk into column kp, then W and L
+    const double* colk = use_t1_as_k ? t1 : t0;
+    double D1 = 0, D2 = 0, OD = 0, d21 = 0, d11 = 0, d22 = 0, rk = 0;
+    if (kstep == 1) {
+      const int sk = swap ? (k == kk ? kp : k) : k;
+      D1 = colk[sk];
+      rk = 1.0 / D1;
+    } else {
+      // post-interchange values of W(k,k), W(k+1,k), W(k+1,k+1)
+      D1 = t0[k];
+      OD = t0[kp];       // row k+1 after the interchange holds row kp
+      D2 = t1[kp];
+      d21 = OD;
+      d11 = D2 / d21;
+      d22 = D1 / d21;
+      const double tt = 1.0 / (d11 * d22 - 1.0);
+      d21 = tt / d21;
+    }
+    for (int i = k + gt; i < w; i += GT) {
+      const int si2 = !swap ? i : (i == kk ? kp : (i == kp ? kk : i));
+      if (swap) {
+        if (i == kk) AA(kp, kp) = AA(kk, kk);
+        else if (i > kk && i < kp) AA(kp, i) = AA(i, kk);
+        else if (i > kp) AA(i, kp) = AA(i, kk);
+      }
+      if (kstep == 1) {
+        const double wv = colk[si2];
+        WW(i, k) = wv;
+        if (i == k) {
+          AA(k, k) = 1.0;
+          dval[c0 + k] = wv;
+          dval[n + c0 + k] = 0.0;
+        } else {
+          AA(i, k) = zero ? wv : wv * rk;
+        }
+      } else {
+        const double w0 = t0[si2], w1 = t1[si2];
+        WW(i, k) = w0;
+        WW(i, k + 1) = w1;
+        if (i == k) {
+          AA(k, k) = 1.0;
+          dval[c0 + k] = D1;
+          dval[n + c0 + k] = OD;
+        } else if (i == k + 1) {
+          AA(k + 1, k) = 0.0;
+          AA(k + 1, k + 1) = 1.0;
+          dval[c0 + k + 1] = D2;
+          dval[n + c0 + k + 1] = 0.0;
+        } else {
+          AA(i, k) = d21 * (d11 * w0 - w1);
+          AA(i, k + 1) = d21 * (d22 * w1 - w0);
+        }
+      }
+    }
+    group_barrier(ctr, G, epoch, status);
+    k += kstep;
+  }
+  if (gt == 0) {
+    kstart[s] = k0;
+    kpos[s] = k;
+  }
+#undef AA
+#undef WW
+}
+void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double* valL, double* dval,
+                     int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+                     int* status, bool cooperative, hipStream_t st) {
+  if (count <= 0) return;
+  if (!cooperative || G == 1) {
+    k_bk_panel<<<count * G, 256, 0, st>>>(list, G, a.super, a.pi, a.px, valL, dval, perm_piv, W, woff,
+                                          kpos, kstart, sync, a.n, status);
+  } else {
+    int n = a.n;
+    void* args[] = {(void*)&list, (void*)&G, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
+                    (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
+                    (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&n, (void*)&status};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel, dim3(count * G), dim3(256), args,
+                                              0, st);
+    if (e != hipSuccess) throw std::runtime_error(std::string("bk_panel coop: ") + hipGetErrorString(e));
+  }
+  HIP_LAUNCH_CHECK("bk_panel");
+}
+
+// --------------------------------------------------------------------------
+// FP64 MFMA tile engine: a 256-thread workgroup computes a 64x64 block
+// C(i,j) = sum_k A(i,k) B(j,k) with A and B staged in LDS as [k][64]
+// (k-major).  Each wave owns a 32x32 quadrant = 2x2 blocks of
+// v_mfma_f64_16x16x4f64.  Operand lanes: A(row = lane%16, k = lane/16),
+// B(col = lane%16, k = lane/16); accumulator element e of a lane holds
+// row 4*e + lane/16, column lane%16 (checked by lblg_selftest_mfma).
+// --------------------------------------------------------------------------
+#define KC 32          // k-chunk staged in LDS
+#define LDT 66         // padded row length of a staged tile
+struct TileAcc {
+  d4 c[2][2];
+};
+__device__ inline void tile_zero(TileAcc& acc) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+// As, Bs: [KC][LDT]; kc = number of valid k (multiple of 4 after zero padding)
+__device__ inline void tile_mma(TileAcc& acc, const double* As, const double* Bs, int kc) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
+  const int li = ln & 15, lk = ln >> 4;
+  for (int k = 0; k < kc; k += 4) {
+    const double a0 = As[(k + lk) * LDT + r0 + li];
+    const double a1 = As[(k + lk) * LDT + r0 + 16 + li];
+    const double b0 = Bs[(k + lk) * LDT + q0 + li];
+    const double b1 = Bs[(k + lk) * LDT + q0 + 16 + li];
+    acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.c[0][0], 0, 0, 0);
+    acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.c[0][1], 0, 0, 0);
+    acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.c[1][0], 0, 0, 0);
+    acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc.c[1][1], 0, 0, 0);
+  }
+}
+// visit every accumulator element: f(row_in_tile, col_in_tile, value)
+template <typename F>
+__device__ inline void tile_foreach(const TileAcc& acc, F f) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + 16 * a + 4 * e + (ln >> 4);
+        const int col = q0 + 16 * b + (ln & 15);
+        f(row, col, acc.c[a][b][e]);
+      }
+}
+
+__global__ void k_mfma_selftest(const double* A, const double* B, double* C) {
+  // one wave: C(16x16) = A(16x4) * B(4x16), A row-major, B row-major
+  const int ln = threadIdx.x;
+  d4 acc = {0, 0, 0, 0};
+  const double a = A[(ln & 15) * 4 + (ln >> 4)];
+  const double b = B[(ln >> 4) * 16 + (ln & 15)];
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int e = 0; e < 4; ++e) C[(4 * e + (ln >> 4)) * 16 + (ln & 15)] = acc[e];
+}
+void launch_mfma_selftest(const double* A, const double* B, double* C, hipStream_t st) {
+  k_mfma_selftest<<<1, 64, 0, st>>>(A, B, C);
+  HIP_LAUNCH_CHECK("mfma_selftest");
+}
+
+// --------------------------------------------------------------------------
+// trailing update of a wide diagonal block after a panel:
+// A(i,j) -= sum_{l in [k0,k1)} L(i,l) W(j,l)  for k1 <= j <= i < w
+// (dlasyf's dgemm/dgemv update of A22).  Tiles cover [lo, w)^2, lo <= k1.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per_sn, int lo,
+                                                  const int* super, const int64_t* pi, const int64_t* px,
+                                                  double* valL, const double* Wbase, const int64_t* woff,
+                                                  const int* kpos, const int* kstart) {
+  __shared__ double As[KC * LDT], Bs[KC * LDT];
+  const int sidx = blockIdx.x / tiles_per_sn, tix = blockIdx.x % tiles_per_sn;
+  const int s = list[sidx];
+  const int w = super[s + 1] - super[s];
+  const int64_t r = pi[s + 1] - pi[s];
+  const int k0 = kstart[s], k1 = kpos[s];
+  if (k1 >= w || k1 <= k0) return;
+  // lower-triangular tile index -> (ti, tj), tj <= ti
+  int ti = (int)((sqrt(8.0 * tix + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= tix) ++ti;
+  while (ti * (ti + 1) / 2 > tix) --ti;
+  const int tj = tix - ti * (ti + 1) / 2;
+  const int i0 = lo + 64 * ti, j0 = lo + 64 * tj;
+  if (i0 >= w || i0 + 63 < k1 || j0 >= w || j0 + 63 < k1) return;
+  double* A = valL + px[s];
+  const double* Wp = Wbase + woff[s];
+  const int kb = k1 - k0;
+  TileAcc acc;
+  tile_zero(acc);
+  for (int kc0 = 0; kc0 < kb; kc0 += KC) {
+    const int kc = min(KC, kb - kc0);
+    const int kcp = (kc + 3) & ~3;
+    for (int idx = threadIdx.x; idx < kcp * 64; idx += 256) {
+      const int kk = idx / 64, x = idx % 64;
+      const int i = i0 + x, j = j0 + x;
+      double va = 0.0, vb = 0.0;
+      if (kk < kc) {
+        const int l = k0 + kc0 + kk;
+        if (i < w) va = A[(int64_t)l * r + i];
+        if (j < w) vb = Wp[(int64_t)(l - k0) * w + j];
+      }
+      As[kk * LDT + x] = va;
+      Bs[kk * LDT + x] = vb;
+    }
+    __syncthreads();
+    tile_mma(acc, As, Bs, kcp);
+    __syncthreads();
+  }
+  tile_foreach(acc, [&](int ri, int ci, double v) {
+    const int i = i0 + ri, j = j0 + ci;
+    if (i < w && j < w && j >= k1 && i >= j) A[(int64_t)j * r + i] -= v;
+  });
+}
+void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
+                     const double* W, const int64_t* woff, const int* kpos, const int* kstart,
+                     hipStream_t st) {
+  if (count <= 0 || tiles_per_sn <= 0) return;
+  k_trailing<<<(int64_t)count * tiles_per_sn, 256, 0, st>>>(list, tiles_per_sn, lo, a.super, a.pi, a.px,
+                                                            valL, W, woff, kpos, kstart);
+  HIP_LAUNCH_CHECK("trailing");
+}
+
+// --------------------------------------------------------------------------
+// off-diagonal rows: B := B P L^{-T} D^{-1}
+// (LAPACKE_dlapmt + DTRSM(R,L,T,U) + blocked_2by2_solver,
+//  parallel_blocked_ldlt_02.cpp:164-188).  One workgroup per 64-row block.
+// gather != 0: columns are permuted here (narrow supernodes); otherwise the
+// panel kernel has already permuted them.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk_prefix, int count,
+                                                 const int* super, const int64_t* pi, const int64_t* px,
+                                                 double* valL, const double* dval, const int* perm_piv,
+                                                 int n) {
+  __shared__ double X[64 * 65];   // X[row + col*65] for a 64 x 64 column block
+  __shared__ double Lt[64 * 65];  // L(cb rows, cb cols)
+  __shared__ double As[KC * LDT], Bs[KC * LDT];
+  // locate (supernode, row block)
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const int s = list[lo];
+  const int rb = blockIdx.x - blk_prefix[lo];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  double* P = valL + px[s];
+  const int64_t q0 = w + 64 * (int64_t)rb;
+  const int nr = (int)min((int64_t)64, r - q0);
+  const int t = threadIdx.x;
+  const bool gather = w <= SMALL_W;
+  for (int cb0 = 0; cb0 < w; cb0 += 64) {
+    const int cw = min(64, w - cb0);
+    // load B block (with column gather for narrow supernodes)
+    for (int idx = t; idx < 64 * cw; idx += 256) {
+      const int i = idx % 64, j = idx / 64;
+      double v = 0.0;
+      if (i < nr) {
+        const int srcc = gather ? (perm_piv[c0 + cb0 + j] - c0) : (cb0 + j);
+        v = P[q0 + i + (int64_t)srcc * r];
+      }
+      X[i + j * 65] = v;
+    }
+    for (int idx = t; idx < cw * cw; idx += 256) {
+      const int i = idx % cw, j = idx / cw;
+      Lt[i + j * 65] = (i > j) ? P[(cb0 + i) + (int64_t)(cb0 + j) * r] : 0.0;
+    }
+    __syncthreads();
+    // X(:,cb) -= X(:, <cb) * L(cb, <cb)^T with MFMA (only for wide supernodes)
+    if (cb0 > 0) {
+      TileAcc acc;
+      tile_zero(acc);
+      for (int kc0 = 0; kc0 < cb0; kc0 += KC) {
+        const int kc = min(KC, cb0 - kc0);
+        for (int idx = t; idx < KC * 64; idx += 256) {
+          const int kk = idx / 64, x = idx % 64;
+          double va = 0.0, vb = 0.0;
+          if (kk < kc) {
+            const int l = kc0 + kk;
+            if (x < nr) va = P[q0 + x + (int64_t)l * r];
+            if (x < cw) vb = P[(cb0 + x) + (int64_t)l * r];
+          }
+          As[kk * LDT + x] = va;
+          Bs[kk * LDT + x] = vb;
+        }
+        __syncthreads();
+        tile_mma(acc, As, Bs, (kc + 3) & ~3);
+        __syncthreads();
+      }
+      tile_foreach(acc, [&](int ri, int ci, double v) {
+        if (ci < cw) X[ri + ci * 65] -= v;
+      });
+      __syncthreads();
+    }
+    // triangular solve inside the column block: thread per row
+    if (t < nr) {
+      for (int j = 0; j < cw; ++j) {
+        double v = X[t + j * 65];
+        for (int l = 0; l < j; ++l) v -= X[t + l * 65] * Lt[j + l * 65];
+        X[t + j * 65] = v;
+      }
+    }
+    __syncthreads();
+    for (int idx = t; idx < 64 * cw; idx += 256) {
+      const int i = idx % 64, j = idx / 64;
+      if (i < nr) P[q0 + i + (int64_t)(cb0 + j) * r] = X[i + j * 65];
+    }
+    __syncthreads();
+  }
+  // D^{-1}: 1x1 scaling or 2x2 Cramer (blocked_2by2_solver, Sym_BLAS.cpp:492-529)
+  if (t < nr) {
+    for (int l = 0; l < w; ++l) {
+      const double od = dval[n + c0 + l];
+      if (od == 0.0) {
+        const double inv = 1.0 / dval[c0 + l];
+        P[q0 + t + (int64_t)l * r] *= inv;
+      } else {
+        const double d1 = dval[c0 + l], d2 = dval[c0 + l + 1];
+        const double inv = 1.0 / (d1 * d2 - od * od);
+        const double x1 = P[q0 + t + (int64_t)l * r], x2 = P[q0 + t + (int64_t)(l + 1) * r];
+        P[q0 + t + (int64_t)l * r] = (x1 * d2 - x2 * od) * inv;
+        P[q0 + t + (int64_t)(l + 1) * r] = (x2 * d1 - x1 * od) * inv;
+        ++l;
+      }
+    }
+  }
+}
+void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
+                    double* valL, const double* dval, const int* perm_piv, hipStream_t st) {
+  if (count <= 0 || nblocks <= 0) return;
+  k_offdiag<<<nblocks, 256, 0, st>>>(list, blk_prefix, count, a.super, a.pi, a.px, valL, dval, perm_piv,
+                                     a.n);
+  HIP_LAUNCH_CHECK("offdiag");
+}
+
+// --------------------------------------------------------------------------
+// Schur update of a factored supernode s onto its ancestors:
+// U = L_off D L_off^T (lower tiles, MFMA FP64; the D scaling of
+// blocked_2by2_mult is applied while staging the B operand), scattered with
+// atomics into the ancestor panels (the DGEMM + scatter-subtract of
+// parallel_blocked_ldlt_02.cpp:128-153, applied right-looking).
+// --------------------------------------------------------------------------
+__device__ inline int64_t row_pos(const int* rows, int64_t b, int64_t e, int key) {
+  // position of key in the sorted rows[b, e); key is present
+  int64_t lo = b, hi = e - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rows[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo - b;
+}
+__global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* tile_prefix, int count,
+                                               const int* super, const int64_t* pi, const int64_t* px,
+                                               const int* rows, const int* col2sup, double* valL,
+                                               const double* dval, int n) {
+  __shared__ double As[KC * LDT], Bs[KC * LDT];
+  __shared__ int rlab[64], clab[64];
+  int lo = 0, hi = count - 1;
+  const int64_t bid = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_prefix[mid] <= bid) lo = mid; else hi = mid - 1;
+  }
+  const int s = list[lo];
+  const int64_t tix = bid - tile_prefix[lo];
+  int ti = (int)((sqrt(8.0 * (double)tix + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(ti + 1) * (ti + 2) / 2 <= tix) ++ti;
+  while ((int64_t)ti * (ti + 1) / 2 > tix) --ti;
+  const int tj = (int)(tix - (int64_t)ti * (ti + 1) / 2);
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  const int m = (int)(r - w);
+  const double* P = valL + px[s];
+  const int i0 = 64 * ti, j0 = 64 * tj;  // within the m off-diagonal rows
+  const int t = threadIdx.x;
+  if (t < 64) {
+    rlab[t] = (i0 + t < m) ? rows[pi[s] + w + i0 + t] : -1;
+    clab[t] = (j0 + t < m) ? rows[pi[s] + w + j0 + t] : -1;
+  }
+  TileAcc acc;
+  tile_zero(acc);
+  for (int kc0 = 0; kc0 < w; kc0 += KC) {
+    const int kc = min(KC, w - kc0);
+    for (int idx = t; idx < KC * 64; idx += 256) {
+      const int kk = idx / 64, x = idx % 64;
+      double va = 0.0, vb = 0.0;
+      if (kk < kc) {
+        const int l = kc0 + kk;
+        const int i = i0 + x, j = j0 + x;
+        if (i < m) va = P[w + i + (int64_t)l * r];
+        if (j < m) {
+          // (L D)(j, l): 1x1 or the matching row of a 2x2 block
+          const double od = dval[n + c0 + l];
+          const double odp = (l > 0) ? dval[n + c0 + l - 1] : 0.0;
+          const double x0 = P[w + j + (int64_t)l * r];
+          if (od != 0.0) {
+            vb = dval[c0 + l] * x0 + od * P[w + j + (int64_t)(l + 1) * r];
+          } else if (odp != 0.0) {
+            vb = odp * P[w + j + (int64_t)(l - 1) * r] + dval[c0 + l] * x0;
+          } else {
+            vb = dval[c0 + l] * x0;
+          }
+        }
+      }
+      As[kk * LDT + x] = va;
+      Bs[kk * LDT + x] = vb;
+    }
+    __syncthreads();
+    tile_mma(acc, As, Bs, (kc + 3) & ~3);
+    __syncthreads();
+  }
+  tile_foreach(acc, [&](int ri, int ci, double v) {
+    const int R = rlab[ri], C = clab[ci];
+    if (R < 0 || C < 0 || R < C) return;
+    const int tg = col2sup[C];
+    const int64_t rt = pi[tg + 1] - pi[tg];
+    const int64_t pos = row_pos(rows, pi[tg], pi[tg + 1], R);
+    atomicAdd(valL + px[tg] + (int64_t)(C - super[tg]) * rt + pos, -v);
+  });
+}
+void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
+                  double* valL, const double* dval, hipStream_t st) {
+  if (count <= 0 || ntiles <= 0) return;
+  if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur: too many tiles");
+  k_schur<<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                            a.col2sup, valL, dval, a.n);
+  HIP_LAUNCH_CHECK("schur");
+}
+
+// --------------------------------------------------------------------------
+// after all supernodes are factored: relabel off-diagonal row indices into
+// the pivoted ordering (row_reordering, Sym_BLAS.cpp:327-490, done as index
+// relabelling) and compose the final ordering (reorder_matrix,
+// linear_solver_wrapper.cpp:1432-1460)
+// --------------------------------------------------------------------------
+__global__ void k_newpos(const int* perm_piv, int* newpos, const int* perm0, int* perm_f, int* iperm_f,
+                         int n) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    newpos[perm_piv[k]] = k;
+    const int o = perm0[perm_piv[k]];
+    perm_f[k] = o;
+    iperm_f[o] = k;
+  }
+}
+__global__ void k_relabel(int nsuper, const int* super, const int64_t* pi, const int* rows0, int* rows,
+                          const int* newpos) {
+  const int s = blockIdx.x;
+  if (s >= nsuper) return;
+  const int c0 = super[s], w = super[s + 1] - c0;
+  for (int64_t q = pi[s] + threadIdx.x; q < pi[s + 1]; q += blockDim.x) {
+    const int64_t loc = q - pi[s];
+    rows[q] = loc < w ? c0 + (int)loc : newpos[rows0[q]];
+  }
+}
+void launch_relabel(const SnArgs& a, int nsuper, const int* rows0, int* rows, const int* perm_piv,
+                    int* newpos, const int* perm0, int* perm_f, int* iperm_f, hipStream_t st) {
+  k_newpos<<<grid_for(a.n, 256), 256, 0, st>>>(perm_piv, newpos, perm0, perm_f, iperm_f, a.n);
+  HIP_LAUNCH_CHECK("newpos");
+  k_relabel<<<nsuper, 256, 0, st>>>(nsuper, a.super, a.pi, rows0, rows, newpos);
+  HIP_LAUNCH_CHECK("relabel");
+}
+
+// --------------------------------------------------------------------------
+// sweeps over solve pieces (column blocks of at most SOLVE_W of a supernode)
+// X, Y, Z are n x nrhs, row-interleaved (element (i,c) at i*nrhs + c).
+// Rows of a piece below its triangle: local rows q in [c0l+pw, r) with label
+// super+q for q < w and rows[pi+q] otherwise.
+// --------------------------------------------------------------------------
+#define MAXRHS 8
+__device__ inline int row_label(const int* rows, int64_t pis, int c0, int w, int64_t q) {
+  return q < w ? c0 + (int)q : rows[pis + q];
+}
+
+// forward triangle of a piece: Y(p) = L_pp^{-1} X(p)
+__global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
+                                                 const int* super, const int64_t* pi, const int64_t* px,
+                                                 const double* valL, const double* X, double* Y, int nrhs) {
+  __shared__ double xs[SOLVE_W * MAXRHS];
+  const int p = pieces[blockIdx.x];
+  const int s = piece_sn[p], c0l = piece_c0[p];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int pw = min(SOLVE_W, w - c0l);
+  const int64_t r = pi[s + 1] - pi[s];
+  const double* P = valL + px[s];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < pw * nrhs; idx += 256) xs[idx] = X[(int64_t)(c0 + c0l) * nrhs + idx];
+  __syncthreads();
+  for (int j = 0; j < pw - 1; ++j) {
+    for (int idx = t; idx < (pw - j - 1) * nrhs; idx += 256) {
+      const int i = j + 1 + idx / nrhs, c = idx % nrhs;
+      xs[i * nrhs + c] -= P[(c0l + i) + (int64_t)(c0l + j) * r] * xs[j * nrhs + c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < pw * nrhs; idx += 256) Y[(int64_t)(c0 + c0l) * nrhs + idx] = xs[idx];
+}
+// forward update of the rows below a piece: X(label q) -= L(q, p) Y(p)
+__global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* blk_prefix, int np,
+                                                  const int* piece_sn, const int* piece_c0, const int* super,
+                                                  const int64_t* pi, const int64_t* px, const int* rows,
+                                                  const double* valL, const double* Y, double* X, int nrhs) {
+  __shared__ double ys[SOLVE_W * MAXRHS];
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const int p = pieces[lo], rb = blockIdx.x - blk_prefix[lo];
+  const int s = piece_sn[p], c0l = piece_c0[p];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int pw = min(SOLVE_W, w - c0l);
+  const int64_t r = pi[s + 1] - pi[s];
+  const double* P = valL + px[s];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < pw * nrhs; idx += 256) ys[idx] = Y[(int64_t)(c0 + c0l) * nrhs + idx];
+  __syncthreads();
+  const int64_t q = c0l + pw + 256 * (int64_t)rb + t;
+  if (q >= r) return;
+  double acc[MAXRHS];
+  for (int c = 0; c < nrhs; ++c) acc[c] = 0.0;
+  for (int j = 0; j < pw; ++j) {
+    const double l = P[q + (int64_t)(c0l + j) * r];
+    for (int c = 0; c < nrhs; ++c) acc[c] += l * ys[j * nrhs + c];
+  }
+  const int lab = row_label(rows, pi[s], c0, w, q);
+  for (int c = 0; c < nrhs; ++c) atomicAdd(X + (int64_t)lab * nrhs + c, -acc[c]);
+}
+// D solve: Z = D^{-1} Y (blocked_2by2_solver with n_rhs = 1)
+__global__ void k_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double od = dval[n + i];
+    if (od != 0.0) {
+      const double d1 = dval[i], d2 = dval[i + 1];
+      const double inv = 1.0 / (d1 * d2 - od * od);
+      for (int c = 0; c < nrhs; ++c) {
+        const double x1 = Y[(int64_t)i * nrhs + c], x2 = Y[(int64_t)(i + 1) * nrhs + c];
+        Z[(int64_t)i * nrhs + c] = (x1 * d2 - x2 * od) * inv;
+        Z[(int64_t)(i + 1) * nrhs + c] = (x2 * d1 - x1 * od) * inv;
+      }
+    } else if (i == 0 || dval[n + i - 1] == 0.0) {
+      const double inv = 1.0 / dval[i];
+      for (int c = 0; c < nrhs; ++c) Z[(int64_t)i * nrhs + c] = Y[(int64_t)i * nrhs + c] * inv;
+    }
+  }
+}
+// backward gather: acc(p) += L(q, p)^T X(label q) over the rows below a piece
+__global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* blk_prefix, int np,
+                                                  const int* piece_sn, const int* piece_c0, const int* super,
+                                                  const int64_t* pi, const int64_t* px, const int* rows,
+                                                  const double* valL, const double* Xf, double* acc,
+                                                  int nrhs) {
+  __shared__ double xq[256 * MAXRHS];
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const int p = pieces[lo], rb = blockIdx.x - blk_prefix[lo];
+  const int s = piece_sn[p], c0l = piece_c0[p];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int pw = min(SOLVE_W, w - c0l);
+  const int64_t r = pi[s + 1] - pi[s];
+  const double* P = valL + px[s];
+  const int t = threadIdx.x;
+  const int64_t qb = c0l + pw + 256 * (int64_t)rb;
+  const int nq = (int)min((int64_t)256, r - qb);
+  if (t < nq) {
+    const int lab = row_label(rows, pi[s], c0, w, qb + t);
+    for (int c = 0; c < nrhs; ++c) xq[t * nrhs + c] = Xf[(int64_t)lab * nrhs + c];
+  }
+  __syncthreads();
+  const int wv = t >> 6, ln = t & 63;
+  for (int j = wv; j < pw; j += 4) {
+    double a[MAXRHS];
+    for (int c = 0; c < nrhs; ++c) a[c] = 0.0;
+    for (int qq = ln; qq < nq; qq += 64) {
+      const double l = P[qb + qq + (int64_t)(c0l + j) * r];
+      for (int c = 0; c < nrhs; ++c) a[c] += l * xq[qq * nrhs + c];
+    }
+    for (int c = 0; c < nrhs; ++c) {
+      const double v = wave_sum(a[c]);
+      if (ln == 0) atomicAdd(acc + (int64_t)(c0 + c0l + j) * nrhs + c, v);
+    }
+  }
+}
+// backward triangle: X(p) = L_pp^{-T} (Z(p) - acc(p))
+__global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
+                                                 const int* super, const int64_t* pi, const int64_t* px,
+                                                 const double* valL, const double* Z, const double* acc,
+                                                 double* Xf, int nrhs) {
+  __shared__ double xs[SOLVE_W * MAXRHS];
+  const int p = pieces[blockIdx.x];
+  const int s = piece_sn[p], c0l = piece_c0[p];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int pw = min(SOLVE_W, w - c0l);
+  const int64_t r = pi[s + 1] - pi[s];
+  const double* P = valL + px[s];
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)(c0 + c0l) * nrhs;
+  for (int idx = t; idx < pw * nrhs; idx += 256) xs[idx] = Z[base + idx] - acc[base + idx];
+  __syncthreads();
+  for (int j = pw - 1; j > 0; --j) {
+    for (int idx = t; idx < j * nrhs; idx += 256) {
+      const int i = idx / nrhs, c = idx % nrhs;
+      xs[i * nrhs + c] -= P[(c0l + j) + (int64_t)(c0l + i) * r] * xs[j * nrhs + c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < pw * nrhs; idx += 256) Xf[base + idx] = xs[idx];
+}
+
+void launch_fwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+                    const double* valL, const double* X, double* Y, int nrhs, hipStream_t st) {
+  if (np <= 0) return;
+  if (nrhs > MAXRHS) throw std::runtime_error("too many right-hand sides");
+  k_fwd_tri<<<np, 256, 0, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X, Y, nrhs);
+  HIP_LAUNCH_CHECK("fwd_tri");
+}
+void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Y, double* X,
+                     int nrhs, hipStream_t st) {
+  if (np <= 0 || nblk <= 0) return;
+  k_fwd_gemv<<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
+                                   valL, Y, X, nrhs);
+  HIP_LAUNCH_CHECK("fwd_gemv");
+}
+void launch_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs, hipStream_t st) {
+  k_dsolve<<<grid_for(n, 256), 256, 0, st>>>(n, dval, Y, Z, nrhs);
+  HIP_LAUNCH_CHECK("dsolve");
+}
+void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Xf, double* acc,
+                     int nrhs, hipStream_t st) {
+  if (np <= 0 || nblk <= 0) return;
+  k_bwd_gemv<<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
+                                   valL, Xf, acc, nrhs);
+  HIP_LAUNCH_CHECK("bwd_gemv");
+}
+void launch_bwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+                    const double* valL, const double* Z, const double* acc, double* Xf, int nrhs,
+                    hipStream_t st) {
+  if (np <= 0) return;
+  k_bwd_tri<<<np, 256, 0, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z, acc, Xf, nrhs);
+  HIP_LAUNCH_CHECK("bwd_tri");
+}
+
+// --------------------------------------------------------------------------
+// y_ord = P K0 P^T x_ord with the full CSR of K0 in the original ordering
+// --------------------------------------------------------------------------
+__global__ void k_spmv(int n, const int* rp, const int* ci, const double* v, const int* perm_f,
+                       const int* iperm_f, const double* x, double* y, int nrhs) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int row = perm_f[i];
+    double acc[MAXRHS];
+    for (int c = 0; c < nrhs; ++c) acc[c] = 0.0;
+    for (int q = rp[row]; q < rp[row + 1]; ++q) {
+      const double a = v[q];
+      const int j = iperm_f[ci[q]];
+      for (int c = 0; c < nrhs; ++c) acc[c] += a * x[(int64_t)j * nrhs + c];
+    }
+    for (int c = 0; c < nrhs; ++c) y[(int64_t)i * nrhs + c] = acc[c];
+  }
+}
+void launch_spmv(int n, const int* rp, const int* ci, const double* v, const int* perm_f, const int* iperm_f,
+                 const double* x, double* y, int nrhs, hipStream_t st) {
+  k_spmv<<<grid_for(n, 256), 256, 0, st>>>(n, rp, ci, v, perm_f, iperm_f, x, y, nrhs);
+  HIP_LAUNCH_CHECK("spmv");
+}
+
+// ordered <-> original vector layouts
+__global__ void k_gather_rhs(int n, int nrhs, const double* B, int64_t ldb, const int* perm_f, double* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < nrhs; ++c) out[(int64_t)i * nrhs + c] = B[(int64_t)c * ldb + perm_f[i]];
+}
+__global__ void k_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, double* X, int64_t ldx) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < nrhs; ++c) X[(int64_t)c * ldx + perm_f[i]] = Xo[(int64_t)i * nrhs + c];
+}
+void launch_gather_rhs(int n, int nrhs, const double* B, int64_t ldb, const int* perm_f, double* out,
+                       hipStream_t st) {
+  k_gather_rhs<<<grid_for(n, 256), 256, 0, st>>>(n, nrhs, B, ldb, perm_f, out);
+  HIP_LAUNCH_CHECK("gather_rhs");
+}
+void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, double* X, int64_t ldx,
+                        hipStream_t st) {
+  k_scatter_sol<<<grid_for(n, 256), 256, 0, st>>>(n, nrhs, Xo, perm_f, X, ldx);
+  HIP_LAUNCH_CHECK("scatter_sol");
+}
+
+// --------------------------------------------------------------------------
+// GMRES(mr) on the device (pmgmres_ldlt_auto, mgmres.cpp:1015-1211), one
+// independent iteration per right-hand-side column.  Small state per column:
+// --------------------------------------------------------------------------
+// state layout per column (doubles), MR = max restart length:
+//   h[(MR+1)*MR] | cs[MR+1] | sn[MR+1] | g[MR+1] | y[MR+1] | dots[MR+2] |
+//   rho, rho_tol, av, flags(active, inner_done, k_last), used
+__device__ inline double* gs_col(double* gm, int c, int MR) {
+  return gm + (int64_t)c * ((MR + 1) * MR + 4 * (MR + 1) + (MR + 2) + 8);
+}
+#define GS_H(st, i, j, MR) st[(i) * (MR) + (j)]
+#define GS_CS(st, MR) (st + (MR + 1) * (MR))
+#define GS_SN(st, MR) (GS_CS(st, MR) + (MR + 1))
+#define GS_G(st, MR) (GS_SN(st, MR) + (MR + 1))
+#define GS_Y(st, MR) (GS_G(st, MR) + (MR + 1))
+#define GS_D(st, MR) (GS_Y(st, MR) + (MR + 1))
+#define GS_S(st, MR) (GS_D(st, MR) + (MR + 2))  // scalars: [0]rho [1]rho_tol [2]av [3]active [4]inner_done [5]k_last [6]used [7]norm_tmp
+
+// partial dot products of nd pairs (a_k, b_k) for every column; part[blk][k][c]
+__global__ void k_dot_partial(int n, int nrhs, const double* const* as, const double* const* bs, int nd,
+                              double* part) {
+  __shared__ double sh[4][MAXRHS * 4];
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  for (int d = 0; d < nd; ++d) {
+    const double* a = as[d];
+    const double* b = bs[d];
+    double acc[MAXRHS];
+    for (int c = 0; c < nrhs; ++c) acc[c] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x)
+      for (int c = 0; c < nrhs; ++c) acc[c] += a[(int64_t)i * nrhs + c] * b[(int64_t)i * nrhs + c];
+    for (int c = 0; c < nrhs; ++c) {
+      const double v = wave_sum(acc[c]);
+      if (ln == 0) sh[wv][c] = v;
+    }
+    __syncthreads();
+    if (t < nrhs) part[((int64_t)blockIdx.x * nd + d) * nrhs + t] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
+    __syncthreads();
+  }
+}
+// ordered sums of partials: out[d*nrhs + c]
+__global__ void k_dot_final(const double* part, int nblk, int nd, int nrhs, double* out) {
+  const int t = threadIdx.x;
+  if (t < nd * nrhs) {
+    const int d = t / nrhs, c = t % nrhs;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[((int64_t)b * nd + d) * nrhs + c];
+    out[t] = s;
+  }
+}
+void launch_dots(int n, int nrhs, const double* const* as_dev, const double* const* bs_dev, int nd,
+                 double* part, int nblk, double* out, hipStream_t st) {
+  k_dot_partial<<<nblk, 256, 0, st>>>(n, nrhs, as_dev, bs_dev, nd, part);
+  HIP_LAUNCH_CHECK("dot_partial");
+  k_dot_final<<<1, 64, 0, st>>>(part, nblk, nd, nrhs, out);
+  HIP_LAUNCH_CHECK("dot_final");
+}
+
+// elementwise: r = b - y
+__global__ void k_sub(int64_t count, const double* b, const double* y, double* r) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = b[i] - y[i];
+}
+void launch_sub(int64_t count, const double* b, const double* y, double* r, hipStream_t st) {
+  k_sub<<<grid_for(count, 256), 256, 0, st>>>(count, b, y, r);
+  HIP_LAUNCH_CHECK("sub");
+}
+
+// y(:,c) = alpha_c * x(:,c) + beta_c * y(:,c) with per-column coefficients
+// read from the device (coef[c*2], coef[c*2+1]); columns whose mask entry is
+// zero are left untouched
+__global__ void k_axpby(int n, int nrhs, const double* coefs, int cstride, const double* x, double* y,
+                        const double* mask, int mstride, int mask_off, int mode) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < nrhs; ++c) {
+      if (mask && mask[(int64_t)c * mstride + mask_off] == 0.0) continue;
+      const double a = coefs[(int64_t)c * cstride];
+      const int64_t e = (int64_t)i * nrhs + c;
+      if (mode == 0) y[e] -= a * x[e];          // y -= a x
+      else if (mode == 1) {                     // y = x / a (skipped when a == 0)
+        if (a != 0.0) y[e] = x[e] / a;
+      }
+      else y[e] += a * x[e];                    // y += a x
+    }
+}
+void launch_axpby(int n, int nrhs, const double* coefs, int cstride, const double* x, double* y,
+                  const double* mask, int mstride, int mask_off, int mode, hipStream_t st) {
+  k_axpby<<<grid_for(n, 256), 256, 0, st>>>(n, nrhs, coefs, cstride, x, y, mask, mstride, mask_off, mode);
+  HIP_LAUNCH_CHECK("axpby");
+}
+
+// scalar GMRES bookkeeping kernels (one thread per column)
+// op 0: after r = M(b - K x): rho = sqrt(d0); stop if rho < tol_abs; set rho_tol at itr 0;
+//       g = [rho,0..]; h = 0; coefficient for v0 = r / rho
+// op 1: av = sqrt(d0)
+// op 2: h(j,k) = d0 (MGS coefficient)  [j,k given]
+// op 3: h(k+1,k) = sqrt(d0); reorth flag
+// op 4: reorth: h(j,k) += d0
+// op 5: h(k+1,k) = sqrt(d0) after reorth
+// op 6: Givens for column k, convergence check
+// op 7: back substitution -> y
+__global__ void k_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k,
+                               int itr, double tol_abs, double tol_rel) {
+  const int c = threadIdx.x;
+  if (c >= nrhs) return;
+  double* st = gs_col(gm, c, MR);
+  double* S = GS_S(st, MR);
+  double* H = st;
+  double* cs = GS_CS(st, MR);
+  double* sn = GS_SN(st, MR);
+  double* g = GS_G(st, MR);
+  double* y = GS_Y(st, MR);
+  double* D = GS_D(st, MR);  // D[0]: coefficient slot used by vector kernels
+  const double d0 = dots[c];
+  const double delta = 1.0e-3;
+  if (op == 0) {
+    S[7] = 0.0;
+    if (S[3] == 0.0) return;  // column already finished
+    const double rho = sqrt(d0);
+    S[0] = rho;
+    if (rho < tol_abs) {
+      S[3] = 0.0;
+      S[5] = -1.0;
+      return;
+    }
+    S[7] = 1.0;
+    if (itr == 0) S[1] = rho * tol_rel;
+    g[0] = rho;
+    for (int i = 1; i <= MR; ++i) g[i] = 0.0;
+    for (int i = 0; i <= MR; ++i)
+      for (int q = 0; q < MR; ++q) GS_H(H, i, q, MR) = 0.0;
+    S[4] = 0.0;  // inner not done
+    S[5] = -1.0;
+    D[0] = rho;
+  } else if (op == 1) {
+    if (S[7] == 0.0) return;
+    S[2] = sqrt(d0);
+  } else if (op == 2) {
+    if (S[7] == 0.0) return;
+    GS_H(H, j, k, MR) = d0;
+    D[0] = d0;
+  } else if (op == 3) {
+    D[1] = 0.0;
+    if (S[7] == 0.0) return;
+    const double hk = sqrt(d0);
+    GS_H(H, k + 1, k, MR) = hk;
+    D[1] = (S[2] + delta * hk == S[2]) ? 1.0 : 0.0;  // re-orthogonalise?
+    D[0] = hk;
+  } else if (op == 4) {
+    if (D[1] == 0.0) return;
+    GS_H(H, j, k, MR) += d0;
+    D[0] = d0;
+  } else if (op == 5) {
+    if (D[1] == 0.0) return;
+    const double hk = sqrt(d0);
+    GS_H(H, k + 1, k, MR) = hk;
+    D[0] = hk;
+  } else if (op == 6) {
+    if (S[7] == 0.0) return;
+    S[5] = (double)k;
+    if (k > 0) {
+      for (int i = 0; i < k + 2; ++i) y[i] = GS_H(H, i, k, MR);
+      for (int q = 0; q < k; ++q) {
+        const double g1 = cs[q] * y[q] - sn[q] * y[q + 1];
+        const double g2 = sn[q] * y[q] + cs[q] * y[q + 1];
+        y[q] = g1;
+        y[q + 1] = g2;
+      }
+      for (int i = 0; i < k + 2; ++i) GS_H(H, i, k, MR) = y[i];
+    }
+    const double hkk = GS_H(H, k, k, MR), hk1 = GS_H(H, k + 1, k, MR);
+    const double mu = sqrt(hkk * hkk + hk1 * hk1);
+    if (mu == 0.0) {
+      S[3] = 0.0;  // breakdown: stop this column (reference returns)
+      S[5] = -2.0;
+      S[7] = 0.0;
+      return;
+    }
+    cs[k] = hkk / mu;
+    sn[k] = -hk1 / mu;
+    GS_H(H, k, k, MR) = cs[k] * hkk - sn[k] * hk1;
+    GS_H(H, k + 1, k, MR) = 0.0;
+    const double g1 = cs[k] * g[k] - sn[k] * g[k + 1];
+    const double g2 = sn[k] * g[k] + cs[k] * g[k + 1];
+    g[k] = g1;
+    g[k + 1] = g2;
+    S[0] = fabs(g[k + 1]);
+    S[6] += 1.0;
+    if (S[0] <= S[1] && S[0] <= tol_abs) {
+      S[4] = 1.0;
+      S[7] = 0.0;
+    }
+  } else if (op == 7) {
+    // back substitution for y; coefficients written to y[0..k]
+    const int kl = (int)S[5];
+    if (S[3] == 0.0 || kl < 0) {
+      for (int i = 0; i <= MR; ++i) y[i] = 0.0;
+      return;
+    }
+    if (GS_H(H, kl, kl, MR) == 0.0) {
+      for (int i = 0; i <= MR; ++i) y[i] = 0.0;
+      S[3] = 0.0;
+      return;
+    }
+    for (int i = kl + 1; i <= MR; ++i) y[i] = 0.0;
+    y[kl] = g[kl] / GS_H(H, kl, kl, MR);
+    for (int i = kl - 1; i >= 0; --i) {
+      y[i] = g[i];
+      for (int q = i + 1; q < kl + 1; ++q) y[i] -= GS_H(H, i, q, MR) * y[q];
+      y[i] /= GS_H(H, i, i, MR);
+    }
+    if (S[4] != 0.0) S[3] = 0.0;  // converged: stop outer loop for this column
+  }
+}
+void launch_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k, int itr,
+                         double tol_abs, double tol_rel, hipStream_t st) {
+  k_gmres_scalar<<<1, 64, 0, st>>>(op, nrhs, MR, gm, dots, j, k, itr, tol_abs, tol_rel);
+  HIP_LAUNCH_CHECK("gmres_scalar");
+}
+
+// x += V y (columns with active or just-converged state)
+__global__ void k_gmres_update(int n, int nrhs, int MR, const double* V, const double* gm, double* x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < nrhs; ++c) {
+      const double* st = gm + (int64_t)c * ((MR + 1) * MR + 4 * (MR + 1) + (MR + 2) + 8);
+      const double* y = st + (MR + 1) * MR + 3 * (MR + 1);
+      double acc = x[(int64_t)i * nrhs + c];
+      for (int j = 0; j <= MR; ++j)
+        if (y[j] != 0.0) acc += V[((int64_t)j * n + i) * nrhs + c] * y[j];
+      x[(int64_t)i * nrhs + c] = acc;
+    }
+}
+void launch_gmres_update(int n, int nrhs, int MR, const double* V, const double* gm, double* x,
+                         hipStream_t st) {
+  k_gmres_update<<<grid_for(n, 256), 256, 0, st>>>(n, nrhs, MR, V, gm, x);
+  HIP_LAUNCH_CHECK("gmres_update");
+}
+
+// backward-error ingredients per column: sum|r|, sum|x|, sum|b|
+__global__ void k_abs_sums(int n, int nrhs, const double* r, const double* x, const double* b, double* part) {
+  __shared__ double sh[4][3 * MAXRHS];
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  double acc[3 * MAXRHS];
+  for (int c = 0; c < 3 * nrhs; ++c) acc[c] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < nrhs; ++c) {
+      acc[c] += fabs(r[(int64_t)i * nrhs + c]);
+      acc[nrhs + c] += fabs(x[(int64_t)i * nrhs + c]);
+      acc[2 * nrhs + c] += fabs(b[(int64_t)i * nrhs + c]);
+    }
+  for (int c = 0; c < 3 * nrhs; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (ln == 0) sh[wv][c] = v;
+  }
+  __syncthreads();
+  if (t < 3 * nrhs) part[(int64_t)blockIdx.x * 3 * nrhs + t] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
+}
+void launch_abs_sums(int n, int nrhs, const double* r, const double* x, const double* b, double* part,
+                     int nblk, double* out, hipStream_t st) {
+  k_abs_sums<<<nblk, 256, 0, st>>>(n, nrhs, r, x, b, part);
+  HIP_LAUNCH_CHECK("abs_sums");
+  k_dot_final<<<1, 64, 0, st>>>(part, nblk, 3, nrhs, out);
+  HIP_LAUNCH_CHECK("abs_final");
+}
+
+}  // namespace lblg
+
+namespace lblg {
+// --------------------------------------------------------------------------
+// value-dependent preparation: perturbation flags (find_perturbation), the
+// unperturbed refinement matrix K0 (set_diags(0)) and its infinity norm
+// (norm_sparse_int, src/common/Norm.cpp:115-185)
+// --------------------------------------------------------------------------
+__global__ void k_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg,
+                             unsigned char* pert) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x)
+    pert[q] = (isdiag[q] && fabs(Kx[q]) < reg) ? 1 : 0;
+}
+void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, int regularization,
+                       unsigned char* pert, hipStream_t st) {
+  if (nnz <= 0) return;
+  k_pert_flags<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, isdiag, Kx, regularization == 1 ? reg : -1.0, pert);
+  HIP_LAUNCH_CHECK("pert_flags");
+}
+__global__ void k_k0_values(int64_t nnzf, const int64_t* src, const double* Kx, const unsigned char* pert,
+                            double* Kv) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnzf; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = src[e];
+    Kv[e] = pert[q] ? 0.0 : Kx[q];
+  }
+}
+__global__ void k_rowabs_max(int n, const int* rp, const double* Kv, double* part) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int q = rp[i]; q < rp[i + 1]; ++q) s += fabs(Kv[q]);
+    m = fmax(m, s);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+__global__ void k_max_final(const double* part, int nblk, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double m = 0.0;
+    for (int b = 0; b < nblk; ++b) m = fmax(m, part[b]);
+    out[0] = m;
+  }
+}
+void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const double* Kx,
+               const unsigned char* pert, double* Kv, double* part, int nblk, double* norm_out, hipStream_t st) {
+  k_k0_values<<<grid_for(nnzf, 256), 256, 0, st>>>(nnzf, src, Kx, pert, Kv);
+  HIP_LAUNCH_CHECK("k0_values");
+  k_rowabs_max<<<nblk, 256, 0, st>>>(n, rp, Kv, part);
+  HIP_LAUNCH_CHECK("rowabs");
+  k_max_final<<<1, 64, 0, st>>>(part, nblk, norm_out);
+  HIP_LAUNCH_CHECK("max_final");
+}
+}  // namespace lblg

@@ -1,0 +1,84 @@
+// Launch wrappers of the gfx950 kernels (kernels.hip).  All launches are
+// asynchronous on the given stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lblg {
+
+constexpr int SMALL_W = 64;    // widest supernode factored entirely in LDS
+constexpr int PANEL_NB = 32;   // Bunch-Kaufman panel width for wider supernodes
+constexpr int SOLVE_W = 128;   // column width of a solve piece
+constexpr int MAX_RHS = 8;     // right-hand sides per solve call
+
+struct SnArgs {  // symbolic arrays on the device
+  const int* super;
+  const int64_t* pi;
+  const int64_t* px;
+  const int* rows;
+  const int* col2sup;
+  int n;
+};
+
+void launch_fill(double* p, int64_t count, double v, hipStream_t st);
+void launch_fill_int(int* p, int64_t count, int v, hipStream_t st);
+void launch_iota(int* p, int count, hipStream_t st);
+void launch_copy(double* dst, const double* src, int64_t count, hipStream_t st);
+void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
+                     double* valL, hipStream_t st);
+void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, double* dval,
+                     int* perm_piv, int* status, hipStream_t st);
+void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double* valL, double* dval,
+                     int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+                     int* status, bool cooperative, hipStream_t st);
+void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
+                     const double* W, const int64_t* woff, const int* kpos, const int* kstart,
+                     hipStream_t st);
+void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
+                    double* valL, const double* dval, const int* perm_piv, hipStream_t st);
+void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
+                  double* valL, const double* dval, hipStream_t st);
+void launch_relabel(const SnArgs& a, int nsuper, const int* rows0, int* rows, const int* perm_piv,
+                    int* newpos, const int* perm0, int* perm_f, int* iperm_f, hipStream_t st);
+void launch_fwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+                    const double* valL, const double* X, double* Y, int nrhs, hipStream_t st);
+void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Y, double* X,
+                     int nrhs, hipStream_t st);
+void launch_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs, hipStream_t st);
+void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Xf, double* acc,
+                     int nrhs, hipStream_t st);
+void launch_bwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+                    const double* valL, const double* Z, const double* acc, double* Xf, int nrhs,
+                    hipStream_t st);
+void launch_spmv(int n, const int* rp, const int* ci, const double* v, const int* perm_f, const int* iperm_f,
+                 const double* x, double* y, int nrhs, hipStream_t st);
+void launch_gather_rhs(int n, int nrhs, const double* B, int64_t ldb, const int* perm_f, double* out,
+                       hipStream_t st);
+void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, double* X, int64_t ldx,
+                        hipStream_t st);
+void launch_dots(int n, int nrhs, const double* const* as_dev, const double* const* bs_dev, int nd,
+                 double* part, int nblk, double* out, hipStream_t st);
+void launch_sub(int64_t count, const double* b, const double* y, double* r, hipStream_t st);
+void launch_axpby(int n, int nrhs, const double* coefs, int cstride, const double* x, double* y,
+                  const double* mask, int mstride, int mask_off, int mode, hipStream_t st);
+void launch_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k, int itr,
+                         double tol_abs, double tol_rel, hipStream_t st);
+void launch_gmres_update(int n, int nrhs, int MR, const double* V, const double* gm, double* x,
+                         hipStream_t st);
+void launch_abs_sums(int n, int nrhs, const double* r, const double* x, const double* b, double* part,
+                     int nblk, double* out, hipStream_t st);
+void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, int regularization,
+                       unsigned char* pert, hipStream_t st);
+void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const double* Kx,
+               const unsigned char* pert, double* Kv, double* part, int nblk, double* norm_out, hipStream_t st);
+void launch_mfma_selftest(const double* A, const double* B, double* C, hipStream_t st);
+
+// GMRES state per right-hand-side column (doubles)
+inline int64_t gmres_state_size(int MR) { return (int64_t)(MR + 1) * MR + 4 * (MR + 1) + (MR + 2) + 8; }
+inline int64_t gmres_off_D(int MR) { return (int64_t)(MR + 1) * MR + 4 * (MR + 1); }
+inline int64_t gmres_off_S(int MR) { return gmres_off_D(MR) + (MR + 2); }
+
+}  // namespace lblg

@@ -1,0 +1,106 @@
+"""Synthetic KKT / QP inputs of the benchmark configurations.
+
+All generators draw from one seeded numpy Generator and return matrices as
+symmetric lower-triangular CSC arrays (diagonal first in every column, rows
+sorted), the layout the reference's CSC type expects
+(include/nasoq/common/def.h:73-122; find_perturbation reads x[p[j]] as the
+diagonal, src/QP/linear_solver_wrapper.cpp:352-363).
+"""
+import numpy as np
+import scipy.sparse as sp
+
+
+def lower_csc(K):
+    """Symmetric scipy matrix -> (n, Ap, Ai, Ax) lower CSC, explicit diagonal first."""
+    K = sp.csc_matrix(K)
+    n = K.shape[0]
+    L = sp.tril(K, format="csc")
+    # make every diagonal entry structurally present (explicit zeros allowed)
+    d = L.diagonal()
+    L = (L + sp.csc_matrix((np.full(n, 1.0), (np.arange(n), np.arange(n))), shape=(n, n))).tocsc()
+    L.sort_indices()
+    Ap = L.indptr.astype(np.int32)
+    Ai = L.indices.astype(np.int32)
+    Ax = L.data.astype(np.float64)
+    # restore the diagonal values (may be exact zeros)
+    for j in range(n):
+        assert Ai[Ap[j]] == j
+    Ax[Ap[:-1]] = d
+    return n, Ap, Ai, Ax
+
+
+def lower_to_full(n, Ap, Ai, Ax):
+    L = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    return (L + sp.tril(L, -1).T).tocsc()
+
+
+def laplacian_2d(r, c, shift):
+    e = lambda m: sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    H = sp.kron(sp.identity(c), e(r)) + sp.kron(e(c), sp.identity(r))
+    return (H + shift * sp.identity(r * c)).tocsc()
+
+
+def laplacian_3d(g, shift):
+    e = sp.diags([-np.ones(g - 1), 2 * np.ones(g), -np.ones(g - 1)], [-1, 0, 1])
+    I = sp.identity(g)
+    H = (sp.kron(sp.kron(I, I), e) + sp.kron(sp.kron(I, e), I) + sp.kron(sp.kron(e, I), I))
+    return (H + shift * sp.identity(g ** 3)).tocsc()
+
+
+def random_rows(rng, m, n, k):
+    """m x n sparse matrix with k distinct uniform random columns per row, N(0,1) values."""
+    cols = np.empty((m, k), dtype=np.int64)
+    for r in range(m):
+        cols[r] = rng.choice(n, size=k, replace=False)
+    vals = rng.standard_normal((m, k))
+    rows = np.repeat(np.arange(m), k)
+    return sp.csr_matrix((vals.ravel(), (rows, cols.ravel())), shape=(m, n))
+
+
+def kkt(H, A):
+    """K = [H A'; A 0] with an explicitly stored zero (2,2) diagonal."""
+    m = A.shape[0]
+    Z = sp.csc_matrix((m, m))
+    return sp.bmat([[H, A.T], [A, Z]], format="csc")
+
+
+def kkt_config(grid, m, nnz_per_row, shift=1e-3, seed=0, nrhs=4, ill=False):
+    """KKT of a 3-D 7-point Laplacian on grid^3 plus m random constraint rows
+    (the plain-LBL and mixed-precision configurations).  ill=True scales H's
+    diagonal by 10^U[-3,3] (the ill-conditioned variant)."""
+    rng = np.random.default_rng(seed)
+    H = laplacian_3d(grid, shift)
+    n = H.shape[0]
+    if ill:
+        s = 10.0 ** rng.uniform(-3, 3, size=n)
+        H = (H + sp.diags(H.diagonal() * (s - 1.0))).tocsc()
+    A = random_rows(rng, m, n, nnz_per_row)
+    K = kkt(H, A)
+    B = rng.uniform(-1.0, 1.0, size=(K.shape[0], nrhs))
+    return lower_csc(K) + (B,)
+
+
+def kkt_small(n_h, m, seed=0, density=0.05, nrhs=1, shift=1.0):
+    """Small random KKT [H A'; A 0] with H SPD (test sizes)."""
+    rng = np.random.default_rng(seed)
+    R = sp.random(n_h, n_h, density=density, random_state=rng, format="csc")
+    H = (R @ R.T + shift * sp.identity(n_h)).tocsc()
+    A = sp.random(m, n_h, density=min(1.0, 4.0 / max(n_h, 1)), random_state=rng, format="csr")
+    # make sure each constraint row has at least one entry
+    for r in range(m):
+        if A.indptr[r + 1] == A.indptr[r]:
+            A = A.tolil()
+            A[r, rng.integers(n_h)] = rng.standard_normal()
+            A = A.tocsr()
+    K = kkt(H, A)
+    B = rng.uniform(-1.0, 1.0, size=(K.shape[0], nrhs))
+    return lower_csc(K) + (B,)
+
+
+def checksum(*arrays):
+    """Order-sensitive checksum of generated arrays (for recording inputs)."""
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()[:16]

@@ -1,0 +1,140 @@
+"""Device LBL engine vs the CPU oracle (parity tests; need a gfx950 GPU).
+
+Bar: the 1x1/2x2 block structure of D and the inertia are identical to the
+oracle's; L D L^T reproduces P K~ P^T to within a growth-scaled rounding
+bound; solutions agree with the oracle's and the refined backward error
+(SolverSettings::backward_error) reaches 1e-10 or better.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from nasoq_amd import lbl
+from nasoq_amd import problems as P
+
+from conftest import dense_from_factor
+
+pytestmark = pytest.mark.gpu
+
+
+def engine(n, Ap, Ai, Ax, reg=1e-9, perm=None, **kw):
+    s = lbl.SolverSettings(n, Ap, Ai, Ax)
+    s.reg_diag = reg
+    for k, v in kw.items():
+        setattr(s, k, v)
+    assert s.symbolic_analysis(perm=perm) == 1
+    assert s.numerical_factorization() == 1
+    return s
+
+
+def oracle_factor(O, s, n, Ap, Ai, Ax, reg):
+    return O.Factor(n, Ap, Ai, Ax, reg, s.sym_ints("Perm"), s.sym_ints("super"), s.sym_longs("pi"),
+                    s.sym_ints("s"), s.sym_longs("p"))
+
+
+def check_factor(O, s, n, Ap, Ai, Ax, reg):
+    F = oracle_factor(O, s, n, Ap, Ai, Ax, reg)
+    valL, dval, rows, permf = s.get_factor()
+    # identical pivot block structure and inertia
+    assert np.array_equal(dval[n:] != 0, F.d_val[n:] != 0)
+    assert s.inertia() == F.inertia()
+    sup, pi = s.sym_ints("super"), s.sym_longs("pi")
+    px = s.sym_longs("p")[sup[:-1]]
+    L, D = dense_from_factor(n, sup, pi, px, valL, rows, dval)
+    K = P.lower_to_full(n, Ap, Ai, Ax).toarray()
+    Kt = K.copy()
+    d = np.diag(K)
+    Kt[np.arange(n), np.arange(n)] = np.where(np.abs(d) < reg, reg, d)
+    Pm = np.eye(n)[:, permf]
+    R = Pm @ L @ D @ L.T @ Pm.T
+    growth = np.abs(L) @ np.abs(D) @ np.abs(L).T
+    assert np.max(np.abs(R - Kt)) <= 1e-12 * max(1.0, growth.max())
+    # the oracle's factor obeys the same bound
+    Lo, Do = F.dense_L(), F.dense_D()
+    Po = np.eye(n)[:, F.perm]
+    Ro = Po @ Lo @ Do @ Lo.T @ Po.T
+    assert np.max(np.abs(Ro - Kt)) <= 1e-12 * max(1.0, (np.abs(Lo) @ np.abs(Do) @ np.abs(Lo).T).max())
+    return F
+
+
+def test_mfma_f64_operand_layout():
+    import ctypes
+    from nasoq_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    C = np.zeros((16, 16))
+    assert lib.lblg_selftest_mfma(_lib.dptr(np.ascontiguousarray(A)), _lib.dptr(np.ascontiguousarray(B)),
+                                  _lib.dptr(C)) == 0
+    assert np.allclose(C, A @ B, rtol=1e-14, atol=1e-14)
+
+
+def test_reference_example_lbl_main():
+    # examples/lbl_main.cpp:26-57: H = 2I, q = (-4,-4) -> x = (-2,-2); q = (8,8) -> x = (4,4)
+    Ap = np.array([0, 1, 2], np.int32)
+    Ai = np.array([0, 1], np.int32)
+    Ax = np.array([2.0, 2.0])
+    s = engine(2, Ap, Ai, Ax, reg=1e-9)
+    assert np.allclose(s.solve_only(np.array([-4.0, -4.0])), [-2, -2], atol=1e-14)
+    assert np.allclose(s.solve_only(np.array([8.0, 8.0])), [4, 4], atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_small_kkt_parity(oracle, seed):
+    n, Ap, Ai, Ax, B = P.kkt_small(50 + 30 * seed, 12 + 6 * seed, seed=seed, nrhs=2)
+    print(f"case small_kkt seed={seed} n={n}", flush=True)
+    reg = 1e-9
+    s = engine(n, Ap, Ai, Ax, reg=reg)
+    F = check_factor(oracle, s, n, Ap, Ai, Ax, reg)
+    # unrefined solve agrees with the oracle's, refined solve reaches 1e-10
+    s.req_ref_iter = 0
+    x0 = s.solve_only(B[:, 0])
+    xo = F.solve(B[:, 0])
+    assert np.linalg.norm(x0 - xo) <= 1e-6 * max(1.0, np.linalg.norm(xo))
+    s.req_ref_iter = 2
+    X, bwd = s.solve_only(B, return_bwd=True)
+    for c in range(B.shape[1]):
+        e = oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c])
+        assert e <= 1e-10, e
+        # device-reported backward error agrees (both are rounding-level near 1e-16)
+        assert abs(e - bwd[c]) <= max(1e-3 * e, 1e-16)
+        xr, _ = F.refine(B[:, c], F.solve(B[:, c]))
+        assert np.linalg.norm(X[:, c] - xr) <= 1e-8 * max(1.0, np.linalg.norm(xr))
+
+
+@pytest.mark.parametrize("group_w", [2048, 96])
+def test_wide_supernodes(oracle, group_w, monkeypatch):
+    # a 3-D Laplacian KKT whose separators give supernodes wider than 64
+    # columns (panel path); group_w=96 forces the cooperative multi-workgroup path
+    monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
+    monkeypatch.setenv("LBLG_GROUPS", "4")
+    n, Ap, Ai, Ax, B = P.kkt_config(9, 60, 6, seed=5, nrhs=3)
+    print(f"case wide n={n} group_w={group_w}", flush=True)
+    reg = 1e-9
+    s = engine(n, Ap, Ai, Ax, reg=reg)
+    assert s.sym_scalar("max_sup_wid") > 64
+    check_factor(oracle, s, n, Ap, Ai, Ax, reg)
+    X, bwd = s.solve_only(B, return_bwd=True)
+    for c in range(3):
+        assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
+
+
+def test_refine_to_target(oracle):
+    n, Ap, Ai, Ax, B = P.kkt_config(10, 150, 8, seed=9, nrhs=4)
+    s = engine(n, Ap, Ai, Ax, reg=1e-9, target_bwd_err=1e-13)
+    X, bwd = s.solve_only(B, return_bwd=True)
+    assert np.all(bwd <= 1e-13)
+    for c in range(4):
+        assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-12
+
+
+def test_spd_matches_dense_solve():
+    n, Ap, Ai, Ax = P.lower_csc(P.laplacian_2d(23, 17, 1e-2))
+    s = engine(n, Ap, Ai, Ax, reg=1e-9)
+    assert s.inertia() == (n, 0, 0)
+    b = np.random.default_rng(1).uniform(-1, 1, n)
+    x = s.solve_only(b)
+    K = P.lower_to_full(n, Ap, Ai, Ax).toarray()
+    assert np.allclose(x, np.linalg.solve(K, b), rtol=1e-12, atol=1e-12)
