@@ -168,6 +168,14 @@ int lblg_inertia(lblg_handle* h, int64_t* pos, int64_t* neg, int64_t* zero);
  * of the last lblg_factor. */
 int lblg_factor_info(const lblg_handle* h, int* zero_pivot, double* k0_norm_inf, double* factor_seconds);
 
+/* Per-kernel timing with HIP events recorded on the handle's stream (engine
+ * diagnostics; no reference counterpart).  lblg_kernel_time returns the
+ * accumulated milliseconds and launch count of a kernel family since timing
+ * was enabled: bk_small, bk_panel, bk_panel_group, trailing, offdiag, schur,
+ * fwd_tri, fwd_gemv, bwd_gemv, bwd_tri, spmv. */
+int lblg_set_kernel_timing(lblg_handle* h, int on);
+int lblg_kernel_time(lblg_handle* h, const char* name, double* total_ms, int64_t* launches);
+
 /* Device self-test of the FP64 MFMA operand layout: C(16x16) = A(16x4) B(4x16),
  * all row-major.  No reference counterpart (engine diagnostics). */
 int lblg_selftest_mfma(const double* A, const double* B, double* C);

@@ -200,3 +200,22 @@ extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C) {
     return LBLG_OK;
   });
 }
+
+extern "C" int lblg_set_kernel_timing(lblg_handle* hh, int on) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !h->dev) return LBLG_ERR_STATE;
+  h->dev->resolve_timing();
+  h->dev->timing = on != 0;
+  if (on) h->dev->ktime.clear();
+  return LBLG_OK;
+}
+
+extern "C" int lblg_kernel_time(lblg_handle* hh, const char* name, double* total_ms, int64_t* launches) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !h->dev || !name) return LBLG_ERR_ARG;
+  h->dev->resolve_timing();
+  auto it = h->dev->ktime.find(name);
+  if (total_ms) *total_ms = it == h->dev->ktime.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == h->dev->ktime.end() ? 0 : it->second.second;
+  return LBLG_OK;
+}

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <string>
 #include <vector>
 
 namespace lblg {
@@ -112,6 +114,15 @@ struct DeviceFactor {
   int* piece_c0 = nullptr;      // piece -> first local column within the supernode
   int npieces = 0;
   std::vector<void*> owned;     // device allocations freed at destruction
+  // optional per-kernel timing with events on the engine stream
+  bool timing = false;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<double, int64_t>> ktime;  // total ms, launches
+  void resolve_timing();
   ~DeviceFactor();
 };
 

@@ -30,8 +30,41 @@ namespace lblg {
   } while (0)
 
 DeviceFactor::~DeviceFactor() {
+  resolve_timing();
   for (void* p : owned) hipFree(p);
   if (stream) hipStreamDestroy(stream);
+}
+
+void DeviceFactor::resolve_timing() {
+  if (pending.empty()) return;
+  hipStreamSynchronize(stream);
+  for (auto& e : pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e.a, e.b);
+    auto& t = ktime[e.name];
+    t.first += ms;
+    t.second += 1;
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  pending.clear();
+}
+
+// run `launch` bracketed by events on the engine stream when timing is on
+template <typename F>
+static void timed(DeviceFactor& D, const char* name, F launch) {
+  if (!D.timing) {
+    launch();
+    return;
+  }
+  DeviceFactor::Pending p{name, nullptr, nullptr};
+  hipEventCreate(&p.a);
+  hipEventCreate(&p.b);
+  hipEventRecord(p.a, D.stream);
+  launch();
+  hipEventRecord(p.b, D.stream);
+  D.pending.push_back(p);
+  if (D.pending.size() > 4096) D.resolve_timing();
 }
 
 template <typename T>
@@ -292,20 +325,24 @@ void factor_device(Handle& h) {
   const int groups = env_int("LBLG_GROUPS", 32);
   for (size_t l = 0; l < D.flevels.size(); ++l) {
     FactorLevel& F = D.flevels[l];
-    launch_bk_small(F.small.d, F.small.count, a, D.valL, D.dval, D.perm_piv, D.status, st);
+    timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, D.valL, D.dval, D.perm_piv, D.status, st); });
     // medium supernodes: one workgroup each, panel by panel
     if (!F.medium.empty()) {
       int wmax = 0;
       for (int s : F.medium) wmax = std::max(wmax, S.super[s + 1] - S.super[s]);
       const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
       for (int p = 0; p < rounds; ++p) {
-        launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, D.valL, D.dval, D.perm_piv, D.W, D.woff,
-                        D.kpos, D.kstart, D.sync, D.status, false, st);
+        timed(D, "bk_panel", [&] {
+          launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, D.valL, D.dval, D.perm_piv, D.W, D.woff,
+                          D.kpos, D.kstart, D.sync, D.status, false, st);
+        });
         const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
         const int T = (wmax - lo + 63) / 64;
         if (T > 0)
-          launch_trailing(F.d_medium, (int)F.medium.size(), T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff,
-                          D.kpos, D.kstart, st);
+          timed(D, "trailing", [&] {
+            launch_trailing(F.d_medium, (int)F.medium.size(), T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff,
+                            D.kpos, D.kstart, st);
+          });
       }
     }
     // large supernodes: a cooperative group of workgroups each, one at a time
@@ -315,18 +352,26 @@ void factor_device(Handle& h) {
       const int rounds = (w + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
       for (int p = 0; p < rounds; ++p) {
         launch_fill_int(D.sync, 1, 0, st);
-        launch_bk_panel(F.d_large + q, 1, groups, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
-                        D.kstart, D.sync, D.status, true, st);
+        timed(D, "bk_panel_group", [&] {
+          launch_bk_panel(F.d_large + q, 1, groups, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
+                          D.kstart, D.sync, D.status, true, st);
+        });
         const int lo = std::min(w, (p + 1) * (PANEL_NB - 1));
         const int T = (w - lo + 63) / 64;
         if (T > 0)
-          launch_trailing(F.d_large + q, 1, T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff, D.kpos, D.kstart,
-                          st);
+          timed(D, "trailing", [&] {
+            launch_trailing(F.d_large + q, 1, T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff, D.kpos, D.kstart,
+                            st);
+          });
       }
     }
-    launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, D.valL, D.dval,
-                   D.perm_piv, st);
-    launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, D.valL, D.dval, st);
+    timed(D, "offdiag", [&] {
+      launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, D.valL, D.dval,
+                     D.perm_piv, st);
+    });
+    timed(D, "schur", [&] {
+      launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, D.valL, D.dval, st);
+    });
   }
   launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
   // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
@@ -373,22 +418,30 @@ void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
   launch_copy(D.X, in, nn, st);
   for (size_t l = 0; l < D.slevels.size(); ++l) {
     const SolveLevel& L = D.slevels[l];
-    launch_fwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.X, D.Y, nrhs, st);
-    launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, D.Y, D.X,
-                    nrhs, st);
+    timed(D, "fwd_tri", [&] {
+      launch_fwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.X, D.Y, nrhs, st);
+    });
+    timed(D, "fwd_gemv", [&] {
+      launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, D.Y, D.X,
+                      nrhs, st);
+    });
   }
   launch_dsolve(D.n, D.dval, D.Y, D.Z, nrhs, st);
   launch_fill(D.acc, nn, 0.0, st);
   for (size_t l = D.slevels.size(); l-- > 0;) {
     const SolveLevel& L = D.slevels[l];
-    launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, out, D.acc,
-                    nrhs, st);
-    launch_bwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.Z, D.acc, out, nrhs, st);
+    timed(D, "bwd_gemv", [&] {
+      launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, out, D.acc,
+                      nrhs, st);
+    });
+    timed(D, "bwd_tri", [&] {
+      launch_bwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.Z, D.acc, out, nrhs, st);
+    });
   }
 }
 
 static void spmv_ordered(DeviceFactor& D, const double* x, double* y, int nrhs) {
-  launch_spmv(D.n, D.Krp, D.Kci, D.Kv, D.perm_f, D.iperm_f, x, y, nrhs, D.stream);
+  timed(D, "spmv", [&] { launch_spmv(D.n, D.Krp, D.Kci, D.Kv, D.perm_f, D.iperm_f, x, y, nrhs, D.stream); });
 }
 
 static void dots(DeviceFactor& D, int nrhs, const std::vector<std::pair<const double*, const double*>>& pairs,
