@@ -40,6 +40,7 @@ struct FactorLevel {
   std::vector<int> medium_dummy;
   int* d_medium = nullptr;
   int* d_large = nullptr;
+  std::vector<int> large_G, large_nb;  // workgroups and panel width per large supernode
 };
 
 // solve pieces: supernodes split into column blocks of at most SOLVE_W

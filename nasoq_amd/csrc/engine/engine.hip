@@ -211,10 +211,27 @@ static void setup(Handle& h) {
       if (w <= SMALL_W) {
         small.push_back(s);
       } else {
-        if (w <= group_w) F.medium.push_back(s); else F.large.push_back(s);
-        const int G = w <= group_w ? 1 : env_int("LBLG_GROUPS", 32);
         woff[s] = wsum;
-        wsum += (int64_t)w * PANEL_NB + 2 * (int64_t)w + 4 * G + 8;
+        if (w <= group_w) {
+          F.medium.push_back(s);
+          wsum += (int64_t)w * PANEL_NB + 2 * (int64_t)w + 16;
+        } else {
+          // cooperative LDS panel: each workgroup caches <= LDS_ROWS rows of nb columns
+          int nb = 64, G = (w + 239) / 240;
+          if (G > 256) {
+            nb = 32;
+            G = (w + 479) / 480;
+          }
+          const int gmin = std::min(env_int("LBLG_MIN_GROUPS", 8), std::max(1, w / 64));
+          G = std::max(G, gmin);
+          if ((w + G - 1) / G > 512 || (int64_t)((w + G - 1) / G) * nb > 240 * 64)
+            throw std::runtime_error("cooperative panel: rows per workgroup exceed the LDS panel");
+          if (G > 256) throw std::runtime_error("supernode too wide for the cooperative panel kernel");
+          F.large.push_back(s);
+          F.large_G.push_back(G);
+          F.large_nb.push_back(nb);
+          wsum += (int64_t)w * nb + 4 * (int64_t)w + 8 * G + 256;
+        }
       }
       if (r > w) {
         off.push_back(s);
@@ -322,7 +339,6 @@ void factor_device(Handle& h) {
   launch_fill_int(D.kpos, D.nsuper, 0, st);
   launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, h.set.regularization, D.pert, st);
   launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg, D.valL, st);
-  const int groups = env_int("LBLG_GROUPS", 32);
   for (size_t l = 0; l < D.flevels.size(); ++l) {
     FactorLevel& F = D.flevels[l];
     timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, D.valL, D.dval, D.perm_piv, D.status, st); });
@@ -349,14 +365,15 @@ void factor_device(Handle& h) {
     for (size_t q = 0; q < F.large.size(); ++q) {
       const int s = F.large[q];
       const int w = S.super[s + 1] - S.super[s];
-      const int rounds = (w + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
+      const int G = F.large_G[q], nb = F.large_nb[q];
+      const int rounds = (w + nb - 2) / (nb - 1) + 1;
       for (int p = 0; p < rounds; ++p) {
         launch_fill_int(D.sync, 1, 0, st);
         timed(D, "bk_panel_group", [&] {
-          launch_bk_panel(F.d_large + q, 1, groups, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
-                          D.kstart, D.sync, D.status, true, st);
+          launch_bk_panel_lds(F.d_large + q, G, nb, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
+                              D.kstart, D.sync, D.status, st);
         });
-        const int lo = std::min(w, (p + 1) * (PANEL_NB - 1));
+        const int lo = std::min(w, (p + 1) * (nb - 1));
         const int T = (w - lo + 63) / 64;
         if (T > 0)
           timed(D, "trailing", [&] {

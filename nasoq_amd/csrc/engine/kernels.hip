@@ -335,11 +335,11 @@ void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, 
 // proceeds, so no wave can wait forever.
 // --------------------------------------------------------------------------
 __device__ inline void group_barrier(int* ctr, int G, int& epoch, int* status) {
-  __threadfence();
   __syncthreads();
   ++epoch;
   if (G > 1) {
     if (threadIdx.x == 0) {
+      __threadfence();  // release: the workgroup's stores (all in L2 after the barrier) become device-visible
       atomicAdd(ctr, 1);
       const int target = epoch * G;
       long spins = 0;
@@ -350,9 +350,9 @@ __device__ inline void group_barrier(int* ctr, int G, int& epoch, int* status) {
           break;
         }
       }
+      __threadfence();  // acquire: drop stale cache lines before the group reads shared data
     }
     __syncthreads();
-    __threadfence();
   }
 }
 
@@ -577,9 +577,12 @@ void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double*
 // FP64 MFMA tile engine: a 256-thread workgroup computes a 64x64 block
 // C(i,j) = sum_k A(i,k) B(j,k) with A and B staged in LDS as [k][64]
 // (k-major).  Each wave owns a 32x32 quadrant = 2x2 blocks of
-// v_mfma_f64_16x16x4f64.  Operand lanes: A(row = lane%16, k = lane/16),
-// B(col = lane%16, k = lane/16); accumulator element e of a lane holds
-// row 4*e + lane/16, column lane%16 (checked by lblg_selftest_mfma).
+// v_mfma_f64_16x16x4f64.  Instruction lanes: first operand (row = lane%16,
+// k = lane/16), second operand (col = lane%16, k = lane/16), accumulator
+// element e of a lane = row 4*e + lane/16, column lane%16 (checked by
+// lblg_selftest_mfma).  The tile engine passes B first so that the
+// accumulator holds C^T: lanes 0..15 then cover 16 consecutive rows of C and
+// the epilogues touch contiguous memory.
 // --------------------------------------------------------------------------
 #define KC 32          // k-chunk staged in LDS
 #define LDT 66         // padded row length of a staged tile
@@ -600,10 +603,11 @@ __device__ inline void tile_mma(TileAcc& acc, const double* As, const double* Bs
     const double a1 = As[(k + lk) * LDT + r0 + 16 + li];
     const double b0 = Bs[(k + lk) * LDT + q0 + li];
     const double b1 = Bs[(k + lk) * LDT + q0 + 16 + li];
-    acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.c[0][0], 0, 0, 0);
-    acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.c[0][1], 0, 0, 0);
-    acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.c[1][0], 0, 0, 0);
-    acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc.c[1][1], 0, 0, 0);
+    // D = B_blk A_blk^T, i.e. the transpose of the C block: lane (ln & 15) walks rows
+    acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc.c[0][0], 0, 0, 0);
+    acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc.c[0][1], 0, 0, 0);
+    acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc.c[1][0], 0, 0, 0);
+    acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc.c[1][1], 0, 0, 0);
   }
 }
 // visit every accumulator element: f(row_in_tile, col_in_tile, value)
@@ -614,8 +618,8 @@ __device__ inline void tile_foreach(const TileAcc& acc, F f) {
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int e = 0; e < 4; ++e) {
-        const int row = r0 + 16 * a + 4 * e + (ln >> 4);
-        const int col = q0 + 16 * b + (ln & 15);
+        const int row = r0 + 16 * a + (ln & 15);
+        const int col = q0 + 16 * b + 4 * e + (ln >> 4);
         f(row, col, acc.c[a][b][e]);
       }
 }
@@ -829,7 +833,11 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
                                                const int* rows, const int* col2sup, double* valL,
                                                const double* dval, int n) {
   __shared__ double As[KC * LDT], Bs[KC * LDT];
-  __shared__ int rlab[64], clab[64];
+  __shared__ double halo[2][64];
+  __shared__ double dd[KC + 2], oo[KC + 2];
+  __shared__ int rlab[64], clab[64], ctg[64], dtcol[64];
+  __shared__ int tpos[4][64];
+  __shared__ int dtl[4], ntg;
   int lo = 0, hi = count - 1;
   const int64_t bid = blockIdx.x;
   while (lo < hi) {
@@ -851,34 +859,73 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   if (t < 64) {
     rlab[t] = (i0 + t < m) ? rows[pi[s] + w + i0 + t] : -1;
     clab[t] = (j0 + t < m) ? rows[pi[s] + w + j0 + t] : -1;
+    ctg[t] = clab[t] >= 0 ? col2sup[clab[t]] : -1;
+  }
+  __syncthreads();
+  // distinct target supernodes of the tile's columns: labels are sorted, so
+  // targets come in runs; run starts are found with one wave ballot
+  if (t < 64) {
+    const int tg = ctg[t];
+    const bool start = tg >= 0 && (t == 0 || ctg[t - 1] != tg);
+    const unsigned long long mask = __ballot(start);
+    const int run = __popcll(mask & (~0ull >> (63 - t))) - 1;  // run index of column t
+    dtcol[t] = (tg >= 0 && run < 4) ? run : -1;
+    if (start && run < 4) dtl[run] = tg;
+    if (t == 0) ntg = min(4, __popcll(mask));
+  }
+  __syncthreads();
+  for (int idx = t; idx < ntg * 64; idx += 256) {
+    const int dt = idx / 64, i = idx % 64;
+    const int tg = dtl[dt];
+    const int R = rlab[i];
+    tpos[dt][i] = (R >= super[tg]) ? (int)row_pos(rows, pi[tg], pi[tg + 1], R) : -1;
   }
   TileAcc acc;
   tile_zero(acc);
   for (int kc0 = 0; kc0 < w; kc0 += KC) {
     const int kc = min(KC, w - kc0);
+    // raw L columns of the chunk, plus one halo column on each side (2x2 blocks)
     for (int idx = t; idx < KC * 64; idx += 256) {
       const int kk = idx / 64, x = idx % 64;
-      double va = 0.0, vb = 0.0;
+      const int la = kc0 + kk;
+      const int i = i0 + x, j = j0 + x;
+      As[kk * LDT + x] = (kk < kc && i < m) ? P[w + i + (int64_t)la * r] : 0.0;
+      Bs[kk * LDT + x] = (kk < kc && j < m) ? P[w + j + (int64_t)la * r] : 0.0;
+    }
+    if (t < 128) {
+      const int side = t >> 6, x = t & 63;
+      const int l = side == 0 ? kc0 - 1 : kc0 + kc;
+      const int j = j0 + x;
+      halo[side][x] = (l >= 0 && l < w && j < m) ? P[w + j + (int64_t)l * r] : 0.0;
+    }
+    if (t < KC + 2) {
+      const int l = kc0 - 1 + t;
+      dd[t] = (l >= 0 && l < w) ? dval[c0 + l] : 0.0;
+      oo[t] = (l >= 0 && l < w) ? dval[n + c0 + l] : 0.0;
+    }
+    __syncthreads();
+    // B = (L D) columns: 1x1 scaling or the matching row of a 2x2 block,
+    // computed in registers, then written back in place
+    double bv[KC * 64 / 256];
+    for (int q = 0; q < KC * 64 / 256; ++q) {
+      const int idx = t + 256 * q;
+      const int kk = idx / 64, x = idx % 64;
+      double vb = 0.0;
       if (kk < kc) {
-        const int l = kc0 + kk;
-        const int i = i0 + x, j = j0 + x;
-        if (i < m) va = P[w + i + (int64_t)l * r];
-        if (j < m) {
-          // (L D)(j, l): 1x1 or the matching row of a 2x2 block
-          const double od = dval[n + c0 + l];
-          const double odp = (l > 0) ? dval[n + c0 + l - 1] : 0.0;
-          const double x0 = P[w + j + (int64_t)l * r];
-          if (od != 0.0) {
-            vb = dval[c0 + l] * x0 + od * P[w + j + (int64_t)(l + 1) * r];
-          } else if (odp != 0.0) {
-            vb = odp * P[w + j + (int64_t)(l - 1) * r] + dval[c0 + l] * x0;
-          } else {
-            vb = dval[c0 + l] * x0;
-          }
-        }
+        const int h = kk + 1;  // index into dd / oo
+        const double x0 = Bs[kk * LDT + x];
+        const double xn = (kk + 1 < kc) ? Bs[(kk + 1) * LDT + x] : halo[1][x];
+        const double xp = (kk > 0) ? Bs[(kk - 1) * LDT + x] : halo[0][x];
+        if (oo[h] != 0.0) vb = dd[h] * x0 + oo[h] * xn;
+        else if (oo[h - 1] != 0.0) vb = oo[h - 1] * xp + dd[h] * x0;
+        else vb = dd[h] * x0;
       }
-      As[kk * LDT + x] = va;
-      Bs[kk * LDT + x] = vb;
+      bv[q] = vb;
+    }
+    __syncthreads();
+    for (int q = 0; q < KC * 64 / 256; ++q) {
+      const int idx = t + 256 * q;
+      Bs[(idx / 64) * LDT + idx % 64] = bv[q];
     }
     __syncthreads();
     tile_mma(acc, As, Bs, (kc + 3) & ~3);
@@ -887,9 +934,10 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   tile_foreach(acc, [&](int ri, int ci, double v) {
     const int R = rlab[ri], C = clab[ci];
     if (R < 0 || C < 0 || R < C) return;
-    const int tg = col2sup[C];
+    const int tg = ctg[ci];
     const int64_t rt = pi[tg + 1] - pi[tg];
-    const int64_t pos = row_pos(rows, pi[tg], pi[tg + 1], R);
+    const int dt = dtcol[ci];
+    const int64_t pos = dt >= 0 ? tpos[dt][ri] : row_pos(rows, pi[tg], pi[tg + 1], R);
     atomicAdd(valL + px[tg] + (int64_t)(C - super[tg]) * rt + pos, -v);
   });
 }
@@ -1481,5 +1529,286 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
   HIP_LAUNCH_CHECK("rowabs");
   k_max_final<<<1, 64, 0, st>>>(part, nblk, norm_out);
   HIP_LAUNCH_CHECK("max_final");
+}
+}  // namespace lblg
+
+namespace lblg {
+// --------------------------------------------------------------------------
+// Cooperative panel Bunch-Kaufman for very wide diagonal blocks (the root
+// separators of KKT matrices).  G workgroups each own a contiguous chunk of
+// the rows [k0, w) of the panel and keep their rows of the panel's L in LDS,
+// so a column step reads only LDS, one column of A and two broadcast rows of
+// W.  Cross-workgroup traffic per step: the partial maxima, the updated
+// candidate columns t0/t1 (global, double-buffered by step parity) and, for
+// interchanges, the L rows of the two swapped rows (staged in swap slots
+// before the decision is known).  Steps without an interchange need one
+// group barrier, steps that inspect column imax two, interchanges three.
+// The pivot rules are those of k_bk_panel (dsytf2 / dlasyf).
+//
+// workspace at Wbase + woff[s]:
+//   Wp (w x nb) | tb (4w: t0[2][w], t1[2][w]) | pv (4G) | pix (4G ints) | slots (3 x 64)
+// --------------------------------------------------------------------------
+#define LDS_PANEL_DOUBLES (240 * 64)
+__global__ __launch_bounds__(256) void k_bk_panel_lds(const int* list, int G, int nb, const int* super,
+                                                      const int64_t* pi, const int64_t* px, double* valL,
+                                                      double* dval, int* perm_piv, double* Wbase,
+                                                      const int64_t* woff, int* kpos, int* kstart, int* sync,
+                                                      int n, int* status) {
+  __shared__ double Lc[LDS_PANEL_DOUBLES];
+  __shared__ double wk[64], wi[64];
+  __shared__ double sv[8];
+  __shared__ int si[8];
+  const int g = blockIdx.x;
+  const int s = list[0];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  double* A = valL + px[s];
+  double* Wp = Wbase + woff[s];
+  double* tb = Wp + (int64_t)w * nb;
+  double* pv = tb + 4 * (int64_t)w;
+  int* pix = reinterpret_cast<int*>(pv + 4 * G);
+  double* slots = pv + 4 * G + 2 * G + 2;
+  int* ctr = sync;
+  const int t = threadIdx.x, GT = G * 256, gt = g * 256 + t;
+  const int k0 = kpos[s];
+  if (k0 >= w) return;
+  const bool last = (w - k0) <= nb;
+  const int chunk = (w - k0 + G - 1) / G;  // <= 512 rows: at most two per thread
+  const int rbeg = k0 + g * chunk, rend = min(w, rbeg + chunk);
+  int epoch = 0;
+#define AA(i, j) A[(int64_t)(j) * r + (i)]
+#define WW(i, j) Wp[(int64_t)((j) - k0) * w + (i)]
+#define LC(i, j) Lc[((i) - rbeg) * nb + ((j) - k0)]
+#define T0(p, i) tb[(int64_t)(p) * w + (i)]
+#define T1(p, i) tb[(int64_t)(2 + (p)) * w + (i)]
+  // previous step bookkeeping for the broadcast rows of W
+  int pk = -1, pkind = 0, ppar = 0;  // pkind 1: 1x1 no swap, 2: 2x2 no swap, 0: synced
+  int k = k0, step = 0;
+  while (k < w && (last || k - k0 < nb - 1)) {
+    const int par = step & 1;
+    // prefetch the own rows of column k (at most two per thread) while the
+    // broadcast row of W is gathered
+    const int ia = max(rbeg, k) + t, ib = ia + 256;
+    double pa = 0.0, pb = 0.0;
+    if (ia < rend) pa = AA(ia, k);
+    if (ib < rend) pb = AA(ib, k);
+    // ---- broadcast row k of W for the panel columns [k0, k)
+    for (int j = k0 + t; j < k; j += 256) {
+      double v;
+      if (pkind == 1 && j == pk) v = T0(ppar, k);
+      else if (pkind == 2 && j == pk) v = T0(ppar, k);
+      else if (pkind == 2 && j == pk + 1) v = T1(ppar, k);
+      else v = WW(k, j);
+      wk[j - k0] = v;
+    }
+    __syncthreads();
+    // ---- phase A: column k for own rows
+    double bv = -1.0;
+    int bi = 0x7fffffff;
+    for (int i = ia, q = 0; i < rend && q < 2; i += 256, ++q) {
+      double v = q == 0 ? pa : pb;
+      for (int j = k0; j < k; ++j) v -= LC(i, j) * wk[j - k0];
+      T0(par, i) = v;
+      if (i > k) {
+        double av = fabs(v);
+        if (!(av == av)) av = -1.0;
+        better(bv, bi, av, i);
+      }
+    }
+    block_argmax(bv, bi, sv, si);
+    if (t == 0) {
+      pv[(par * 2 + 0) * G + g] = bv;
+      pix[(par * 2 + 0) * G + g] = bi;
+    }
+    group_barrier(ctr, G, epoch, status);
+    double colmax = -1.0;
+    int imax = 0x7fffffff;
+    for (int q = 0; q < G; ++q) better(colmax, imax, pv[(par * 2) * G + q], pix[(par * 2) * G + q]);
+    if (k == w - 1 || colmax < 0) {
+      colmax = 0.0;
+      imax = k;
+    }
+    const double absakk = fabs(T0(par, k));
+    int kp = k, kstep = 1, zero = 0;
+    bool use_t1 = false;
+    if (fmax(absakk, colmax) == 0.0 || !(absakk == absakk)) {
+      zero = 1;
+    } else if (!(absakk >= BK_ALPHA * colmax)) {
+      // ---- phase B: column imax for own rows; stage candidate swap rows
+      double qa = 0.0, qb = 0.0;
+      if (ia < rend) qa = (ia < imax) ? AA(imax, ia) : AA(ia, imax);
+      if (ib < rend) qb = (ib < imax) ? AA(imax, ib) : AA(ib, imax);
+      for (int j = k0 + t; j < k; j += 256) {
+        double v;
+        if (pkind == 1 && j == pk) v = T0(ppar, imax);
+        else if (pkind == 2 && j == pk) v = T0(ppar, imax);
+        else if (pkind == 2 && j == pk + 1) v = T1(ppar, imax);
+        else v = WW(imax, j);
+        wi[j - k0] = v;
+      }
+      __syncthreads();
+      double rv = -1.0;
+      int ri = 0x7fffffff;
+      for (int i = ia, q = 0; i < rend && q < 2; i += 256, ++q) {
+        double v = q == 0 ? qa : qb;
+        for (int j = k0; j < k; ++j) v -= LC(i, j) * wi[j - k0];
+        T1(par, i) = v;
+        if (i != imax) {
+          double av = fabs(v);
+          if (!(av == av)) av = -1.0;
+          better(rv, ri, av, i);
+        }
+      }
+      // candidate rows for an interchange: imax, k and k+1
+      for (int j = k0 + t; j < k; j += 256) {
+        if (imax >= rbeg && imax < rend) slots[j - k0] = LC(imax, j);
+        if (k >= rbeg && k < rend) slots[64 + j - k0] = LC(k, j);
+        if (k + 1 < w && k + 1 >= rbeg && k + 1 < rend) slots[128 + j - k0] = LC(k + 1, j);
+      }
+      block_argmax(rv, ri, sv, si);
+      if (t == 0) {
+        pv[(par * 2 + 1) * G + g] = rv;
+        pix[(par * 2 + 1) * G + g] = ri;
+      }
+      group_barrier(ctr, G, epoch, status);
+      double rowmax = -1.0;
+      int rj = 0x7fffffff;
+      for (int q = 0; q < G; ++q) better(rowmax, rj, pv[(par * 2 + 1) * G + q], pix[(par * 2 + 1) * G + q]);
+      if (rowmax < 0) rowmax = 0.0;
+      if (absakk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+        kp = k;
+      } else if (fabs(T1(par, imax)) >= BK_ALPHA * rowmax) {
+        kp = imax;
+        use_t1 = true;
+      } else {
+        kp = imax;
+        kstep = 2;
+      }
+    }
+    const int kk = k + kstep - 1;
+    const bool swap = kp != kk;
+    // ---- phase C
+    if (zero && gt == 0) atomicOr(status, 1);
+    if (swap) {
+      // LDS rows of the current panel (from the staged slots)
+      for (int j = k0 + t; j < k; j += 256) {
+        if (kk >= rbeg && kk < rend) LC(kk, j) = slots[j - k0];
+        if (kp >= rbeg && kp < rend) LC(kp, j) = slots[(kk == k ? 64 : 128) + j - k0];
+      }
+      // global rows kk/kp of all earlier L columns and of the panel's W
+      for (int j = gt; j < k; j += GT) {
+        const double x = AA(kk, j);
+        AA(kk, j) = AA(kp, j);
+        AA(kp, j) = x;
+      }
+      for (int j = k0 + gt; j < k; j += GT) {
+        const double x = WW(kk, j);
+        WW(kk, j) = WW(kp, j);
+        WW(kp, j) = x;
+      }
+      for (int64_t q = w + gt; q < r; q += GT) {
+        const double x = AA(q, kk);
+        AA(q, kk) = AA(q, kp);
+        AA(q, kp) = x;
+      }
+      if (gt == 0) {
+        const int tp = perm_piv[c0 + kk];
+        perm_piv[c0 + kk] = perm_piv[c0 + kp];
+        perm_piv[c0 + kp] = tp;
+      }
+    }
+    double D1 = 0, D2 = 0, OD = 0, d21 = 0, d11 = 0, d22 = 0, rk = 0;
+    if (kstep == 1) {
+      const int sk = swap ? kp : k;
+      D1 = use_t1 ? T1(par, sk) : T0(par, sk);
+      rk = 1.0 / D1;
+    } else {
+      D1 = T0(par, k);
+      OD = T0(par, kp);
+      D2 = T1(par, kp);
+      d21 = OD;
+      d11 = D2 / d21;
+      d22 = D1 / d21;
+      const double tt = 1.0 / (d11 * d22 - 1.0);
+      d21 = tt / d21;
+    }
+    for (int i = max(rbeg, k) + t; i < rend; i += 256) {
+      const int si2 = !swap ? i : (i == kk ? kp : (i == kp ? kk : i));
+      if (swap) {
+        if (i == kk) AA(kp, kp) = AA(kk, kk);
+        else if (i > kk && i < kp) AA(kp, i) = AA(i, kk);
+        else if (i > kp) AA(i, kp) = AA(i, kk);
+      }
+      if (kstep == 1) {
+        const double wv = use_t1 ? T1(par, si2) : T0(par, si2);
+        WW(i, k) = wv;
+        if (i == k) {
+          AA(k, k) = 1.0;
+          LC(k, k) = 1.0;
+          dval[c0 + k] = wv;
+          dval[n + c0 + k] = 0.0;
+        } else {
+          const double l = zero ? wv : wv * rk;
+          AA(i, k) = l;
+          LC(i, k) = l;
+        }
+      } else {
+        const double w0 = T0(par, si2), w1 = T1(par, si2);
+        WW(i, k) = w0;
+        WW(i, k + 1) = w1;
+        if (i == k) {
+          AA(k, k) = 1.0;
+          LC(k, k) = 1.0;
+          LC(k, k + 1) = 0.0;
+          dval[c0 + k] = D1;
+          dval[n + c0 + k] = OD;
+        } else if (i == k + 1) {
+          AA(k + 1, k) = 0.0;
+          AA(k + 1, k + 1) = 1.0;
+          LC(k + 1, k) = 0.0;
+          LC(k + 1, k + 1) = 1.0;
+          dval[c0 + k + 1] = D2;
+          dval[n + c0 + k + 1] = 0.0;
+        } else {
+          const double l0 = d21 * (d11 * w0 - w1), l1 = d21 * (d22 * w1 - w0);
+          AA(i, k) = l0;
+          AA(i, k + 1) = l1;
+          LC(i, k) = l0;
+          LC(i, k + 1) = l1;
+        }
+      }
+    }
+    if (swap) {
+      group_barrier(ctr, G, epoch, status);
+      pkind = 0;
+    } else {
+      __syncthreads();
+      pkind = kstep;
+      pk = k;
+      ppar = par;
+    }
+    k += kstep;
+    ++step;
+  }
+  if (gt == 0) {
+    kstart[s] = k0;
+    kpos[s] = k;
+  }
+#undef AA
+#undef WW
+#undef LC
+#undef T0
+#undef T1
+}
+void launch_bk_panel_lds(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval,
+                         int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+                         int* status, hipStream_t st) {
+  if (nb > 64 || nb < 4) throw std::runtime_error("bk_panel_lds: panel width out of range");
+  int n = a.n;
+  void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
+                  (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
+                  (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&n, (void*)&status};
+  hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel_lds, dim3(G), dim3(256), args, 0, st);
+  if (e != hipSuccess) throw std::runtime_error(std::string("bk_panel_lds: ") + hipGetErrorString(e));
 }
 }  // namespace lblg

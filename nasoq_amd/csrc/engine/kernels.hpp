@@ -32,6 +32,9 @@ void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, 
 void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double* valL, double* dval,
                      int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                      int* status, bool cooperative, hipStream_t st);
+void launch_bk_panel_lds(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval,
+                         int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+                         int* status, hipStream_t st);
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
                      const double* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st);

@@ -104,14 +104,15 @@ def test_small_kkt_parity(oracle, seed):
         assert np.linalg.norm(X[:, c] - xr) <= 1e-8 * max(1.0, np.linalg.norm(xr))
 
 
-@pytest.mark.parametrize("group_w", [2048, 96])
-def test_wide_supernodes(oracle, group_w, monkeypatch):
-    # a 3-D Laplacian KKT whose separators give supernodes wider than 64
-    # columns (panel path); group_w=96 forces the cooperative multi-workgroup path
+@pytest.mark.parametrize("group_w,grid,m", [(2048, 9, 60), (96, 9, 60), (96, 12, 300), (256, 14, 500)])
+def test_wide_supernodes(oracle, group_w, grid, m, monkeypatch):
+    # 3-D Laplacian KKTs whose separators give supernodes wider than 64
+    # columns (panel path); small group_w sends them to the cooperative
+    # multi-workgroup LDS panel kernel
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
-    monkeypatch.setenv("LBLG_GROUPS", "4")
-    n, Ap, Ai, Ax, B = P.kkt_config(9, 60, 6, seed=5, nrhs=3)
-    print(f"case wide n={n} group_w={group_w}", flush=True)
+    monkeypatch.setenv("LBLG_MIN_GROUPS", "8")
+    n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 6, seed=5, nrhs=3)
+    print(f"case wide n={n} group_w={group_w} grid={grid}", flush=True)
     reg = 1e-9
     s = engine(n, Ap, Ai, Ax, reg=reg)
     assert s.sym_scalar("max_sup_wid") > 64
@@ -138,3 +139,17 @@ def test_spd_matches_dense_solve():
     x = s.solve_only(b)
     K = P.lower_to_full(n, Ap, Ai, Ax).toarray()
     assert np.allclose(x, np.linalg.solve(K, b), rtol=1e-12, atol=1e-12)
+
+
+def test_cpp_dropin_lbl_main(tmp_path):
+    # the reference's examples/lbl_main.cpp flow, compiled against the C++
+    # drop-in nasoq::SolverSettings header and run as its own process
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "lbl_main")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "lbl_main.cpp"), "-L", os.path.join(root, "nasoq_amd"),
+                    "-llblg", "-Wl,-rpath," + os.path.join(root, "nasoq_amd"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "Solution: 4,4" in out.stdout
