@@ -54,13 +54,13 @@ def trailing_flops(sup, small_w=64):
 
 def cpu_baseline(config, full_fl):
     """Oracle (FP64 CPU restatement, one thread) on a bounded sample of the same
-    workload: the same KKT construction on a 28^3 grid, scaled to the full
+    workload: the same KKT construction on a 36^3 grid, scaled to the full
     configuration by the ratio of sum c_j^2."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from nasoq_amd import lbl
     from nasoq_amd import problems as P
-    g = 28
+    g = 36
     m = int(round(CONFIGS[config][1] * (g / CONFIGS[config][0]) ** 3))
     n, Ap, Ai, Ax, B = P.kkt_config(g, m, CONFIGS[config][2], seed=0, nrhs=4)
     s = lbl.SolverSettings(n, Ap, Ai, Ax)
@@ -211,6 +211,10 @@ def main():
         "kernel_ms_per_step": {k: v[0] for k, v in kt.items()},
         "roofline": roof,
     }
+    dval = np.zeros(2 * n)
+    _lib.check(lib.lblg_get_factor(s.h, None, _lib.dptr(dval), None, None), "get_factor")
+    out["pivots_2x2"] = int(np.count_nonzero(dval[n:]))
+    out["inertia"] = list(s.inertia())
     s.close()
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

@@ -104,3 +104,24 @@ def checksum(*arrays):
     for a in arrays:
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()[:16]
+
+
+def block_indefinite(nblk, bs, seed=0, coupling=0.3, nrhs=1):
+    """Block-tridiagonal symmetric indefinite matrix whose diagonal blocks are
+    dense with an all-zero diagonal (so Bunch-Kaufman must take 2x2 pivots
+    inside supernodes), coupled by sparse random off-diagonal blocks."""
+    rng = np.random.default_rng(seed)
+    n = nblk * bs
+    blocks = []
+    for b in range(nblk):
+        D = rng.standard_normal((bs, bs))
+        D = D + D.T
+        np.fill_diagonal(D, 0.0)
+        blocks.append(sp.coo_matrix(D))
+    K = sp.block_diag(blocks, format="lil")
+    for b in range(nblk - 1):
+        C = sp.random(bs, bs, density=coupling, random_state=rng).toarray()
+        K[(b + 1) * bs:(b + 2) * bs, b * bs:(b + 1) * bs] = C
+        K[b * bs:(b + 1) * bs, (b + 1) * bs:(b + 2) * bs] = C.T
+    B = rng.uniform(-1.0, 1.0, size=(n, nrhs))
+    return lower_csc(K.tocsc()) + (B,)

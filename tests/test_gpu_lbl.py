@@ -122,6 +122,28 @@ def test_wide_supernodes(oracle, group_w, grid, m, monkeypatch):
         assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
 
 
+@pytest.mark.parametrize("nb,bs,order,group_w", [(8, 10, None, 2048), (20, 12, "nat", 2048), (6, 40, "nat", 2048),
+                                                  (4, 200, "nat", 2048), (4, 200, "nat", 96),
+                                                  (3, 600, "nat", 256)])
+def test_two_by_two_pivots(oracle, nb, bs, order, group_w, monkeypatch):
+    # dense zero-diagonal blocks force Bunch-Kaufman 2x2 pivots in every panel
+    # path: w <= 64 (LDS kernel), 64 < w <= group_w (single-workgroup panel),
+    # w > group_w (cooperative multi-workgroup panel)
+    monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
+    monkeypatch.setenv("LBLG_MIN_GROUPS", "8")
+    n, Ap, Ai, Ax, B = P.block_indefinite(nb, bs, seed=1, nrhs=2)
+    perm = np.arange(n, dtype=np.int32) if order == "nat" else None
+    print(f"case 2x2 n={n} nb={nb} bs={bs} order={order} group_w={group_w}", flush=True)
+    reg = 1e-9
+    s = engine(n, Ap, Ai, Ax, reg=reg, perm=perm)
+    F = check_factor(oracle, s, n, Ap, Ai, Ax, reg)
+    _, dval, _, _ = s.get_factor()
+    assert np.count_nonzero(dval[n:]) > 0 and np.count_nonzero(F.d_val[n:]) > 0
+    X, bwd = s.solve_only(B, return_bwd=True)
+    for c in range(2):
+        assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
+
+
 def test_refine_to_target(oracle):
     n, Ap, Ai, Ax, B = P.kkt_config(10, 150, 8, seed=9, nrhs=4)
     s = engine(n, Ap, Ai, Ax, reg=1e-9, target_bwd_err=1e-13)
