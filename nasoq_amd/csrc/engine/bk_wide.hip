@@ -1,0 +1,562 @@
+// Cooperative panel Bunch-Kaufman for very wide diagonal blocks (the root
+// separators of KKT matrices), LAPACK dlasyf-style: the columns of a panel
+// of nb <= 64 columns are formed left-looking from the panel's L and
+// W = L D, pivots follow the dsytf2 rules (alpha = (1+sqrt(17))/8) on the
+// fully updated columns, and the rank-nb trailing update runs afterwards in
+// k_trailing.  Reference: LAPACKE_dsytrf on the diagonal block at
+// src/ldl/parallel_blocked_ldlt_02.cpp:155 and the column permutation of the
+// rows below it (LAPACKE_dlapmt, :164-165).
+//
+// Work split.  The rows [k0, w) of the panel are the nb "diagonal" rows
+// [k0, k0+nb) and the "far" rows below them.  G workgroups each own up to
+// WCH far rows (one per thread of waves 0-2) and keep those rows of the
+// panel (L for eliminated columns, A for pending ones) in LDS.  Every
+// workgroup also keeps the diagonal rows (nb x nb) and computes their
+// updated values itself, so the W row needed by each column step is always
+// local.
+//
+// Exchange.  Per column step the workgroups publish one small record (the
+// largest candidate of the column) as flag-in-word 64-bit values written
+// and polled with agent-scope atomics; the flag is a per-step epoch, so the
+// record is its own barrier.  A step that has to inspect a second column
+// (absakk < alpha * colmax) adds the candidate row of the winning group
+// (published speculatively with the record) and a second record round.
+//
+// Interchanges.  Within the panel nothing is written to the matrix: the
+// interchange of logical rows/columns kk and kp only updates a
+// logical-to-physical map (the diagonal rows' map and a table of moved far
+// rows) and the LDS row copies.  At the end of the panel the accumulated
+// permutation is applied in place, through a scratch copy between two group
+// barriers, to the trailing block, to the earlier columns of L and to the
+// columns of the rows below the diagonal block; then L and W of the panel
+// are written.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "engine/device_util.hpp"
+#include "engine/kernels.hpp"
+
+namespace lblg {
+
+#define WCH 192                        // far rows per workgroup
+#define WNB 64                         // widest panel
+#define WMM (2 * WNB)                  // moved indices per panel
+#define WLDA (WNB + 1)                 // padded row length of the diagonal rows
+#define MBITS_WORDS (256 * WCH / 32)   // moved-far-row bitmap
+#define LL_SPIN (1L << 22)
+
+// exchange buffer layout, in 64-bit words (see kernels.hpp: LL_WORDS)
+#define LL_RECA 0                            // [2][256][4]
+#define LL_RECB (LL_RECA + 2 * 256 * 4)      // [2][256][8]
+#define LL_PAY (LL_RECB + 2 * 256 * 8)       // [2][256][2 * WNB]
+
+static_assert(LL_PAY + 2 * 256 * 2 * WNB == LL_WORDS, "exchange buffer size");
+
+__device__ inline void ll_store(uint64_t* p, uint32_t v, uint32_t flag) {
+  __hip_atomic_store(p, ((uint64_t)flag << 32) | (uint64_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait (bounded) for the word to carry this epoch's flag; abort != 0 skips the wait
+__device__ inline uint32_t ll_load(uint64_t* p, uint32_t flag, int& abort) {
+  long spins = 0;
+  while (!abort) {
+    const uint64_t x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(x >> 32) == flag) return (uint32_t)x;
+    if (++spins > LL_SPIN) abort = 1;
+  }
+  return 0u;
+}
+__device__ inline void ll_store_d(uint64_t* p, double v, uint32_t flag) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  ll_store(p, (uint32_t)b, flag);
+  ll_store(p + 1, (uint32_t)(b >> 32), flag);
+}
+__device__ inline double ll_load_d(uint64_t* p, uint32_t flag, int& abort) {
+  const uint64_t lo = ll_load(p, flag, abort);
+  const uint64_t hi = ll_load(p + 1, flag, abort);
+  return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+// W(i, j) = sum_c L(i, c) D(c, j) for an eliminated panel column j; l(j) reads L(i, j)
+template <typename F>
+__device__ inline double w_entry(F l, int j, const double* dd, const double* dsb) {
+  if (dsb[j] != 0.0) return l(j) * dd[j] + l(j + 1) * dsb[j];
+  if (j > 0 && dsb[j - 1] != 0.0) return l(j - 1) * dsb[j - 1] + l(j) * dd[j];
+  return l(j) * dd[j];
+}
+
+__device__ inline int far_phys(int a, int fstart, const unsigned* mbits, const int* mlog, const int* mphys,
+                               int nm) {
+  const int b = a - fstart;
+  if (!(mbits[b >> 5] & (1u << (b & 31)))) return a;
+  for (int q = 0; q < nm; ++q)
+    if (mlog[q] == a) return mphys[q];
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb, const int* super,
+                                                 const int64_t* pi, const int64_t* px, double* valL,
+                                                 double* dval, int* perm_piv, double* Wbase,
+                                                 const int64_t* woff, int* kpos, int* kstart, int* sync,
+                                                 uint64_t* ll, uint32_t ebase, int n, int* status) {
+  __shared__ double LC[WNB * WCH];       // own far rows, column-major [column][row]
+  __shared__ double Ad[WNB * WLDA];      // diagonal rows: L (eliminated) | symmetric A (pending)
+  __shared__ double wk[WNB], wi[WNB], td0[WNB], td1[WNB], dd[WNB], dsb[WNB], prow[WNB];
+  __shared__ int dphys[WNB];
+  __shared__ int mlog[WMM], mphys[WMM], Ml[WMM], Mp[WMM];
+  __shared__ unsigned mbits[MBITS_WORDS];
+  __shared__ double redv[4];
+  __shared__ int redi[4];
+  __shared__ double bcd[4];
+  __shared__ int bci[4];
+  __shared__ int s_nm, s_abort, s_nM, s_nMa;
+
+  const int g = blockIdx.x, t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int s = list[0];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  double* A = valL + px[s];
+  double* Wp = Wbase + woff[s];
+  double* scr = Wp + (int64_t)w * nb;
+  const int k0 = kpos[s];
+  if (k0 >= w) return;
+  const int nbe = min(nb, w - k0);
+  const bool last = (w - k0) <= nb;
+  const int fstart = k0 + nbe, nfar = w - fstart;
+  const int chunk = nfar > 0 ? (nfar + G - 1) / G : 0;
+  if (chunk > WCH || nb > WNB) {  // uniform over the grid: every workgroup returns
+    if (g == 0 && t == 0) atomicOr(status, 8);
+    return;
+  }
+  const int fbeg = fstart + g * chunk;
+  const int fcnt = max(0, min(chunk, w - fbeg));
+  const bool hasrow = t < fcnt;
+  const int fr = fbeg + t;
+  int myphys = fr;
+#define AG(i, j) A[(int64_t)(j) * r + (i)]
+#define LCR(j) LC[(j) * WCH + t]
+#define ADR(i, j) Ad[(i) * WLDA + (j)]
+
+  // ---- load the panel rows
+  if (hasrow)
+    for (int j = 0; j < nbe; ++j) LCR(j) = AG(fr, k0 + j);
+  for (int idx = t; idx < nbe * nbe; idx += 256) {
+    const int i = idx % nbe, j = idx / nbe;
+    ADR(i, j) = (i >= j) ? AG(k0 + i, k0 + j) : AG(k0 + j, k0 + i);
+  }
+  if (t < WNB) {
+    dphys[t] = k0 + t;
+    dd[t] = 0.0;
+    dsb[t] = 0.0;
+  }
+  for (int q = t; q < MBITS_WORDS; q += 256) mbits[q] = 0u;
+  if (t == 0) {
+    s_nm = 0;
+    s_abort = 0;
+  }
+  __syncthreads();
+
+  int k = k0, step = 0;
+  while (k < w && (last || k - k0 < nb - 1)) {
+    const int kr = k - k0, par = step & 1;
+    const uint32_t fA = ebase + 3u * (uint32_t)step + 1u, fB = fA + 1u;
+    int abort = s_abort;
+    // ---- W row k over the eliminated panel columns
+    if (t < kr) wk[t] = w_entry([&](int j) { return ADR(kr, j); }, t, dd, dsb);
+    __syncthreads();
+    // ---- column k updated: own far rows and (wave 3) the diagonal rows
+    double v0 = 0.0, v1 = 0.0;
+    if (hasrow) {
+      double v = LCR(kr);
+      for (int j = 0; j < kr; ++j) v -= LCR(j) * wk[j];
+      v0 = v;
+    }
+    if (wv == 3 && ln >= kr && ln < nbe) {
+      double v = ADR(ln, kr);
+      for (int j = 0; j < kr; ++j) v -= ADR(ln, j) * wk[j];
+      td0[ln] = v;
+    }
+    double av = -1.0;
+    int ai = 0x7fffffff;
+    if (hasrow) {
+      av = fabs(v0);
+      if (!(av == av)) av = -1.0;
+      ai = fr;
+    }
+    __syncthreads();
+    if (wv == 3 && ln > kr && ln < nbe) {
+      av = fabs(td0[ln]);
+      if (!(av == av)) av = -1.0;
+      ai = k0 + ln;
+    }
+    block_argmax(av, ai, redv, redi);
+    const double absakk = fabs(td0[kr]);
+    // ---- record A: the group's largest candidate; candidate row for a second column
+    {
+      uint64_t* rec = ll + LL_RECA + ((int64_t)par * 256 + g) * 4;
+      const bool cand = av >= 0.0;
+      const bool own_far = cand && hasrow && ai == fr;
+      const bool own_diag = cand && wv == 3 && ai == k0 + ln;
+      if (own_far || own_diag) {
+        ll_store_d(rec, own_far ? v0 : td0[ln], fA);
+        ll_store(rec + 2, (uint32_t)ai, fA);
+      } else if (!cand && t == 0) {
+        ll_store_d(rec, 0.0, fA);
+        ll_store(rec + 2, 0x7fffffffu, fA);
+      }
+      if (cand && ai >= fbeg && ai < fbeg + fcnt && absakk < BK_ALPHA * av && t < nbe) {
+        uint64_t* pay = ll + LL_PAY + ((int64_t)par * 256 + g) * 2 * WNB;
+        ll_store_d(pay + 2 * t, LC[t * WCH + (ai - fbeg)], fA);
+      }
+    }
+    if (wv == 0) {
+      double ba = -1.0, bv = 0.0;
+      int bi = 0x7fffffff, bg = 0;
+      for (int q = ln; q < G; q += 64) {
+        uint64_t* rec = ll + LL_RECA + ((int64_t)par * 256 + q) * 4;
+        const double v = ll_load_d(rec, fA, abort);
+        const int idx = (int)ll_load(rec + 2, fA, abort);
+        double x = fabs(v);
+        if (!(x == x) || idx == 0x7fffffff) x = -1.0;
+        if (x > ba || (x == ba && idx < bi)) {
+          ba = x;
+          bi = idx;
+          bv = v;
+          bg = q;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double oa = __shfl_xor(ba, off, 64), ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64), og = __shfl_xor(bg, off, 64);
+        if (oa > ba || (oa == ba && oi < bi)) {
+          ba = oa;
+          bi = oi;
+          bv = ov;
+          bg = og;
+        }
+      }
+      const bool any_abort = __ballot(abort) != 0;
+      if (ln == 0) {
+        bcd[0] = ba;
+        bcd[1] = bv;
+        bci[0] = bi;
+        bci[1] = bg;
+        if (any_abort) s_abort = 1;
+      }
+    }
+    __syncthreads();
+    double colmax = bcd[0];
+    int imax = bci[0];
+    const int gwin = bci[1];
+    if (k == w - 1 || colmax < 0.0) {
+      colmax = 0.0;
+      imax = k;
+    }
+    const int imr = imax - k0;
+    const bool im_diag = imr < nbe;
+    const double t0imax = im_diag ? td0[imr] : bcd[1];
+    int kp = k, kstep = 1, zero = 0;
+    bool use_t1 = false;
+    double t1imax = 0.0;
+    abort = s_abort;
+    if (fmax(absakk, colmax) == 0.0 || !(absakk == absakk)) {
+      zero = 1;
+    } else if (!(absakk >= BK_ALPHA * colmax)) {
+      // ---- second column: row imax of L / A, then column imax updated
+      if (im_diag) {
+        if (t < nbe) prow[t] = ADR(imr, t);
+      } else if (wv == 0) {
+        uint64_t* pay = ll + LL_PAY + ((int64_t)par * 256 + gwin) * 2 * WNB;
+        for (int j = ln; j < nbe; j += 64) prow[j] = ll_load_d(pay + 2 * j, fA, abort);
+      }
+      __syncthreads();
+      if (t < kr) wi[t] = w_entry([&](int j) { return prow[j]; }, t, dd, dsb);
+      const int pim = im_diag ? dphys[imr] : far_phys(imax, fstart, mbits, mlog, mphys, s_nm);
+      __syncthreads();
+      if (hasrow) {
+        const int pa = myphys;
+        double v = (pa >= pim) ? AG(pa, pim) : AG(pim, pa);
+        for (int j = 0; j < kr; ++j) v -= LCR(j) * wi[j];
+        v1 = v;
+      }
+      if (wv == 3 && ln >= kr && ln < nbe) {
+        double v = im_diag ? ADR(ln, imr) : prow[ln];
+        for (int j = 0; j < kr; ++j) v -= ADR(ln, j) * wi[j];
+        td1[ln] = v;
+      }
+      double bv2 = -1.0;
+      int bi2 = 0x7fffffff;
+      if (hasrow && fr != imax) {
+        bv2 = fabs(v1);
+        if (!(bv2 == bv2)) bv2 = -1.0;
+        bi2 = fr;
+      }
+      __syncthreads();
+      if (wv == 3 && ln >= kr && ln < nbe && k0 + ln != imax) {
+        bv2 = fabs(td1[ln]);
+        if (!(bv2 == bv2)) bv2 = -1.0;
+        bi2 = k0 + ln;
+      }
+      block_argmax(bv2, bi2, redv, redi);
+      {
+        uint64_t* rec = ll + LL_RECB + ((int64_t)par * 256 + g) * 8;
+        if (t == 0) ll_store_d(rec, bv2, fB);
+        if (hasrow && fr == imax) ll_store_d(rec + 4, v1, fB);
+      }
+      if (wv == 0) {
+        double rm = -1.0;
+        for (int q = ln; q < G; q += 64) {
+          const double x = ll_load_d(ll + LL_RECB + ((int64_t)par * 256 + q) * 8, fB, abort);
+          rm = fmax(rm, x);
+        }
+        for (int off = 32; off > 0; off >>= 1) rm = fmax(rm, __shfl_xor(rm, off, 64));
+        double ti = 0.0;
+        if (!im_diag && ln == 0) ti = ll_load_d(ll + LL_RECB + ((int64_t)par * 256 + gwin) * 8 + 4, fB, abort);
+        const bool any_abort = __ballot(abort) != 0;
+        if (ln == 0) {
+          bcd[2] = rm;
+          bcd[3] = ti;
+          if (any_abort) s_abort = 1;
+        }
+      }
+      __syncthreads();
+      double rowmax = bcd[2];
+      if (rowmax < 0.0) rowmax = 0.0;
+      t1imax = im_diag ? td1[imr] : bcd[3];
+      if (absakk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+        kp = k;
+      } else if (fabs(t1imax) >= BK_ALPHA * rowmax) {
+        kp = imax;
+        use_t1 = true;
+      } else {
+        kp = imax;
+        kstep = 2;
+      }
+    }
+    const int kk = k + kstep - 1, kkr = kk - k0, kpr = kp - k0;
+    const bool swap = kp != kk;
+    const bool kp_diag = kpr < nbe;
+    double D1 = 0, D2 = 0, OD = 0, d11 = 0, d22 = 0, d21 = 0, rk = 0;
+    if (kstep == 1) {
+      D1 = use_t1 ? t1imax : td0[kr];
+      rk = 1.0 / D1;
+    } else {
+      D1 = td0[kr];
+      OD = t0imax;
+      D2 = t1imax;
+      d21 = OD;
+      d11 = D2 / d21;
+      d22 = D1 / d21;
+      const double tt = 1.0 / (d11 * d22 - 1.0);
+      d21 = tt / d21;
+    }
+    // ---- interchange kk <-> kp: row histories and the logical-to-physical map
+    bool recv = false;  // this thread's far row becomes the old diagonal row kk
+    if (swap) {
+      if (kp_diag) {
+        if (hasrow) LCR(kpr) = LCR(kkr);
+      } else if (hasrow && fr == kp) {
+        recv = true;
+        myphys = dphys[kkr];
+        for (int j = 0; j < nbe; ++j)
+          if (j < kr || j > kkr) LCR(j) = ADR(kkr, j);
+      }
+      __syncthreads();
+      if (kp_diag) {
+        if (t < kr) {
+          const double x = ADR(kkr, t);
+          ADR(kkr, t) = ADR(kpr, t);
+          ADR(kpr, t) = x;
+        } else if (t > kk - k0 && t < nbe && t != kpr) {
+          const double x = ADR(kkr, t);
+          ADR(kpr, t) = x;
+          ADR(t, kpr) = x;
+        }
+        if (t == 0) {
+          ADR(kpr, kpr) = ADR(kkr, kkr);
+          const int x = dphys[kkr];
+          dphys[kkr] = dphys[kpr];
+          dphys[kpr] = x;
+        }
+      } else {
+        if (t < kr) ADR(kkr, t) = prow[t];
+        if (t == 0) {
+          const int old_kk = dphys[kkr];
+          const int b = kp - fstart;
+          int q = -1;
+          if (mbits[b >> 5] & (1u << (b & 31)))
+            for (int x = 0; x < s_nm; ++x)
+              if (mlog[x] == kp) q = x;
+          int old_kp = kp;
+          if (q >= 0) {
+            old_kp = mphys[q];
+            mphys[q] = old_kk;
+          } else {
+            mlog[s_nm] = kp;
+            mphys[s_nm] = old_kk;
+            ++s_nm;
+            mbits[b >> 5] |= 1u << (b & 31);
+          }
+          dphys[kkr] = old_kp;
+        }
+      }
+      __syncthreads();
+    }
+    // ---- L column(s) of this step
+    if (kstep == 1) {
+      if (hasrow) {
+        const double x = recv ? (use_t1 ? td1[kkr] : td0[kkr]) : (use_t1 ? v1 : v0);
+        LCR(kr) = zero ? x : x * rk;
+      }
+      if (t > kr && t < nbe) {
+        const int sr = (swap && kp_diag && t == kpr) ? kkr : t;
+        const double x = use_t1 ? td1[sr] : td0[sr];
+        ADR(t, kr) = zero ? x : x * rk;
+      }
+    } else {
+      if (hasrow) {
+        const double a0 = recv ? td0[kkr] : v0, a1 = recv ? td1[kkr] : v1;
+        LCR(kr) = d21 * (d11 * a0 - a1);
+        LCR(kr + 1) = d21 * (d22 * a1 - a0);
+      }
+      if (t > kr + 1 && t < nbe) {
+        const int sr = (swap && kp_diag && t == kpr) ? kkr : t;
+        ADR(t, kr) = d21 * (d11 * td0[sr] - td1[sr]);
+        ADR(t, kr + 1) = d21 * (d22 * td1[sr] - td0[sr]);
+      }
+    }
+    if (t == 0) {
+      ADR(kr, kr) = 1.0;
+      dd[kr] = D1;
+      dsb[kr] = kstep == 2 ? OD : 0.0;
+      if (kstep == 2) {
+        ADR(kr + 1, kr) = 0.0;
+        ADR(kr + 1, kr + 1) = 1.0;
+        dd[kr + 1] = D2;
+        dsb[kr + 1] = 0.0;
+      }
+      if (g == 0) {
+        dval[c0 + k] = D1;
+        dval[n + c0 + k] = kstep == 2 ? OD : 0.0;
+        if (kstep == 2) {
+          dval[c0 + k + 1] = D2;
+          dval[n + c0 + k + 1] = 0.0;
+        }
+        if (swap) {
+          const int x = perm_piv[c0 + kk];
+          perm_piv[c0 + kk] = perm_piv[c0 + kp];
+          perm_piv[c0 + kp] = x;
+        }
+        if (zero) atomicOr(status, 1);
+      }
+    }
+    __syncthreads();
+    k += kstep;
+    ++step;
+  }
+  const int kend = k, ker = kend - k0;
+  if (t == 0 && s_abort) atomicOr(status, 4);
+
+  // ---- moved indices of this panel: (logical, physical)
+  if (t == 0) {
+    int c = 0, ca = 0;
+    for (int i = 0; i < nbe; ++i)
+      if (dphys[i] != k0 + i && k0 + i >= kend) {
+        Ml[c] = k0 + i;
+        Mp[c++] = dphys[i];
+      }
+    for (int q = 0; q < s_nm; ++q)
+      if (mphys[q] != mlog[q]) {
+        Ml[c] = mlog[q];
+        Mp[c++] = mphys[q];
+      }
+    ca = c;  // the first ca entries lie in the trailing block
+    for (int i = 0; i < nbe; ++i)
+      if (dphys[i] != k0 + i && k0 + i < kend) {
+        Ml[c] = k0 + i;
+        Mp[c++] = dphys[i];
+      }
+    s_nM = c;
+    s_nMa = ca;
+  }
+  __syncthreads();
+  const int nM = s_nM, nMa = s_nMa;
+  const int GT = G * 256, gt = g * 256 + t;
+  const int nt = w - kend;                 // trailing block size
+  const int64_t no = r - w;                // rows below the diagonal block
+  double* S1 = scr;                                  // [nMa][nt]
+  double* S2 = scr + (int64_t)WMM * w;               // [nM][no]
+  double* S3 = S2 + (int64_t)WMM * (no > 0 ? no : 0);  // [nM][k0]
+  auto phys = [&](int j) -> int {
+    if (j < k0 + nbe) return dphys[j - k0];
+    return far_phys(j, fstart, mbits, mlog, mphys, s_nm);
+  };
+  int epoch = 0;
+  if (nM > 0) {
+    // gather the permuted values from the untouched panel-start storage
+    for (int64_t idx = gt; idx < (int64_t)nMa * nt; idx += GT) {
+      const int m = (int)(idx / nt), j = kend + (int)(idx % nt);
+      const int pa = Mp[m], pj = phys(j);
+      S1[idx] = pa >= pj ? AG(pa, pj) : AG(pj, pa);
+    }
+    for (int64_t idx = gt; idx < (int64_t)nM * no; idx += GT) {
+      const int m = (int)(idx / no);
+      S2[idx] = AG(w + idx % no, Mp[m]);
+    }
+    for (int64_t idx = gt; idx < (int64_t)nM * k0; idx += GT) {
+      const int m = (int)(idx / k0), j = (int)(idx % k0);
+      S3[idx] = AG(Mp[m], j);
+    }
+    group_barrier(sync, G, epoch, status);
+    for (int64_t idx = gt; idx < (int64_t)nMa * nt; idx += GT) {
+      const int m = (int)(idx / nt), j = kend + (int)(idx % nt);
+      const int a = Ml[m];
+      if (a >= j) AG(a, j) = S1[idx];
+      else AG(j, a) = S1[idx];
+    }
+    for (int64_t idx = gt; idx < (int64_t)nM * no; idx += GT) {
+      const int m = (int)(idx / no);
+      AG(w + idx % no, Ml[m]) = S2[idx];
+    }
+    for (int64_t idx = gt; idx < (int64_t)nM * k0; idx += GT) {
+      const int m = (int)(idx / k0), j = (int)(idx % k0);
+      AG(Ml[m], j) = S3[idx];
+    }
+  }
+  // ---- L of the panel and W = L D for the rows of the trailing block
+  if (hasrow) {
+    for (int j = 0; j < ker; ++j) {
+      AG(fr, k0 + j) = LCR(j);
+      Wp[(int64_t)j * w + fr] = w_entry([&](int c) { return LCR(c); }, j, dd, dsb);
+    }
+  }
+  for (int idx = gt; idx < nbe * ker; idx += GT) {
+    const int i = idx % nbe, j = idx / nbe;
+    if (j <= i) AG(k0 + i, k0 + j) = ADR(i, j);
+    if (k0 + i >= kend) Wp[(int64_t)j * w + k0 + i] = w_entry([&](int c) { return ADR(i, c); }, j, dd, dsb);
+  }
+  if (g == 0 && t == 0) {
+    kstart[s] = k0;
+    kpos[s] = kend;
+  }
+#undef AG
+#undef LCR
+#undef ADR
+}
+
+void launch_bk_wide(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval, int* perm_piv,
+                    double* W, const int64_t* woff, int* kpos, int* kstart, int* sync, uint64_t* ll,
+                    uint32_t ebase, int* status, hipStream_t st) {
+  if (nb > WNB || nb < 4 || G < 1 || G > 256) throw std::runtime_error("bk_wide: launch shape out of range");
+  int n = a.n;
+  void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
+                  (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
+                  (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&ll, (void*)&ebase, (void*)&n,
+                  (void*)&status};
+  hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_wide, dim3(G), dim3(256), args, 0, st);
+  if (e != hipSuccess) throw std::runtime_error(std::string("bk_wide: ") + hipGetErrorString(e));
+}
+
+}  // namespace lblg

@@ -89,6 +89,8 @@ struct DeviceFactor {
   int64_t nnzf = 0;             // entries of the full CSR
   double* k0norm = nullptr;     // ||K0||_inf on the device
   int* sync = nullptr;          // group-barrier counters
+  uint64_t* ll = nullptr;       // flag-in-word exchange buffer of the wide panel kernel
+  uint32_t ll_epoch = 0;        // next epoch base (monotone for the handle's lifetime)
   double* red = nullptr;        // reduction scratch
   int* status = nullptr;        // device flags (zero pivots etc.)
   // solve

@@ -216,21 +216,14 @@ static void setup(Handle& h) {
           F.medium.push_back(s);
           wsum += (int64_t)w * PANEL_NB + 2 * (int64_t)w + 16;
         } else {
-          // cooperative LDS panel: each workgroup caches <= LDS_ROWS rows of nb columns
-          int nb = 64, G = (w + 239) / 240;
-          if (G > 256) {
-            nb = 32;
-            G = (w + 479) / 480;
-          }
-          const int gmin = std::min(env_int("LBLG_MIN_GROUPS", 8), std::max(1, w / 64));
-          G = std::max(G, gmin);
-          if ((w + G - 1) / G > 512 || (int64_t)((w + G - 1) / G) * nb > 240 * 64)
-            throw std::runtime_error("cooperative panel: rows per workgroup exceed the LDS panel");
+          // cooperative panel over G workgroups, <= WIDE_ROWS far rows each
+          const int nb = WIDE_NB;
+          const int G = std::max(1, (w - nb + WIDE_ROWS - 1) / WIDE_ROWS);
           if (G > 256) throw std::runtime_error("supernode too wide for the cooperative panel kernel");
+          wsum += (int64_t)w * nb + (int64_t)WIDE_MOVED * (w + r) + 64;
           F.large.push_back(s);
           F.large_G.push_back(G);
           F.large_nb.push_back(nb);
-          wsum += (int64_t)w * nb + 4 * (int64_t)w + 8 * G + 256;
         }
       }
       if (r > w) {
@@ -263,6 +256,9 @@ static void setup(Handle& h) {
   D.kpos = dalloc<int>(D, ns);
   D.kstart = dalloc<int>(D, ns);
   D.sync = dalloc<int>(D, std::max<int64_t>(ns, 1));
+  D.ll = reinterpret_cast<uint64_t*>(dalloc<double>(D, LL_WORDS));
+  launch_fill(reinterpret_cast<double*>(D.ll), LL_WORDS, 0.0, D.stream);
+  D.ll_epoch = 0;
 
   // ---- solve pieces and their levels
   std::vector<int> psn, pc0, first_piece(ns), piece_par;
@@ -369,9 +365,14 @@ void factor_device(Handle& h) {
       const int rounds = (w + nb - 2) / (nb - 1) + 1;
       for (int p = 0; p < rounds; ++p) {
         launch_fill_int(D.sync, 1, 0, st);
+        // far rows of this panel are at most w - p (nb - 1) - nb (each panel eliminates >= nb - 1 columns)
+        const int far = w - p * (nb - 1) - nb;
+        const int Gp = std::min(G, std::max(1, (far + WIDE_ROWS - 1) / WIDE_ROWS));
+        const uint32_t eb = D.ll_epoch;
+        D.ll_epoch += 3u * (uint32_t)nb + 8u;
         timed(D, "bk_panel_group", [&] {
-          launch_bk_panel_lds(F.d_large + q, G, nb, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
-                              D.kstart, D.sync, D.status, st);
+          launch_bk_wide(F.d_large + q, Gp, nb, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos, D.kstart,
+                         D.sync, D.ll, eb, D.status, st);
         });
         const int lo = std::min(w, (p + 1) * (nb - 1));
         const int T = (w - lo + 63) / 64;
@@ -398,7 +399,8 @@ void factor_device(Handle& h) {
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   h.zero_pivot = stat[0] & 1;
-  if (stat[0] & 4) throw std::runtime_error("group barrier timed out");
+  if (stat[0] & 4) throw std::runtime_error("cross-workgroup exchange timed out");
+  if (stat[0] & 8) throw std::runtime_error("wide panel launch shape does not fit the supernode");
   h.factored = true;
 }
 

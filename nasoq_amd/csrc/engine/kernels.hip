@@ -21,19 +21,9 @@
 #include <string>
 
 #include "engine/kernels.hpp"
+#include "engine/device_util.hpp"
 
 namespace lblg {
-
-#define HIP_LAUNCH_CHECK(what)                                                                     \
-  do {                                                                                             \
-    hipError_t e_ = hipGetLastError();                                                             \
-    if (e_ != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-#define BK_ALPHA 0.6403882032022076  // (1 + sqrt(17)) / 8
-#define SFMIN 2.2250738585072014e-308
 
 static int grid_for(int64_t count, int block) {
   int64_t g = (count + block - 1) / block;
@@ -100,42 +90,6 @@ void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char*
   if (nnz <= 0) return;
   k_assemble<<<grid_for(nnz, 256), 256, 0, st>>>(amap, Kx, pert, nnz, reg, valL);
   HIP_LAUNCH_CHECK("assemble");
-}
-
-// --------------------------------------------------------------------------
-// reductions with LAPACK idamax tie-breaking (first index of the maximum)
-// --------------------------------------------------------------------------
-__device__ inline void better(double& v, int& i, double ov, int oi) {
-  if (ov > v || (ov == v && oi < i)) {
-    v = ov;
-    i = oi;
-  }
-}
-__device__ inline void wave_argmax(double& v, int& i) {
-  for (int off = 32; off > 0; off >>= 1) {
-    double ov = __shfl_xor(v, off, 64);
-    int oi = __shfl_xor(i, off, 64);
-    better(v, i, ov, oi);
-  }
-}
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-// workgroup argmax over 256 threads; result valid in all threads
-__device__ inline void block_argmax(double& v, int& i, double* sv, int* si) {
-  wave_argmax(v, i);
-  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-  __syncthreads();
-  if (ln == 0) {
-    sv[wv] = v;
-    si[wv] = i;
-  }
-  __syncthreads();
-  v = sv[0];
-  i = si[0];
-  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) better(v, i, sv[q], si[q]);
-  __syncthreads();
 }
 
 // --------------------------------------------------------------------------
@@ -327,33 +281,6 @@ void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, 
   if (count <= 0) return;
   k_bk_small<<<count, 256, 0, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n, status);
   HIP_LAUNCH_CHECK("bk_small");
-}
-
-// --------------------------------------------------------------------------
-// group barrier among the G workgroups that share one supernode.  The wait
-// is bounded: after the spin limit the barrier flags status bit 2 and
-// proceeds, so no wave can wait forever.
-// --------------------------------------------------------------------------
-__device__ inline void group_barrier(int* ctr, int G, int& epoch, int* status) {
-  __syncthreads();
-  ++epoch;
-  if (G > 1) {
-    if (threadIdx.x == 0) {
-      __threadfence();  // release: the workgroup's stores (all in L2 after the barrier) become device-visible
-      atomicAdd(ctr, 1);
-      const int target = epoch * G;
-      long spins = 0;
-      while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1L << 26)) {
-          atomicOr(status, 4);
-          break;
-        }
-      }
-      __threadfence();  // acquire: drop stale cache lines before the group reads shared data
-    }
-    __syncthreads();
-  }
 }
 
 // --------------------------------------------------------------------------
@@ -1532,283 +1459,3 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
 }
 }  // namespace lblg
 
-namespace lblg {
-// --------------------------------------------------------------------------
-// Cooperative panel Bunch-Kaufman for very wide diagonal blocks (the root
-// separators of KKT matrices).  G workgroups each own a contiguous chunk of
-// the rows [k0, w) of the panel and keep their rows of the panel's L in LDS,
-// so a column step reads only LDS, one column of A and two broadcast rows of
-// W.  Cross-workgroup traffic per step: the partial maxima, the updated
-// candidate columns t0/t1 (global, double-buffered by step parity) and, for
-// interchanges, the L rows of the two swapped rows (staged in swap slots
-// before the decision is known).  Steps without an interchange need one
-// group barrier, steps that inspect column imax two, interchanges three.
-// The pivot rules are those of k_bk_panel (dsytf2 / dlasyf).
-//
-// workspace at Wbase + woff[s]:
-//   Wp (w x nb) | tb (4w: t0[2][w], t1[2][w]) | pv (4G) | pix (4G ints) | slots (3 x 64)
-// --------------------------------------------------------------------------
-#define LDS_PANEL_DOUBLES (240 * 64)
-__global__ __launch_bounds__(256) void k_bk_panel_lds(const int* list, int G, int nb, const int* super,
-                                                      const int64_t* pi, const int64_t* px, double* valL,
-                                                      double* dval, int* perm_piv, double* Wbase,
-                                                      const int64_t* woff, int* kpos, int* kstart, int* sync,
-                                                      int n, int* status) {
-  __shared__ double Lc[LDS_PANEL_DOUBLES];
-  __shared__ double wk[64], wi[64];
-  __shared__ double sv[8];
-  __shared__ int si[8];
-  const int g = blockIdx.x;
-  const int s = list[0];
-  const int c0 = super[s], w = super[s + 1] - c0;
-  const int64_t r = pi[s + 1] - pi[s];
-  double* A = valL + px[s];
-  double* Wp = Wbase + woff[s];
-  double* tb = Wp + (int64_t)w * nb;
-  double* pv = tb + 4 * (int64_t)w;
-  int* pix = reinterpret_cast<int*>(pv + 4 * G);
-  double* slots = pv + 4 * G + 2 * G + 2;
-  int* ctr = sync;
-  const int t = threadIdx.x, GT = G * 256, gt = g * 256 + t;
-  const int k0 = kpos[s];
-  if (k0 >= w) return;
-  const bool last = (w - k0) <= nb;
-  const int chunk = (w - k0 + G - 1) / G;  // <= 512 rows: at most two per thread
-  const int rbeg = k0 + g * chunk, rend = min(w, rbeg + chunk);
-  int epoch = 0;
-#define AA(i, j) A[(int64_t)(j) * r + (i)]
-#define WW(i, j) Wp[(int64_t)((j) - k0) * w + (i)]
-#define LC(i, j) Lc[((i) - rbeg) * nb + ((j) - k0)]
-#define T0(p, i) tb[(int64_t)(p) * w + (i)]
-#define T1(p, i) tb[(int64_t)(2 + (p)) * w + (i)]
-  // previous step bookkeeping for the broadcast rows of W
-  int pk = -1, pkind = 0, ppar = 0;  // pkind 1: 1x1 no swap, 2: 2x2 no swap, 0: synced
-  int k = k0, step = 0;
-  while (k < w && (last || k - k0 < nb - 1)) {
-    const int par = step & 1;
-    // prefetch the own rows of column k (at most two per thread) while the
-    // broadcast row of W is gathered
-    const int ia = max(rbeg, k) + t, ib = ia + 256;
-    double pa = 0.0, pb = 0.0;
-    if (ia < rend) pa = AA(ia, k);
-    if (ib < rend) pb = AA(ib, k);
-    // ---- broadcast row k of W for the panel columns [k0, k)
-    for (int j = k0 + t; j < k; j += 256) {
-      double v;
-      if (pkind == 1 && j == pk) v = T0(ppar, k);
-      else if (pkind == 2 && j == pk) v = T0(ppar, k);
-      else if (pkind == 2 && j == pk + 1) v = T1(ppar, k);
-      else v = WW(k, j);
-      wk[j - k0] = v;
-    }
-    __syncthreads();
-    // ---- phase A: column k for own rows
-    double bv = -1.0;
-    int bi = 0x7fffffff;
-    for (int i = ia, q = 0; i < rend && q < 2; i += 256, ++q) {
-      double v = q == 0 ? pa : pb;
-      for (int j = k0; j < k; ++j) v -= LC(i, j) * wk[j - k0];
-      T0(par, i) = v;
-      if (i > k) {
-        double av = fabs(v);
-        if (!(av == av)) av = -1.0;
-        better(bv, bi, av, i);
-      }
-    }
-    block_argmax(bv, bi, sv, si);
-    if (t == 0) {
-      pv[(par * 2 + 0) * G + g] = bv;
-      pix[(par * 2 + 0) * G + g] = bi;
-    }
-    group_barrier(ctr, G, epoch, status);
-    double colmax = -1.0;
-    int imax = 0x7fffffff;
-    for (int q = 0; q < G; ++q) better(colmax, imax, pv[(par * 2) * G + q], pix[(par * 2) * G + q]);
-    if (k == w - 1 || colmax < 0) {
-      colmax = 0.0;
-      imax = k;
-    }
-    const double absakk = fabs(T0(par, k));
-    int kp = k, kstep = 1, zero = 0;
-    bool use_t1 = false;
-    if (fmax(absakk, colmax) == 0.0 || !(absakk == absakk)) {
-      zero = 1;
-    } else if (!(absakk >= BK_ALPHA * colmax)) {
-      // ---- phase B: column imax for own rows; stage candidate swap rows
-      double qa = 0.0, qb = 0.0;
-      if (ia < rend) qa = (ia < imax) ? AA(imax, ia) : AA(ia, imax);
-      if (ib < rend) qb = (ib < imax) ? AA(imax, ib) : AA(ib, imax);
-      for (int j = k0 + t; j < k; j += 256) {
-        double v;
-        if (pkind == 1 && j == pk) v = T0(ppar, imax);
-        else if (pkind == 2 && j == pk) v = T0(ppar, imax);
-        else if (pkind == 2 && j == pk + 1) v = T1(ppar, imax);
-        else v = WW(imax, j);
-        wi[j - k0] = v;
-      }
-      __syncthreads();
-      double rv = -1.0;
-      int ri = 0x7fffffff;
-      for (int i = ia, q = 0; i < rend && q < 2; i += 256, ++q) {
-        double v = q == 0 ? qa : qb;
-        for (int j = k0; j < k; ++j) v -= LC(i, j) * wi[j - k0];
-        T1(par, i) = v;
-        if (i != imax) {
-          double av = fabs(v);
-          if (!(av == av)) av = -1.0;
-          better(rv, ri, av, i);
-        }
-      }
-      // candidate rows for an interchange: imax, k and k+1
-      for (int j = k0 + t; j < k; j += 256) {
-        if (imax >= rbeg && imax < rend) slots[j - k0] = LC(imax, j);
-        if (k >= rbeg && k < rend) slots[64 + j - k0] = LC(k, j);
-        if (k + 1 < w && k + 1 >= rbeg && k + 1 < rend) slots[128 + j - k0] = LC(k + 1, j);
-      }
-      block_argmax(rv, ri, sv, si);
-      if (t == 0) {
-        pv[(par * 2 + 1) * G + g] = rv;
-        pix[(par * 2 + 1) * G + g] = ri;
-      }
-      group_barrier(ctr, G, epoch, status);
-      double rowmax = -1.0;
-      int rj = 0x7fffffff;
-      for (int q = 0; q < G; ++q) better(rowmax, rj, pv[(par * 2 + 1) * G + q], pix[(par * 2 + 1) * G + q]);
-      if (rowmax < 0) rowmax = 0.0;
-      if (absakk >= BK_ALPHA * colmax * (colmax / rowmax)) {
-        kp = k;
-      } else if (fabs(T1(par, imax)) >= BK_ALPHA * rowmax) {
-        kp = imax;
-        use_t1 = true;
-      } else {
-        kp = imax;
-        kstep = 2;
-      }
-    }
-    const int kk = k + kstep - 1;
-    const bool swap = kp != kk;
-    // ---- phase C
-    if (zero && gt == 0) atomicOr(status, 1);
-    if (swap) {
-      // LDS rows of the current panel (from the staged slots)
-      for (int j = k0 + t; j < k; j += 256) {
-        if (kk >= rbeg && kk < rend) LC(kk, j) = slots[j - k0];
-        if (kp >= rbeg && kp < rend) LC(kp, j) = slots[(kk == k ? 64 : 128) + j - k0];
-      }
-      // global rows kk/kp of all earlier L columns and of the panel's W
-      for (int j = gt; j < k; j += GT) {
-        const double x = AA(kk, j);
-        AA(kk, j) = AA(kp, j);
-        AA(kp, j) = x;
-      }
-      for (int j = k0 + gt; j < k; j += GT) {
-        const double x = WW(kk, j);
-        WW(kk, j) = WW(kp, j);
-        WW(kp, j) = x;
-      }
-      for (int64_t q = w + gt; q < r; q += GT) {
-        const double x = AA(q, kk);
-        AA(q, kk) = AA(q, kp);
-        AA(q, kp) = x;
-      }
-      if (gt == 0) {
-        const int tp = perm_piv[c0 + kk];
-        perm_piv[c0 + kk] = perm_piv[c0 + kp];
-        perm_piv[c0 + kp] = tp;
-      }
-    }
-    double D1 = 0, D2 = 0, OD = 0, d21 = 0, d11 = 0, d22 = 0, rk = 0;
-    if (kstep == 1) {
-      const int sk = swap ? kp : k;
-      D1 = use_t1 ? T1(par, sk) : T0(par, sk);
-      rk = 1.0 / D1;
-    } else {
-      D1 = T0(par, k);
-      OD = T0(par, kp);
-      D2 = T1(par, kp);
-      d21 = OD;
-      d11 = D2 / d21;
-      d22 = D1 / d21;
-      const double tt = 1.0 / (d11 * d22 - 1.0);
-      d21 = tt / d21;
-    }
-    for (int i = max(rbeg, k) + t; i < rend; i += 256) {
-      const int si2 = !swap ? i : (i == kk ? kp : (i == kp ? kk : i));
-      if (swap) {
-        if (i == kk) AA(kp, kp) = AA(kk, kk);
-        else if (i > kk && i < kp) AA(kp, i) = AA(i, kk);
-        else if (i > kp) AA(i, kp) = AA(i, kk);
-      }
-      if (kstep == 1) {
-        const double wv = use_t1 ? T1(par, si2) : T0(par, si2);
-        WW(i, k) = wv;
-        if (i == k) {
-          AA(k, k) = 1.0;
-          LC(k, k) = 1.0;
-          dval[c0 + k] = wv;
-          dval[n + c0 + k] = 0.0;
-        } else {
-          const double l = zero ? wv : wv * rk;
-          AA(i, k) = l;
-          LC(i, k) = l;
-        }
-      } else {
-        const double w0 = T0(par, si2), w1 = T1(par, si2);
-        WW(i, k) = w0;
-        WW(i, k + 1) = w1;
-        if (i == k) {
-          AA(k, k) = 1.0;
-          LC(k, k) = 1.0;
-          LC(k, k + 1) = 0.0;
-          dval[c0 + k] = D1;
-          dval[n + c0 + k] = OD;
-        } else if (i == k + 1) {
-          AA(k + 1, k) = 0.0;
-          AA(k + 1, k + 1) = 1.0;
-          LC(k + 1, k) = 0.0;
-          LC(k + 1, k + 1) = 1.0;
-          dval[c0 + k + 1] = D2;
-          dval[n + c0 + k + 1] = 0.0;
-        } else {
-          const double l0 = d21 * (d11 * w0 - w1), l1 = d21 * (d22 * w1 - w0);
-          AA(i, k) = l0;
-          AA(i, k + 1) = l1;
-          LC(i, k) = l0;
-          LC(i, k + 1) = l1;
-        }
-      }
-    }
-    if (swap) {
-      group_barrier(ctr, G, epoch, status);
-      pkind = 0;
-    } else {
-      __syncthreads();
-      pkind = kstep;
-      pk = k;
-      ppar = par;
-    }
-    k += kstep;
-    ++step;
-  }
-  if (gt == 0) {
-    kstart[s] = k0;
-    kpos[s] = k;
-  }
-#undef AA
-#undef WW
-#undef LC
-#undef T0
-#undef T1
-}
-void launch_bk_panel_lds(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval,
-                         int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
-                         int* status, hipStream_t st) {
-  if (nb > 64 || nb < 4) throw std::runtime_error("bk_panel_lds: panel width out of range");
-  int n = a.n;
-  void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
-                  (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
-                  (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&n, (void*)&status};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel_lds, dim3(G), dim3(256), args, 0, st);
-  if (e != hipSuccess) throw std::runtime_error(std::string("bk_panel_lds: ") + hipGetErrorString(e));
-}
-}  // namespace lblg

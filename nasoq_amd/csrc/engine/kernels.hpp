@@ -32,9 +32,14 @@ void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, 
 void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double* valL, double* dval,
                      int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                      int* status, bool cooperative, hipStream_t st);
-void launch_bk_panel_lds(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval,
-                         int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
-                         int* status, hipStream_t st);
+// wide panels with flag-in-word exchange (bk_wide.hip)
+constexpr int WIDE_NB = 64;         // panel width
+constexpr int WIDE_ROWS = 192;      // far rows per workgroup
+constexpr int WIDE_MOVED = 128;     // moved indices per panel (scratch rows)
+constexpr int64_t LL_WORDS = 2 * 256 * 4 + 2 * 256 * 8 + 2 * 256 * 2 * 64;
+void launch_bk_wide(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval, int* perm_piv,
+                    double* W, const int64_t* woff, int* kpos, int* kstart, int* sync, uint64_t* ll,
+                    uint32_t ebase, int* status, hipStream_t st);
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
                      const double* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st);

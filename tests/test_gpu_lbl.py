@@ -110,7 +110,6 @@ def test_wide_supernodes(oracle, group_w, grid, m, monkeypatch):
     # columns (panel path); small group_w sends them to the cooperative
     # multi-workgroup LDS panel kernel
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
-    monkeypatch.setenv("LBLG_MIN_GROUPS", "8")
     n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 6, seed=5, nrhs=3)
     print(f"case wide n={n} group_w={group_w} grid={grid}", flush=True)
     reg = 1e-9
@@ -130,7 +129,6 @@ def test_two_by_two_pivots(oracle, nb, bs, order, group_w, monkeypatch):
     # path: w <= 64 (LDS kernel), 64 < w <= group_w (single-workgroup panel),
     # w > group_w (cooperative multi-workgroup panel)
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
-    monkeypatch.setenv("LBLG_MIN_GROUPS", "8")
     n, Ap, Ai, Ax, B = P.block_indefinite(nb, bs, seed=1, nrhs=2)
     perm = np.arange(n, dtype=np.int32) if order == "nat" else None
     print(f"case 2x2 n={n} nb={nb} bs={bs} order={order} group_w={group_w}", flush=True)
