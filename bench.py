@@ -168,7 +168,7 @@ def main():
     ms_step = dt / args.steps * 1e3
 
     kt = {}
-    for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "schur", "fwd_tri", "fwd_gemv",
+    for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "scale_ld", "schur", "fwd_tri", "fwd_gemv",
                  "bwd_gemv", "bwd_tri", "spmv"]:
         tot, cnt = ctypes.c_double(), ctypes.c_int64()
         lib.lblg_kernel_time(s.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt))

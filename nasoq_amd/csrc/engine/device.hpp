@@ -36,6 +36,12 @@ struct FactorLevel {
   SnList schur;                    // supernodes with an update (m > 0)
   int64_t schur_tiles = 0;
   int64_t* d_schur_tile = nullptr; // prefix of 64x64 lower tiles per schur supernode
+  SnList schur_big;                // Schur updates with >= 256 off-diagonal rows: 128x128 tiles
+  int64_t schur_big_tiles = 0;
+  int64_t* d_schur_big_tile = nullptr;
+  int* d_schur_big_cols = nullptr;    // prefix of columns (for the L D scaling launch)
+  int schur_big_ncols = 0;
+  int64_t* d_schur_big_ldoff = nullptr;  // offset of each supernode's L D block in DeviceFactor::LD
   int max_w = 0;
   std::vector<int> medium_dummy;
   int* d_medium = nullptr;
@@ -81,6 +87,7 @@ struct DeviceFactor {
   int64_t* Ksrc = nullptr;   // CSR entry -> lower CSC entry (to refresh values)
   // workspace
   double* W = nullptr;          // BK panel workspace (max over concurrent supernodes)
+  double* LD = nullptr;         // L_off D of the large Schur updates of one level
   int64_t W_elems = 0;
   int64_t* woff = nullptr;      // per-supernode offset into W
   int* kpos = nullptr;          // next column to factor (wide supernodes)

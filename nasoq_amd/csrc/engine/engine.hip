@@ -195,15 +195,17 @@ static void setup(Handle& h) {
   std::vector<std::vector<int>> bylev(nlev);
   for (int s = 0; s < ns; ++s) bylev[lev[s]].push_back(s);
   const int group_w = env_int("LBLG_GROUP_W", 2048);
+  int64_t ld_max = 0;
   int64_t wmax_total = 0;
   std::vector<int64_t> woff(ns, 0);
   D.flevels.resize(nlev);
   for (int l = 0; l < nlev; ++l) {
     FactorLevel& F = D.flevels[l];
-    std::vector<int> small, off, sch, offblk, big;
-    std::vector<int64_t> tiles;
-    int blk = 0;
-    int64_t tl = 0, wsum = 0;
+    std::vector<int> small, off, sch, schb, offblk, big;
+    std::vector<int64_t> tiles, tilesb, ldoffb;
+    std::vector<int> colsb;
+    int blk = 0, ncolb = 0;
+    int64_t tl = 0, tlb = 0, wsum = 0, ldsum = 0;
     for (int s : bylev[l]) {
       const int w = S.super[s + 1] - S.super[s];
       const int64_t r = S.pi[s + 1] - S.pi[s];
@@ -230,10 +232,21 @@ static void setup(Handle& h) {
         off.push_back(s);
         offblk.push_back(blk);
         blk += (int)((r - w + 63) / 64);
-        sch.push_back(s);
-        tiles.push_back(tl);
-        const int64_t T = (r - w + 63) / 64;
-        tl += T * (T + 1) / 2;
+        if (r - w >= 256) {
+          schb.push_back(s);
+          tilesb.push_back(tlb);
+          colsb.push_back(ncolb);
+          ldoffb.push_back(ldsum);
+          ncolb += w;
+          ldsum += (r - w) * (int64_t)w;
+          const int64_t T = (r - w + 127) / 128;
+          tlb += T * (T + 1) / 2;
+        } else {
+          sch.push_back(s);
+          tiles.push_back(tl);
+          const int64_t T = (r - w + 63) / 64;
+          tl += T * (T + 1) / 2;
+        }
       }
     }
     wmax_total = std::max(wmax_total, wsum);
@@ -247,10 +260,19 @@ static void setup(Handle& h) {
     F.schur.d = dupload(D, sch);
     F.schur_tiles = tl;
     F.d_schur_tile = dupload(D, tiles);
+    F.schur_big.count = (int)schb.size();
+    F.schur_big.d = dupload(D, schb);
+    F.schur_big_tiles = tlb;
+    F.d_schur_big_tile = dupload(D, tilesb);
+    F.d_schur_big_cols = dupload(D, colsb);
+    F.schur_big_ncols = ncolb;
+    F.d_schur_big_ldoff = dupload(D, ldoffb);
+    ld_max = std::max(ld_max, ldsum);
     F.d_medium = dupload(D, F.medium);
     F.d_large = dupload(D, F.large);
   }
   D.W = dalloc<double>(D, wmax_total);
+  D.LD = dalloc<double>(D, std::max<int64_t>(ld_max, 1));
   D.W_elems = wmax_total;
   D.woff = dupload(D, woff);
   D.kpos = dalloc<int>(D, ns);
@@ -375,11 +397,10 @@ void factor_device(Handle& h) {
                          D.sync, D.ll, eb, D.status, st);
         });
         const int lo = std::min(w, (p + 1) * (nb - 1));
-        const int T = (w - lo + 63) / 64;
+        const int T = (w - lo + 127) / 128;
         if (T > 0)
           timed(D, "trailing", [&] {
-            launch_trailing(F.d_large + q, 1, T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff, D.kpos, D.kstart,
-                            st);
+            launch_trailing_big(F.d_large + q, T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
           });
       }
     }
@@ -390,6 +411,16 @@ void factor_device(Handle& h) {
     timed(D, "schur", [&] {
       launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, D.valL, D.dval, st);
     });
+    if (F.schur_big.count > 0) {
+      timed(D, "scale_ld", [&] {
+        launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, D.valL, D.dval,
+                        D.LD, F.d_schur_big_ldoff, st);
+      });
+      timed(D, "schur", [&] {
+        launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, D.valL, D.LD,
+                         F.d_schur_big_ldoff, st);
+      });
+    }
   }
   launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
   // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV

@@ -1,0 +1,301 @@
+// 128 x 128 FP64 MFMA tiles for the large dense updates of the factorization:
+//
+//   k_schur_big     U = L_off D L_off^T of a supernode, scatter-added into
+//                   the ancestors' panels (the DGEMM + scatter of
+//                   ldl_left_sn_parallel_02, src/ldl/parallel_blocked_ldlt_02.cpp:128-153,
+//                   in push form)
+//   k_trailing_big  A22 -= L21 W21^T after a wide Bunch-Kaufman panel
+//                   (dlasyf's trailing update inside LAPACKE_dsytrf, :155)
+//
+// A workgroup (4 waves) owns a 128 x 128 tile; each wave a 64 x 64 quadrant
+// = 4 x 4 blocks of v_mfma_f64_16x16x4f64.  Operands are staged through LDS
+// in chunks of 16 columns of k, k-major with a row pitch of 144 doubles
+// (rows of one k land in distinct bank halves for the four k of a step), and
+// the next chunk is prefetched into registers while the current one is
+// multiplied.  As in the 64 x 64 engine, the second operand is passed first
+// so the accumulator holds the transposed block and lanes 0..15 cover 16
+// consecutive rows in the epilogue.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "engine/device_util.hpp"
+#include "engine/kernels.hpp"
+
+namespace lblg {
+
+#define BT 128
+#define BKC 16
+#define BLD 144
+#define BRUNS 8  // target supernodes per tile with cached row positions
+
+struct BigAcc {
+  d4 c[4][4];
+};
+
+__device__ inline void big_mma(BigAcc& acc, const double* As, const double* Bs) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
+  const int li = ln & 15, lk = ln >> 4;
+#pragma unroll
+  for (int k = 0; k < BKC; k += 4) {
+    double a[4], b[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      a[x] = As[(k + lk) * BLD + r0 + 16 * x + li];
+      b[x] = Bs[(k + lk) * BLD + q0 + 16 * x + li];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+        acc.c[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[y], a[x], acc.c[x][y], 0, 0, 0);
+  }
+}
+
+// acc = sum_k A(x, k) B(y, k) over k < K; la(k, x) / lb(k, x) return the
+// operands (zero outside the matrix).  Thread t stages row t % 128 of k
+// entries (t / 128) * 8 .. +8 of each chunk.
+template <typename LA, typename LB>
+__device__ inline void big_gemm(BigAcc& acc, int K, double* As, double* Bs, LA la, LB lb) {
+  const int t = threadIdx.x, x = t & 127, kq = (t >> 7) * 8;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  double ra[8], rb[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    ra[q] = la(kq + q, x);
+    rb[q] = lb(kq + q, x);
+  }
+  for (int kc0 = 0; kc0 < K; kc0 += BKC) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      As[(kq + q) * BLD + x] = ra[q];
+      Bs[(kq + q) * BLD + x] = rb[q];
+    }
+    __syncthreads();
+    if (kc0 + BKC < K) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        ra[q] = la(kc0 + BKC + kq + q, x);
+        rb[q] = lb(kc0 + BKC + kq + q, x);
+      }
+    }
+    big_mma(acc, As, Bs);
+  }
+}
+
+template <typename F>
+__device__ inline void big_foreach(const BigAcc& acc, F f) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) f(r0 + 16 * a + (ln & 15), q0 + 16 * b + 4 * e + (ln >> 4), acc.c[a][b][e]);
+}
+
+__device__ inline void tile_of(int64_t tix, int& ti, int& tj) {
+  ti = (int)((sqrt(8.0 * (double)tix + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(ti + 1) * (ti + 2) / 2 <= tix) ++ti;
+  while ((int64_t)ti * (ti + 1) / 2 > tix) --ti;
+  tj = (int)(tix - (int64_t)ti * (ti + 1) / 2);
+}
+
+__device__ inline int64_t big_row_pos(const int* rows, int64_t b, int64_t e, int key) {
+  int64_t lo = b, hi = e - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rows[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo - b;
+}
+
+__global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_t* tile_prefix, int count,
+                                                   const int* super, const int64_t* pi, const int64_t* px,
+                                                   const int* rows, const int* col2sup, double* valL,
+                                                   const double* LD, const int64_t* ldoff) {
+  __shared__ double As[BKC * BLD], Bs[BKC * BLD];
+  __shared__ int rlab[BT], clab[BT], ctg[BT], dtcol[BT];
+  __shared__ int tpos[BRUNS][BT];
+  __shared__ int64_t cbase[BT];
+  __shared__ int dtl[BRUNS], ntg;
+  int lo = 0, hi = count - 1;
+  const int64_t bid = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_prefix[mid] <= bid) lo = mid; else hi = mid - 1;
+  }
+  const int s = list[lo];
+  int ti, tj;
+  tile_of(bid - tile_prefix[lo], ti, tj);
+  const int w = super[s + 1] - super[s];
+  const int64_t r = pi[s + 1] - pi[s];
+  const int m = (int)(r - w);
+  const double* P = valL + px[s];
+  const int i0 = BT * ti, j0 = BT * tj;  // within the m off-diagonal rows
+  const int t = threadIdx.x;
+  if (t < BT) {
+    rlab[t] = (i0 + t < m) ? rows[pi[s] + w + i0 + t] : -1;
+    clab[t] = (j0 + t < m) ? rows[pi[s] + w + j0 + t] : -1;
+    ctg[t] = clab[t] >= 0 ? col2sup[clab[t]] : -1;
+  }
+  __syncthreads();
+  // distinct target supernodes of the tile's columns come in runs (labels are sorted)
+  if (t == 0) {
+    int run = -1, prev = -2;
+    for (int x = 0; x < BT; ++x) {
+      const int tg = ctg[x];
+      if (tg >= 0 && tg != prev) {
+        ++run;
+        if (run < BRUNS) dtl[run] = tg;
+        prev = tg;
+      }
+      dtcol[x] = (tg >= 0 && run < BRUNS) ? run : -1;
+    }
+    ntg = min(BRUNS, run + 1);
+  }
+  __syncthreads();
+  for (int idx = t; idx < ntg * BT; idx += 256) {
+    const int dt = idx / BT, i = idx % BT;
+    const int tg = dtl[dt];
+    const int R = rlab[i];
+    tpos[dt][i] = (R >= super[tg]) ? (int)big_row_pos(rows, pi[tg], pi[tg + 1], R) : -1;
+  }
+  if (t < BT) {
+    const int tg = ctg[t];
+    cbase[t] = tg >= 0 ? px[tg] + (int64_t)(clab[t] - super[tg]) * (pi[tg + 1] - pi[tg]) : 0;
+  }
+  const double* LDs = LD + ldoff[lo];
+  BigAcc acc;
+  big_gemm(
+      acc, w, As, Bs,
+      [&](int k, int x) -> double { return (k < w && i0 + x < m) ? P[w + i0 + x + (int64_t)k * r] : 0.0; },
+      [&](int k, int x) -> double { return (k < w && j0 + x < m) ? LDs[(int64_t)k * m + j0 + x] : 0.0; });
+  // scatter-add: no global reads on the common path (target bases and row
+  // positions come from LDS), so the no-return atomics stream out
+  big_foreach(acc, [&](int ri, int ci, double v) {
+    const int R = rlab[ri], C = clab[ci];
+    if (R < 0 || C < 0 || R < C) return;
+    const int dt = dtcol[ci];
+    int64_t pos;
+    if (dt >= 0) {
+      pos = tpos[dt][ri];
+    } else {
+      const int tg = ctg[ci];
+      pos = big_row_pos(rows, pi[tg], pi[tg + 1], R);
+    }
+    atomicAdd(valL + cbase[ci] + pos, -v);
+  });
+}
+
+// A(i, j) -= sum_{l in [k0, k1)} L(i, l) W(j, l) for k1 <= j <= i < w;
+// 128 x 128 lower tiles cover [lo, w)^2 with lo <= k1
+__global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, const int* super,
+                                                      const int64_t* pi, const int64_t* px, double* valL,
+                                                      const double* Wbase, const int64_t* woff,
+                                                      const int* kpos, const int* kstart) {
+  __shared__ double As[BKC * BLD], Bs[BKC * BLD];
+  const int s = list[0];
+  const int w = super[s + 1] - super[s];
+  const int64_t r = pi[s + 1] - pi[s];
+  const int k0 = kstart[s], k1 = kpos[s];
+  if (k1 >= w || k1 <= k0) return;
+  int ti, tj;
+  tile_of(blockIdx.x, ti, tj);
+  const int i0 = lo + BT * ti, j0 = lo + BT * tj;
+  if (i0 >= w || i0 + BT - 1 < k1 || j0 >= w || j0 + BT - 1 < k1) return;
+  double* A = valL + px[s];
+  const double* Wp = Wbase + woff[s];
+  const int kb = k1 - k0;
+  BigAcc acc;
+  big_gemm(
+      acc, kb, As, Bs,
+      [&](int k, int x) -> double { return (k < kb && i0 + x < w) ? A[(int64_t)(k0 + k) * r + i0 + x] : 0.0; },
+      [&](int k, int x) -> double { return (k < kb && j0 + x < w) ? Wp[(int64_t)k * w + j0 + x] : 0.0; });
+  // read-modify-write in batches of 16 values per lane: the loads use a
+  // safe address for masked elements so that they issue back to back
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double cur[16];
+    int64_t off[16];
+    bool ok[16];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
+        const int q = 4 * b + e;
+        ok[q] = i < w && j < w && j >= k1 && i >= j;
+        off[q] = ok[q] ? (int64_t)j * r + i : 0;
+        cur[q] = A[off[q]];
+      }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int q = 4 * b + e;
+        if (ok[q]) A[off[q]] = cur[q] - acc.c[a][b][e];
+      }
+  }
+}
+
+// LD(:, k) = (L_off D)(:, k) of each listed supernode, column-major m x w at
+// LD + ldoff[q] (the D scaling of blocked_2by2_mult, Sym_BLAS.cpp:573-599)
+__global__ __launch_bounds__(256) void k_scale_ld(const int* list, const int* col_prefix, int count,
+                                                  const int* super, const int64_t* pi, const int64_t* px,
+                                                  const double* valL, const double* dval, int n, double* LD,
+                                                  const int64_t* ldoff) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (col_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const int s = list[lo], k = blockIdx.x - col_prefix[lo];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t r = pi[s + 1] - pi[s];
+  const int m = (int)(r - w);
+  const double* col = valL + px[s] + w;
+  const double dk = dval[c0 + k], ok = dval[n + c0 + k], op = k > 0 ? dval[n + c0 + k - 1] : 0.0;
+  double* out = LD + ldoff[lo] + (int64_t)k * m;
+  for (int j = threadIdx.x; j < m; j += 256) {
+    const double l0 = col[(int64_t)k * r + j];
+    double v;
+    if (ok != 0.0) v = l0 * dk + col[(int64_t)(k + 1) * r + j] * ok;
+    else if (op != 0.0) v = col[(int64_t)(k - 1) * r + j] * op + l0 * dk;
+    else v = l0 * dk;
+    out[j] = v;
+  }
+}
+
+void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncols, const SnArgs& a,
+                     const double* valL, const double* dval, double* LD, const int64_t* ldoff, hipStream_t st) {
+  if (count <= 0 || ncols <= 0) return;
+  k_scale_ld<<<ncols, 256, 0, st>>>(list, col_prefix, count, a.super, a.pi, a.px, valL, dval, a.n, LD, ldoff);
+  HIP_LAUNCH_CHECK("scale_ld");
+}
+
+void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
+                      double* valL, const double* LD, const int64_t* ldoff, hipStream_t st) {
+  if (count <= 0 || ntiles <= 0) return;
+  if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur_big: too many tiles");
+  k_schur_big<<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                                a.col2sup, valL, LD, ldoff);
+  HIP_LAUNCH_CHECK("schur_big");
+}
+
+void launch_trailing_big(const int* list, int ntiles, int lo, const SnArgs& a, double* valL, const double* W,
+                         const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
+  if (ntiles <= 0) return;
+  k_trailing_big<<<ntiles, 256, 0, st>>>(list, lo, a.super, a.pi, a.px, valL, W, woff, kpos, kstart);
+  HIP_LAUNCH_CHECK("trailing_big");
+}
+
+}  // namespace lblg

@@ -612,10 +612,28 @@ __global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per
     tile_mma(acc, As, Bs, kcp);
     __syncthreads();
   }
-  tile_foreach(acc, [&](int ri, int ci, double v) {
-    const int i = i0 + ri, j = j0 + ci;
-    if (i < w && j < w && j >= k1 && i >= j) A[(int64_t)j * r + i] -= v;
-  });
+  // batched read-modify-write: masked elements load from a safe address so
+  // the 16 loads of a lane issue back to back
+  {
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
+    double cur[16];
+    int64_t off[16];
+    bool ok[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int a = q >> 3, b = (q >> 2) & 1, e = q & 3;
+      const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
+      ok[q] = i < w && j < w && j >= k1 && i >= j;
+      off[q] = ok[q] ? (int64_t)j * r + i : 0;
+      cur[q] = A[off[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int a = q >> 3, b = (q >> 2) & 1, e = q & 3;
+      if (ok[q]) A[off[q]] = cur[q] - acc.c[a][b][e];
+    }
+  }
 }
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
                      const double* W, const int64_t* woff, const int* kpos, const int* kstart,
@@ -713,21 +731,40 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
     }
     __syncthreads();
   }
-  // D^{-1}: 1x1 scaling or 2x2 Cramer (blocked_2by2_solver, Sym_BLAS.cpp:492-529)
+  // D^{-1}: 1x1 scaling or 2x2 Cramer (blocked_2by2_solver, Sym_BLAS.cpp:492-529),
+  // 16 columns per batch so that a row's loads issue together; a 2x2 pair
+  // may straddle two batches (look-ahead column / previous raw value)
   if (t < nr) {
-    for (int l = 0; l < w; ++l) {
-      const double od = dval[n + c0 + l];
-      if (od == 0.0) {
-        const double inv = 1.0 / dval[c0 + l];
-        P[q0 + t + (int64_t)l * r] *= inv;
-      } else {
-        const double d1 = dval[c0 + l], d2 = dval[c0 + l + 1];
-        const double inv = 1.0 / (d1 * d2 - od * od);
-        const double x1 = P[q0 + t + (int64_t)l * r], x2 = P[q0 + t + (int64_t)(l + 1) * r];
-        P[q0 + t + (int64_t)l * r] = (x1 * d2 - x2 * od) * inv;
-        P[q0 + t + (int64_t)(l + 1) * r] = (x2 * d1 - x1 * od) * inv;
-        ++l;
+    double* row = P + q0 + t;
+    const double* dg = dval + c0;
+    const double* od = dval + n + c0;
+    double xprev = 0.0;
+    for (int l0 = 0; l0 < w; l0 += 16) {
+      double xv[17], yv[16];
+#pragma unroll
+      for (int q = 0; q < 17; ++q) xv[q] = (l0 + q < w) ? row[(int64_t)(l0 + q) * r] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int l = l0 + q;
+        yv[q] = 0.0;
+        if (l < w) {
+          const double o = od[l], op = l > 0 ? od[l - 1] : 0.0;
+          if (o != 0.0) {
+            const double d1 = dg[l], d2 = dg[l + 1];
+            yv[q] = (xv[q] * d2 - xv[q + 1] * o) / (d1 * d2 - o * o);
+          } else if (op != 0.0) {
+            const double d1 = dg[l - 1], d2 = dg[l];
+            const double x1 = q > 0 ? xv[q - 1] : xprev;
+            yv[q] = (xv[q] * d1 - x1 * op) / (d1 * d2 - op * op);
+          } else {
+            yv[q] = xv[q] / dg[l];
+          }
+        }
       }
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (l0 + q < w) row[(int64_t)(l0 + q) * r] = yv[q];
+      xprev = xv[15];
     }
   }
 }
@@ -764,6 +801,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   __shared__ double dd[KC + 2], oo[KC + 2];
   __shared__ int rlab[64], clab[64], ctg[64], dtcol[64];
   __shared__ int tpos[4][64];
+  __shared__ int64_t cbase[64];
   __shared__ int dtl[4], ntg;
   int lo = 0, hi = count - 1;
   const int64_t bid = blockIdx.x;
@@ -806,6 +844,10 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
     const int tg = dtl[dt];
     const int R = rlab[i];
     tpos[dt][i] = (R >= super[tg]) ? (int)row_pos(rows, pi[tg], pi[tg + 1], R) : -1;
+  }
+  if (t < 64) {
+    const int tg = ctg[t];
+    cbase[t] = tg >= 0 ? px[tg] + (int64_t)(clab[t] - super[tg]) * (pi[tg + 1] - pi[tg]) : 0;
   }
   TileAcc acc;
   tile_zero(acc);
@@ -858,14 +900,19 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
     tile_mma(acc, As, Bs, (kc + 3) & ~3);
     __syncthreads();
   }
+  // target bases and row positions come from LDS on the common path
   tile_foreach(acc, [&](int ri, int ci, double v) {
     const int R = rlab[ri], C = clab[ci];
     if (R < 0 || C < 0 || R < C) return;
-    const int tg = ctg[ci];
-    const int64_t rt = pi[tg + 1] - pi[tg];
     const int dt = dtcol[ci];
-    const int64_t pos = dt >= 0 ? tpos[dt][ri] : row_pos(rows, pi[tg], pi[tg + 1], R);
-    atomicAdd(valL + px[tg] + (int64_t)(C - super[tg]) * rt + pos, -v);
+    int64_t pos;
+    if (dt >= 0) {
+      pos = tpos[dt][ri];
+    } else {
+      const int tg = ctg[ci];
+      pos = row_pos(rows, pi[tg], pi[tg + 1], R);
+    }
+    atomicAdd(valL + cbase[ci] + pos, -v);
   });
 }
 void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,

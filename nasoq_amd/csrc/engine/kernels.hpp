@@ -43,6 +43,13 @@ void launch_bk_wide(const int* list, int G, int nb, const SnArgs& a, double* val
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
                      const double* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st);
+// 128 x 128 MFMA tiles (gemm_big.hip)
+void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncols, const SnArgs& a,
+                     const double* valL, const double* dval, double* LD, const int64_t* ldoff, hipStream_t st);
+void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
+                      double* valL, const double* LD, const int64_t* ldoff, hipStream_t st);
+void launch_trailing_big(const int* list, int ntiles, int lo, const SnArgs& a, double* valL, const double* W,
+                         const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
 void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
                     double* valL, const double* dval, const int* perm_piv, hipStream_t st);
 void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
