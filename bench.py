@@ -99,14 +99,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--groups", type=int, default=None, help="workgroups per wide supernode")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (reference traffic for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
-    if args.groups:
-        os.environ["LBLG_GROUPS"] = str(args.groups)
 
     from nasoq_amd import _lib, lbl
     from nasoq_amd import problems as P
@@ -215,6 +225,9 @@ def main():
     _lib.check(lib.lblg_get_factor(s.h, None, _lib.dptr(dval), None, None), "get_factor")
     out["pivots_2x2"] = int(np.count_nonzero(dval[n:]))
     out["inertia"] = list(s.inertia())
+    a2, a3 = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(lib.lblg_pivot_stats(s.h, ctypes.byref(a2), ctypes.byref(a3)), "pivot_stats")
+    out["wide_panel_pivoting"] = {"second_column_steps": a2.value, "interchanges": a3.value}
     s.close()
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

@@ -168,6 +168,12 @@ int lblg_inertia(lblg_handle* h, int64_t* pos, int64_t* neg, int64_t* zero);
  * of the last lblg_factor. */
 int lblg_factor_info(const lblg_handle* h, int* zero_pivot, double* k0_norm_inf, double* factor_seconds);
 
+/* Pivoting statistics of the wide (cooperative) panels of the last factor:
+ * Bunch-Kaufman steps that had to inspect a second column
+ * (|a_kk| < alpha * colmax) and row/column interchanges.  Engine
+ * diagnostics; the reference has no counterpart. */
+int lblg_pivot_stats(const lblg_handle* h, int64_t* second_column_steps, int64_t* interchanges);
+
 /* Per-kernel timing with HIP events recorded on the handle's stream (engine
  * diagnostics; no reference counterpart).  lblg_kernel_time returns the
  * accumulated milliseconds and launch count of a kernel family since timing

@@ -183,6 +183,15 @@ extern "C" int lblg_factor_info(const lblg_handle* hh, int* zero_pivot, double* 
   return LBLG_OK;
 }
 
+extern "C" int lblg_pivot_stats(const lblg_handle* hh, int64_t* second_column_steps, int64_t* interchanges) {
+  const Handle* h = reinterpret_cast<const Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  if (second_column_steps) *second_column_steps = h->second_col_steps;
+  if (interchanges) *interchanges = h->interchanges;
+  return LBLG_OK;
+}
+
 extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C) {
   if (!A || !B || !C) return LBLG_ERR_ARG;
   return guarded([&] {

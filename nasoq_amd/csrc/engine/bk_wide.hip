@@ -96,7 +96,7 @@ __device__ inline int far_phys(int a, int fstart, const unsigned* mbits, const i
   return a;
 }
 
-__global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb, const int* super,
+__global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb, int sub, int wcols, const int* super,
                                                  const int64_t* pi, const int64_t* px, double* valL,
                                                  double* dval, int* perm_piv, double* Wbase,
                                                  const int64_t* woff, int* kpos, int* kstart, int* sync,
@@ -119,14 +119,22 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   const int64_t r = pi[s + 1] - pi[s];
   double* A = valL + px[s];
   double* Wp = Wbase + woff[s];
-  double* scr = Wp + (int64_t)w * nb;
+  const double* Pp = Wp + (int64_t)w * wcols;  // this panel's columns with the block's earlier panels applied
+  double* scr = Wp + (int64_t)w * (wcols + nb);
   const int k0 = kpos[s];
   if (k0 >= w) return;
+  // sub > 0: this panel continues the block of panels that started at K0.
+  // The matrix holds no contribution of the block's earlier panels (the
+  // trailing block is updated once per block); this panel's columns with
+  // those contributions applied are in Pp, and W holds the earlier panels'
+  // columns [0, off)
+  const int K0 = sub == 0 ? k0 : kstart[s];
+  const int off = k0 - K0;
   const int nbe = min(nb, w - k0);
   const bool last = (w - k0) <= nb;
   const int fstart = k0 + nbe, nfar = w - fstart;
   const int chunk = nfar > 0 ? (nfar + G - 1) / G : 0;
-  if (chunk > WCH || nb > WNB) {  // uniform over the grid: every workgroup returns
+  if (chunk > WCH || nb > WNB || off < 0 || off + nb > wcols) {  // uniform over the grid: every workgroup returns
     if (g == 0 && t == 0) atomicOr(status, 8);
     return;
   }
@@ -140,11 +148,13 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 #define ADR(i, j) Ad[(i) * WLDA + (j)]
 
   // ---- load the panel rows
+  // panel column j, row i >= j (with the block's earlier panels applied)
+  auto pcol = [&](int i, int j) -> double { return sub ? Pp[(int64_t)j * w + i] : AG(i, k0 + j); };
   if (hasrow)
-    for (int j = 0; j < nbe; ++j) LCR(j) = AG(fr, k0 + j);
+    for (int j = 0; j < nbe; ++j) LCR(j) = pcol(fr, j);
   for (int idx = t; idx < nbe * nbe; idx += 256) {
     const int i = idx % nbe, j = idx / nbe;
-    ADR(i, j) = (i >= j) ? AG(k0 + i, k0 + j) : AG(k0 + j, k0 + i);
+    ADR(i, j) = (i >= j) ? pcol(k0 + i, j) : pcol(k0 + j, i);
   }
   if (t < WNB) {
     dphys[t] = k0 + t;
@@ -278,6 +288,8 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       if (hasrow) {
         const int pa = myphys;
         double v = (pa >= pim) ? AG(pa, pim) : AG(pim, pa);
+        // the matrix lacks the block's earlier panels
+        for (int c = 0; c < off; ++c) v -= AG(pa, K0 + c) * Wp[(int64_t)c * w + pim];
         for (int j = 0; j < kr; ++j) v -= LCR(j) * wi[j];
         v1 = v;
       }
@@ -450,6 +462,8 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
           perm_piv[c0 + kp] = x;
         }
         if (zero) atomicOr(status, 1);
+        if (colmax > 0.0 && !(absakk >= BK_ALPHA * colmax)) atomicAdd(status + 2, 1);
+        if (swap) atomicAdd(status + 3, 1);
       }
     }
     __syncthreads();
@@ -489,6 +503,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   double* S1 = scr;                                  // [nMa][nt]
   double* S2 = scr + (int64_t)WMM * w;               // [nM][no]
   double* S3 = S2 + (int64_t)WMM * (no > 0 ? no : 0);  // [nM][k0]
+  double* S4 = S3 + (int64_t)WMM * w;                 // [nM][off] rows of W of the block's earlier panels
   auto phys = [&](int j) -> int {
     if (j < k0 + nbe) return dphys[j - k0];
     return far_phys(j, fstart, mbits, mlog, mphys, s_nm);
@@ -509,6 +524,10 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       const int m = (int)(idx / k0), j = (int)(idx % k0);
       S3[idx] = AG(Mp[m], j);
     }
+    for (int64_t idx = gt; idx < (int64_t)nM * off; idx += GT) {
+      const int m = (int)(idx / off), c = (int)(idx % off);
+      S4[idx] = Wp[(int64_t)c * w + Mp[m]];
+    }
     group_barrier(sync, G, epoch, status);
     for (int64_t idx = gt; idx < (int64_t)nMa * nt; idx += GT) {
       const int m = (int)(idx / nt), j = kend + (int)(idx % nt);
@@ -524,21 +543,25 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       const int m = (int)(idx / k0), j = (int)(idx % k0);
       AG(Ml[m], j) = S3[idx];
     }
+    for (int64_t idx = gt; idx < (int64_t)nM * off; idx += GT) {
+      const int m = (int)(idx / off), c = (int)(idx % off);
+      Wp[(int64_t)c * w + Ml[m]] = S4[idx];
+    }
   }
   // ---- L of the panel and W = L D for the rows of the trailing block
   if (hasrow) {
     for (int j = 0; j < ker; ++j) {
       AG(fr, k0 + j) = LCR(j);
-      Wp[(int64_t)j * w + fr] = w_entry([&](int c) { return LCR(c); }, j, dd, dsb);
+      Wp[(int64_t)(off + j) * w + fr] = w_entry([&](int c) { return LCR(c); }, j, dd, dsb);
     }
   }
   for (int idx = gt; idx < nbe * ker; idx += GT) {
     const int i = idx % nbe, j = idx / nbe;
     if (j <= i) AG(k0 + i, k0 + j) = ADR(i, j);
-    if (k0 + i >= kend) Wp[(int64_t)j * w + k0 + i] = w_entry([&](int c) { return ADR(i, c); }, j, dd, dsb);
+    if (k0 + i >= kend) Wp[(int64_t)(off + j) * w + k0 + i] = w_entry([&](int c) { return ADR(i, c); }, j, dd, dsb);
   }
   if (g == 0 && t == 0) {
-    kstart[s] = k0;
+    kstart[s] = K0;
     kpos[s] = kend;
   }
 #undef AG
@@ -546,12 +569,12 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 #undef ADR
 }
 
-void launch_bk_wide(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval, int* perm_piv,
-                    double* W, const int64_t* woff, int* kpos, int* kstart, int* sync, uint64_t* ll,
-                    uint32_t ebase, int* status, hipStream_t st) {
+void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, double* valL, double* dval,
+                    int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+                    uint64_t* ll, uint32_t ebase, int* status, hipStream_t st) {
   if (nb > WNB || nb < 4 || G < 1 || G > 256) throw std::runtime_error("bk_wide: launch shape out of range");
   int n = a.n;
-  void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
+  void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&sub, (void*)&wcols, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
                   (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
                   (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&ll, (void*)&ebase, (void*)&n,
                   (void*)&status};

@@ -91,7 +91,7 @@ struct DeviceFactor {
   int64_t W_elems = 0;
   int64_t* woff = nullptr;      // per-supernode offset into W
   int* kpos = nullptr;          // next column to factor (wide supernodes)
-  int* kstart = nullptr;        // first column of the last panel
+  int* kstart = nullptr;        // first column of the current block of panels
   unsigned char* isdiag = nullptr;
   int64_t nnzf = 0;             // entries of the full CSR
   double* k0norm = nullptr;     // ||K0||_inf on the device

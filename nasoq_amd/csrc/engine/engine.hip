@@ -222,7 +222,8 @@ static void setup(Handle& h) {
           const int nb = WIDE_NB;
           const int G = std::max(1, (w - nb + WIDE_ROWS - 1) / WIDE_ROWS);
           if (G > 256) throw std::runtime_error("supernode too wide for the cooperative panel kernel");
-          wsum += (int64_t)w * nb + (int64_t)WIDE_MOVED * (w + r) + 64;
+          const int64_t wcols = (int64_t)WIDE_SUB * nb;
+          wsum += (int64_t)w * (wcols + nb) + (int64_t)WIDE_MOVED * (w + r + wcols) + 64;
           F.large.push_back(s);
           F.large_G.push_back(G);
           F.large_nb.push_back(nb);
@@ -385,7 +386,20 @@ void factor_device(Handle& h) {
       const int w = S.super[s + 1] - S.super[s];
       const int G = F.large_G[q], nb = F.large_nb[q];
       const int rounds = (w + nb - 2) / (nb - 1) + 1;
+      const int wcols = WIDE_SUB * nb;
       for (int p = 0; p < rounds; ++p) {
+        // panels run in blocks of WIDE_SUB: before panel p of a block, its columns
+        // receive the block's earlier panels; the rest of the trailing block is
+        // updated once at the end of the block (rank up to WIDE_SUB * nb)
+        const int sub = p % WIDE_SUB;
+        const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
+        if (sub > 0) {
+          const int Tr = (w - lo + 127) / 128;
+          if (Tr > 0)
+            timed(D, "trailing", [&] {
+              launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
+            });
+        }
         launch_fill_int(D.sync, 1, 0, st);
         // far rows of this panel are at most w - p (nb - 1) - nb (each panel eliminates >= nb - 1 columns)
         const int far = w - p * (nb - 1) - nb;
@@ -393,15 +407,18 @@ void factor_device(Handle& h) {
         const uint32_t eb = D.ll_epoch;
         D.ll_epoch += 3u * (uint32_t)nb + 8u;
         timed(D, "bk_panel_group", [&] {
-          launch_bk_wide(F.d_large + q, Gp, nb, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos, D.kstart,
-                         D.sync, D.ll, eb, D.status, st);
+          launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
+                         D.kstart, D.sync, D.ll, eb, D.status, st);
         });
-        const int lo = std::min(w, (p + 1) * (nb - 1));
-        const int T = (w - lo + 127) / 128;
-        if (T > 0)
-          timed(D, "trailing", [&] {
-            launch_trailing_big(F.d_large + q, T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
-          });
+        if (sub == WIDE_SUB - 1 || p == rounds - 1) {
+          const int lo2 = std::min(w, (p + 1) * (nb - 1));
+          const int T = (w - lo2 + 127) / 128;
+          if (T > 0)
+            timed(D, "trailing", [&] {
+              launch_trailing_big(F.d_large + q, T * (T + 1) / 2, lo2, 0, wcols, a, D.valL, D.W, D.woff, D.kpos,
+                                  D.kstart, st);
+            });
+        }
       }
     }
     timed(D, "offdiag", [&] {
@@ -425,11 +442,13 @@ void factor_device(Handle& h) {
   launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
   // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
   launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
-  int stat[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  int stat[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(stat), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   h.zero_pivot = stat[0] & 1;
+  h.second_col_steps = stat[2];
+  h.interchanges = stat[3];
   if (stat[0] & 4) throw std::runtime_error("cross-workgroup exchange timed out");
   if (stat[0] & 8) throw std::runtime_error("wide panel launch shape does not fit the supernode");
   h.factored = true;

@@ -194,11 +194,17 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
   });
 }
 
-// A(i, j) -= sum_{l in [k0, k1)} L(i, l) W(j, l) for k1 <= j <= i < w;
-// 128 x 128 lower tiles cover [lo, w)^2 with lo <= k1
-__global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, const int* super,
+// A(i, j) -= sum_{l in [k0, k1)} L(i, l) W(j, l) for k1 <= j <= i < w, with
+// W(j, l) at Wp[(l - k0) w + j] (k0 = kstart, k1 = kpos).
+//   strip = 0: in place on the whole trailing block; 128 x 128 lower tiles
+//              cover [lo, w)^2, lo <= k1
+//   strip > 0: the next panel's columns j < k1 + strip (strip <= 128) only,
+//              written to the panel buffer Pp[(j - k1) w + i] (the matrix itself
+//              stays without the block's updates until the block ends); one
+//              column of tiles starting at k1
+__global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, int strip, int wcols, const int* super,
                                                       const int64_t* pi, const int64_t* px, double* valL,
-                                                      const double* Wbase, const int64_t* woff,
+                                                      double* Wbase, const int64_t* woff,
                                                       const int* kpos, const int* kstart) {
   __shared__ double As[BKC * BLD], Bs[BKC * BLD];
   const int s = list[0];
@@ -206,12 +212,21 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, c
   const int64_t r = pi[s + 1] - pi[s];
   const int k0 = kstart[s], k1 = kpos[s];
   if (k1 >= w || k1 <= k0) return;
-  int ti, tj;
-  tile_of(blockIdx.x, ti, tj);
-  const int i0 = lo + BT * ti, j0 = lo + BT * tj;
-  if (i0 >= w || i0 + BT - 1 < k1 || j0 >= w || j0 + BT - 1 < k1) return;
+  int i0, j0;
+  if (strip) {
+    i0 = k1 + BT * (int)blockIdx.x;
+    j0 = k1;
+  } else {
+    int ti, tj;
+    tile_of(blockIdx.x, ti, tj);
+    i0 = lo + BT * ti;
+    j0 = lo + BT * tj;
+  }
+  const int jend = strip ? min(w, k1 + strip) : w;
+  if (i0 >= w || i0 + BT - 1 < k1 || j0 >= jend || j0 + BT - 1 < k1) return;
   double* A = valL + px[s];
   const double* Wp = Wbase + woff[s];
+  double* Pp = Wbase + woff[s] + (int64_t)w * wcols;
   const int kb = k1 - k0;
   BigAcc acc;
   big_gemm(
@@ -233,17 +248,28 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, c
       for (int e = 0; e < 4; ++e) {
         const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
         const int q = 4 * b + e;
-        ok[q] = i < w && j < w && j >= k1 && i >= j;
+        ok[q] = i < w && j < jend && j >= k1 && i >= j;
         off[q] = ok[q] ? (int64_t)j * r + i : 0;
         cur[q] = A[off[q]];
       }
+    if (strip) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+      for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int q = 4 * b + e;
-        if (ok[q]) A[off[q]] = cur[q] - acc.c[a][b][e];
-      }
+        for (int e = 0; e < 4; ++e) {
+          const int q = 4 * b + e;
+          const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
+          if (ok[q]) Pp[(int64_t)(j - k1) * w + i] = cur[q] - acc.c[a][b][e];
+        }
+    } else {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int q = 4 * b + e;
+          if (ok[q]) A[off[q]] = cur[q] - acc.c[a][b][e];
+        }
+    }
   }
 }
 
@@ -291,10 +317,11 @@ void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, in
   HIP_LAUNCH_CHECK("schur_big");
 }
 
-void launch_trailing_big(const int* list, int ntiles, int lo, const SnArgs& a, double* valL, const double* W,
-                         const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
+void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, double* valL,
+                         double* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
   if (ntiles <= 0) return;
-  k_trailing_big<<<ntiles, 256, 0, st>>>(list, lo, a.super, a.pi, a.px, valL, W, woff, kpos, kstart);
+  if (strip < 0 || strip > BT) throw std::runtime_error("trailing_big: strip wider than a tile");
+  k_trailing_big<<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff, kpos, kstart);
   HIP_LAUNCH_CHECK("trailing_big");
 }
 

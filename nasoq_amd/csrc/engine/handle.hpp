@@ -26,6 +26,8 @@ struct Handle {
   double k0_norm_inf = 0.0;
   int num_ref_iter = 0;
   int zero_pivot = 0;
+  int64_t second_col_steps = 0;  // wide panels: pivot steps that inspected a second column
+  int64_t interchanges = 0;      // wide panels: row/column interchanges
   double factor_seconds = 0.0;
   std::unique_ptr<DeviceFactor> dev;
   Handle();

@@ -36,10 +36,11 @@ void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double*
 constexpr int WIDE_NB = 64;         // panel width
 constexpr int WIDE_ROWS = 192;      // far rows per workgroup
 constexpr int WIDE_MOVED = 128;     // moved indices per panel (scratch rows)
+constexpr int WIDE_SUB = 4;         // panels per trailing update (W holds WIDE_SUB * WIDE_NB columns)
 constexpr int64_t LL_WORDS = 2 * 256 * 4 + 2 * 256 * 8 + 2 * 256 * 2 * 64;
-void launch_bk_wide(const int* list, int G, int nb, const SnArgs& a, double* valL, double* dval, int* perm_piv,
-                    double* W, const int64_t* woff, int* kpos, int* kstart, int* sync, uint64_t* ll,
-                    uint32_t ebase, int* status, hipStream_t st);
+void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, double* valL, double* dval,
+                    int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+                    uint64_t* ll, uint32_t ebase, int* status, hipStream_t st);
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
                      const double* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st);
@@ -48,8 +49,8 @@ void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncol
                      const double* valL, const double* dval, double* LD, const int64_t* ldoff, hipStream_t st);
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
                       double* valL, const double* LD, const int64_t* ldoff, hipStream_t st);
-void launch_trailing_big(const int* list, int ntiles, int lo, const SnArgs& a, double* valL, const double* W,
-                         const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
+void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, double* valL,
+                         double* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
 void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
                     double* valL, const double* dval, const int* perm_piv, hipStream_t st);
 void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
