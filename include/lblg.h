@@ -186,6 +186,88 @@ int lblg_kernel_time(lblg_handle* h, const char* name, double* total_ms, int64_t
  * all row-major.  No reference counterpart (engine diagnostics). */
 int lblg_selftest_mfma(const double* A, const double* B, double* C);
 
+/* ---------------------------------------------------------------------------
+ * NASOQ QP solver over the engine:  min 1/2 x'Hx + q'x  s.t.  Ax = a, Bx <= b
+ * (nasoq::Nasoq, include/nasoq/nasoq.h:61-493, src/nasoq.cpp).  The active-set
+ * loop runs on the host; every KKT factorization and solve is a device call
+ * on the super-KKT [H A' B'; A 0 0; B 0 0] (build_super_matrix,
+ * src/QP/linear_solver_wrapper.cpp:246-350).
+ * ------------------------------------------------------------------------- */
+typedef struct lblg_qp lblg_qp;
+
+/* variants (nasoq_mode, nasoq.h:44-46) and statuses (nasoq_status, :47-49) */
+#define LBLG_QP_FIXED 0
+#define LBLG_QP_AUTO 1
+#define LBLG_QP_TUNED 2
+#define LBLG_QP_PREDET 3
+#define LBLG_QP_INFEASIBLE 0
+#define LBLG_QP_OPTIMAL 1
+#define LBLG_QP_INACCURATE 2
+#define LBLG_QP_NOT_CONVERGED 3
+
+/* Public settings of Nasoq (Nasoq::default_setting, nasoq.cpp:186-199). */
+typedef struct lblg_qp_settings {
+  double diag_perturb;  /* default 1e-9 */
+  double eps_abs;       /* default 1e-3 */
+  double stop_tol;      /* default 1e-15 */
+  int max_iter;         /* refinement iterations of the PREDET variant, default 0 */
+  int max_iter_nas;     /* active-set iteration cap, default 4000 */
+  int variant;          /* LBLG_QP_*, default LBLG_QP_FIXED */
+  int ordering;         /* LBLG_ORDER_ND (default) or LBLG_ORDER_NATURAL */
+} lblg_qp_settings;
+
+/* Results and KKT checks of the last solve (the reference's public fields). */
+typedef struct lblg_qp_info {
+  int status;            /* LBLG_QP_* status (check_solve_status, nasoq.cpp:712-728) */
+  int num_iter;          /* active-set iterations */
+  int n_active;          /* active inequalities */
+  int inner_iter_ref;    /* refinement budget used (after the variant's choice) */
+  int outer_iter_ref;
+  double diag_perturb;   /* perturbation used */
+  double stop_tol;
+  double primal_obj, dual_obj, objective;
+  double lag_res;               /* ||Hx + q + A'y + B'z||_inf */
+  double cons_sat_norm;         /* max(||max(Bx - b, 0)||_inf, ||Ax - a||_inf) */
+  double non_negativity_infn;   /* ||min(z, 0)||_inf */
+  double complementarity_infn;  /* ||z .* (Bx - b)||_inf */
+  int64_t factorizations;       /* device KKT factorizations */
+  int64_t kkt_solves;           /* device KKT solves */
+} lblg_qp_info;
+
+void lblg_qp_default_settings(lblg_qp_settings* s);
+
+/* Nasoq(H_size, Hp, Hi, Hx, q, A_size1, A_size2, Ap, Ai, Ax, a, B_size1, B_size2,
+ * Bp, Bi, Bx, b) (nasoq.cpp:36-135): H is n x n lower CSC; A is me x n and B
+ * is mi x n, both CSC with n columns (me or mi may be 0, pointers NULL then).
+ * All arrays are copied. */
+int lblg_qp_create(lblg_qp** qp, int n, const int* Hp, const int* Hi, const double* Hx, const double* q, int me,
+                   const int* Ap, const int* Ai, const double* Ax, const double* a, int mi, const int* Bp,
+                   const int* Bi, const double* Bx, const double* b);
+int lblg_qp_destroy(lblg_qp* qp);
+int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* s);
+
+/* Nasoq::symbolic_QP (nasoq.cpp:201-278) alone: the variant's refinement
+ * settings, the super-KKT and its symbolic analysis (host only). */
+int lblg_qp_analyze(lblg_qp* qp);
+
+/* Nasoq::solve() (nasoq.cpp:656-710); *status receives the LBLG_QP_* status. */
+int lblg_qp_solve(lblg_qp* qp, int* status);
+
+/* Nasoq::solve(primal, dual, active_set) (nasoq.cpp:740-760): warm start from
+ * x0 (n), z0 (mi), the active inequalities (n_active) and y0 (me, may be NULL).
+ * *converged receives 1 when the active-set loop converged. */
+int lblg_qp_solve_warm(lblg_qp* qp, const double* x0, const double* z0, const int* active, int n_active,
+                       const double* y0, int* converged);
+
+/* primal (n), dual_ineq (mi), dual_eq (me), active set (n_active ints); any may be NULL. */
+int lblg_qp_get(const lblg_qp* qp, double* primal, double* dual_ineq, double* dual_eq, int* active);
+int lblg_qp_get_info(const lblg_qp* qp, lblg_qp_info* info);
+
+/* The super-KKT of the last solve, lower CSC of order n + me + mi, with the
+ * current values (active inequalities carry B, the others zero).  Returns the
+ * number of entries; copies when the pointers are not NULL. */
+int64_t lblg_qp_super_kkt(const lblg_qp* qp, int* p, int* i, double* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,41 @@ class Settings(ctypes.Structure):
     ]
 
 
+class QPSettings(ctypes.Structure):
+    """Mirror of lblg_qp_settings."""
+    _fields_ = [
+        ("diag_perturb", ctypes.c_double),
+        ("eps_abs", ctypes.c_double),
+        ("stop_tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int),
+        ("max_iter_nas", ctypes.c_int),
+        ("variant", ctypes.c_int),
+        ("ordering", ctypes.c_int),
+    ]
+
+
+class QPInfo(ctypes.Structure):
+    """Mirror of lblg_qp_info."""
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("num_iter", ctypes.c_int),
+        ("n_active", ctypes.c_int),
+        ("inner_iter_ref", ctypes.c_int),
+        ("outer_iter_ref", ctypes.c_int),
+        ("diag_perturb", ctypes.c_double),
+        ("stop_tol", ctypes.c_double),
+        ("primal_obj", ctypes.c_double),
+        ("dual_obj", ctypes.c_double),
+        ("objective", ctypes.c_double),
+        ("lag_res", ctypes.c_double),
+        ("cons_sat_norm", ctypes.c_double),
+        ("non_negativity_infn", ctypes.c_double),
+        ("complementarity_infn", ctypes.c_double),
+        ("factorizations", ctypes.c_int64),
+        ("kkt_solves", ctypes.c_int64),
+    ]
+
+
 # (name, restype, argtypes) for every exported entry point of include/lblg.h
 SIGNATURES = [
     ("lblg_default_settings", None, [ctypes.POINTER(Settings)]),
@@ -69,6 +104,17 @@ SIGNATURES = [
     ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p]),
     ("lblg_set_kernel_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("lblg_kernel_time", ctypes.c_int, [_vp, ctypes.c_char_p, _f64p, _i64p]),
+    ("lblg_qp_default_settings", None, [ctypes.POINTER(QPSettings)]),
+    ("lblg_qp_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,
+                                      _i32p, _i32p, _f64p, _f64p, ctypes.c_int, _i32p, _i32p, _f64p, _f64p]),
+    ("lblg_qp_destroy", ctypes.c_int, [_vp]),
+    ("lblg_qp_set_settings", ctypes.c_int, [_vp, ctypes.POINTER(QPSettings)]),
+    ("lblg_qp_analyze", ctypes.c_int, [_vp]),
+    ("lblg_qp_solve", ctypes.c_int, [_vp, _i32p]),
+    ("lblg_qp_solve_warm", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, ctypes.c_int, _f64p, _i32p]),
+    ("lblg_qp_get", ctypes.c_int, [_vp, _f64p, _f64p, _f64p, _i32p]),
+    ("lblg_qp_get_info", ctypes.c_int, [_vp, ctypes.POINTER(QPInfo)]),
+    ("lblg_qp_super_kkt", ctypes.c_int64, [_vp, _i32p, _i32p, _f64p]),
 ]
 
 _lib = None

@@ -24,6 +24,13 @@ void set_error(const char* what) {
   std::strncpy(g_last_error, what, sizeof(g_last_error) - 1);
   g_last_error[sizeof(g_last_error) - 1] = 0;
 }
+}  // namespace
+
+namespace lblg {
+void set_last_error(const char* what) { set_error(what); }
+}  // namespace lblg
+
+namespace {
 template <typename F>
 int guarded(F f) {
   try {
