@@ -268,6 +268,28 @@ int lblg_qp_get_info(const lblg_qp* qp, lblg_qp_info* info);
  * number of entries; copies when the pointers are not NULL. */
 int64_t lblg_qp_super_kkt(const lblg_qp* qp, int* p, int* i, double* x);
 
+/* ---------------------------------------------------------------------------
+ * Mode-1 LBL (the super-KKT with inequality rows added and dropped, the
+ * SolverSettings expert mode: constructors SolverSettings(H, rhs, B[, BT, C,
+ * CT]), linear_solver_wrapper.cpp:100-132; update_somod / downdate_somod
+ * :995-1018; add_del_matrix_qp :1020-1137; update_factorization :1151-1290).
+ * The handle is an lblg_qp created by lblg_qp_create (q, a, b unused here).
+ * ------------------------------------------------------------------------- */
+
+/* Build the super-KKT and analyse it with the LBL settings s (reg_diag,
+ * refinement budget, ordering); every inequality row starts inactive. */
+int lblg_kkt_analyze(lblg_qp* qp, const lblg_settings* s);
+/* Order of the super-KKT (n + me + mi), or negative before lblg_kkt_analyze. */
+int64_t lblg_kkt_size(const lblg_qp* qp);
+/* Activate (add = 1) or deactivate (add = 0) inequality rows (add_del_matrix_qp). */
+int lblg_kkt_edit(lblg_qp* qp, int add, int count, const int* rows);
+/* Factor the super-KKT with the current active rows (update_factorization /
+ * numerical_factorization): inactive rows are decoupled, which equals the
+ * reference's factor with their supernodes hidden. */
+int lblg_kkt_factor(lblg_qp* qp);
+/* Solve the super-KKT (solve_only, mode 1): rhs and x have n + me + mi entries. */
+int lblg_kkt_solve(lblg_qp* qp, const double* rhs, double* x, double* bwd_err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,11 @@ SIGNATURES = [
     ("lblg_qp_get", ctypes.c_int, [_vp, _f64p, _f64p, _f64p, _i32p]),
     ("lblg_qp_get_info", ctypes.c_int, [_vp, ctypes.POINTER(QPInfo)]),
     ("lblg_qp_super_kkt", ctypes.c_int64, [_vp, _i32p, _i32p, _f64p]),
+    ("lblg_kkt_analyze", ctypes.c_int, [_vp, ctypes.POINTER(Settings)]),
+    ("lblg_kkt_size", ctypes.c_int64, [_vp]),
+    ("lblg_kkt_edit", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
+    ("lblg_kkt_factor", ctypes.c_int, [_vp]),
+    ("lblg_kkt_solve", ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
 ]
 
 _lib = None

@@ -198,3 +198,57 @@ extern "C" int64_t lblg_qp_super_kkt(const lblg_qp* qp, int* p, int* i, double* 
   if (x) std::memcpy(x, s->SM.x.data(), sizeof(double) * nnz);
   return nnz;
 }
+
+extern "C" int lblg_kkt_analyze(lblg_qp* qp, const lblg_settings* st) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s || !st) return LBLG_ERR_ARG;
+  if (st->req_ref_iter < 0 || st->max_inner_iter < 0 || !(st->reg_diag >= 0)) return LBLG_ERR_ARG;
+  return guarded([&] {
+    s->analyze_lbl(*st);
+    return LBLG_OK;
+  });
+}
+
+extern "C" int64_t lblg_kkt_size(const lblg_qp* qp) {
+  const QpSolver* s = reinterpret_cast<const QpSolver*>(qp);
+  if (!s || !s->analysed) return LBLG_ERR_STATE;
+  return s->SM.ncol;
+}
+
+extern "C" int lblg_kkt_edit(lblg_qp* qp, int add, int count, const int* rows) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s || count < 0 || (count > 0 && !rows)) return LBLG_ERR_ARG;
+  if (!s->analysed) return LBLG_ERR_STATE;
+  for (int k = 0; k < count; ++k)
+    if (rows[k] < 0 || rows[k] >= s->mi) return LBLG_ERR_ARG;
+  for (int k = 0; k < count; ++k) {
+    const int c = rows[k];
+    s->edit_kkt(add != 0, c);
+    if (add && !s->used[c]) s->update_active_set(true, c);
+    else if (!add && s->used[c]) s->update_active_set(false, c);
+  }
+  return LBLG_OK;
+}
+
+extern "C" int lblg_kkt_factor(lblg_qp* qp) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s) return LBLG_ERR_ARG;
+  if (!s->analysed) return LBLG_ERR_STATE;
+  int rc = lblg_set_values(s->h, s->SM.x.data());
+  if (rc) return rc;
+  rc = lblg_factor(s->h);
+  if (rc == LBLG_OK) ++s->factorizations;
+  return rc;
+}
+
+extern "C" int lblg_kkt_solve(lblg_qp* qp, const double* rhs, double* x, double* bwd_err) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s || !rhs || !x) return LBLG_ERR_ARG;
+  if (!s->analysed) return LBLG_ERR_STATE;
+  const int64_t N = s->SM.ncol;
+  double bwd = 0.0;
+  const int rc = lblg_solve(s->h, 1, rhs, N, x, N, &bwd);
+  if (bwd_err) *bwd_err = bwd;
+  if (rc == LBLG_OK) ++s->kkt_solves;
+  return rc;
+}

@@ -200,6 +200,30 @@ void QpSolver::symbolic() {
   analysed = true;
 }
 
+// SolverSettings::symbolic_analysis in mode 1 (linear_solver_wrapper.cpp:396-536)
+void QpSolver::analyze_lbl(const lblg_settings& st) {
+  AT = transpose(A);
+  BT = transpose(B);
+  diag_perturb = zero_thresh = st.reg_diag;
+  outer_iter_ref = st.req_ref_iter;
+  inner_iter_ref = st.max_inner_iter;
+  stop_tol = st.tol_abs;
+  build_super_matrix();
+  const int N = SM.ncol;
+  if (h) lblg_destroy(h);
+  h = nullptr;
+  check(lblg_create(&h, N, SM.p.data(), SM.i.data(), SM.x.data()), "kkt: create");
+  std::vector<int> extra(N, 0);
+  for (int j = n + me; j < N; ++j) extra[j] = 1;
+  lblg_settings s2 = st;
+  if (s2.ordering == LBLG_ORDER_GIVEN) s2.ordering = LBLG_ORDER_ND;
+  check(lblg_analyze(h, &s2, nullptr, extra.data()), "kkt: analyze");
+  used.assign(mi, 0);
+  active_set.clear();
+  n_active = 0;
+  analysed = true;
+}
+
 // Nasoq::solve_kkt (nasoq.cpp:918-949): refactor (UPDATE / REFACTOR) or reuse
 // the factor (SOLVE), then the refined solve
 void QpSolver::solve_kkt(bool refactor) {

@@ -58,6 +58,7 @@ struct QpSolver {
 
   void build_super_matrix();
   void symbolic();
+  void analyze_lbl(const lblg_settings& st);  // mode-1 LBL: super-KKT with given settings
   void initialize_x();
   int numeric();
   int solve();

@@ -111,3 +111,39 @@ def test_random_qp_kkt_and_warm_start(seed):
     assert w.num_iter == 0 and np.allclose(w.primal_vars, x, atol=1e-12)
     m.close()
     w.close()
+
+
+def _build_cpp(name, tmp_path):
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / name)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", name + ".cpp"), "-L", os.path.join(root, "nasoq_amd"),
+                    "-llblg", "-Wl,-rpath," + os.path.join(root, "nasoq_amd"), "-o", exe], check=True)
+    return exe
+
+
+def test_cpp_dropin_nasoq_main(tmp_path):
+    import subprocess
+    out = subprocess.run([_build_cpp("nasoq_main", tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+
+
+@pytest.mark.parametrize("zero", [True, False])
+def test_cpp_dropin_somod(tmp_path, zero):
+    # examples/lbl_somod_main.cpp: with zero right-hand sides for the added
+    # rows the solutions are the example's listed vectors {-2,-2}, {0,-2,-4},
+    # {0,0,-4,-4}, {-2,0,0,-4}; with d = (2, 1) they solve the same bordered
+    # systems with those right-hand sides
+    import subprocess
+    exe = _build_cpp("lbl_somod_main", tmp_path)
+    out = subprocess.run([exe] + (["zero"] if zero else []), capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = [np.array([float(v) for v in ln.split()]) for ln in out.stdout.strip().splitlines()]
+    if zero:
+        want = [[-2, -2], [0, -2, -4, 0], [0, 0, -4, -4], [-2, 0, 0, -4]]
+    else:
+        want = [[-2, -2], [2, -2, -8, 0], [2, 1, -8, -6], [-2, 1, 0, -6]]
+    assert len(got) == 4
+    for g, w in zip(got, want):
+        assert np.allclose(g, w, atol=1e-12), (g, w)
