@@ -7,10 +7,13 @@
 // src/ldl/parallel_blocked_ldlt_02.cpp:155 and the column permutation of the
 // rows below it (LAPACKE_dlapmt, :164-165).
 //
-// Work split.  The rows [k0, w) of the panel are the nb "diagonal" rows
-// [k0, k0+nb) and the "far" rows below them.  G workgroups each own up to
-// WCH far rows (one per thread of waves 0-2) and keep those rows of the
-// panel (L for eliminated columns, A for pending ones) in LDS.  Every
+// Work split.  The rows [k0, r) of the panel are the nb "diagonal" rows
+// [k0, k0+nb) and the "far" rows below them: the rest of the diagonal block
+// and the supernode's off-diagonal rows [w, r), which get their L in the same
+// steps (so no separate off-diagonal solve is needed) but are never pivot
+// candidates.  G workgroups each own up to WCH far rows (one per thread of
+// waves 0-2) and keep those rows of the panel (L for eliminated columns, A
+// for pending ones) in LDS.  Every
 // workgroup also keeps the diagonal rows (nb x nb) and computes their
 // updated values itself, so the W row needed by each column step is always
 // local.
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   double* A = valL + px[s];
   double* Wp = Wbase + woff[s];
   const double* Pp = Wp + (int64_t)w * wcols;  // this panel's columns with the block's earlier panels applied
-  double* scr = Wp + (int64_t)w * (wcols + nb);
+  double* scr = Wp + (int64_t)w * wcols + (int64_t)r * nb;
   const int k0 = kpos[s];
   if (k0 >= w) return;
   // sub > 0: this panel continues the block of panels that started at K0.
@@ -132,14 +135,16 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   const int off = k0 - K0;
   const int nbe = min(nb, w - k0);
   const bool last = (w - k0) <= nb;
-  const int fstart = k0 + nbe, nfar = w - fstart;
+  // far rows: the rest of the diagonal block, then the rows below it (those
+  // take part in every update and get their L here, but are never pivots)
+  const int fstart = k0 + nbe, nfar = (int)(r - fstart);
   const int chunk = nfar > 0 ? (nfar + G - 1) / G : 0;
   if (chunk > WCH || nb > WNB || off < 0 || off + nb > wcols) {  // uniform over the grid: every workgroup returns
     if (g == 0 && t == 0) atomicOr(status, 8);
     return;
   }
   const int fbeg = fstart + g * chunk;
-  const int fcnt = max(0, min(chunk, w - fbeg));
+  const int fcnt = max(0, min(chunk, (int)r - fbeg));
   const bool hasrow = t < fcnt;
   const int fr = fbeg + t;
   int myphys = fr;
@@ -149,7 +154,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 
   // ---- load the panel rows
   // panel column j, row i >= j (with the block's earlier panels applied)
-  auto pcol = [&](int i, int j) -> double { return sub ? Pp[(int64_t)j * w + i] : AG(i, k0 + j); };
+  auto pcol = [&](int i, int j) -> double { return sub ? Pp[(int64_t)j * r + i] : AG(i, k0 + j); };
   if (hasrow)
     for (int j = 0; j < nbe; ++j) LCR(j) = pcol(fr, j);
   for (int idx = t; idx < nbe * nbe; idx += 256) {
@@ -190,7 +195,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
     }
     double av = -1.0;
     int ai = 0x7fffffff;
-    if (hasrow) {
+    if (hasrow && fr < w) {
       av = fabs(v0);
       if (!(av == av)) av = -1.0;
       ai = fr;
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       }
       double bv2 = -1.0;
       int bi2 = 0x7fffffff;
-      if (hasrow && fr != imax) {
+      if (hasrow && fr < w && fr != imax) {
         bv2 = fabs(v1);
         if (!(bv2 == bv2)) bv2 = -1.0;
         bi2 = fr;
@@ -516,7 +521,9 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       const int pa = Mp[m], pj = phys(j);
       S1[idx] = pa >= pj ? AG(pa, pj) : AG(pj, pa);
     }
-    for (int64_t idx = gt; idx < (int64_t)nM * no; idx += GT) {
+    // rows below the diagonal block: only the trailing columns move (their
+    // panel columns hold L in logical order already)
+    for (int64_t idx = gt; idx < (int64_t)nMa * no; idx += GT) {
       const int m = (int)(idx / no);
       S2[idx] = AG(w + idx % no, Mp[m]);
     }
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       if (a >= j) AG(a, j) = S1[idx];
       else AG(j, a) = S1[idx];
     }
-    for (int64_t idx = gt; idx < (int64_t)nM * no; idx += GT) {
+    for (int64_t idx = gt; idx < (int64_t)nMa * no; idx += GT) {
       const int m = (int)(idx / no);
       AG(w + idx % no, Ml[m]) = S2[idx];
     }
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   if (hasrow) {
     for (int j = 0; j < ker; ++j) {
       AG(fr, k0 + j) = LCR(j);
-      Wp[(int64_t)(off + j) * w + fr] = w_entry([&](int c) { return LCR(c); }, j, dd, dsb);
+      if (fr < w) Wp[(int64_t)(off + j) * w + fr] = w_entry([&](int c) { return LCR(c); }, j, dd, dsb);
     }
   }
   for (int idx = gt; idx < nbe * ker; idx += GT) {

@@ -52,7 +52,8 @@ struct FactorLevel {
 // solve pieces: supernodes split into column blocks of at most SOLVE_W
 struct SolveLevel {
   int npieces = 0;
-  int nblocks = 0;       // total row blocks (256 rows) of all pieces
+  int maxpw = 1;         // widest piece of the level
+  int nblocks = 0;       // total row blocks (SOLVE_ROWS rows) of all pieces
   int* d_piece = nullptr;   // piece ids
   int* d_blk = nullptr;     // prefix of row blocks per piece
 };

@@ -220,19 +220,23 @@ static void setup(Handle& h) {
         } else {
           // cooperative panel over G workgroups, <= WIDE_ROWS far rows each
           const int nb = WIDE_NB;
-          const int G = std::max(1, (w - nb + WIDE_ROWS - 1) / WIDE_ROWS);
-          if (G > 256) throw std::runtime_error("supernode too wide for the cooperative panel kernel");
+          // far rows: the diagonal block below the panel and the off-diagonal rows
+          const int G = std::max(1, (int)((r - nb + WIDE_ROWS - 1) / WIDE_ROWS));
+          if (G > 256) throw std::runtime_error("supernode too tall for the cooperative panel kernel");
           const int64_t wcols = (int64_t)WIDE_SUB * nb;
-          wsum += (int64_t)w * (wcols + nb) + (int64_t)WIDE_MOVED * (w + r + wcols) + 64;
+          wsum += (int64_t)w * wcols + r * nb + (int64_t)WIDE_MOVED * (w + r + wcols) + 64;
           F.large.push_back(s);
           F.large_G.push_back(G);
           F.large_nb.push_back(nb);
         }
       }
       if (r > w) {
-        off.push_back(s);
-        offblk.push_back(blk);
-        blk += (int)((r - w + 63) / 64);
+        // the cooperative panel computes the off-diagonal rows of wide supernodes itself
+        if (w <= group_w) {
+          off.push_back(s);
+          offblk.push_back(blk);
+          blk += (int)((r - w + 63) / 64);
+        }
         if (r - w >= 256) {
           schb.push_back(s);
           tilesb.push_back(tlb);
@@ -320,7 +324,8 @@ static void setup(Handle& h) {
       const int64_t r = S.pi[s + 1] - S.pi[s];
       const int pw = std::min(SOLVE_W, w - pc0[p]);
       pre.push_back(nb);
-      nb += (int)((r - pc0[p] - pw + 255) / 256);
+      nb += (int)((r - pc0[p] - pw + SOLVE_ROWS - 1) / SOLVE_ROWS);
+      L.maxpw = std::max(L.maxpw, pw);
     }
     L.npieces = (int)pbylev[l].size();
     L.nblocks = nb;
@@ -384,6 +389,7 @@ void factor_device(Handle& h) {
     for (size_t q = 0; q < F.large.size(); ++q) {
       const int s = F.large[q];
       const int w = S.super[s + 1] - S.super[s];
+      const int64_t r = S.pi[s + 1] - S.pi[s];
       const int G = F.large_G[q], nb = F.large_nb[q];
       const int rounds = (w + nb - 2) / (nb - 1) + 1;
       const int wcols = WIDE_SUB * nb;
@@ -394,15 +400,16 @@ void factor_device(Handle& h) {
         const int sub = p % WIDE_SUB;
         const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
         if (sub > 0) {
-          const int Tr = (w - lo + 127) / 128;
+          const int Tr = (int)((r - lo + 127) / 128);
           if (Tr > 0)
             timed(D, "trailing", [&] {
               launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
             });
         }
         launch_fill_int(D.sync, 1, 0, st);
-        // far rows of this panel are at most w - p (nb - 1) - nb (each panel eliminates >= nb - 1 columns)
-        const int far = w - p * (nb - 1) - nb;
+        // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
+        // panel eliminates >= nb - 1 columns); the last panel may be narrower
+        const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
         const int Gp = std::min(G, std::max(1, (far + WIDE_ROWS - 1) / WIDE_ROWS));
         const uint32_t eb = D.ll_epoch;
         D.ll_epoch += 3u * (uint32_t)nb + 8u;
@@ -412,11 +419,11 @@ void factor_device(Handle& h) {
         });
         if (sub == WIDE_SUB - 1 || p == rounds - 1) {
           const int lo2 = std::min(w, (p + 1) * (nb - 1));
-          const int T = (w - lo2 + 127) / 128;
-          if (T > 0)
+          const int Tj = (w - lo2 + 127) / 128, Ti = (int)((r - lo2 + 127) / 128);
+          const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
+          if (Tj > 0)
             timed(D, "trailing", [&] {
-              launch_trailing_big(F.d_large + q, T * (T + 1) / 2, lo2, 0, wcols, a, D.valL, D.W, D.woff, D.kpos,
-                                  D.kstart, st);
+              launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
             });
         }
       }
@@ -488,7 +495,7 @@ void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
   for (size_t l = 0; l < D.slevels.size(); ++l) {
     const SolveLevel& L = D.slevels[l];
     timed(D, "fwd_tri", [&] {
-      launch_fwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.X, D.Y, nrhs, st);
+      launch_fwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, D.valL, D.X, D.Y, nrhs, st);
     });
     timed(D, "fwd_gemv", [&] {
       launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, D.Y, D.X,
@@ -504,7 +511,7 @@ void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
                       nrhs, st);
     });
     timed(D, "bwd_tri", [&] {
-      launch_bwd_tri(L.d_piece, L.npieces, D.piece_sn, D.piece_c0, a, D.valL, D.Z, D.acc, out, nrhs, st);
+      launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, D.valL, D.Z, D.acc, out, nrhs, st);
     });
   }
 }

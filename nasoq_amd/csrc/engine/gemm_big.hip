@@ -194,12 +194,14 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
   });
 }
 
-// A(i, j) -= sum_{l in [k0, k1)} L(i, l) W(j, l) for k1 <= j <= i < w, with
-// W(j, l) at Wp[(l - k0) w + j] (k0 = kstart, k1 = kpos).
-//   strip = 0: in place on the whole trailing block; 128 x 128 lower tiles
-//              cover [lo, w)^2, lo <= k1
+// A(i, j) -= sum_{l in [k0, k1)} L(i, l) W(j, l) for k1 <= j < w, j <= i < r
+// (the trailing diagonal block and the rows below it), with W(j, l) at
+// Wp[(l - k0) w + j] (k0 = kstart, k1 = kpos).
+//   strip = 0: in place; 128 x 128 tiles over rows [lo, r) and columns
+//              [lo, w), lo <= k1: the lower triangle of the square part, then
+//              the full rows below
 //   strip > 0: the next panel's columns j < k1 + strip (strip <= 128) only,
-//              written to the panel buffer Pp[(j - k1) w + i] (the matrix itself
+//              written to the panel buffer Pp[(j - k1) r + i] (the matrix itself
 //              stays without the block's updates until the block ends); one
 //              column of tiles starting at k1
 __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, int strip, int wcols, const int* super,
@@ -217,21 +219,29 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
     i0 = k1 + BT * (int)blockIdx.x;
     j0 = k1;
   } else {
+    const int Tj = (w - lo + BT - 1) / BT;
+    const int64_t ntri = (int64_t)Tj * (Tj + 1) / 2;
     int ti, tj;
-    tile_of(blockIdx.x, ti, tj);
+    if ((int64_t)blockIdx.x < ntri) {
+      tile_of(blockIdx.x, ti, tj);
+    } else {
+      const int64_t b = blockIdx.x - ntri;
+      ti = Tj + (int)(b / Tj);
+      tj = (int)(b % Tj);
+    }
     i0 = lo + BT * ti;
     j0 = lo + BT * tj;
   }
   const int jend = strip ? min(w, k1 + strip) : w;
-  if (i0 >= w || i0 + BT - 1 < k1 || j0 >= jend || j0 + BT - 1 < k1) return;
+  if (i0 >= r || i0 + BT - 1 < k1 || j0 >= jend || j0 + BT - 1 < k1) return;
   double* A = valL + px[s];
   const double* Wp = Wbase + woff[s];
-  double* Pp = Wbase + woff[s] + (int64_t)w * wcols;
+  double* Pp = Wbase + woff[s] + (int64_t)w * wcols;  // r x strip, pitch r
   const int kb = k1 - k0;
   BigAcc acc;
   big_gemm(
       acc, kb, As, Bs,
-      [&](int k, int x) -> double { return (k < kb && i0 + x < w) ? A[(int64_t)(k0 + k) * r + i0 + x] : 0.0; },
+      [&](int k, int x) -> double { return (k < kb && i0 + x < r) ? A[(int64_t)(k0 + k) * r + i0 + x] : 0.0; },
       [&](int k, int x) -> double { return (k < kb && j0 + x < w) ? Wp[(int64_t)k * w + j0 + x] : 0.0; });
   // read-modify-write in batches of 16 values per lane: the loads use a
   // safe address for masked elements so that they issue back to back
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
       for (int e = 0; e < 4; ++e) {
         const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
         const int q = 4 * b + e;
-        ok[q] = i < w && j < jend && j >= k1 && i >= j;
+        ok[q] = i < r && j < jend && j >= k1 && i >= j;
         off[q] = ok[q] ? (int64_t)j * r + i : 0;
         cur[q] = A[off[q]];
       }
@@ -259,7 +269,7 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
         for (int e = 0; e < 4; ++e) {
           const int q = 4 * b + e;
           const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
-          if (ok[q]) Pp[(int64_t)(j - k1) * w + i] = cur[q] - acc.c[a][b][e];
+          if (ok[q]) Pp[(int64_t)(j - k1) * r + i] = cur[q] - acc.c[a][b][e];
         }
     } else {
 #pragma unroll

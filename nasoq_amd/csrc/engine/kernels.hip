@@ -968,11 +968,20 @@ __device__ inline int row_label(const int* rows, int64_t pis, int c0, int w, int
   return q < w ? c0 + (int)q : rows[pis + q];
 }
 
+// Sweeps over pieces of <= SOLVE_W columns of a supernode.  Triangles: the
+// piece's unit lower triangle is staged in LDS (pitch lp >= pw + 1, dynamic)
+// and one wave substitutes with its 64 lanes holding rows (lane, lane + 64);
+// the solved entry of each step is broadcast with a lane shuffle.  Row
+// blocks below a piece: GROWS rows per workgroup, the four waves split the
+// piece's columns, partial sums meet in LDS.
+#define GROWS SOLVE_ROWS
+__device__ inline double bcast(double v, int lane) { return __shfl(v, lane, 64); }
+
 // forward triangle of a piece: Y(p) = L_pp^{-1} X(p)
 __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
                                                  const int* super, const int64_t* pi, const int64_t* px,
-                                                 const double* valL, const double* X, double* Y, int nrhs) {
-  __shared__ double xs[SOLVE_W * MAXRHS];
+                                                 const double* valL, const double* X, double* Y, int nrhs, int lp) {
+  extern __shared__ double Lt[];  // Lt[j * lp + i] = L(i, j), i > j
   const int p = pieces[blockIdx.x];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
@@ -980,16 +989,37 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
   const int64_t r = pi[s + 1] - pi[s];
   const double* P = valL + px[s];
   const int t = threadIdx.x;
-  for (int idx = t; idx < pw * nrhs; idx += 256) xs[idx] = X[(int64_t)(c0 + c0l) * nrhs + idx];
-  __syncthreads();
-  for (int j = 0; j < pw - 1; ++j) {
-    for (int idx = t; idx < (pw - j - 1) * nrhs; idx += 256) {
-      const int i = j + 1 + idx / nrhs, c = idx % nrhs;
-      xs[i * nrhs + c] -= P[(c0l + i) + (int64_t)(c0l + j) * r] * xs[j * nrhs + c];
-    }
-    __syncthreads();
+  for (int idx = t; idx < pw * pw; idx += 256) {
+    const int i = idx % pw, j = idx / pw;
+    if (i > j) Lt[j * lp + i] = P[(c0l + i) + (int64_t)(c0l + j) * r];
   }
-  for (int idx = t; idx < pw * nrhs; idx += 256) Y[(int64_t)(c0 + c0l) * nrhs + idx] = xs[idx];
+  __syncthreads();
+  if (t >= 64) return;
+  const int64_t base = (int64_t)(c0 + c0l) * nrhs;
+  double x0[MAXRHS], x1[MAXRHS];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) {
+    x0[c] = (c < nrhs && t < pw) ? X[base + (int64_t)t * nrhs + c] : 0.0;
+    x1[c] = (c < nrhs && t + 64 < pw) ? X[base + (int64_t)(t + 64) * nrhs + c] : 0.0;
+  }
+  for (int j = 0; j < pw - 1; ++j) {
+    const double* col = Lt + j * lp;
+    const double l0 = (t > j && t < pw) ? col[t] : 0.0;
+    const double l1 = (t + 64 > j && t + 64 < pw) ? col[t + 64] : 0.0;
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c) {
+      if (c >= nrhs) break;
+      const double xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
+      x0[c] -= l0 * xj;
+      x1[c] -= l1 * xj;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) {
+    if (c >= nrhs) break;
+    if (t < pw) Y[base + (int64_t)t * nrhs + c] = x0[c];
+    if (t + 64 < pw) Y[base + (int64_t)(t + 64) * nrhs + c] = x1[c];
+  }
 }
 // forward update of the rows below a piece: X(label q) -= L(q, p) Y(p)
 __global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* blk_prefix, int np,
@@ -997,6 +1027,7 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* 
                                                   const int64_t* pi, const int64_t* px, const int* rows,
                                                   const double* valL, const double* Y, double* X, int nrhs) {
   __shared__ double ys[SOLVE_W * MAXRHS];
+  __shared__ double part[4][GROWS * MAXRHS];
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1008,19 +1039,39 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* 
   const int pw = min(SOLVE_W, w - c0l);
   const int64_t r = pi[s + 1] - pi[s];
   const double* P = valL + px[s];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   for (int idx = t; idx < pw * nrhs; idx += 256) ys[idx] = Y[(int64_t)(c0 + c0l) * nrhs + idx];
   __syncthreads();
-  const int64_t q = c0l + pw + 256 * (int64_t)rb + t;
-  if (q >= r) return;
+  const int64_t q = c0l + pw + (int64_t)GROWS * rb + ln;
+  const bool ok = q < r;
   double acc[MAXRHS];
-  for (int c = 0; c < nrhs; ++c) acc[c] = 0.0;
-  for (int j = 0; j < pw; ++j) {
-    const double l = P[q + (int64_t)(c0l + j) * r];
-    for (int c = 0; c < nrhs; ++c) acc[c] += l * ys[j * nrhs + c];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) acc[c] = 0.0;
+  const int jb = wv * (SOLVE_W / 4), je = min(pw, jb + SOLVE_W / 4);
+  if (ok) {
+    for (int j = jb; j < je; j += 8) {
+      double l[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) l[u] = (j + u < je) ? P[q + (int64_t)(c0l + j + u) * r] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int c = 0; c < MAXRHS; ++c)
+          if (c < nrhs && j + u < je) acc[c] += l[u] * ys[(j + u) * nrhs + c];
+    }
   }
-  const int lab = row_label(rows, pi[s], c0, w, q);
-  for (int c = 0; c < nrhs; ++c) atomicAdd(X + (int64_t)lab * nrhs + c, -acc[c]);
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c)
+    if (c < nrhs) part[wv][ln * nrhs + c] = acc[c];
+  __syncthreads();
+  if (wv == 0 && ok) {
+    const int lab = row_label(rows, pi[s], c0, w, q);
+    for (int c = 0; c < nrhs; ++c) {
+      const double v = part[0][ln * nrhs + c] + part[1][ln * nrhs + c] + part[2][ln * nrhs + c] +
+                       part[3][ln * nrhs + c];
+      atomicAdd(X + (int64_t)lab * nrhs + c, -v);
+    }
+  }
 }
 // D solve: Z = D^{-1} Y (blocked_2by2_solver with n_rhs = 1)
 __global__ void k_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs) {
@@ -1040,13 +1091,17 @@ __global__ void k_dsolve(int n, const double* dval, const double* Y, double* Z, 
     }
   }
 }
-// backward gather: acc(p) += L(q, p)^T X(label q) over the rows below a piece
+// backward gather: acc(p) += L(q, p)^T X(label q) over the rows below a piece.
+// The GROWS x pw block is staged in LDS (coalesced along the rows), then
+// each thread sums a column over half of the rows.
 __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* blk_prefix, int np,
                                                   const int* piece_sn, const int* piece_c0, const int* super,
                                                   const int64_t* pi, const int64_t* px, const int* rows,
                                                   const double* valL, const double* Xf, double* acc,
                                                   int nrhs) {
-  __shared__ double xq[256 * MAXRHS];
+  __shared__ double Lb[SOLVE_W * (GROWS + 1)];  // Lb[j * (GROWS + 1) + qq]
+  __shared__ double xq[GROWS * MAXRHS];
+  __shared__ double part[SOLVE_W * MAXRHS];
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1059,33 +1114,46 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
   const int64_t r = pi[s + 1] - pi[s];
   const double* P = valL + px[s];
   const int t = threadIdx.x;
-  const int64_t qb = c0l + pw + 256 * (int64_t)rb;
-  const int nq = (int)min((int64_t)256, r - qb);
+  const int64_t qb = c0l + pw + (int64_t)GROWS * rb;
+  const int nq = (int)min((int64_t)GROWS, r - qb);
   if (t < nq) {
     const int lab = row_label(rows, pi[s], c0, w, qb + t);
     for (int c = 0; c < nrhs; ++c) xq[t * nrhs + c] = Xf[(int64_t)lab * nrhs + c];
+  } else if (t < GROWS) {
+    for (int c = 0; c < nrhs; ++c) xq[t * nrhs + c] = 0.0;  // rows past the panel: 0 * stale LDS could be NaN
+  }
+  for (int idx = t; idx < GROWS * pw; idx += 256) {
+    const int qq = idx % GROWS, j = idx / GROWS;
+    Lb[j * (GROWS + 1) + qq] = qq < nq ? P[qb + qq + (int64_t)(c0l + j) * r] : 0.0;
   }
   __syncthreads();
-  const int wv = t >> 6, ln = t & 63;
-  for (int j = wv; j < pw; j += 4) {
-    double a[MAXRHS];
-    for (int c = 0; c < nrhs; ++c) a[c] = 0.0;
-    for (int qq = ln; qq < nq; qq += 64) {
-      const double l = P[qb + qq + (int64_t)(c0l + j) * r];
-      for (int c = 0; c < nrhs; ++c) a[c] += l * xq[qq * nrhs + c];
-    }
-    for (int c = 0; c < nrhs; ++c) {
-      const double v = wave_sum(a[c]);
-      if (ln == 0) atomicAdd(acc + (int64_t)(c0 + c0l + j) * nrhs + c, v);
+  const int j = t & (SOLVE_W - 1), half = t >> 7;  // 2 x 128 threads: column j, rows of one half
+  double a[MAXRHS];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) a[c] = 0.0;
+  if (j < pw) {
+    const int q0 = half * (GROWS / 2);
+    for (int qq = q0; qq < q0 + GROWS / 2; ++qq) {
+      const double l = Lb[j * (GROWS + 1) + qq];
+#pragma unroll
+      for (int c = 0; c < MAXRHS; ++c)
+        if (c < nrhs) a[c] += l * xq[qq * nrhs + c];
     }
   }
+  if (half == 1)
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c)
+      if (c < nrhs) part[j * nrhs + c] = a[c];
+  __syncthreads();
+  if (half == 0 && j < pw)
+    for (int c = 0; c < nrhs; ++c) atomicAdd(acc + (int64_t)(c0 + c0l + j) * nrhs + c, a[c] + part[j * nrhs + c]);
 }
 // backward triangle: X(p) = L_pp^{-T} (Z(p) - acc(p))
 __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
                                                  const int* super, const int64_t* pi, const int64_t* px,
                                                  const double* valL, const double* Z, const double* acc,
-                                                 double* Xf, int nrhs) {
-  __shared__ double xs[SOLVE_W * MAXRHS];
+                                                 double* Xf, int nrhs, int lp) {
+  extern __shared__ double Lt[];  // Lt[i * lp + j] = L(j, i), j > i (column i of L)
   const int p = pieces[blockIdx.x];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
@@ -1093,24 +1161,55 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
   const int64_t r = pi[s + 1] - pi[s];
   const double* P = valL + px[s];
   const int t = threadIdx.x;
-  const int64_t base = (int64_t)(c0 + c0l) * nrhs;
-  for (int idx = t; idx < pw * nrhs; idx += 256) xs[idx] = Z[base + idx] - acc[base + idx];
-  __syncthreads();
-  for (int j = pw - 1; j > 0; --j) {
-    for (int idx = t; idx < j * nrhs; idx += 256) {
-      const int i = idx / nrhs, c = idx % nrhs;
-      xs[i * nrhs + c] -= P[(c0l + j) + (int64_t)(c0l + i) * r] * xs[j * nrhs + c];
-    }
-    __syncthreads();
+  for (int idx = t; idx < pw * pw; idx += 256) {
+    const int jj = idx % pw, i = idx / pw;
+    if (jj > i) Lt[i * lp + jj] = P[(c0l + jj) + (int64_t)(c0l + i) * r];
   }
-  for (int idx = t; idx < pw * nrhs; idx += 256) Xf[base + idx] = xs[idx];
+  __syncthreads();
+  if (t >= 64) return;
+  const int64_t base = (int64_t)(c0 + c0l) * nrhs;
+  double x0[MAXRHS], x1[MAXRHS];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) {
+    x0[c] = (c < nrhs && t < pw) ? Z[base + (int64_t)t * nrhs + c] - acc[base + (int64_t)t * nrhs + c] : 0.0;
+    x1[c] = (c < nrhs && t + 64 < pw) ? Z[base + (int64_t)(t + 64) * nrhs + c] - acc[base + (int64_t)(t + 64) * nrhs + c]
+                                      : 0.0;
+  }
+  for (int j = pw - 1; j > 0; --j) {
+    // rows i < j: x_i -= L(j, i) x_j
+    const double l0 = (t < j) ? Lt[t * lp + j] : 0.0;
+    const double l1 = (t + 64 < j) ? Lt[(t + 64) * lp + j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c) {
+      if (c >= nrhs) break;
+      const double xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
+      x0[c] -= l0 * xj;
+      x1[c] -= l1 * xj;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) {
+    if (c >= nrhs) break;
+    if (t < pw) Xf[base + (int64_t)t * nrhs + c] = x0[c];
+    if (t + 64 < pw) Xf[base + (int64_t)(t + 64) * nrhs + c] = x1[c];
+  }
 }
 
-void launch_fwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
                     const double* valL, const double* X, double* Y, int nrhs, hipStream_t st) {
   if (np <= 0) return;
   if (nrhs > MAXRHS) throw std::runtime_error("too many right-hand sides");
-  k_fwd_tri<<<np, 256, 0, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X, Y, nrhs);
+  if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("fwd_tri: piece width out of range");
+  const int lp = maxpw + 1;
+  static bool attr = false;
+  if (!attr) {  // the widest triangle (128 x 129 doubles) needs more than the default dynamic LDS
+    if (hipFuncSetAttribute((const void*)k_fwd_tri, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(double) * SOLVE_W * (SOLVE_W + 1))) != hipSuccess)
+      throw std::runtime_error("fwd_tri: cannot raise the dynamic LDS limit");
+    attr = true;
+  }
+  k_fwd_tri<<<np, 256, sizeof(double) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X,
+                                                          Y, nrhs, lp);
   HIP_LAUNCH_CHECK("fwd_tri");
 }
 void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
@@ -1133,11 +1232,21 @@ void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk,
                                    valL, Xf, acc, nrhs);
   HIP_LAUNCH_CHECK("bwd_gemv");
 }
-void launch_bwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+void launch_bwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
                     const double* valL, const double* Z, const double* acc, double* Xf, int nrhs,
                     hipStream_t st) {
   if (np <= 0) return;
-  k_bwd_tri<<<np, 256, 0, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z, acc, Xf, nrhs);
+  if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("bwd_tri: piece width out of range");
+  const int lp = maxpw + 1;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void*)k_bwd_tri, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(double) * SOLVE_W * (SOLVE_W + 1))) != hipSuccess)
+      throw std::runtime_error("bwd_tri: cannot raise the dynamic LDS limit");
+    attr = true;
+  }
+  k_bwd_tri<<<np, 256, sizeof(double) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z,
+                                                          acc, Xf, nrhs, lp);
   HIP_LAUNCH_CHECK("bwd_tri");
 }
 

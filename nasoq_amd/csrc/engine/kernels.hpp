@@ -57,7 +57,8 @@ void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_
                   double* valL, const double* dval, hipStream_t st);
 void launch_relabel(const SnArgs& a, int nsuper, const int* rows0, int* rows, const int* perm_piv,
                     int* newpos, const int* perm0, int* perm_f, int* iperm_f, hipStream_t st);
-void launch_fwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+constexpr int SOLVE_ROWS = 64;  // rows per workgroup in the sweeps below a piece
+void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
                     const double* valL, const double* X, double* Y, int nrhs, hipStream_t st);
 void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
                      const int* piece_c0, const SnArgs& a, const double* valL, const double* Y, double* X,
@@ -66,7 +67,7 @@ void launch_dsolve(int n, const double* dval, const double* Y, double* Z, int nr
 void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
                      const int* piece_c0, const SnArgs& a, const double* valL, const double* Xf, double* acc,
                      int nrhs, hipStream_t st);
-void launch_bwd_tri(const int* pieces, int np, const int* piece_sn, const int* piece_c0, const SnArgs& a,
+void launch_bwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
                     const double* valL, const double* Z, const double* acc, double* Xf, int nrhs,
                     hipStream_t st);
 void launch_spmv(int n, const int* rp, const int* ci, const double* v, const int* perm_f, const int* iperm_f,
