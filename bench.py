@@ -227,7 +227,10 @@ def main():
     out["inertia"] = list(s.inertia())
     a2, a3 = ctypes.c_int64(), ctypes.c_int64()
     _lib.check(lib.lblg_pivot_stats(s.h, ctypes.byref(a2), ctypes.byref(a3)), "pivot_stats")
-    out["wide_panel_pivoting"] = {"second_column_steps": a2.value, "interchanges": a3.value}
+    p1, p2 = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(lib.lblg_panel_stats(s.h, ctypes.byref(p1), ctypes.byref(p2)), "panel_stats")
+    out["wide_panel_pivoting"] = {"second_column_steps": a2.value, "interchanges": a3.value,
+                                  "panels": p1.value, "panels_with_pivot_search": p2.value}
     s.close()
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

@@ -173,6 +173,11 @@ int lblg_factor_info(const lblg_handle* h, int* zero_pivot, double* k0_norm_inf,
  * (|a_kk| < alpha * colmax) and row/column interchanges.  Engine
  * diagnostics; the reference has no counterpart. */
 int lblg_pivot_stats(const lblg_handle* h, int64_t* second_column_steps, int64_t* interchanges);
+/* Cooperative-panel counts of the last factorization (engine diagnostics; no
+ * reference counterpart): panels factored, and panels in which some column
+ * failed the 1x1 no-interchange test so that the step-by-step pivot search
+ * ran from that column on. */
+int lblg_panel_stats(const lblg_handle* h, int64_t* panels, int64_t* panels_pivoted);
 
 /* Per-kernel timing with HIP events recorded on the handle's stream (engine
  * diagnostics; no reference counterpart).  lblg_kernel_time returns the

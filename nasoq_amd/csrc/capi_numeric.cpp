@@ -199,6 +199,15 @@ extern "C" int lblg_pivot_stats(const lblg_handle* hh, int64_t* second_column_st
   return LBLG_OK;
 }
 
+extern "C" int lblg_panel_stats(const lblg_handle* hh, int64_t* panels, int64_t* panels_pivoted) {
+  const Handle* h = reinterpret_cast<const Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  if (panels) *panels = h->wide_panels;
+  if (panels_pivoted) *panels_pivoted = h->wide_panels_pivoted;
+  return LBLG_OK;
+}
+
 extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C) {
   if (!A || !B || !C) return LBLG_ERR_ARG;
   return guarded([&] {

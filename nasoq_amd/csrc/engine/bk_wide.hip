@@ -48,6 +48,7 @@ namespace lblg {
 #define WNB 64                         // widest panel
 #define WMM (2 * WNB)                  // moved indices per panel
 #define WLDA (WNB + 1)                 // padded row length of the diagonal rows
+#define OSB 16                         // columns per sub-block of the optimistic pass
 #define MBITS_WORDS (256 * WCH / 32)   // moved-far-row bitmap
 #define LL_SPIN (1L << 22)
 
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   __shared__ int redi[4];
   __shared__ double bcd[4];
   __shared__ int bci[4];
-  __shared__ int s_nm, s_abort, s_nM, s_nMa;
+  __shared__ int s_nm, s_abort, s_nM, s_nMa, s_fail;
 
   const int g = blockIdx.x, t = threadIdx.x, wv = t >> 6, ln = t & 63;
   const int s = list[0];
@@ -170,10 +171,136 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   if (t == 0) {
     s_nm = 0;
     s_abort = 0;
+    s_fail = nbe;
   }
   __syncthreads();
 
-  int k = k0, step = 0;
+  // ---- optimistic pass.  Bunch-Kaufman keeps the diagonal pivot of column
+  // k without interchange when |a_kk| >= alpha * max_{i>k, i<w} |a_ik| on the
+  // updated column.  Every workgroup runs all steps of the panel on that
+  // assumption with no exchange, and each thread notes the first step at
+  // which one of its candidate rows breaks the test.  One exchange then
+  // gives the first failing step J: steps [0, J) are the steps the pivoting
+  // loop below would take on the same values, and that loop resumes at J
+  // from the panel's original columns.
+  // The steps run in sub-blocks of OSB columns.  A sub-block's columns first
+  // receive the earlier sub-blocks' contribution as 16x16 FP64 MFMA tiles
+  // (far rows: a(i,j) -= sum_c L(i,c) W(j,c); diagonal rows:
+  // a(i,j) -= sum_c W(i,c) L(j,c)); inside the sub-block every wave computes
+  // the diagonal rows itself, so the steps need no block barrier.  During
+  // the pass the upper triangle of the diagonal rows holds
+  // W(i, j) = L(i, j) d_j (row j, column i) and wk holds 1 / d_j.
+  {
+    const bool drow = ln < nbe;
+    const int li = ln & 15, lk = ln >> 4;
+    int myfail = nbe;
+    for (int cb = 0; cb < nbe; cb += OSB) {
+      const int ce = min(nbe, cb + OSB);
+      if (cb > 0) {
+        __syncthreads();
+        const int nft = (fcnt + 15) >> 4;      // far-row tiles (rows < WCH)
+        const int ndt = (nbe - cb + 15) >> 4;  // diagonal-row tiles, rows [cb, 64)
+        for (int tile = wv; tile < nft + ndt; tile += 4) {
+          const bool isfar = tile < nft;
+          const int i0 = isfar ? tile * 16 : cb + (tile - nft) * 16;
+          d4 acc = {0.0, 0.0, 0.0, 0.0};
+          for (int c = 0; c < cb; c += 4) {
+            const int cc = c + lk;
+            // first operand row = column j of the sub-block, second operand col = row i
+            const double x = isfar ? ADR(cc, cb + li) : ADR(cc, cb + li) * wk[cc];
+            const double y = isfar ? LC[cc * WCH + i0 + li] : ADR(cc, i0 + li);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
+          }
+          // acc[e] = C(j = cb + 4e + lk, i = i0 + li)
+          const int i = i0 + li;
+          for (int e = 0; e < 4; ++e) {
+            const int j = cb + 4 * e + lk;
+            if (j < ce) {
+              if (isfar) {
+                if (i < fcnt) LC[j * WCH + i] -= acc[e];
+              } else if (i < nbe && i >= j) {
+                ADR(i, j) -= acc[e];
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      for (int kr = cb; kr < ce; ++kr) {
+        double vd = 0.0;
+        if (drow && ln >= kr) {
+          double v = ADR(ln, kr);
+          for (int c = cb; c < kr; ++c) v -= (ADR(c, ln) * wk[c]) * ADR(c, kr);
+          vd = v;
+        }
+        double v0 = 0.0;
+        if (hasrow) {
+          double v = LCR(kr);
+          for (int c = cb; c < kr; ++c) v -= LCR(c) * ADR(c, kr);
+          v0 = v;
+        }
+        const double d = __shfl(vd, kr, 64);
+        const double absakk = fabs(d);
+        if (!(absakk > 0.0)) myfail = min(myfail, kr);
+        if (wv == 3 && drow && ln > kr) {
+          const double x = fabs(vd);
+          if (x == x && !(absakk >= BK_ALPHA * x)) myfail = min(myfail, kr);
+        }
+        if (hasrow && fr < w) {
+          const double x = fabs(v0);
+          if (x == x && !(absakk >= BK_ALPHA * x)) myfail = min(myfail, kr);
+        }
+        const double rk = 1.0 / d;
+        if (hasrow) LCR(kr) = v0 * rk;
+        if (drow && ln > kr) ADR(kr, ln) = vd;  // every wave stores the same value
+        if (ln == 0) {
+          wk[kr] = rk;
+          dd[kr] = d;
+        }
+        // LDS accesses of one wave complete in order; keep the compiler from
+        // moving the next step's loads above these stores
+        __asm__ volatile("" ::: "memory");
+      }
+    }
+    if (myfail < nbe) atomicMin(&s_fail, myfail);
+    __syncthreads();
+    if (wv == 0) {
+      const uint32_t fO = ebase + 3u * (uint32_t)nb + 1u;
+      int abort = 0;
+      if (ln == 0) ll_store(ll + LL_RECA + (256 + (int64_t)g) * 4, (uint32_t)s_fail, fO);
+      int jm = nbe;
+      for (int q = ln; q < G; q += 64) jm = min(jm, (int)ll_load(ll + LL_RECA + (256 + (int64_t)q) * 4, fO, abort));
+      for (int o = 32; o > 0; o >>= 1) jm = min(jm, __shfl_xor(jm, o, 64));
+      const bool any_abort = __ballot(abort) != 0;
+      if (ln == 0) {
+        bci[2] = any_abort ? 0 : jm;
+        if (any_abort) s_abort = 1;
+      }
+    }
+    __syncthreads();
+  }
+  const int J = bci[2];
+  // back to the pivoting loop's layout: L below the diagonal for the
+  // accepted columns, the original columns from J on
+  if (J < nbe && hasrow)
+    for (int j = J; j < nbe; ++j) LCR(j) = pcol(fr, j);
+  for (int idx = t; idx < nbe * nbe; idx += 256) {
+    const int i = idx % nbe, j = idx / nbe;
+    if (min(i, j) >= J) ADR(i, j) = i >= j ? pcol(k0 + i, j) : pcol(k0 + j, i);
+    else if (i > j) ADR(i, j) = ADR(j, i) * wk[j];
+    else if (i == j) ADR(i, i) = 1.0;
+  }
+  if (g == 0 && t < J) {
+    dval[c0 + k0 + t] = dd[t];
+    dval[n + c0 + k0 + t] = 0.0;
+  }
+  if (g == 0 && t == 0) {
+    atomicAdd(status + 4, 1);
+    if (J < nbe) atomicAdd(status + 5, 1);
+  }
+  __syncthreads();
+
+  int k = k0 + J, step = 0;
   while (k < w && (last || k - k0 < nb - 1)) {
     const int kr = k - k0, par = step & 1;
     const uint32_t fA = ebase + 3u * (uint32_t)step + 1u, fB = fA + 1u;

@@ -449,13 +449,15 @@ void factor_device(Handle& h) {
   launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
   // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
   launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
-  int stat[4] = {0, 0, 0, 0};
+  int stat[6] = {0, 0, 0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(stat), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   h.zero_pivot = stat[0] & 1;
   h.second_col_steps = stat[2];
   h.interchanges = stat[3];
+  h.wide_panels = stat[4];
+  h.wide_panels_pivoted = stat[5];
   if (stat[0] & 4) throw std::runtime_error("cross-workgroup exchange timed out");
   if (stat[0] & 8) throw std::runtime_error("wide panel launch shape does not fit the supernode");
   h.factored = true;

@@ -28,6 +28,8 @@ struct Handle {
   int zero_pivot = 0;
   int64_t second_col_steps = 0;  // wide panels: pivot steps that inspected a second column
   int64_t interchanges = 0;      // wide panels: row/column interchanges
+  int64_t wide_panels = 0;       // wide panels factored
+  int64_t wide_panels_pivoted = 0;  // wide panels whose 1x1 no-interchange pass stopped early
   double factor_seconds = 0.0;
   std::unique_ptr<DeviceFactor> dev;
   Handle();
