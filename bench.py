@@ -37,12 +37,15 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
-def schur_flops(sup, pi):
-    """Algorithmic flops of all Schur-complement updates: sum_s m_s (m_s + 1) w_s."""
+def schur_flops(sup, pi, big=None):
+    """Algorithmic flops of the Schur-complement updates: sum_s m_s (m_s + 1) w_s
+    over all supernodes, or over those with m_s >= 256 (big=True: the 128-row
+    tile kernel) / m_s < 256 (big=False: the 64-row tile kernel)."""
     w = np.diff(sup).astype(np.float64)
     r = np.diff(pi).astype(np.float64)
     m = r - w
-    return float(np.sum(m * (m + 1.0) * w))
+    sel = np.ones_like(m, dtype=bool) if big is None else ((m >= 256) if big else (m < 256))
+    return float(np.sum((m * (m + 1.0) * w)[sel]))
 
 
 def trailing_flops(sup, small_w=64):
@@ -50,6 +53,22 @@ def trailing_flops(sup, small_w=64):
     w = np.diff(sup).astype(np.float64)
     w = w[w > small_w]
     return float(np.sum(w ** 3 / 3.0))
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` at the 64^3 KKT, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/kkt64_hbm_traffic.txt,
+    a line 'traffic_bytes_per_launch <kernel> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", "kkt64_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for ln in f:
+                fld = ln.split()
+                if len(fld) == 3 and fld[0] == "traffic_bytes_per_launch" and fld[1] == kernel:
+                    return float(fld[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(config, full_fl):
@@ -178,24 +197,26 @@ def main():
     ms_step = dt / args.steps * 1e3
 
     kt = {}
-    for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "scale_ld", "schur", "fwd_tri", "fwd_gemv",
-                 "bwd_gemv", "bwd_tri", "spmv"]:
+    for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "schur", "scale_ld", "schur_big",
+                 "fwd_tri", "fwd_gemv", "bwd_gemv", "bwd_tri", "spmv"]:
         tot, cnt = ctypes.c_double(), ctypes.c_int64()
         lib.lblg_kernel_time(s.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
         kt[name] = (tot.value / args.steps, cnt.value // max(args.steps, 1))
     print("[bench] per-step kernel ms: " + ", ".join(f"{k}={v[0]:.2f}({v[1]})" for k, v in kt.items()),
           file=sys.stderr, flush=True)
 
-    # roofline of the dominant MFMA kernel (Schur-complement updates)
-    sf = schur_flops(sup, pi)
+    # roofline of the dominant MFMA kernel (128-row-tile Schur-complement
+    # updates, or the trailing updates of the wide diagonal blocks)
+    sf = schur_flops(sup, pi, big=True)
     tf = trailing_flops(sup)
-    dom = "schur" if kt["schur"][0] >= kt["trailing"][0] else "trailing"
-    work = sf if dom == "schur" else tf
+    dom = "schur_big" if kt["schur_big"][0] >= kt["trailing"][0] else "trailing"
+    work = sf if dom == "schur_big" else tf
     launches = max(kt[dom][1], 1)
     avg_ms = kt[dom][0] / launches
     achieved = (work / launches) / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     roof = {"bound": "mfma", "kernel": "k_" + dom, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
+            "traffic": committed_traffic("k_" + dom) if args.config == "kkt64" else None,
             "avg_launch_ms": avg_ms, "launches_per_step": launches,
             "algorithmic_flops_per_launch": work / launches}
 

@@ -182,8 +182,9 @@ int lblg_panel_stats(const lblg_handle* h, int64_t* panels, int64_t* panels_pivo
 /* Per-kernel timing with HIP events recorded on the handle's stream (engine
  * diagnostics; no reference counterpart).  lblg_kernel_time returns the
  * accumulated milliseconds and launch count of a kernel family since timing
- * was enabled: bk_small, bk_panel, bk_panel_group, trailing, offdiag, schur,
- * fwd_tri, fwd_gemv, bwd_gemv, bwd_tri, spmv. */
+ * was enabled: bk_small, bk_panel, bk_panel_group, trailing, offdiag, schur
+ * (64-row tiles), scale_ld, schur_big (128-row tiles), fwd_tri, fwd_gemv,
+ * bwd_gemv, bwd_tri, spmv. */
 int lblg_set_kernel_timing(lblg_handle* h, int on);
 int lblg_kernel_time(lblg_handle* h, const char* name, double* total_ms, int64_t* launches);
 

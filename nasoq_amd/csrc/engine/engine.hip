@@ -440,7 +440,7 @@ void factor_device(Handle& h) {
         launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, D.valL, D.dval,
                         D.LD, F.d_schur_big_ldoff, st);
       });
-      timed(D, "schur", [&] {
+      timed(D, "schur_big", [&] {
         launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, D.valL, D.LD,
                          F.d_schur_big_ldoff, st);
       });
