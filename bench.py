@@ -111,6 +111,46 @@ def cpu_baseline(config, full_fl):
     }
 
 
+def mixed_leg(s, lib, step, bwd, dX, x64, args, torch):
+    """The same inputs with the FP32 factor (FP32 MFMA updates, FP32 sweeps)
+    and FP64 refinement (residual, GMRES, solution update) to the same 1e-10
+    backward error; forward error per right-hand side against the FP64
+    solution of the plain path (itself refined to 1e-10)."""
+    from nasoq_amd import _lib
+    s.precision = 1
+    s.numerical_factorization()  # re-creates the value buffers in FP32
+    for _ in range(args.warmup):
+        step()
+    _lib.check(lib.lblg_synchronize(s.h), "sync")
+    _lib.check(lib.lblg_set_kernel_timing(s.h, 1), "timing")  # the FP32 factor has its own device state
+    torch.cuda.synchronize()
+    worst, rounds = 0.0, []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        worst = max(worst, float(bwd.max()))
+        rounds.append(int(lib.lblg_num_ref_iter(s.h)))
+    _lib.check(lib.lblg_synchronize(s.h), "sync")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert worst <= 1e-10, f"FP32-factor backward error {worst} above 1e-10"
+    x32 = dX.cpu().numpy()
+    fwd = [float(np.linalg.norm(x32[c] - x64[c]) / max(np.linalg.norm(x64[c]), 1e-300)) for c in range(x64.shape[0])]
+    kt = {}
+    for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "schur", "scale_ld", "schur_big",
+                 "fwd_tri", "fwd_gemv", "bwd_gemv", "bwd_tri", "spmv"]:
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        lib.lblg_kernel_time(s.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
+        kt[name] = tot.value / args.steps
+    print(f"[bench] mixed FP32 factor: {dt / args.steps * 1e3:.1f} ms/step rounds={rounds} worst={worst:.2e} "
+          f"fwd={max(fwd):.2e}", file=sys.stderr, flush=True)
+    s.precision = 0
+    return {"variant": "FP32 factor (FP32 MFMA updates, FP32 sweeps) + FP64 GMRES refinement",
+            "value": args.steps / dt, "unit": "KKT solves/s", "ms_per_step": dt / args.steps * 1e3,
+            "dtype": "f32 factor / f64 refinement", "worst_backward_error": worst, "refine_rounds": rounds,
+            "forward_error_vs_fp64_per_rhs": fwd, "kernel_ms_per_step": kt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +158,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor + FP64-refinement leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference traffic for HBM counters)")
     args = ap.parse_args()
@@ -246,12 +287,15 @@ def main():
     _lib.check(lib.lblg_get_factor(s.h, None, _lib.dptr(dval), None, None), "get_factor")
     out["pivots_2x2"] = int(np.count_nonzero(dval[n:]))
     out["inertia"] = list(s.inertia())
+    x64 = dX.cpu().numpy().copy()
     a2, a3 = ctypes.c_int64(), ctypes.c_int64()
     _lib.check(lib.lblg_pivot_stats(s.h, ctypes.byref(a2), ctypes.byref(a3)), "pivot_stats")
     p1, p2 = ctypes.c_int64(), ctypes.c_int64()
     _lib.check(lib.lblg_panel_stats(s.h, ctypes.byref(p1), ctypes.byref(p2)), "panel_stats")
     out["wide_panel_pivoting"] = {"second_column_steps": a2.value, "interchanges": a3.value,
                                   "panels": p1.value, "panels_with_pivot_search": p2.value}
+    if not args.no_mixed:
+        out["mixed"] = mixed_leg(s, lib, step, bwd, dX, x64, args, torch)
     s.close()
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

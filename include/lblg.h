@@ -55,7 +55,9 @@ typedef struct lblg_settings {
   int n_relax[3];         /* default {4,16,48} */
   double z_relax[3];      /* default {0.8,0.1,0.05} */
   int ordering;           /* LBLG_ORDER_* */
-  int precision;          /* LBLG_F64 / LBLG_F32 / LBLG_BF16 factor storage and MFMA type */
+  int precision;          /* factor storage and MFMA type: LBLG_F64 (plain), LBLG_F32
+                             (FP32 factor and sweeps; residual, GMRES and the
+                             solution update stay FP64); LBLG_BF16 is rejected */
   double target_bwd_err;  /* >0: keep refining until the backward error is <= target
                              (at most max_refine_rounds GMRES cycles); 0: the reference's
                              fixed GMRES budget */
@@ -188,9 +190,11 @@ int lblg_panel_stats(const lblg_handle* h, int64_t* panels, int64_t* panels_pivo
 int lblg_set_kernel_timing(lblg_handle* h, int on);
 int lblg_kernel_time(lblg_handle* h, const char* name, double* total_ms, int64_t* launches);
 
-/* Device self-test of the FP64 MFMA operand layout: C(16x16) = A(16x4) B(4x16),
- * all row-major.  No reference counterpart (engine diagnostics). */
-int lblg_selftest_mfma(const double* A, const double* B, double* C);
+/* Device self-test of the 16x16x4 MFMA lane layout the engine assumes, for
+ * the FP64 (precision LBLG_F64) or FP32 (LBLG_F32) instruction:
+ * C(16x16) = A(16x4) B(4x16), all row-major (inputs rounded to the type).
+ * No reference counterpart (engine diagnostics). */
+int lblg_selftest_mfma(const double* A, const double* B, double* C, int precision);
 
 /* ---------------------------------------------------------------------------
  * NASOQ QP solver over the engine:  min 1/2 x'Hx + q'x  s.t.  Ax = a, Bx <= b

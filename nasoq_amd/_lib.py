@@ -102,7 +102,7 @@ SIGNATURES = [
     ("lblg_factor_info", ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
     ("lblg_pivot_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
     ("lblg_panel_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
-    ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p]),
+    ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p, ctypes.c_int]),
     ("lblg_set_kernel_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("lblg_kernel_time", ctypes.c_int, [_vp, ctypes.c_char_p, _f64p, _i64p]),
     ("lblg_qp_default_settings", None, [ctypes.POINTER(QPSettings)]),

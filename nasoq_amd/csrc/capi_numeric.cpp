@@ -62,7 +62,7 @@ extern "C" int lblg_factor(lblg_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return LBLG_ERR_ARG;
   if (!h->analyzed) return LBLG_ERR_STATE;
-  if (h->set.precision != LBLG_F64) return LBLG_ERR_ARG;
+  if (h->set.precision != LBLG_F64 && h->set.precision != LBLG_F32) return LBLG_ERR_ARG;
   return guarded([&] {
     auto t0 = std::chrono::steady_clock::now();
     lblg::factor(*h);
@@ -135,8 +135,21 @@ extern "C" int lblg_get_factor(lblg_handle* hh, double* valL, double* d_val, int
     lblg::DeviceFactor& D = *h->dev;
     hipStream_t st = D.stream;
     const lblg::Symbolic& S = h->sym;
-    if (valL) hipMemcpyAsync(valL, D.valL, sizeof(double) * S.p[h->n], hipMemcpyDeviceToHost, st);
-    if (d_val) hipMemcpyAsync(d_val, D.dval, sizeof(double) * 2 * h->n, hipMemcpyDeviceToHost, st);
+    // values of a float factor are widened on the host
+    auto fetch = [&](double* dst, const void* src, int64_t count) {
+      if (D.prec == 0) {
+        if (hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToHost, st) != hipSuccess)
+          throw std::runtime_error("copy failed");
+        return;
+      }
+      std::vector<float> tmp(count);
+      if (hipMemcpyAsync(tmp.data(), src, sizeof(float) * count, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        throw std::runtime_error("copy failed");
+      for (int64_t i = 0; i < count; ++i) dst[i] = tmp[i];
+    };
+    if (valL) fetch(valL, D.valL, S.p[h->n]);
+    if (d_val) fetch(d_val, D.dval, 2 * (int64_t)h->n);
     if (rows) hipMemcpyAsync(rows, D.rows, sizeof(int) * S.pi[S.nsuper], hipMemcpyDeviceToHost, st);
     if (perm_final) hipMemcpyAsync(perm_final, D.perm_f, sizeof(int) * h->n, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("copy failed");
@@ -208,8 +221,8 @@ extern "C" int lblg_panel_stats(const lblg_handle* hh, int64_t* panels, int64_t*
   return LBLG_OK;
 }
 
-extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C) {
-  if (!A || !B || !C) return LBLG_ERR_ARG;
+extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C, int precision) {
+  if (!A || !B || !C || (precision != LBLG_F64 && precision != LBLG_F32)) return LBLG_ERR_ARG;
   return guarded([&] {
     double *dA, *dB, *dC;
     if (hipMalloc(&dA, 64 * 8) != hipSuccess || hipMalloc(&dB, 64 * 8) != hipSuccess ||
@@ -217,7 +230,7 @@ extern "C" int lblg_selftest_mfma(const double* A, const double* B, double* C) {
       throw std::runtime_error("alloc");
     hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice);
     hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice);
-    lblg::launch_mfma_selftest(dA, dB, dC, nullptr);
+    lblg::launch_mfma_selftest(dA, dB, dC, precision == LBLG_F32, nullptr);
     hipMemcpy(C, dC, 256 * 8, hipMemcpyDeviceToHost);
     hipFree(dA);
     hipFree(dB);

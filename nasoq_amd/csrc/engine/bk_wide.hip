@@ -84,8 +84,8 @@ __device__ inline double ll_load_d(uint64_t* p, uint32_t flag, int& abort) {
 }
 
 // W(i, j) = sum_c L(i, c) D(c, j) for an eliminated panel column j; l(j) reads L(i, j)
-template <typename F>
-__device__ inline double w_entry(F l, int j, const double* dd, const double* dsb) {
+template <typename F, typename T>
+__device__ inline double w_entry(F l, int j, const T* dd, const T* dsb) {
   if (dsb[j] != 0.0) return l(j) * dd[j] + l(j + 1) * dsb[j];
   if (j > 0 && dsb[j - 1] != 0.0) return l(j - 1) * dsb[j - 1] + l(j) * dd[j];
   return l(j) * dd[j];
@@ -100,14 +100,15 @@ __device__ inline int far_phys(int a, int fstart, const unsigned* mbits, const i
   return a;
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb, int sub, int wcols, const int* super,
-                                                 const int64_t* pi, const int64_t* px, double* valL,
-                                                 double* dval, int* perm_piv, double* Wbase,
+                                                 const int64_t* pi, const int64_t* px, T* valL,
+                                                 T* dval, int* perm_piv, T* Wbase,
                                                  const int64_t* woff, int* kpos, int* kstart, int* sync,
                                                  uint64_t* ll, uint32_t ebase, int n, int* status) {
-  __shared__ double LC[WNB * WCH];       // own far rows, column-major [column][row]
-  __shared__ double Ad[WNB * WLDA];      // diagonal rows: L (eliminated) | symmetric A (pending)
-  __shared__ double wk[WNB], wi[WNB], td0[WNB], td1[WNB], dd[WNB], dsb[WNB], prow[WNB];
+  __shared__ T LC[WNB * WCH];       // own far rows, column-major [column][row]
+  __shared__ T Ad[WNB * WLDA];      // diagonal rows: L (eliminated) | symmetric A (pending)
+  __shared__ T wk[WNB], wi[WNB], td0[WNB], td1[WNB], dd[WNB], dsb[WNB], prow[WNB];
   __shared__ int dphys[WNB];
   __shared__ int mlog[WMM], mphys[WMM], Ml[WMM], Mp[WMM];
   __shared__ unsigned mbits[MBITS_WORDS];
@@ -121,10 +122,10 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   const int s = list[0];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
-  double* A = valL + px[s];
-  double* Wp = Wbase + woff[s];
-  const double* Pp = Wp + (int64_t)w * wcols;  // this panel's columns with the block's earlier panels applied
-  double* scr = Wp + (int64_t)w * wcols + (int64_t)r * nb;
+  T* A = valL + px[s];
+  T* Wp = Wbase + woff[s];
+  const T* Pp = Wp + (int64_t)w * wcols;  // this panel's columns with the block's earlier panels applied
+  T* scr = Wp + (int64_t)w * wcols + (int64_t)r * nb;
   const int k0 = kpos[s];
   if (k0 >= w) return;
   // sub > 0: this panel continues the block of panels that started at K0.
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 
   // ---- load the panel rows
   // panel column j, row i >= j (with the block's earlier panels applied)
-  auto pcol = [&](int i, int j) -> double { return sub ? Pp[(int64_t)j * r + i] : AG(i, k0 + j); };
+  auto pcol = [&](int i, int j) -> T { return sub ? Pp[(int64_t)j * r + i] : AG(i, k0 + j); };
   if (hasrow)
     for (int j = 0; j < nbe; ++j) LCR(j) = pcol(fr, j);
   for (int idx = t; idx < nbe * nbe; idx += 256) {
@@ -203,18 +204,18 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
         for (int tile = wv; tile < nft + ndt; tile += 4) {
           const bool isfar = tile < nft;
           const int i0 = isfar ? tile * 16 : cb + (tile - nft) * 16;
-          d4 acc = {0.0, 0.0, 0.0, 0.0};
+          typename Mfma<T>::acc acc = {0, 0, 0, 0};
           for (int c = 0; c < cb; c += 4) {
             const int cc = c + lk;
             // first operand row = column j of the sub-block, second operand col = row i
-            const double x = isfar ? ADR(cc, cb + li) : ADR(cc, cb + li) * wk[cc];
-            const double y = isfar ? LC[cc * WCH + i0 + li] : ADR(cc, i0 + li);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
+            const T x = isfar ? ADR(cc, cb + li) : ADR(cc, cb + li) * wk[cc];
+            const T y = isfar ? LC[cc * WCH + i0 + li] : ADR(cc, i0 + li);
+            acc = Mfma<T>::mma(x, y, acc);
           }
-          // acc[e] = C(j = cb + 4e + lk, i = i0 + li)
+          // acc[e] = C(j = cb + orow(e), i = i0 + li)
           const int i = i0 + li;
           for (int e = 0; e < 4; ++e) {
-            const int j = cb + 4 * e + lk;
+            const int j = cb + Mfma<T>::orow(e, ln);
             if (j < ce) {
               if (isfar) {
                 if (i < fcnt) LC[j * WCH + i] -= acc[e];
@@ -632,10 +633,10 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   const int GT = G * 256, gt = g * 256 + t;
   const int nt = w - kend;                 // trailing block size
   const int64_t no = r - w;                // rows below the diagonal block
-  double* S1 = scr;                                  // [nMa][nt]
-  double* S2 = scr + (int64_t)WMM * w;               // [nM][no]
-  double* S3 = S2 + (int64_t)WMM * (no > 0 ? no : 0);  // [nM][k0]
-  double* S4 = S3 + (int64_t)WMM * w;                 // [nM][off] rows of W of the block's earlier panels
+  T* S1 = scr;                                  // [nMa][nt]
+  T* S2 = scr + (int64_t)WMM * w;               // [nM][no]
+  T* S3 = S2 + (int64_t)WMM * (no > 0 ? no : 0);  // [nM][k0]
+  T* S4 = S3 + (int64_t)WMM * w;                 // [nM][off] rows of W of the block's earlier panels
   auto phys = [&](int j) -> int {
     if (j < k0 + nbe) return dphys[j - k0];
     return far_phys(j, fstart, mbits, mlog, mphys, s_nm);
@@ -703,8 +704,9 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 #undef ADR
 }
 
-void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, double* valL, double* dval,
-                    int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+template <typename T>
+void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
+                    int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                     uint64_t* ll, uint32_t ebase, int* status, hipStream_t st) {
   if (nb > WNB || nb < 4 || G < 1 || G > 256) throw std::runtime_error("bk_wide: launch shape out of range");
   int n = a.n;
@@ -712,8 +714,12 @@ void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const Sn
                   (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
                   (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&ll, (void*)&ebase, (void*)&n,
                   (void*)&status};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_wide, dim3(G), dim3(256), args, 0, st);
+  hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_wide<T>, dim3(G), dim3(256), args, 0, st);
   if (e != hipSuccess) throw std::runtime_error(std::string("bk_wide: ") + hipGetErrorString(e));
 }
+template void launch_bk_wide<double>(const int*, int, int, int, int, const SnArgs&, double*, double*, int*, double*,
+                                     const int64_t*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
+template void launch_bk_wide<float>(const int*, int, int, int, int, const SnArgs&, float*, float*, int*, float*,
+                                    const int64_t*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
 
 }  // namespace lblg

@@ -68,9 +68,12 @@ struct DeviceFactor {
   int* rows = nullptr;     // row labels (ssize), relabelled after factor
   int* rows0 = nullptr;    // pre-pivot row labels (kept for re-factorization)
   int* col2sup = nullptr;
-  // numeric
-  double* valL = nullptr;
-  double* dval = nullptr;
+  // numeric: the factor's value type is fixed at setup (prec 0: double,
+  // 1: float); valL, dval, W, LD and the sweep vectors X, Y, Z, acc, O hold
+  // that type
+  int prec = 0;
+  void* valL = nullptr;
+  void* dval = nullptr;
   int* perm_piv = nullptr;   // local: new position -> old position (global indices)
   int* newpos = nullptr;     // inverse of perm_piv
   int* perm_f = nullptr;     // final ordering: ordered index -> original index
@@ -87,8 +90,8 @@ struct DeviceFactor {
   double* Kv = nullptr;
   int64_t* Ksrc = nullptr;   // CSR entry -> lower CSC entry (to refresh values)
   // workspace
-  double* W = nullptr;          // BK panel workspace (max over concurrent supernodes)
-  double* LD = nullptr;         // L_off D of the large Schur updates of one level
+  void* W = nullptr;            // BK panel workspace (max over concurrent supernodes)
+  void* LD = nullptr;           // L_off D of the large Schur updates of one level
   int64_t W_elems = 0;
   int64_t* woff = nullptr;      // per-supernode offset into W
   int* kpos = nullptr;          // next column to factor (wide supernodes)
@@ -102,10 +105,11 @@ struct DeviceFactor {
   double* red = nullptr;        // reduction scratch
   int* status = nullptr;        // device flags (zero pivots etc.)
   // solve
-  double* X = nullptr;          // n x nrhs working vectors (row-interleaved)
-  double* Y = nullptr;
-  double* Z = nullptr;
-  double* acc = nullptr;
+  void* X = nullptr;            // n x nrhs sweep vectors in the factor's type (row-interleaved)
+  void* Y = nullptr;
+  void* Z = nullptr;
+  void* acc = nullptr;
+  void* O = nullptr;            // sweep result (float factor; double writes the caller's block)
   double* acc2 = nullptr;       // ordered solution block
   double* T = nullptr;
   double* Xs = nullptr;

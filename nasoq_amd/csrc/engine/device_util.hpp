@@ -15,9 +15,40 @@ namespace lblg {
   } while (0)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 #define BK_ALPHA 0.6403882032022076  // (1 + sqrt(17)) / 8
-#define SFMIN 2.2250738585072014e-308
+
+// --------------------------------------------------------------------------
+// The factor is stored and computed in one value type T (double: the plain
+// FP64 factor; float: the FP32 factor that GMRES refines in FP64).  Mfma<T>
+// is the 16x16x4 matrix instruction of T: v_mfma_f64_16x16x4f64 or
+// v_mfma_f32_16x16x4f32.  Operand lanes are the same for both (first
+// operand: row = lane % 16, k = lane / 16; second operand: column =
+// lane % 16, k = lane / 16); accumulator element e of a lane holds output
+// column lane % 16 and output row orow(e, lane), which differs between the
+// two instructions (checked by lblg_selftest_mfma).
+// --------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef d4 acc;
+  static __device__ inline d4 mma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int orow(int e, int ln) { return 4 * e + (ln >> 4); }
+  static __device__ inline double sfmin() { return 2.2250738585072014e-308; }  // smallest normal
+};
+template <>
+struct Mfma<float> {
+  typedef f4 acc;
+  static __device__ inline f4 mma(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int orow(int e, int ln) { return 4 * (ln >> 4) + e; }
+  static __device__ inline float sfmin() { return 1.17549435e-38f; }
+};
 
 // --------------------------------------------------------------------------
 // reductions with LAPACK idamax tie-breaking (first index of the maximum)

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "engine/device.hpp"
 #include "engine/handle.hpp"
@@ -83,6 +84,11 @@ static T* dupload(DeviceFactor& D, const std::vector<T>& v) {
   return p;
 }
 
+// an array of the factor's value type
+static void* valloc(DeviceFactor& D, int64_t count) {
+  return D.prec == 1 ? (void*)dalloc<float>(D, count) : (void*)dalloc<double>(D, count);
+}
+
 static SnArgs snargs(const DeviceFactor& D) {
   return SnArgs{D.super, D.pi, D.px, D.rows, D.col2sup, D.n};
 }
@@ -117,8 +123,9 @@ static void setup(Handle& h) {
   D.rows = dalloc<int>(D, (int64_t)rows0.size());
   D.col2sup = dupload(D, S.col2sup);
   D.perm0 = dupload(D, S.perm);
-  D.valL = dalloc<double>(D, S.p[n]);
-  D.dval = dalloc<double>(D, 2 * (int64_t)n);
+  D.prec = h.set.precision == LBLG_F32 ? 1 : 0;
+  D.valL = valloc(D, S.p[n]);
+  D.dval = valloc(D, 2 * (int64_t)n);
   D.perm_piv = dalloc<int>(D, n);
   D.newpos = dalloc<int>(D, n);
   D.perm_f = dalloc<int>(D, n);
@@ -276,8 +283,8 @@ static void setup(Handle& h) {
     F.d_medium = dupload(D, F.medium);
     F.d_large = dupload(D, F.large);
   }
-  D.W = dalloc<double>(D, wmax_total);
-  D.LD = dalloc<double>(D, std::max<int64_t>(ld_max, 1));
+  D.W = valloc(D, wmax_total);
+  D.LD = valloc(D, std::max<int64_t>(ld_max, 1));
   D.W_elems = wmax_total;
   D.woff = dupload(D, woff);
   D.kpos = dalloc<int>(D, ns);
@@ -342,30 +349,44 @@ static void setup(Handle& h) {
 // numeric factorization
 // ---------------------------------------------------------------------------
 void factor(Handle& h) {
+  if (h.dev && h.dev->prec != (h.set.precision == LBLG_F32 ? 1 : 0)) h.dev.reset();  // value type changed
   if (!h.dev) setup(h);
   DeviceFactor& D = *h.dev;
   HIPCHK(hipMemcpyAsync(D.Kx, h.Ax.data(), sizeof(double) * D.nnzK, hipMemcpyHostToDevice, D.stream));
+  double md = 0.0;
+  for (double v : h.Ax) md = std::max(md, std::fabs(v));
+  h.max_abs_entry = md;
   factor_device(h);
 }
 
-void factor_device(Handle& h) {
+template <typename T>
+static void factor_device_t(Handle& h) {
   DeviceFactor& D = *h.dev;
+  T* const valL = static_cast<T*>(D.valL);
+  T* const dval = static_cast<T*>(D.dval);
+  T* const Wb = static_cast<T*>(D.W);
+  T* const LDb = static_cast<T*>(D.LD);
   const Symbolic& S = h.sym;
   const int n = h.n;
   hipStream_t st = D.stream;
   const double reg = h.set.reg_diag;
+  // The FP32 factor perturbs tiny diagonals up to the static-pivoting
+  // threshold sqrt(eps_f32) max_ij |K_ij| (2^-12; pivots of reg_diag = 1e-9
+  // would give growth ~1e9, beyond FP32); the refinement still runs on the
+  // reference's K0, so only the preconditioner changes.
+  const double reg_factor = D.prec == 1 ? std::max(reg, std::ldexp(h.max_abs_entry, -12)) : reg;
   SnArgs a = snargs(D);
   a.rows = D.rows0;  // pre-pivot labels during the factorization
-  launch_fill(D.valL, S.p[n], 0.0, st);
-  launch_fill(D.dval, 2 * (int64_t)n, 0.0, st);
+  launch_fill(valL, S.p[n], 0.0, st);
+  launch_fill(dval, 2 * (int64_t)n, 0.0, st);
   launch_iota(D.perm_piv, n, st);
   launch_fill_int(D.status, 16, 0, st);
   launch_fill_int(D.kpos, D.nsuper, 0, st);
-  launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, h.set.regularization, D.pert, st);
-  launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg, D.valL, st);
+  launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, reg_factor, h.set.regularization, D.pert, st);
+  launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg_factor, valL, st);
   for (size_t l = 0; l < D.flevels.size(); ++l) {
     FactorLevel& F = D.flevels[l];
-    timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, D.valL, D.dval, D.perm_piv, D.status, st); });
+    timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, valL, dval, D.perm_piv, D.status, st); });
     // medium supernodes: one workgroup each, panel by panel
     if (!F.medium.empty()) {
       int wmax = 0;
@@ -373,14 +394,14 @@ void factor_device(Handle& h) {
       const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
       for (int p = 0; p < rounds; ++p) {
         timed(D, "bk_panel", [&] {
-          launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, D.valL, D.dval, D.perm_piv, D.W, D.woff,
+          launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, valL, dval, D.perm_piv, Wb, D.woff,
                           D.kpos, D.kstart, D.sync, D.status, false, st);
         });
         const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
-        const int T = (wmax - lo + 63) / 64;
-        if (T > 0)
+        const int nt = (wmax - lo + 63) / 64;
+        if (nt > 0)
           timed(D, "trailing", [&] {
-            launch_trailing(F.d_medium, (int)F.medium.size(), T * (T + 1) / 2, lo, a, D.valL, D.W, D.woff,
+            launch_trailing(F.d_medium, (int)F.medium.size(), nt * (nt + 1) / 2, lo, a, valL, Wb, D.woff,
                             D.kpos, D.kstart, st);
           });
       }
@@ -403,7 +424,7 @@ void factor_device(Handle& h) {
           const int Tr = (int)((r - lo + 127) / 128);
           if (Tr > 0)
             timed(D, "trailing", [&] {
-              launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
+              launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
             });
         }
         launch_fill_int(D.sync, 1, 0, st);
@@ -414,7 +435,7 @@ void factor_device(Handle& h) {
         const uint32_t eb = D.ll_epoch;
         D.ll_epoch += 3u * (uint32_t)nb + 8u;
         timed(D, "bk_panel_group", [&] {
-          launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, D.valL, D.dval, D.perm_piv, D.W, D.woff, D.kpos,
+          launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
                          D.kstart, D.sync, D.ll, eb, D.status, st);
         });
         if (sub == WIDE_SUB - 1 || p == rounds - 1) {
@@ -423,25 +444,25 @@ void factor_device(Handle& h) {
           const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
           if (Tj > 0)
             timed(D, "trailing", [&] {
-              launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, D.valL, D.W, D.woff, D.kpos, D.kstart, st);
+              launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
             });
         }
       }
     }
     timed(D, "offdiag", [&] {
-      launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, D.valL, D.dval,
+      launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, valL, dval,
                      D.perm_piv, st);
     });
     timed(D, "schur", [&] {
-      launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, D.valL, D.dval, st);
+      launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, valL, dval, st);
     });
     if (F.schur_big.count > 0) {
       timed(D, "scale_ld", [&] {
-        launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, D.valL, D.dval,
-                        D.LD, F.d_schur_big_ldoff, st);
+        launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, valL, dval,
+                        LDb, F.d_schur_big_ldoff, st);
       });
       timed(D, "schur_big", [&] {
-        launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, D.valL, D.LD,
+        launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, valL, LDb,
                          F.d_schur_big_ldoff, st);
       });
     }
@@ -463,6 +484,13 @@ void factor_device(Handle& h) {
   h.factored = true;
 }
 
+void factor_device(Handle& h) {
+  if (h.dev->prec == 1)
+    factor_device_t<float>(h);
+  else
+    factor_device_t<double>(h);
+}
+
 // ---------------------------------------------------------------------------
 // M^{-1}: P L^{-T} D^{-1} L^{-1} P^T applied to an ordered block (in -> out)
 // ---------------------------------------------------------------------------
@@ -471,10 +499,11 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
   const int64_t nn = (int64_t)D.n * std::max(nrhs, D.nrhs_cap);
   const int cap = std::max(nrhs, D.nrhs_cap);
   const int mrc = std::max(mr, std::max(D.mr_cap, 1));
-  D.X = dalloc<double>(D, nn);
-  D.Y = dalloc<double>(D, nn);
-  D.Z = dalloc<double>(D, nn);
-  D.acc = dalloc<double>(D, nn);
+  D.X = valloc(D, nn);
+  D.Y = valloc(D, nn);
+  D.Z = valloc(D, nn);
+  D.acc = valloc(D, nn);
+  D.O = valloc(D, nn);
   D.R = dalloc<double>(D, nn);
   D.B = dalloc<double>(D, nn);
   D.T = dalloc<double>(D, nn);
@@ -489,33 +518,58 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
   D.mr_cap = mrc;
 }
 
-void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
+// sweeps in the factor's type T; in / out are the FP64 ordered blocks of the
+// refinement (converted at entry and exit when T is float)
+template <typename T>
+static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nrhs) {
   hipStream_t st = D.stream;
   SnArgs a = snargs(D);
   const int64_t nn = (int64_t)D.n * nrhs;
-  launch_copy(D.X, in, nn, st);
+  const T* valL = static_cast<const T*>(D.valL);
+  const T* dval = static_cast<const T*>(D.dval);
+  T* X = static_cast<T*>(D.X);
+  T* Y = static_cast<T*>(D.Y);
+  T* Z = static_cast<T*>(D.Z);
+  T* acc = static_cast<T*>(D.acc);
+  T* O;
+  if constexpr (std::is_same<T, double>::value) {
+    launch_copy(X, in, nn, st);
+    O = out;
+  } else {
+    launch_convert(X, in, nn, st);
+    O = static_cast<T*>(D.O);
+  }
   for (size_t l = 0; l < D.slevels.size(); ++l) {
     const SolveLevel& L = D.slevels[l];
     timed(D, "fwd_tri", [&] {
-      launch_fwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, D.valL, D.X, D.Y, nrhs, st);
+      launch_fwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)X, Y, nrhs, st);
     });
     timed(D, "fwd_gemv", [&] {
-      launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, D.Y, D.X,
+      launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)Y, X,
                       nrhs, st);
     });
   }
-  launch_dsolve(D.n, D.dval, D.Y, D.Z, nrhs, st);
-  launch_fill(D.acc, nn, 0.0, st);
+  launch_dsolve(D.n, dval, (const T*)Y, Z, nrhs, st);
+  launch_fill(acc, nn, 0.0, st);
   for (size_t l = D.slevels.size(); l-- > 0;) {
     const SolveLevel& L = D.slevels[l];
     timed(D, "bwd_gemv", [&] {
-      launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, D.valL, out, D.acc,
+      launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)O, acc,
                       nrhs, st);
     });
     timed(D, "bwd_tri", [&] {
-      launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, D.valL, D.Z, D.acc, out, nrhs, st);
+      launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)Z, (const T*)acc, O,
+                     nrhs, st);
     });
   }
+  if constexpr (!std::is_same<T, double>::value) launch_convert(out, (const T*)O, nn, st);
+}
+
+void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
+  if (D.prec == 1)
+    apply_solve_t<float>(D, in, out, nrhs);
+  else
+    apply_solve_t<double>(D, in, out, nrhs);
 }
 
 static void spmv_ordered(DeviceFactor& D, const double* x, double* y, int nrhs) {

@@ -1,4 +1,5 @@
-// 128 x 128 FP64 MFMA tiles for the large dense updates of the factorization:
+// 128 x 128 MFMA tiles (FP64, or FP32 for the FP32 factor) for the large
+// dense updates of the factorization:
 //
 //   k_schur_big     U = L_off D L_off^T of a supernode, scatter-added into
 //                   the ancestors' panels (the DGEMM + scatter of
@@ -8,7 +9,7 @@
 //                   (dlasyf's trailing update inside LAPACKE_dsytrf, :155)
 //
 // A workgroup (4 waves) owns a 128 x 128 tile; each wave a 64 x 64 quadrant
-// = 4 x 4 blocks of v_mfma_f64_16x16x4f64.  Operands are staged through LDS
+// = 4 x 4 blocks of the 16x16x4 MFMA of the value type.  Operands are staged through LDS
 // in chunks of 16 columns of k, k-major with a row pitch of 144 doubles
 // (rows of one k land in distinct bank halves for the four k of a step), and
 // the next chunk is prefetched into registers while the current one is
@@ -31,17 +32,19 @@ namespace lblg {
 #define BLD 144
 #define BRUNS 8  // target supernodes per tile with cached row positions
 
+template <typename T>
 struct BigAcc {
-  d4 c[4][4];
+  typename Mfma<T>::acc c[4][4];
 };
 
-__device__ inline void big_mma(BigAcc& acc, const double* As, const double* Bs) {
+template <typename T>
+__device__ inline void big_mma(BigAcc<T>& acc, const T* As, const T* Bs) {
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
   const int li = ln & 15, lk = ln >> 4;
 #pragma unroll
   for (int k = 0; k < BKC; k += 4) {
-    double a[4], b[4];
+    T a[4], b[4];
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
       a[x] = As[(k + lk) * BLD + r0 + 16 * x + li];
@@ -51,19 +54,19 @@ __device__ inline void big_mma(BigAcc& acc, const double* As, const double* Bs) 
     for (int x = 0; x < 4; ++x)
 #pragma unroll
       for (int y = 0; y < 4; ++y)
-        acc.c[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[y], a[x], acc.c[x][y], 0, 0, 0);
+        acc.c[x][y] = Mfma<T>::mma(b[y], a[x], acc.c[x][y]);
   }
 }
 
 // acc = sum_k A(x, k) B(y, k) over k < K; la(k, x) / lb(k, x) return the
 // operands (zero outside the matrix).  Thread t stages row t % 128 of k
 // entries (t / 128) * 8 .. +8 of each chunk.
-template <typename LA, typename LB>
-__device__ inline void big_gemm(BigAcc& acc, int K, double* As, double* Bs, LA la, LB lb) {
+template <typename T, typename LA, typename LB>
+__device__ inline void big_gemm(BigAcc<T>& acc, int K, T* As, T* Bs, LA la, LB lb) {
   const int t = threadIdx.x, x = t & 127, kq = (t >> 7) * 8;
   for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  double ra[8], rb[8];
+    for (int b = 0; b < 4; ++b) acc.c[a][b] = typename Mfma<T>::acc{0, 0, 0, 0};
+  T ra[8], rb[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     ra[q] = la(kq + q, x);
@@ -88,8 +91,8 @@ __device__ inline void big_gemm(BigAcc& acc, int K, double* As, double* Bs, LA l
   }
 }
 
-template <typename F>
-__device__ inline void big_foreach(const BigAcc& acc, F f) {
+template <typename T, typename F>
+__device__ inline void big_foreach(const BigAcc<T>& acc, F f) {
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
 #pragma unroll
@@ -97,7 +100,7 @@ __device__ inline void big_foreach(const BigAcc& acc, F f) {
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) f(r0 + 16 * a + (ln & 15), q0 + 16 * b + 4 * e + (ln >> 4), acc.c[a][b][e]);
+      for (int e = 0; e < 4; ++e) f(r0 + 16 * a + (ln & 15), q0 + 16 * b + Mfma<T>::orow(e, ln), acc.c[a][b][e]);
 }
 
 __device__ inline void tile_of(int64_t tix, int& ti, int& tj) {
@@ -116,11 +119,12 @@ __device__ inline int64_t big_row_pos(const int* rows, int64_t b, int64_t e, int
   return lo - b;
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_t* tile_prefix, int count,
                                                    const int* super, const int64_t* pi, const int64_t* px,
-                                                   const int* rows, const int* col2sup, double* valL,
-                                                   const double* LD, const int64_t* ldoff) {
-  __shared__ double As[BKC * BLD], Bs[BKC * BLD];
+                                                   const int* rows, const int* col2sup, T* valL,
+                                                   const T* LD, const int64_t* ldoff) {
+  __shared__ T As[BKC * BLD], Bs[BKC * BLD];
   __shared__ int rlab[BT], clab[BT], ctg[BT], dtcol[BT];
   __shared__ int tpos[BRUNS][BT];
   __shared__ int64_t cbase[BT];
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
   const int w = super[s + 1] - super[s];
   const int64_t r = pi[s + 1] - pi[s];
   const int m = (int)(r - w);
-  const double* P = valL + px[s];
+  const T* P = valL + px[s];
   const int i0 = BT * ti, j0 = BT * tj;  // within the m off-diagonal rows
   const int t = threadIdx.x;
   if (t < BT) {
@@ -171,15 +175,15 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
     const int tg = ctg[t];
     cbase[t] = tg >= 0 ? px[tg] + (int64_t)(clab[t] - super[tg]) * (pi[tg + 1] - pi[tg]) : 0;
   }
-  const double* LDs = LD + ldoff[lo];
-  BigAcc acc;
+  const T* LDs = LD + ldoff[lo];
+  BigAcc<T> acc;
   big_gemm(
       acc, w, As, Bs,
-      [&](int k, int x) -> double { return (k < w && i0 + x < m) ? P[w + i0 + x + (int64_t)k * r] : 0.0; },
-      [&](int k, int x) -> double { return (k < w && j0 + x < m) ? LDs[(int64_t)k * m + j0 + x] : 0.0; });
+      [&](int k, int x) -> T { return (k < w && i0 + x < m) ? P[w + i0 + x + (int64_t)k * r] : (T)0; },
+      [&](int k, int x) -> T { return (k < w && j0 + x < m) ? LDs[(int64_t)k * m + j0 + x] : (T)0; });
   // scatter-add: no global reads on the common path (target bases and row
   // positions come from LDS), so the no-return atomics stream out
-  big_foreach(acc, [&](int ri, int ci, double v) {
+  big_foreach(acc, [&](int ri, int ci, T v) {
     const int R = rlab[ri], C = clab[ci];
     if (R < 0 || C < 0 || R < C) return;
     const int dt = dtcol[ci];
@@ -204,11 +208,12 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
 //              written to the panel buffer Pp[(j - k1) r + i] (the matrix itself
 //              stays without the block's updates until the block ends); one
 //              column of tiles starting at k1
+template <typename T>
 __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, int strip, int wcols, const int* super,
-                                                      const int64_t* pi, const int64_t* px, double* valL,
-                                                      double* Wbase, const int64_t* woff,
+                                                      const int64_t* pi, const int64_t* px, T* valL,
+                                                      T* Wbase, const int64_t* woff,
                                                       const int* kpos, const int* kstart) {
-  __shared__ double As[BKC * BLD], Bs[BKC * BLD];
+  __shared__ T As[BKC * BLD], Bs[BKC * BLD];
   const int s = list[0];
   const int w = super[s + 1] - super[s];
   const int64_t r = pi[s + 1] - pi[s];
@@ -234,29 +239,29 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
   }
   const int jend = strip ? min(w, k1 + strip) : w;
   if (i0 >= r || i0 + BT - 1 < k1 || j0 >= jend || j0 + BT - 1 < k1) return;
-  double* A = valL + px[s];
-  const double* Wp = Wbase + woff[s];
-  double* Pp = Wbase + woff[s] + (int64_t)w * wcols;  // r x strip, pitch r
+  T* A = valL + px[s];
+  const T* Wp = Wbase + woff[s];
+  T* Pp = Wbase + woff[s] + (int64_t)w * wcols;  // r x strip, pitch r
   const int kb = k1 - k0;
-  BigAcc acc;
+  BigAcc<T> acc;
   big_gemm(
       acc, kb, As, Bs,
-      [&](int k, int x) -> double { return (k < kb && i0 + x < r) ? A[(int64_t)(k0 + k) * r + i0 + x] : 0.0; },
-      [&](int k, int x) -> double { return (k < kb && j0 + x < w) ? Wp[(int64_t)k * w + j0 + x] : 0.0; });
+      [&](int k, int x) -> T { return (k < kb && i0 + x < r) ? A[(int64_t)(k0 + k) * r + i0 + x] : (T)0; },
+      [&](int k, int x) -> T { return (k < kb && j0 + x < w) ? Wp[(int64_t)k * w + j0 + x] : (T)0; });
   // read-modify-write in batches of 16 values per lane: the loads use a
   // safe address for masked elements so that they issue back to back
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    double cur[16];
+    T cur[16];
     int64_t off[16];
     bool ok[16];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
+        const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + Mfma<T>::orow(e, ln);
         const int q = 4 * b + e;
         ok[q] = i < r && j < jend && j >= k1 && i >= j;
         off[q] = ok[q] ? (int64_t)j * r + i : 0;
@@ -268,7 +273,7 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int q = 4 * b + e;
-          const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
+          const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + Mfma<T>::orow(e, ln);
           if (ok[q]) Pp[(int64_t)(j - k1) * r + i] = cur[q] - acc.c[a][b][e];
         }
     } else {
@@ -285,9 +290,10 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
 
 // LD(:, k) = (L_off D)(:, k) of each listed supernode, column-major m x w at
 // LD + ldoff[q] (the D scaling of blocked_2by2_mult, Sym_BLAS.cpp:573-599)
+template <typename T>
 __global__ __launch_bounds__(256) void k_scale_ld(const int* list, const int* col_prefix, int count,
                                                   const int* super, const int64_t* pi, const int64_t* px,
-                                                  const double* valL, const double* dval, int n, double* LD,
+                                                  const T* valL, const T* dval, int n, T* LD,
                                                   const int64_t* ldoff) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
@@ -298,12 +304,12 @@ __global__ __launch_bounds__(256) void k_scale_ld(const int* list, const int* co
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
   const int m = (int)(r - w);
-  const double* col = valL + px[s] + w;
-  const double dk = dval[c0 + k], ok = dval[n + c0 + k], op = k > 0 ? dval[n + c0 + k - 1] : 0.0;
-  double* out = LD + ldoff[lo] + (int64_t)k * m;
+  const T* col = valL + px[s] + w;
+  const T dk = dval[c0 + k], ok = dval[n + c0 + k], op = k > 0 ? dval[n + c0 + k - 1] : (T)0;
+  T* out = LD + ldoff[lo] + (int64_t)k * m;
   for (int j = threadIdx.x; j < m; j += 256) {
-    const double l0 = col[(int64_t)k * r + j];
-    double v;
+    const T l0 = col[(int64_t)k * r + j];
+    T v;
     if (ok != 0.0) v = l0 * dk + col[(int64_t)(k + 1) * r + j] * ok;
     else if (op != 0.0) v = col[(int64_t)(k - 1) * r + j] * op + l0 * dk;
     else v = l0 * dk;
@@ -311,28 +317,43 @@ __global__ __launch_bounds__(256) void k_scale_ld(const int* list, const int* co
   }
 }
 
+template <typename T>
 void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncols, const SnArgs& a,
-                     const double* valL, const double* dval, double* LD, const int64_t* ldoff, hipStream_t st) {
+                     const T* valL, const T* dval, T* LD, const int64_t* ldoff, hipStream_t st) {
   if (count <= 0 || ncols <= 0) return;
-  k_scale_ld<<<ncols, 256, 0, st>>>(list, col_prefix, count, a.super, a.pi, a.px, valL, dval, a.n, LD, ldoff);
+  k_scale_ld<T><<<ncols, 256, 0, st>>>(list, col_prefix, count, a.super, a.pi, a.px, valL, dval, a.n, LD, ldoff);
   HIP_LAUNCH_CHECK("scale_ld");
 }
 
+template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                      double* valL, const double* LD, const int64_t* ldoff, hipStream_t st) {
+                      T* valL, const T* LD, const int64_t* ldoff, hipStream_t st) {
   if (count <= 0 || ntiles <= 0) return;
   if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur_big: too many tiles");
-  k_schur_big<<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+  k_schur_big<T><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
                                                 a.col2sup, valL, LD, ldoff);
   HIP_LAUNCH_CHECK("schur_big");
 }
 
-void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, double* valL,
-                         double* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
+template <typename T>
+void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
+                         T* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
   if (ntiles <= 0) return;
   if (strip < 0 || strip > BT) throw std::runtime_error("trailing_big: strip wider than a tile");
-  k_trailing_big<<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff, kpos, kstart);
+  k_trailing_big<T><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff, kpos,
+                                            kstart);
   HIP_LAUNCH_CHECK("trailing_big");
 }
+
+#define LBLG_INSTANTIATE(T)                                                                                   \
+  template void launch_scale_ld<T>(const int*, int, const int*, int, const SnArgs&, const T*, const T*, T*,  \
+                                   const int64_t*, hipStream_t);                                            \
+  template void launch_schur_big<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,   \
+                                    const int64_t*, hipStream_t);                                           \
+  template void launch_trailing_big<T>(const int*, int, int, int, int, const SnArgs&, T*, T*, const int64_t*, \
+                                       const int*, const int*, hipStream_t);
+LBLG_INSTANTIATE(double)
+LBLG_INSTANTIATE(float)
+#undef LBLG_INSTANTIATE
 
 }  // namespace lblg

@@ -24,6 +24,7 @@ struct Handle {
   std::vector<int64_t> csr_src;
   std::vector<int> csr_rp;
   double k0_norm_inf = 0.0;
+  double max_abs_entry = 0.0;      // max |K_ij| of the values last uploaded
   int num_ref_iter = 0;
   int zero_pivot = 0;
   int64_t second_col_steps = 0;  // wide panels: pivot steps that inspected a second column

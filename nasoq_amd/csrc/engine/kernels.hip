@@ -35,7 +35,8 @@ static int grid_for(int64_t count, int block) {
 // --------------------------------------------------------------------------
 // small helpers
 // --------------------------------------------------------------------------
-__global__ void k_fill(double* p, int64_t count, double v) {
+template <typename T>
+__global__ void k_fill(T* p, int64_t count, T v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -53,9 +54,10 @@ __global__ void k_copy(double* dst, const double* src, int64_t count) {
        i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
 }
-void launch_fill(double* p, int64_t count, double v, hipStream_t st) {
+template <typename T>
+void launch_fill(T* p, int64_t count, double v, hipStream_t st) {
   if (count <= 0) return;
-  k_fill<<<grid_for(count, 256), 256, 0, st>>>(p, count, v);
+  k_fill<T><<<grid_for(count, 256), 256, 0, st>>>(p, count, (T)v);
   HIP_LAUNCH_CHECK("fill");
 }
 void launch_fill_int(int* p, int64_t count, int v, hipStream_t st) {
@@ -73,22 +75,37 @@ void launch_copy(double* dst, const double* src, int64_t count, hipStream_t st) 
   k_copy<<<grid_for(count, 256), 256, 0, st>>>(dst, src, count);
   HIP_LAUNCH_CHECK("copy");
 }
+// value conversion between the FP64 refinement vectors and the factor's type
+template <typename TO, typename TI>
+__global__ void k_convert(TO* dst, const TI* src, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (TO)src[i];
+}
+template <typename TO, typename TI>
+void launch_convert(TO* dst, const TI* src, int64_t count, hipStream_t st) {
+  if (count <= 0) return;
+  k_convert<TO, TI><<<grid_for(count, 256), 256, 0, st>>>(dst, src, count);
+  HIP_LAUNCH_CHECK("convert");
+}
 
 // --------------------------------------------------------------------------
 // assembly (parallel_blocked_ldlt_02.cpp:81-91) with the diagonal
 // perturbation of find_perturbation (linear_solver_wrapper.cpp:352-363)
 // --------------------------------------------------------------------------
+template <typename T>
 __global__ void k_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz,
-                           double reg, double* valL) {
+                           double reg, T* valL) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
-    const double v = pert[q] ? reg : Kx[q];
-    valL[amap[q]] = v;
+    const double v = (pert[q] & 2) ? reg : Kx[q];
+    valL[amap[q]] = (T)v;
   }
 }
+template <typename T>
 void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
-                     double* valL, hipStream_t st) {
+                     T* valL, hipStream_t st) {
   if (nnz <= 0) return;
-  k_assemble<<<grid_for(nnz, 256), 256, 0, st>>>(amap, Kx, pert, nnz, reg, valL);
+  k_assemble<T><<<grid_for(nnz, 256), 256, 0, st>>>(amap, Kx, pert, nnz, reg, valL);
   HIP_LAUNCH_CHECK("assemble");
 }
 
@@ -101,17 +118,18 @@ void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char*
 // dval[n + c] and the diagonal of the block becomes 1.
 // --------------------------------------------------------------------------
 #define LDA_S 65
+template <typename T>
 __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* super, const int64_t* pi,
-                                                  const int64_t* px, double* valL, double* dval,
+                                                  const int64_t* px, T* valL, T* dval,
                                                   int* perm_piv, int n, int* status) {
-  __shared__ double a[64 * LDA_S];
-  __shared__ double wk[64], wk1[64];
+  __shared__ T a[64 * LDA_S];
+  __shared__ T wk[64], wk1[64];
   __shared__ int lperm[64], kind[64];
   __shared__ int sh_kp, sh_kstep, sh_zero;
   const int s = list[blockIdx.x];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
-  double* P = valL + px[s];
+  T* P = valL + px[s];
   const int t = threadIdx.x;
   for (int idx = t; idx < w * w; idx += 256) {
     const int i = idx % w, j = idx / w;
@@ -169,28 +187,28 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
         if (part == 0) {
           const int i = kp + 1 + x;
           if (i < w) {
-            const double tmp = a[i + kk * LDA_S];
+            const T tmp = a[i + kk * LDA_S];
             a[i + kk * LDA_S] = a[i + kp * LDA_S];
             a[i + kp * LDA_S] = tmp;
           }
         } else if (part == 1) {
           const int j = kk + 1 + x;
           if (j < kp) {
-            const double tmp = a[j + kk * LDA_S];
+            const T tmp = a[j + kk * LDA_S];
             a[j + kk * LDA_S] = a[kp + j * LDA_S];
             a[kp + j * LDA_S] = tmp;
           }
         } else {
           const int j = x;
           if (j < k) {
-            const double tmp = a[kk + j * LDA_S];
+            const T tmp = a[kk + j * LDA_S];
             a[kk + j * LDA_S] = a[kp + j * LDA_S];
             a[kp + j * LDA_S] = tmp;
           }
         }
       }
       if (t == 0) {
-        double tmp = a[kk + kk * LDA_S];
+        T tmp = a[kk + kk * LDA_S];
         a[kk + kk * LDA_S] = a[kp + kp * LDA_S];
         a[kp + kp * LDA_S] = tmp;
         if (kstep == 2) {
@@ -211,9 +229,9 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
     if (!zero) {
       if (kstep == 1) {
         if (k < w - 1) {
-          const double akk = a[k + k * LDA_S];
-          const bool big = fabs(akk) >= SFMIN;
-          const double d11 = 1.0 / akk;
+          const T akk = a[k + k * LDA_S];
+          const bool big = fabs(akk) >= Mfma<T>::sfmin();
+          const T d11 = (T)1 / akk;
           const int m = w - k - 1;
           if (!big) {
             for (int i = k + 1 + t; i < w; i += 256) a[i + k * LDA_S] /= akk;
@@ -222,7 +240,7 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
           for (int idx = t; idx < m * m; idx += 256) {
             const int i = k + 1 + idx % m, j = k + 1 + idx / m;
             if (i >= j) {
-              const double f = big ? -d11 * a[j + k * LDA_S] : -akk * a[j + k * LDA_S];
+              const T f = big ? -d11 * a[j + k * LDA_S] : -akk * a[j + k * LDA_S];
               a[i + j * LDA_S] += a[i + k * LDA_S] * f;
             }
           }
@@ -231,10 +249,10 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
             for (int i = k + 1 + t; i < w; i += 256) a[i + k * LDA_S] *= d11;
         }
       } else if (k < w - 2) {
-        double d21 = a[k + 1 + k * LDA_S];
-        const double d11 = a[k + 1 + (k + 1) * LDA_S] / d21;
-        const double d22 = a[k + k * LDA_S] / d21;
-        const double tt = 1.0 / (d11 * d22 - 1.0);
+        T d21 = a[k + 1 + k * LDA_S];
+        const T d11 = a[k + 1 + (k + 1) * LDA_S] / d21;
+        const T d22 = a[k + k * LDA_S] / d21;
+        const T tt = (T)1 / (d11 * d22 - (T)1);
         d21 = tt / d21;
         for (int j = k + 2 + t; j < w; j += 256) {
           wk[j] = d21 * (d11 * a[j + k * LDA_S] - a[j + (k + 1) * LDA_S]);
@@ -276,10 +294,11 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
     if (i >= j) P[i + j * r] = a[i + j * LDA_S];
   }
 }
-void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, double* dval,
+template <typename T>
+void launch_bk_small(const int* list, int count, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, int* status, hipStream_t st) {
   if (count <= 0) return;
-  k_bk_small<<<count, 256, 0, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n, status);
+  k_bk_small<T><<<count, 256, 0, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n, status);
   HIP_LAUNCH_CHECK("bk_small");
 }
 
@@ -296,9 +315,10 @@ void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, 
 // Per-supernode workspace at W + woff[s]:  Wp (w x NB, ld w) | t0 (w) | t1 (w)
 //                                         | part (2 G doubles) | parti (2 G ints)
 // --------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const int* super,
-                                                  const int64_t* pi, const int64_t* px, double* valL,
-                                                  double* dval, int* perm_piv, double* Wbase,
+                                                  const int64_t* pi, const int64_t* px, T* valL,
+                                                  T* dval, int* perm_piv, T* Wbase,
                                                   const int64_t* woff, int* kpos, int* kstart,
                                                   int* sync, int n, int* status) {
   __shared__ double sv[8];
@@ -307,11 +327,11 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
   const int s = list[sidx];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
-  double* A = valL + px[s];
-  double* Wp = Wbase + woff[s];
-  double* t0 = Wp + (int64_t)w * PANEL_NB;
-  double* t1 = t0 + w;
-  double* part = t1 + w;
+  T* A = valL + px[s];
+  T* Wp = Wbase + woff[s];
+  T* t0 = Wp + (int64_t)w * PANEL_NB;
+  T* t1 = t0 + w;
+  T* part = t1 + w;  // column maxima (exact in T: magnitudes of T values)
   int* parti = reinterpret_cast<int*>(part + 2 * G);
   int* ctr = sync + sidx;
   const int gt = g * 256 + threadIdx.x, GT = G * 256;
@@ -327,7 +347,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
     double bv = -1.0;
     int bi = 0x7fffffff;
     for (int i = k + gt; i < w; i += GT) {
-      double v = AA(i, k);
+      T v = AA(i, k);
       for (int j = k0; j < k; ++j) v -= AA(i, j) * WW(k, j);
       t0[i] = v;
       if (i > k) {
@@ -338,7 +358,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
     }
     block_argmax(bv, bi, sv, si);
     if (threadIdx.x == 0) {
-      part[g] = bv;
+      part[g] = (T)bv;
       parti[g] = bi;
     }
     group_barrier(ctr, G, epoch, status);
@@ -349,7 +369,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       colmax = 0.0;
       imax = k;
     }
-    const double akk_v = t0[k];
+    const T akk_v = t0[k];
     const double absakk = fabs(akk_v);
     int kp = k, kstep = 1, zero = 0;
     bool use_t1_as_k = false;
@@ -360,7 +380,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       double rv = -1.0;
       int ri = 0x7fffffff;
       for (int i = k + gt; i < w; i += GT) {
-        double v = (i < imax) ? AA(imax, i) : AA(i, imax);
+        T v = (i < imax) ? AA(imax, i) : AA(i, imax);
         for (int j = k0; j < k; ++j) v -= AA(i, j) * WW(imax, j);
         t1[i] = v;
         if (i != imax) {
@@ -371,7 +391,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       }
       block_argmax(rv, ri, sv, si);
       if (threadIdx.x == 0) {
-        part[G + g] = rv;
+        part[G + g] = (T)rv;
         parti[G + g] = ri;
       }
       group_barrier(ctr, G, epoch, status);
@@ -396,18 +416,18 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
     if (swap) {
       // rows kk/kp of the earlier L columns and of the panel's W columns
       for (int j = gt; j < k; j += GT) {
-        const double x = AA(kk, j);
+        const T x = AA(kk, j);
         AA(kk, j) = AA(kp, j);
         AA(kp, j) = x;
       }
       for (int j = k0 + gt; j < k; j += GT) {
-        const double x = WW(kk, j);
+        const T x = WW(kk, j);
         WW(kk, j) = WW(kp, j);
         WW(kp, j) = x;
       }
       // columns kk/kp of the rows below the diagonal block
       for (int64_t q = w + gt; q < r; q += GT) {
-        const double x = AA(q, kk);
+        const T x = AA(q, kk);
         AA(q, kk) = AA(q, kp);
         AA(q, kp) = x;
       }
@@ -418,12 +438,12 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       }
     }
     // own rows: non-updated copy of column kk into column kp, then W and L
-    const double* colk = use_t1_as_k ? t1 : t0;
-    double D1 = 0, D2 = 0, OD = 0, d21 = 0, d11 = 0, d22 = 0, rk = 0;
+    const T* colk = use_t1_as_k ? t1 : t0;
+    T D1 = 0, D2 = 0, OD = 0, d21 = 0, d11 = 0, d22 = 0, rk = 0;
     if (kstep == 1) {
       const int sk = swap ? (k == kk ? kp : k) : k;
       D1 = colk[sk];
-      rk = 1.0 / D1;
+      rk = (T)1 / D1;
     } else {
       // post-interchange values of W(k,k), W(k+1,k), W(k+1,k+1)
       D1 = t0[k];
@@ -432,7 +452,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       d21 = OD;
       d11 = D2 / d21;
       d22 = D1 / d21;
-      const double tt = 1.0 / (d11 * d22 - 1.0);
+      const T tt = (T)1 / (d11 * d22 - (T)1);
       d21 = tt / d21;
     }
     for (int i = k + gt; i < w; i += GT) {
@@ -443,7 +463,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
         else if (i > kp) AA(i, kp) = AA(i, kk);
       }
       if (kstep == 1) {
-        const double wv = colk[si2];
+        const T wv = colk[si2];
         WW(i, k) = wv;
         if (i == k) {
           AA(k, k) = 1.0;
@@ -453,7 +473,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
           AA(i, k) = zero ? wv : wv * rk;
         }
       } else {
-        const double w0 = t0[si2], w1 = t1[si2];
+        const T w0 = t0[si2], w1 = t1[si2];
         WW(i, k) = w0;
         WW(i, k + 1) = w1;
         if (i == k) {
@@ -481,19 +501,20 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
 #undef AA
 #undef WW
 }
-void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double* valL, double* dval,
-                     int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+template <typename T>
+void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL, T* dval,
+                     int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                      int* status, bool cooperative, hipStream_t st) {
   if (count <= 0) return;
   if (!cooperative || G == 1) {
-    k_bk_panel<<<count * G, 256, 0, st>>>(list, G, a.super, a.pi, a.px, valL, dval, perm_piv, W, woff,
+    k_bk_panel<T><<<count * G, 256, 0, st>>>(list, G, a.super, a.pi, a.px, valL, dval, perm_piv, W, woff,
                                           kpos, kstart, sync, a.n, status);
   } else {
     int n = a.n;
     void* args[] = {(void*)&list, (void*)&G, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
                     (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
                     (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&n, (void*)&status};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel, dim3(count * G), dim3(256), args,
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel<T>, dim3(count * G), dim3(256), args,
                                               0, st);
     if (e != hipSuccess) throw std::runtime_error(std::string("bk_panel coop: ") + hipGetErrorString(e));
   }
@@ -501,67 +522,73 @@ void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double*
 }
 
 // --------------------------------------------------------------------------
-// FP64 MFMA tile engine: a 256-thread workgroup computes a 64x64 block
+// MFMA tile engine: a 256-thread workgroup computes a 64x64 block
 // C(i,j) = sum_k A(i,k) B(j,k) with A and B staged in LDS as [k][64]
 // (k-major).  Each wave owns a 32x32 quadrant = 2x2 blocks of
-// v_mfma_f64_16x16x4f64.  Instruction lanes: first operand (row = lane%16,
-// k = lane/16), second operand (col = lane%16, k = lane/16), accumulator
-// element e of a lane = row 4*e + lane/16, column lane%16 (checked by
-// lblg_selftest_mfma).  The tile engine passes B first so that the
+// Mfma<T>::mma (16x16x4; lane layout in device_util.hpp).  The tile engine
+// passes B first so that the
 // accumulator holds C^T: lanes 0..15 then cover 16 consecutive rows of C and
 // the epilogues touch contiguous memory.
 // --------------------------------------------------------------------------
 #define KC 32          // k-chunk staged in LDS
 #define LDT 66         // padded row length of a staged tile
+template <typename T>
 struct TileAcc {
-  d4 c[2][2];
+  typename Mfma<T>::acc c[2][2];
 };
-__device__ inline void tile_zero(TileAcc& acc) {
+template <typename T>
+__device__ inline void tile_zero(TileAcc<T>& acc) {
   for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 2; ++b) acc.c[a][b] = typename Mfma<T>::acc{0, 0, 0, 0};
 }
 // As, Bs: [KC][LDT]; kc = number of valid k (multiple of 4 after zero padding)
-__device__ inline void tile_mma(TileAcc& acc, const double* As, const double* Bs, int kc) {
+template <typename T>
+__device__ inline void tile_mma(TileAcc<T>& acc, const T* As, const T* Bs, int kc) {
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
   const int li = ln & 15, lk = ln >> 4;
   for (int k = 0; k < kc; k += 4) {
-    const double a0 = As[(k + lk) * LDT + r0 + li];
-    const double a1 = As[(k + lk) * LDT + r0 + 16 + li];
-    const double b0 = Bs[(k + lk) * LDT + q0 + li];
-    const double b1 = Bs[(k + lk) * LDT + q0 + 16 + li];
+    const T a0 = As[(k + lk) * LDT + r0 + li];
+    const T a1 = As[(k + lk) * LDT + r0 + 16 + li];
+    const T b0 = Bs[(k + lk) * LDT + q0 + li];
+    const T b1 = Bs[(k + lk) * LDT + q0 + 16 + li];
     // D = B_blk A_blk^T, i.e. the transpose of the C block: lane (ln & 15) walks rows
-    acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc.c[0][0], 0, 0, 0);
-    acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc.c[0][1], 0, 0, 0);
-    acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc.c[1][0], 0, 0, 0);
-    acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc.c[1][1], 0, 0, 0);
+    acc.c[0][0] = Mfma<T>::mma(b0, a0, acc.c[0][0]);
+    acc.c[0][1] = Mfma<T>::mma(b1, a0, acc.c[0][1]);
+    acc.c[1][0] = Mfma<T>::mma(b0, a1, acc.c[1][0]);
+    acc.c[1][1] = Mfma<T>::mma(b1, a1, acc.c[1][1]);
   }
 }
 // visit every accumulator element: f(row_in_tile, col_in_tile, value)
-template <typename F>
-__device__ inline void tile_foreach(const TileAcc& acc, F f) {
+template <typename T, typename F>
+__device__ inline void tile_foreach(const TileAcc<T>& acc, F f) {
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int e = 0; e < 4; ++e) {
         const int row = r0 + 16 * a + (ln & 15);
-        const int col = q0 + 16 * b + 4 * e + (ln >> 4);
+        const int col = q0 + 16 * b + Mfma<T>::orow(e, ln);
         f(row, col, acc.c[a][b][e]);
       }
 }
 
+// one wave: C(16x16) = A(16x4) * B(4x16) (row-major) through Mfma<T>, each
+// accumulator element stored at the (row, column) the engine assumes
+template <typename T>
 __global__ void k_mfma_selftest(const double* A, const double* B, double* C) {
-  // one wave: C(16x16) = A(16x4) * B(4x16), A row-major, B row-major
   const int ln = threadIdx.x;
-  d4 acc = {0, 0, 0, 0};
-  const double a = A[(ln & 15) * 4 + (ln >> 4)];
-  const double b = B[(ln >> 4) * 16 + (ln & 15)];
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-  for (int e = 0; e < 4; ++e) C[(4 * e + (ln >> 4)) * 16 + (ln & 15)] = acc[e];
+  typename Mfma<T>::acc acc = {0, 0, 0, 0};
+  const T a = (T)A[(ln & 15) * 4 + (ln >> 4)];
+  const T b = (T)B[(ln >> 4) * 16 + (ln & 15)];
+  acc = Mfma<T>::mma(a, b, acc);
+  for (int e = 0; e < 4; ++e) C[Mfma<T>::orow(e, ln) * 16 + (ln & 15)] = (double)acc[e];
 }
-void launch_mfma_selftest(const double* A, const double* B, double* C, hipStream_t st) {
-  k_mfma_selftest<<<1, 64, 0, st>>>(A, B, C);
+void launch_mfma_selftest(const double* A, const double* B, double* C, int f32, hipStream_t st) {
+  if (f32)
+    k_mfma_selftest<float><<<1, 64, 0, st>>>(A, B, C);
+  else
+    k_mfma_selftest<double><<<1, 64, 0, st>>>(A, B, C);
   HIP_LAUNCH_CHECK("mfma_selftest");
 }
 
@@ -570,11 +597,12 @@ void launch_mfma_selftest(const double* A, const double* B, double* C, hipStream
 // A(i,j) -= sum_{l in [k0,k1)} L(i,l) W(j,l)  for k1 <= j <= i < w
 // (dlasyf's dgemm/dgemv update of A22).  Tiles cover [lo, w)^2, lo <= k1.
 // --------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per_sn, int lo,
                                                   const int* super, const int64_t* pi, const int64_t* px,
-                                                  double* valL, const double* Wbase, const int64_t* woff,
+                                                  T* valL, const T* Wbase, const int64_t* woff,
                                                   const int* kpos, const int* kstart) {
-  __shared__ double As[KC * LDT], Bs[KC * LDT];
+  __shared__ T As[KC * LDT], Bs[KC * LDT];
   const int sidx = blockIdx.x / tiles_per_sn, tix = blockIdx.x % tiles_per_sn;
   const int s = list[sidx];
   const int w = super[s + 1] - super[s];
@@ -588,10 +616,10 @@ __global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per
   const int tj = tix - ti * (ti + 1) / 2;
   const int i0 = lo + 64 * ti, j0 = lo + 64 * tj;
   if (i0 >= w || i0 + 63 < k1 || j0 >= w || j0 + 63 < k1) return;
-  double* A = valL + px[s];
-  const double* Wp = Wbase + woff[s];
+  T* A = valL + px[s];
+  const T* Wp = Wbase + woff[s];
   const int kb = k1 - k0;
-  TileAcc acc;
+  TileAcc<T> acc;
   tile_zero(acc);
   for (int kc0 = 0; kc0 < kb; kc0 += KC) {
     const int kc = min(KC, kb - kc0);
@@ -599,7 +627,7 @@ __global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per
     for (int idx = threadIdx.x; idx < kcp * 64; idx += 256) {
       const int kk = idx / 64, x = idx % 64;
       const int i = i0 + x, j = j0 + x;
-      double va = 0.0, vb = 0.0;
+      T va = 0, vb = 0;
       if (kk < kc) {
         const int l = k0 + kc0 + kk;
         if (i < w) va = A[(int64_t)l * r + i];
@@ -617,13 +645,13 @@ __global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per
   {
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
-    double cur[16];
+    T cur[16];
     int64_t off[16];
     bool ok[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int a = q >> 3, b = (q >> 2) & 1, e = q & 3;
-      const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + 4 * e + (ln >> 4);
+      const int i = i0 + r0 + 16 * a + (ln & 15), j = j0 + q0 + 16 * b + Mfma<T>::orow(e, ln);
       ok[q] = i < w && j < w && j >= k1 && i >= j;
       off[q] = ok[q] ? (int64_t)j * r + i : 0;
       cur[q] = A[off[q]];
@@ -635,11 +663,12 @@ __global__ __launch_bounds__(256) void k_trailing(const int* list, int tiles_per
     }
   }
 }
-void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
-                     const double* W, const int64_t* woff, const int* kpos, const int* kstart,
+template <typename T>
+void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, T* valL,
+                     const T* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st) {
   if (count <= 0 || tiles_per_sn <= 0) return;
-  k_trailing<<<(int64_t)count * tiles_per_sn, 256, 0, st>>>(list, tiles_per_sn, lo, a.super, a.pi, a.px,
+  k_trailing<T><<<(int64_t)count * tiles_per_sn, 256, 0, st>>>(list, tiles_per_sn, lo, a.super, a.pi, a.px,
                                                             valL, W, woff, kpos, kstart);
   HIP_LAUNCH_CHECK("trailing");
 }
@@ -651,13 +680,14 @@ void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const
 // gather != 0: columns are permuted here (narrow supernodes); otherwise the
 // panel kernel has already permuted them.
 // --------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk_prefix, int count,
                                                  const int* super, const int64_t* pi, const int64_t* px,
-                                                 double* valL, const double* dval, const int* perm_piv,
+                                                 T* valL, const T* dval, const int* perm_piv,
                                                  int n) {
-  __shared__ double X[64 * 65];   // X[row + col*65] for a 64 x 64 column block
-  __shared__ double Lt[64 * 65];  // L(cb rows, cb cols)
-  __shared__ double As[KC * LDT], Bs[KC * LDT];
+  __shared__ T X[64 * 65];   // X[row + col*65] for a 64 x 64 column block
+  __shared__ T Lt[64 * 65];  // L(cb rows, cb cols)
+  __shared__ T As[KC * LDT], Bs[KC * LDT];
   // locate (supernode, row block)
   int lo = 0, hi = count - 1;
   while (lo < hi) {
@@ -668,7 +698,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   const int rb = blockIdx.x - blk_prefix[lo];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
-  double* P = valL + px[s];
+  T* P = valL + px[s];
   const int64_t q0 = w + 64 * (int64_t)rb;
   const int nr = (int)min((int64_t)64, r - q0);
   const int t = threadIdx.x;
@@ -678,7 +708,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
     // load B block (with column gather for narrow supernodes)
     for (int idx = t; idx < 64 * cw; idx += 256) {
       const int i = idx % 64, j = idx / 64;
-      double v = 0.0;
+      T v = 0;
       if (i < nr) {
         const int srcc = gather ? (perm_piv[c0 + cb0 + j] - c0) : (cb0 + j);
         v = P[q0 + i + (int64_t)srcc * r];
@@ -692,13 +722,13 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
     __syncthreads();
     // X(:,cb) -= X(:, <cb) * L(cb, <cb)^T with MFMA (only for wide supernodes)
     if (cb0 > 0) {
-      TileAcc acc;
+      TileAcc<T> acc;
       tile_zero(acc);
       for (int kc0 = 0; kc0 < cb0; kc0 += KC) {
         const int kc = min(KC, cb0 - kc0);
         for (int idx = t; idx < KC * 64; idx += 256) {
           const int kk = idx / 64, x = idx % 64;
-          double va = 0.0, vb = 0.0;
+          T va = 0, vb = 0;
           if (kk < kc) {
             const int l = kc0 + kk;
             if (x < nr) va = P[q0 + x + (int64_t)l * r];
@@ -711,7 +741,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
         tile_mma(acc, As, Bs, (kc + 3) & ~3);
         __syncthreads();
       }
-      tile_foreach(acc, [&](int ri, int ci, double v) {
+      tile_foreach(acc, [&](int ri, int ci, T v) {
         if (ci < cw) X[ri + ci * 65] -= v;
       });
       __syncthreads();
@@ -719,7 +749,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
     // triangular solve inside the column block: thread per row
     if (t < nr) {
       for (int j = 0; j < cw; ++j) {
-        double v = X[t + j * 65];
+        T v = X[t + j * 65];
         for (int l = 0; l < j; ++l) v -= X[t + l * 65] * Lt[j + l * 65];
         X[t + j * 65] = v;
       }
@@ -735,12 +765,12 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   // 16 columns per batch so that a row's loads issue together; a 2x2 pair
   // may straddle two batches (look-ahead column / previous raw value)
   if (t < nr) {
-    double* row = P + q0 + t;
-    const double* dg = dval + c0;
-    const double* od = dval + n + c0;
-    double xprev = 0.0;
+    T* row = P + q0 + t;
+    const T* dg = dval + c0;
+    const T* od = dval + n + c0;
+    T xprev = 0;
     for (int l0 = 0; l0 < w; l0 += 16) {
-      double xv[17], yv[16];
+      T xv[17], yv[16];
 #pragma unroll
       for (int q = 0; q < 17; ++q) xv[q] = (l0 + q < w) ? row[(int64_t)(l0 + q) * r] : 0.0;
 #pragma unroll
@@ -748,13 +778,13 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
         const int l = l0 + q;
         yv[q] = 0.0;
         if (l < w) {
-          const double o = od[l], op = l > 0 ? od[l - 1] : 0.0;
+          const T o = od[l], op = l > 0 ? od[l - 1] : (T)0;
           if (o != 0.0) {
-            const double d1 = dg[l], d2 = dg[l + 1];
+            const T d1 = dg[l], d2 = dg[l + 1];
             yv[q] = (xv[q] * d2 - xv[q + 1] * o) / (d1 * d2 - o * o);
           } else if (op != 0.0) {
-            const double d1 = dg[l - 1], d2 = dg[l];
-            const double x1 = q > 0 ? xv[q - 1] : xprev;
+            const T d1 = dg[l - 1], d2 = dg[l];
+            const T x1 = q > 0 ? xv[q - 1] : xprev;
             yv[q] = (xv[q] * d1 - x1 * op) / (d1 * d2 - op * op);
           } else {
             yv[q] = xv[q] / dg[l];
@@ -768,10 +798,11 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
     }
   }
 }
+template <typename T>
 void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
-                    double* valL, const double* dval, const int* perm_piv, hipStream_t st) {
+                    T* valL, const T* dval, const int* perm_piv, hipStream_t st) {
   if (count <= 0 || nblocks <= 0) return;
-  k_offdiag<<<nblocks, 256, 0, st>>>(list, blk_prefix, count, a.super, a.pi, a.px, valL, dval, perm_piv,
+  k_offdiag<T><<<nblocks, 256, 0, st>>>(list, blk_prefix, count, a.super, a.pi, a.px, valL, dval, perm_piv,
                                      a.n);
   HIP_LAUNCH_CHECK("offdiag");
 }
@@ -792,13 +823,14 @@ __device__ inline int64_t row_pos(const int* rows, int64_t b, int64_t e, int key
   }
   return lo - b;
 }
+template <typename T>
 __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* tile_prefix, int count,
                                                const int* super, const int64_t* pi, const int64_t* px,
-                                               const int* rows, const int* col2sup, double* valL,
-                                               const double* dval, int n) {
-  __shared__ double As[KC * LDT], Bs[KC * LDT];
-  __shared__ double halo[2][64];
-  __shared__ double dd[KC + 2], oo[KC + 2];
+                                               const int* rows, const int* col2sup, T* valL,
+                                               const T* dval, int n) {
+  __shared__ T As[KC * LDT], Bs[KC * LDT];
+  __shared__ T halo[2][64];
+  __shared__ T dd[KC + 2], oo[KC + 2];
   __shared__ int rlab[64], clab[64], ctg[64], dtcol[64];
   __shared__ int tpos[4][64];
   __shared__ int64_t cbase[64];
@@ -818,7 +850,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
   const int m = (int)(r - w);
-  const double* P = valL + px[s];
+  const T* P = valL + px[s];
   const int i0 = 64 * ti, j0 = 64 * tj;  // within the m off-diagonal rows
   const int t = threadIdx.x;
   if (t < 64) {
@@ -849,7 +881,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
     const int tg = ctg[t];
     cbase[t] = tg >= 0 ? px[tg] + (int64_t)(clab[t] - super[tg]) * (pi[tg + 1] - pi[tg]) : 0;
   }
-  TileAcc acc;
+  TileAcc<T> acc;
   tile_zero(acc);
   for (int kc0 = 0; kc0 < w; kc0 += KC) {
     const int kc = min(KC, w - kc0);
@@ -858,33 +890,33 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
       const int kk = idx / 64, x = idx % 64;
       const int la = kc0 + kk;
       const int i = i0 + x, j = j0 + x;
-      As[kk * LDT + x] = (kk < kc && i < m) ? P[w + i + (int64_t)la * r] : 0.0;
-      Bs[kk * LDT + x] = (kk < kc && j < m) ? P[w + j + (int64_t)la * r] : 0.0;
+      As[kk * LDT + x] = (kk < kc && i < m) ? P[w + i + (int64_t)la * r] : (T)0;
+      Bs[kk * LDT + x] = (kk < kc && j < m) ? P[w + j + (int64_t)la * r] : (T)0;
     }
     if (t < 128) {
       const int side = t >> 6, x = t & 63;
       const int l = side == 0 ? kc0 - 1 : kc0 + kc;
       const int j = j0 + x;
-      halo[side][x] = (l >= 0 && l < w && j < m) ? P[w + j + (int64_t)l * r] : 0.0;
+      halo[side][x] = (l >= 0 && l < w && j < m) ? P[w + j + (int64_t)l * r] : (T)0;
     }
     if (t < KC + 2) {
       const int l = kc0 - 1 + t;
-      dd[t] = (l >= 0 && l < w) ? dval[c0 + l] : 0.0;
-      oo[t] = (l >= 0 && l < w) ? dval[n + c0 + l] : 0.0;
+      dd[t] = (l >= 0 && l < w) ? dval[c0 + l] : (T)0;
+      oo[t] = (l >= 0 && l < w) ? dval[n + c0 + l] : (T)0;
     }
     __syncthreads();
     // B = (L D) columns: 1x1 scaling or the matching row of a 2x2 block,
     // computed in registers, then written back in place
-    double bv[KC * 64 / 256];
+    T bv[KC * 64 / 256];
     for (int q = 0; q < KC * 64 / 256; ++q) {
       const int idx = t + 256 * q;
       const int kk = idx / 64, x = idx % 64;
-      double vb = 0.0;
+      T vb = 0;
       if (kk < kc) {
         const int h = kk + 1;  // index into dd / oo
-        const double x0 = Bs[kk * LDT + x];
-        const double xn = (kk + 1 < kc) ? Bs[(kk + 1) * LDT + x] : halo[1][x];
-        const double xp = (kk > 0) ? Bs[(kk - 1) * LDT + x] : halo[0][x];
+        const T x0 = Bs[kk * LDT + x];
+        const T xn = (kk + 1 < kc) ? Bs[(kk + 1) * LDT + x] : halo[1][x];
+        const T xp = (kk > 0) ? Bs[(kk - 1) * LDT + x] : halo[0][x];
         if (oo[h] != 0.0) vb = dd[h] * x0 + oo[h] * xn;
         else if (oo[h - 1] != 0.0) vb = oo[h - 1] * xp + dd[h] * x0;
         else vb = dd[h] * x0;
@@ -901,7 +933,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
     __syncthreads();
   }
   // target bases and row positions come from LDS on the common path
-  tile_foreach(acc, [&](int ri, int ci, double v) {
+  tile_foreach(acc, [&](int ri, int ci, T v) {
     const int R = rlab[ri], C = clab[ci];
     if (R < 0 || C < 0 || R < C) return;
     const int dt = dtcol[ci];
@@ -915,11 +947,12 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
     atomicAdd(valL + cbase[ci] + pos, -v);
   });
 }
+template <typename T>
 void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                  double* valL, const double* dval, hipStream_t st) {
+                  T* valL, const T* dval, hipStream_t st) {
   if (count <= 0 || ntiles <= 0) return;
   if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur: too many tiles");
-  k_schur<<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+  k_schur<T><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
                                             a.col2sup, valL, dval, a.n);
   HIP_LAUNCH_CHECK("schur");
 }
@@ -975,19 +1008,22 @@ __device__ inline int row_label(const int* rows, int64_t pis, int c0, int w, int
 // blocks below a piece: GROWS rows per workgroup, the four waves split the
 // piece's columns, partial sums meet in LDS.
 #define GROWS SOLVE_ROWS
-__device__ inline double bcast(double v, int lane) { return __shfl(v, lane, 64); }
+template <typename T>
+__device__ inline T bcast(T v, int lane) { return __shfl(v, lane, 64); }
 
 // forward triangle of a piece: Y(p) = L_pp^{-1} X(p)
+template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
                                                  const int* super, const int64_t* pi, const int64_t* px,
-                                                 const double* valL, const double* X, double* Y, int nrhs, int lp) {
-  extern __shared__ double Lt[];  // Lt[j * lp + i] = L(i, j), i > j
+                                                 const T* valL, const T* X, T* Y, int nrhs, int lp) {
+  extern __shared__ double smem_d[];
+  T* Lt = reinterpret_cast<T*>(smem_d);  // Lt[j * lp + i] = L(i, j), i > j
   const int p = pieces[blockIdx.x];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int pw = min(SOLVE_W, w - c0l);
   const int64_t r = pi[s + 1] - pi[s];
-  const double* P = valL + px[s];
+  const T* P = valL + px[s];
   const int t = threadIdx.x;
   for (int idx = t; idx < pw * pw; idx += 256) {
     const int i = idx % pw, j = idx / pw;
@@ -996,20 +1032,20 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
   __syncthreads();
   if (t >= 64) return;
   const int64_t base = (int64_t)(c0 + c0l) * nrhs;
-  double x0[MAXRHS], x1[MAXRHS];
+  T x0[MAXRHS], x1[MAXRHS];
 #pragma unroll
   for (int c = 0; c < MAXRHS; ++c) {
-    x0[c] = (c < nrhs && t < pw) ? X[base + (int64_t)t * nrhs + c] : 0.0;
-    x1[c] = (c < nrhs && t + 64 < pw) ? X[base + (int64_t)(t + 64) * nrhs + c] : 0.0;
+    x0[c] = (c < nrhs && t < pw) ? X[base + (int64_t)t * nrhs + c] : (T)0;
+    x1[c] = (c < nrhs && t + 64 < pw) ? X[base + (int64_t)(t + 64) * nrhs + c] : (T)0;
   }
   for (int j = 0; j < pw - 1; ++j) {
-    const double* col = Lt + j * lp;
-    const double l0 = (t > j && t < pw) ? col[t] : 0.0;
-    const double l1 = (t + 64 > j && t + 64 < pw) ? col[t + 64] : 0.0;
+    const T* col = Lt + j * lp;
+    const T l0 = (t > j && t < pw) ? col[t] : (T)0;
+    const T l1 = (t + 64 > j && t + 64 < pw) ? col[t + 64] : (T)0;
 #pragma unroll
     for (int c = 0; c < MAXRHS; ++c) {
       if (c >= nrhs) break;
-      const double xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
+      const T xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
       x0[c] -= l0 * xj;
       x1[c] -= l1 * xj;
     }
@@ -1022,12 +1058,13 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
   }
 }
 // forward update of the rows below a piece: X(label q) -= L(q, p) Y(p)
+template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* blk_prefix, int np,
                                                   const int* piece_sn, const int* piece_c0, const int* super,
                                                   const int64_t* pi, const int64_t* px, const int* rows,
-                                                  const double* valL, const double* Y, double* X, int nrhs) {
-  __shared__ double ys[SOLVE_W * MAXRHS];
-  __shared__ double part[4][GROWS * MAXRHS];
+                                                  const T* valL, const T* Y, T* X, int nrhs) {
+  __shared__ T ys[SOLVE_W * MAXRHS];
+  __shared__ T part[4][GROWS * MAXRHS];
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1038,21 +1075,21 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* 
   const int c0 = super[s], w = super[s + 1] - c0;
   const int pw = min(SOLVE_W, w - c0l);
   const int64_t r = pi[s + 1] - pi[s];
-  const double* P = valL + px[s];
+  const T* P = valL + px[s];
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   for (int idx = t; idx < pw * nrhs; idx += 256) ys[idx] = Y[(int64_t)(c0 + c0l) * nrhs + idx];
   __syncthreads();
   const int64_t q = c0l + pw + (int64_t)GROWS * rb + ln;
   const bool ok = q < r;
-  double acc[MAXRHS];
+  T acc[MAXRHS];
 #pragma unroll
   for (int c = 0; c < MAXRHS; ++c) acc[c] = 0.0;
   const int jb = wv * (SOLVE_W / 4), je = min(pw, jb + SOLVE_W / 4);
   if (ok) {
     for (int j = jb; j < je; j += 8) {
-      double l[8];
+      T l[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) l[u] = (j + u < je) ? P[q + (int64_t)(c0l + j + u) * r] : 0.0;
+      for (int u = 0; u < 8; ++u) l[u] = (j + u < je) ? P[q + (int64_t)(c0l + j + u) * r] : (T)0;
 #pragma unroll
       for (int u = 0; u < 8; ++u)
 #pragma unroll
@@ -1067,26 +1104,27 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* 
   if (wv == 0 && ok) {
     const int lab = row_label(rows, pi[s], c0, w, q);
     for (int c = 0; c < nrhs; ++c) {
-      const double v = part[0][ln * nrhs + c] + part[1][ln * nrhs + c] + part[2][ln * nrhs + c] +
+      const T v = part[0][ln * nrhs + c] + part[1][ln * nrhs + c] + part[2][ln * nrhs + c] +
                        part[3][ln * nrhs + c];
       atomicAdd(X + (int64_t)lab * nrhs + c, -v);
     }
   }
 }
 // D solve: Z = D^{-1} Y (blocked_2by2_solver with n_rhs = 1)
-__global__ void k_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs) {
+template <typename T>
+__global__ void k_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double od = dval[n + i];
+    const T od = dval[n + i];
     if (od != 0.0) {
-      const double d1 = dval[i], d2 = dval[i + 1];
-      const double inv = 1.0 / (d1 * d2 - od * od);
+      const T d1 = dval[i], d2 = dval[i + 1];
+      const T inv = (T)1 / (d1 * d2 - od * od);
       for (int c = 0; c < nrhs; ++c) {
-        const double x1 = Y[(int64_t)i * nrhs + c], x2 = Y[(int64_t)(i + 1) * nrhs + c];
+        const T x1 = Y[(int64_t)i * nrhs + c], x2 = Y[(int64_t)(i + 1) * nrhs + c];
         Z[(int64_t)i * nrhs + c] = (x1 * d2 - x2 * od) * inv;
         Z[(int64_t)(i + 1) * nrhs + c] = (x2 * d1 - x1 * od) * inv;
       }
     } else if (i == 0 || dval[n + i - 1] == 0.0) {
-      const double inv = 1.0 / dval[i];
+      const T inv = (T)1 / dval[i];
       for (int c = 0; c < nrhs; ++c) Z[(int64_t)i * nrhs + c] = Y[(int64_t)i * nrhs + c] * inv;
     }
   }
@@ -1094,14 +1132,15 @@ __global__ void k_dsolve(int n, const double* dval, const double* Y, double* Z, 
 // backward gather: acc(p) += L(q, p)^T X(label q) over the rows below a piece.
 // The GROWS x pw block is staged in LDS (coalesced along the rows), then
 // each thread sums a column over half of the rows.
+template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* blk_prefix, int np,
                                                   const int* piece_sn, const int* piece_c0, const int* super,
                                                   const int64_t* pi, const int64_t* px, const int* rows,
-                                                  const double* valL, const double* Xf, double* acc,
+                                                  const T* valL, const T* Xf, T* acc,
                                                   int nrhs) {
-  __shared__ double Lb[SOLVE_W * (GROWS + 1)];  // Lb[j * (GROWS + 1) + qq]
-  __shared__ double xq[GROWS * MAXRHS];
-  __shared__ double part[SOLVE_W * MAXRHS];
+  __shared__ T Lb[SOLVE_W * (GROWS + 1)];  // Lb[j * (GROWS + 1) + qq]
+  __shared__ T xq[GROWS * MAXRHS];
+  __shared__ T part[SOLVE_W * MAXRHS];
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1112,7 +1151,7 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
   const int c0 = super[s], w = super[s + 1] - c0;
   const int pw = min(SOLVE_W, w - c0l);
   const int64_t r = pi[s + 1] - pi[s];
-  const double* P = valL + px[s];
+  const T* P = valL + px[s];
   const int t = threadIdx.x;
   const int64_t qb = c0l + pw + (int64_t)GROWS * rb;
   const int nq = (int)min((int64_t)GROWS, r - qb);
@@ -1124,17 +1163,17 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
   }
   for (int idx = t; idx < GROWS * pw; idx += 256) {
     const int qq = idx % GROWS, j = idx / GROWS;
-    Lb[j * (GROWS + 1) + qq] = qq < nq ? P[qb + qq + (int64_t)(c0l + j) * r] : 0.0;
+    Lb[j * (GROWS + 1) + qq] = qq < nq ? P[qb + qq + (int64_t)(c0l + j) * r] : (T)0;
   }
   __syncthreads();
   const int j = t & (SOLVE_W - 1), half = t >> 7;  // 2 x 128 threads: column j, rows of one half
-  double a[MAXRHS];
+  T a[MAXRHS];
 #pragma unroll
   for (int c = 0; c < MAXRHS; ++c) a[c] = 0.0;
   if (j < pw) {
     const int q0 = half * (GROWS / 2);
     for (int qq = q0; qq < q0 + GROWS / 2; ++qq) {
-      const double l = Lb[j * (GROWS + 1) + qq];
+      const T l = Lb[j * (GROWS + 1) + qq];
 #pragma unroll
       for (int c = 0; c < MAXRHS; ++c)
         if (c < nrhs) a[c] += l * xq[qq * nrhs + c];
@@ -1149,17 +1188,19 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
     for (int c = 0; c < nrhs; ++c) atomicAdd(acc + (int64_t)(c0 + c0l + j) * nrhs + c, a[c] + part[j * nrhs + c]);
 }
 // backward triangle: X(p) = L_pp^{-T} (Z(p) - acc(p))
+template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
                                                  const int* super, const int64_t* pi, const int64_t* px,
-                                                 const double* valL, const double* Z, const double* acc,
-                                                 double* Xf, int nrhs, int lp) {
-  extern __shared__ double Lt[];  // Lt[i * lp + j] = L(j, i), j > i (column i of L)
+                                                 const T* valL, const T* Z, const T* acc,
+                                                 T* Xf, int nrhs, int lp) {
+  extern __shared__ double smem_d[];
+  T* Lt = reinterpret_cast<T*>(smem_d);  // Lt[i * lp + j] = L(j, i), j > i (column i of L)
   const int p = pieces[blockIdx.x];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int pw = min(SOLVE_W, w - c0l);
   const int64_t r = pi[s + 1] - pi[s];
-  const double* P = valL + px[s];
+  const T* P = valL + px[s];
   const int t = threadIdx.x;
   for (int idx = t; idx < pw * pw; idx += 256) {
     const int jj = idx % pw, i = idx / pw;
@@ -1168,21 +1209,21 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
   __syncthreads();
   if (t >= 64) return;
   const int64_t base = (int64_t)(c0 + c0l) * nrhs;
-  double x0[MAXRHS], x1[MAXRHS];
+  T x0[MAXRHS], x1[MAXRHS];
 #pragma unroll
   for (int c = 0; c < MAXRHS; ++c) {
-    x0[c] = (c < nrhs && t < pw) ? Z[base + (int64_t)t * nrhs + c] - acc[base + (int64_t)t * nrhs + c] : 0.0;
+    x0[c] = (c < nrhs && t < pw) ? Z[base + (int64_t)t * nrhs + c] - acc[base + (int64_t)t * nrhs + c] : (T)0;
     x1[c] = (c < nrhs && t + 64 < pw) ? Z[base + (int64_t)(t + 64) * nrhs + c] - acc[base + (int64_t)(t + 64) * nrhs + c]
-                                      : 0.0;
+                                      : (T)0;
   }
   for (int j = pw - 1; j > 0; --j) {
     // rows i < j: x_i -= L(j, i) x_j
-    const double l0 = (t < j) ? Lt[t * lp + j] : 0.0;
-    const double l1 = (t + 64 < j) ? Lt[(t + 64) * lp + j] : 0.0;
+    const T l0 = (t < j) ? Lt[t * lp + j] : (T)0;
+    const T l1 = (t + 64 < j) ? Lt[(t + 64) * lp + j] : (T)0;
 #pragma unroll
     for (int c = 0; c < MAXRHS; ++c) {
       if (c >= nrhs) break;
-      const double xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
+      const T xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
       x0[c] -= l0 * xj;
       x1[c] -= l1 * xj;
     }
@@ -1195,57 +1236,62 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
   }
 }
 
+template <typename T>
 void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
-                    const double* valL, const double* X, double* Y, int nrhs, hipStream_t st) {
+                    const T* valL, const T* X, T* Y, int nrhs, hipStream_t st) {
   if (np <= 0) return;
   if (nrhs > MAXRHS) throw std::runtime_error("too many right-hand sides");
   if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("fwd_tri: piece width out of range");
   const int lp = maxpw + 1;
   static bool attr = false;
   if (!attr) {  // the widest triangle (128 x 129 doubles) needs more than the default dynamic LDS
-    if (hipFuncSetAttribute((const void*)k_fwd_tri, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(sizeof(double) * SOLVE_W * (SOLVE_W + 1))) != hipSuccess)
+    if (hipFuncSetAttribute((const void*)k_fwd_tri<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(T) * SOLVE_W * (SOLVE_W + 1))) != hipSuccess)
       throw std::runtime_error("fwd_tri: cannot raise the dynamic LDS limit");
     attr = true;
   }
-  k_fwd_tri<<<np, 256, sizeof(double) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X,
+  k_fwd_tri<T><<<np, 256, sizeof(T) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X,
                                                           Y, nrhs, lp);
   HIP_LAUNCH_CHECK("fwd_tri");
 }
+template <typename T>
 void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
-                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Y, double* X,
+                     const int* piece_c0, const SnArgs& a, const T* valL, const T* Y, T* X,
                      int nrhs, hipStream_t st) {
   if (np <= 0 || nblk <= 0) return;
-  k_fwd_gemv<<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
+  k_fwd_gemv<T><<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
                                    valL, Y, X, nrhs);
   HIP_LAUNCH_CHECK("fwd_gemv");
 }
-void launch_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs, hipStream_t st) {
-  k_dsolve<<<grid_for(n, 256), 256, 0, st>>>(n, dval, Y, Z, nrhs);
+template <typename T>
+void launch_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs, hipStream_t st) {
+  k_dsolve<T><<<grid_for(n, 256), 256, 0, st>>>(n, dval, Y, Z, nrhs);
   HIP_LAUNCH_CHECK("dsolve");
 }
+template <typename T>
 void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
-                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Xf, double* acc,
+                     const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
                      int nrhs, hipStream_t st) {
   if (np <= 0 || nblk <= 0) return;
-  k_bwd_gemv<<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
+  k_bwd_gemv<T><<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
                                    valL, Xf, acc, nrhs);
   HIP_LAUNCH_CHECK("bwd_gemv");
 }
+template <typename T>
 void launch_bwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
-                    const double* valL, const double* Z, const double* acc, double* Xf, int nrhs,
+                    const T* valL, const T* Z, const T* acc, T* Xf, int nrhs,
                     hipStream_t st) {
   if (np <= 0) return;
   if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("bwd_tri: piece width out of range");
   const int lp = maxpw + 1;
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute((const void*)k_bwd_tri, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(sizeof(double) * SOLVE_W * (SOLVE_W + 1))) != hipSuccess)
+    if (hipFuncSetAttribute((const void*)k_bwd_tri<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(T) * SOLVE_W * (SOLVE_W + 1))) != hipSuccess)
       throw std::runtime_error("bwd_tri: cannot raise the dynamic LDS limit");
     attr = true;
   }
-  k_bwd_tri<<<np, 256, sizeof(double) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z,
+  k_bwd_tri<T><<<np, 256, sizeof(T) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z,
                                                           acc, Xf, nrhs, lp);
   HIP_LAUNCH_CHECK("bwd_tri");
 }
@@ -1566,22 +1612,27 @@ namespace lblg {
 // unperturbed refinement matrix K0 (set_diags(0)) and its infinity norm
 // (norm_sparse_int, src/common/Norm.cpp:115-185)
 // --------------------------------------------------------------------------
+// bit 0: |K_jj| < reg (the reference's rule; those diagonals are zero in K0),
+// bit 1: |K_jj| < reg_factor (replaced by reg_factor in the factored matrix)
 __global__ void k_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg,
-                             unsigned char* pert) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x)
-    pert[q] = (isdiag[q] && fabs(Kx[q]) < reg) ? 1 : 0;
+                             double reg_factor, unsigned char* pert) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+    const double a = fabs(Kx[q]);
+    pert[q] = isdiag[q] ? (unsigned char)((a < reg ? 1 : 0) | (a < reg_factor ? 2 : 0)) : (unsigned char)0;
+  }
 }
-void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, int regularization,
-                       unsigned char* pert, hipStream_t st) {
+void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, double reg_factor,
+                       int regularization, unsigned char* pert, hipStream_t st) {
   if (nnz <= 0) return;
-  k_pert_flags<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, isdiag, Kx, regularization == 1 ? reg : -1.0, pert);
+  const bool on = regularization == 1;
+  k_pert_flags<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, isdiag, Kx, on ? reg : -1.0, on ? reg_factor : -1.0, pert);
   HIP_LAUNCH_CHECK("pert_flags");
 }
 __global__ void k_k0_values(int64_t nnzf, const int64_t* src, const double* Kx, const unsigned char* pert,
                             double* Kv) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnzf; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t q = src[e];
-    Kv[e] = pert[q] ? 0.0 : Kx[q];
+    Kv[e] = (pert[q] & 1) ? 0.0 : Kx[q];
   }
 }
 __global__ void k_rowabs_max(int n, const int* rp, const double* Kv, double* part) {
@@ -1613,5 +1664,37 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
   k_max_final<<<1, 64, 0, st>>>(part, nblk, norm_out);
   HIP_LAUNCH_CHECK("max_final");
 }
+// --------------------------------------------------------------------------
+// instantiations for the two factor value types
+// --------------------------------------------------------------------------
+#define LBLG_INSTANTIATE(T)                                                                                      \
+  template void launch_fill<T>(T*, int64_t, double, hipStream_t);                                              \
+  template void launch_assemble<T>(const int64_t*, const double*, const unsigned char*, int, double, T*,       \
+                                   hipStream_t);                                                               \
+  template void launch_bk_small<T>(const int*, int, const SnArgs&, T*, T*, int*, int*, hipStream_t);           \
+  template void launch_bk_panel<T>(const int*, int, int, const SnArgs&, T*, T*, int*, T*, const int64_t*, int*, \
+                                   int*, int*, int*, bool, hipStream_t);                                       \
+  template void launch_trailing<T>(const int*, int, int, int, const SnArgs&, T*, const T*, const int64_t*,      \
+                                   const int*, const int*, hipStream_t);                                       \
+  template void launch_offdiag<T>(const int*, int, const int*, int, const SnArgs&, T*, const T*, const int*,   \
+                                  hipStream_t);                                                                \
+  template void launch_schur<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,         \
+                                hipStream_t);                                                                  \
+  template void launch_fwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \
+                                  const T*, T*, int, hipStream_t);                                             \
+  template void launch_fwd_gemv<T>(const int*, int, const int*, int, const int*, const int*, const SnArgs&,    \
+                                   const T*, const T*, T*, int, hipStream_t);                                  \
+  template void launch_dsolve<T>(int, const T*, const T*, T*, int, hipStream_t);                               \
+  template void launch_bwd_gemv<T>(const int*, int, const int*, int, const int*, const int*, const SnArgs&,    \
+                                   const T*, const T*, T*, int, hipStream_t);                                  \
+  template void launch_bwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \
+                                  const T*, const T*, T*, int, hipStream_t);
+LBLG_INSTANTIATE(double)
+LBLG_INSTANTIATE(float)
+#undef LBLG_INSTANTIATE
+template void launch_convert<float, double>(float*, const double*, int64_t, hipStream_t);
+template void launch_convert<double, float>(double*, const float*, int64_t, hipStream_t);
+
 }  // namespace lblg
+
 

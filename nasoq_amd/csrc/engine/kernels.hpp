@@ -21,16 +21,25 @@ struct SnArgs {  // symbolic arrays on the device
   int n;
 };
 
-void launch_fill(double* p, int64_t count, double v, hipStream_t st);
+template <typename T>
+void launch_fill(T* p, int64_t count, double v, hipStream_t st);
 void launch_fill_int(int* p, int64_t count, int v, hipStream_t st);
 void launch_iota(int* p, int count, hipStream_t st);
 void launch_copy(double* dst, const double* src, int64_t count, hipStream_t st);
+template <typename TO, typename TI>
+void launch_convert(TO* dst, const TI* src, int64_t count, hipStream_t st);
+// The value kernels are templates over the factor's type T (double: plain
+// FP64 factor, float: FP32 factor), instantiated for both in kernels.hip,
+// bk_wide.hip and gemm_big.hip.
+template <typename T>
 void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
-                     double* valL, hipStream_t st);
-void launch_bk_small(const int* list, int count, const SnArgs& a, double* valL, double* dval,
+                     T* valL, hipStream_t st);
+template <typename T>
+void launch_bk_small(const int* list, int count, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, int* status, hipStream_t st);
-void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, double* valL, double* dval,
-                     int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+template <typename T>
+void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL, T* dval,
+                     int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                      int* status, bool cooperative, hipStream_t st);
 // wide panels with flag-in-word exchange (bk_wide.hip)
 constexpr int WIDE_NB = 64;         // panel width
@@ -38,37 +47,49 @@ constexpr int WIDE_ROWS = 192;      // far rows per workgroup
 constexpr int WIDE_MOVED = 128;     // moved indices per panel (scratch rows)
 constexpr int WIDE_SUB = 4;         // panels per trailing update (W holds WIDE_SUB * WIDE_NB columns)
 constexpr int64_t LL_WORDS = 2 * 256 * 4 + 2 * 256 * 8 + 2 * 256 * 2 * 64;
-void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, double* valL, double* dval,
-                    int* perm_piv, double* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
+template <typename T>
+void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
+                    int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                     uint64_t* ll, uint32_t ebase, int* status, hipStream_t st);
-void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, double* valL,
-                     const double* W, const int64_t* woff, const int* kpos, const int* kstart,
+template <typename T>
+void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, T* valL,
+                     const T* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st);
 // 128 x 128 MFMA tiles (gemm_big.hip)
+template <typename T>
 void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncols, const SnArgs& a,
-                     const double* valL, const double* dval, double* LD, const int64_t* ldoff, hipStream_t st);
+                     const T* valL, const T* dval, T* LD, const int64_t* ldoff, hipStream_t st);
+template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                      double* valL, const double* LD, const int64_t* ldoff, hipStream_t st);
-void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, double* valL,
-                         double* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
+                      T* valL, const T* LD, const int64_t* ldoff, hipStream_t st);
+template <typename T>
+void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
+                         T* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
+template <typename T>
 void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
-                    double* valL, const double* dval, const int* perm_piv, hipStream_t st);
+                    T* valL, const T* dval, const int* perm_piv, hipStream_t st);
+template <typename T>
 void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                  double* valL, const double* dval, hipStream_t st);
+                  T* valL, const T* dval, hipStream_t st);
 void launch_relabel(const SnArgs& a, int nsuper, const int* rows0, int* rows, const int* perm_piv,
                     int* newpos, const int* perm0, int* perm_f, int* iperm_f, hipStream_t st);
 constexpr int SOLVE_ROWS = 64;  // rows per workgroup in the sweeps below a piece
+template <typename T>
 void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
-                    const double* valL, const double* X, double* Y, int nrhs, hipStream_t st);
+                    const T* valL, const T* X, T* Y, int nrhs, hipStream_t st);
+template <typename T>
 void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
-                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Y, double* X,
+                     const int* piece_c0, const SnArgs& a, const T* valL, const T* Y, T* X,
                      int nrhs, hipStream_t st);
-void launch_dsolve(int n, const double* dval, const double* Y, double* Z, int nrhs, hipStream_t st);
+template <typename T>
+void launch_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs, hipStream_t st);
+template <typename T>
 void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
-                     const int* piece_c0, const SnArgs& a, const double* valL, const double* Xf, double* acc,
+                     const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
                      int nrhs, hipStream_t st);
+template <typename T>
 void launch_bwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
-                    const double* valL, const double* Z, const double* acc, double* Xf, int nrhs,
+                    const T* valL, const T* Z, const T* acc, T* Xf, int nrhs,
                     hipStream_t st);
 void launch_spmv(int n, const int* rp, const int* ci, const double* v, const int* perm_f, const int* iperm_f,
                  const double* x, double* y, int nrhs, hipStream_t st);
@@ -87,11 +108,11 @@ void launch_gmres_update(int n, int nrhs, int MR, const double* V, const double*
                          hipStream_t st);
 void launch_abs_sums(int n, int nrhs, const double* r, const double* x, const double* b, double* part,
                      int nblk, double* out, hipStream_t st);
-void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, int regularization,
-                       unsigned char* pert, hipStream_t st);
+void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, double reg_factor,
+                       int regularization, unsigned char* pert, hipStream_t st);
 void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const double* Kx,
                const unsigned char* pert, double* Kv, double* part, int nblk, double* norm_out, hipStream_t st);
-void launch_mfma_selftest(const double* A, const double* B, double* C, hipStream_t st);
+void launch_mfma_selftest(const double* A, const double* B, double* C, int f32, hipStream_t st);
 
 // GMRES state per right-hand-side column (doubles)
 inline int64_t gmres_state_size(int MR) { return (int64_t)(MR + 1) * MR + 4 * (MR + 1) + (MR + 2) + 8; }

@@ -58,8 +58,9 @@ def check_factor(O, s, n, Ap, Ai, Ax, reg):
     return F
 
 
-def test_mfma_f64_operand_layout():
-    import ctypes
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mfma_operand_layout(precision):
+    # the lane layout the tile engines assume, for the FP64 and the FP32 16x16x4 MFMA
     from nasoq_amd import _lib
     lib = _lib.load()
     rng = np.random.default_rng(0)
@@ -67,8 +68,12 @@ def test_mfma_f64_operand_layout():
     B = rng.standard_normal((4, 16))
     C = np.zeros((16, 16))
     assert lib.lblg_selftest_mfma(_lib.dptr(np.ascontiguousarray(A)), _lib.dptr(np.ascontiguousarray(B)),
-                                  _lib.dptr(C)) == 0
-    assert np.allclose(C, A @ B, rtol=1e-14, atol=1e-14)
+                                  _lib.dptr(C), precision) == 0
+    if precision == 0:
+        assert np.allclose(C, A @ B, rtol=1e-14, atol=1e-14)
+    else:
+        ref = A.astype(np.float32).astype(np.float64) @ B.astype(np.float32).astype(np.float64)
+        assert np.allclose(C, ref, rtol=1e-5, atol=1e-5)
 
 
 def test_reference_example_lbl_main():
