@@ -138,7 +138,7 @@ def mixed_leg(s, lib, step, bwd, dX, x64, args, torch):
     fwd = [float(np.linalg.norm(x32[c] - x64[c]) / max(np.linalg.norm(x64[c]), 1e-300)) for c in range(x64.shape[0])]
     kt = {}
     for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "schur", "scale_ld", "schur_big",
-                 "fwd_tri", "fwd_gemv", "bwd_gemv", "bwd_tri", "spmv"]:
+                 "fwd_tri", "fwd_gemv", "fwd_chain", "bwd_chain", "bwd_gemv", "bwd_tri", "spmv"]:
         tot, cnt = ctypes.c_double(), ctypes.c_int64()
         lib.lblg_kernel_time(s.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
         kt[name] = tot.value / args.steps
@@ -239,7 +239,7 @@ def main():
 
     kt = {}
     for name in ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "schur", "scale_ld", "schur_big",
-                 "fwd_tri", "fwd_gemv", "bwd_gemv", "bwd_tri", "spmv"]:
+                 "fwd_tri", "fwd_gemv", "fwd_chain", "bwd_chain", "bwd_gemv", "bwd_tri", "spmv"]:
         tot, cnt = ctypes.c_double(), ctypes.c_int64()
         lib.lblg_kernel_time(s.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
         kt[name] = (tot.value / args.steps, cnt.value // max(args.steps, 1))

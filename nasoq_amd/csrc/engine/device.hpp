@@ -51,6 +51,7 @@ struct FactorLevel {
 
 // solve pieces: supernodes split into column blocks of at most SOLVE_W
 struct SolveLevel {
+  std::vector<int> chains;  // wide supernodes swept by one cooperative launch each
   int npieces = 0;
   int maxpw = 1;         // widest piece of the level
   int nblocks = 0;       // total row blocks (SOLVE_ROWS rows) of all pieces
@@ -68,6 +69,8 @@ struct DeviceFactor {
   int* rows = nullptr;     // row labels (ssize), relabelled after factor
   int* rows0 = nullptr;    // pre-pivot row labels (kept for re-factorization)
   int* col2sup = nullptr;
+  std::vector<int> hsuper;      // host copies of super / pi (launch shapes)
+  std::vector<int64_t> hpi;
   // numeric: the factor's value type is fixed at setup (prec 0: double,
   // 1: float); valL, dval, W, LD and the sweep vectors X, Y, Z, acc, O hold
   // that type
@@ -125,6 +128,8 @@ struct DeviceFactor {
   // schedules
   std::vector<FactorLevel> flevels;
   std::vector<SolveLevel> slevels;
+  uint64_t* chain_xb = nullptr;     // flag-in-word exchange of the chain sweeps (epoch-stamped words)
+  uint32_t chain_epoch = 0;
   int* piece_sn = nullptr;      // piece -> supernode
   int* piece_c0 = nullptr;      // piece -> first local column within the supernode
   int npieces = 0;

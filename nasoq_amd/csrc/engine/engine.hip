@@ -117,6 +117,8 @@ static void setup(Handle& h) {
   px[ns] = S.p[n];
   D.super = dupload(D, S.super);
   D.pi = dupload(D, S.pi);
+  D.hsuper = S.super;
+  D.hpi = S.pi;
   D.px = dupload(D, px);
   std::vector<int> rows0(S.s.begin(), S.s.begin() + S.pi[ns]);
   D.rows0 = dupload(D, rows0);
@@ -312,15 +314,49 @@ static void setup(Handle& h) {
     if (pc0[p] + SOLVE_W < w) piece_par[p] = p + 1;
     else if (S.sparent[s] >= 0) piece_par[p] = first_piece[S.sparent[s]];
   }
+  // wide supernodes are swept by one cooperative launch ("chain"): all their
+  // pieces take one level
+  const int chain_w = env_int("LBLG_CHAIN_W", 512);
+  std::vector<char> chain(ns, 0);
+  int chain_pmax = 1;
+  for (int s = 0; s < ns; ++s) {
+    const int w = S.super[s + 1] - S.super[s];
+    const int64_t r = S.pi[s + 1] - S.pi[s];
+    if (w > SOLVE_W && w >= chain_w && chain_groups(w, r, true) > 0 && chain_groups(w, r, false) > 0) {
+      chain[s] = 1;
+      chain_pmax = std::max(chain_pmax, (w + SOLVE_W - 1) / SOLVE_W);
+    }
+  }
   std::vector<int> plev(np, 0);
   int nplev = 0;
   for (int p = 0; p < np; ++p) {  // parents come later in this numbering
+    const int s = psn[p];
+    if (chain[s]) {
+      if (p != first_piece[s]) continue;
+      const int w = S.super[s + 1] - S.super[s];
+      const int lastp = p + (w + SOLVE_W - 1) / SOLVE_W - 1;
+      for (int q = p + 1; q <= lastp; ++q) plev[q] = plev[p];
+      nplev = std::max(nplev, plev[p] + 1);
+      const int par = piece_par[lastp];
+      if (par >= 0) plev[par] = std::max(plev[par], plev[p] + 1);
+      continue;
+    }
     nplev = std::max(nplev, plev[p] + 1);
     if (piece_par[p] >= 0) plev[piece_par[p]] = std::max(plev[piece_par[p]], plev[p] + 1);
   }
   std::vector<std::vector<int>> pbylev(nplev);
-  for (int p = 0; p < np; ++p) pbylev[plev[p]].push_back(p);
   D.slevels.resize(nplev);
+  for (int p = 0; p < np; ++p) {
+    const int s = psn[p];
+    if (!chain[s]) pbylev[plev[p]].push_back(p);
+    else if (p == first_piece[s]) D.slevels[plev[p]].chains.push_back(s);
+  }
+  {  // one published block of SOLVE_W x MAX_RHS values per piece, two words per value
+    const int64_t words = (int64_t)chain_pmax * SOLVE_W * MAX_RHS * 2;
+    D.chain_xb = reinterpret_cast<uint64_t*>(dalloc<double>(D, words));
+    launch_fill(reinterpret_cast<double*>(D.chain_xb), words, 0.0, D.stream);
+  }
+  D.chain_epoch = 0;
   for (int l = 0; l < nplev; ++l) {
     SolveLevel& L = D.slevels[l];
     std::vector<int> pre;
@@ -541,26 +577,46 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   }
   for (size_t l = 0; l < D.slevels.size(); ++l) {
     const SolveLevel& L = D.slevels[l];
-    timed(D, "fwd_tri", [&] {
-      launch_fwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)X, Y, nrhs, st);
-    });
-    timed(D, "fwd_gemv", [&] {
-      launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)Y, X,
-                      nrhs, st);
-    });
+    if (L.npieces > 0) {
+      timed(D, "fwd_tri", [&] {
+        launch_fwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)X, Y, nrhs, st);
+      });
+      timed(D, "fwd_gemv", [&] {
+        launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)Y, X,
+                        nrhs, st);
+      });
+    }
+    for (int s : L.chains) {
+      const int w = D.hsuper[s + 1] - D.hsuper[s];
+      const int64_t r = D.hpi[s + 1] - D.hpi[s];
+      const uint32_t ep = ++D.chain_epoch;
+      timed(D, "fwd_chain", [&] {
+        launch_fwd_chain(s, w, r, a, valL, X, Y, nrhs, D.chain_xb, ep, D.status, st);
+      });
+    }
   }
   launch_dsolve(D.n, dval, (const T*)Y, Z, nrhs, st);
   launch_fill(acc, nn, 0.0, st);
   for (size_t l = D.slevels.size(); l-- > 0;) {
     const SolveLevel& L = D.slevels[l];
-    timed(D, "bwd_gemv", [&] {
-      launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)O, acc,
-                      nrhs, st);
-    });
-    timed(D, "bwd_tri", [&] {
-      launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)Z, (const T*)acc, O,
-                     nrhs, st);
-    });
+    for (int s : L.chains) {
+      const int w = D.hsuper[s + 1] - D.hsuper[s];
+      const int64_t r = D.hpi[s + 1] - D.hpi[s];
+      const uint32_t ep = ++D.chain_epoch;
+      timed(D, "bwd_chain", [&] {
+        launch_bwd_chain(s, w, r, a, valL, (const T*)Z, acc, O, nrhs, D.chain_xb, ep, D.status, st);
+      });
+    }
+    if (L.npieces > 0) {
+      timed(D, "bwd_gemv", [&] {
+        launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)O, acc,
+                        nrhs, st);
+      });
+      timed(D, "bwd_tri", [&] {
+        launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)Z, (const T*)acc,
+                       O, nrhs, st);
+      });
+    }
   }
   if constexpr (!std::is_same<T, double>::value) launch_convert(out, (const T*)O, nn, st);
 }
@@ -691,6 +747,14 @@ void solve_ordered(Handle& h, double* xo, int nrhs, double* bwd_out) {
   if (bwd_out) backward_errors(h, D.B, xo, nrhs, bwd_out);
 }
 
+// a chain sweep whose bounded wait ran out leaves status bit 16
+static void check_solve_status(DeviceFactor& D) {
+  int st0 = 0;
+  HIPCHK(hipMemcpyAsync(&st0, D.status, sizeof(int), hipMemcpyDeviceToHost, D.stream));
+  HIPCHK(hipStreamSynchronize(D.stream));
+  if (st0 & 16) throw std::runtime_error("chain sweep: cross-workgroup wait timed out");
+}
+
 void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx, bool device_ptrs,
            double* bwd_out) {
   DeviceFactor& D = *h.dev;
@@ -714,6 +778,7 @@ void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t
   double* xo = D.acc2 ? D.acc2 : (D.acc2 = dalloc<double>(D, (int64_t)D.n * D.nrhs_cap));
   solve_ordered(h, xo, nrhs, bwd_out);
   launch_scatter_sol(D.n, nrhs, xo, D.perm_f, Xd, device_ptrs ? ldx : D.n, st);
+  check_solve_status(D);
   if (!device_ptrs) {
     for (int c = 0; c < nrhs; ++c)
       HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, tmpX + (int64_t)c * D.n, sizeof(double) * D.n,

@@ -91,6 +91,14 @@ template <typename T>
 void launch_bwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
                     const T* valL, const T* Z, const T* acc, T* Xf, int nrhs,
                     hipStream_t st);
+// one wide supernode's sweep in a single cooperative launch (solve_chain.hip)
+int chain_groups(int w, int64_t r, bool forward);  // workgroups, or -1 if too large
+template <typename T>
+void launch_fwd_chain(int s, int w, int64_t r, const SnArgs& a, const T* valL, T* X, T* Y, int nrhs,
+                      uint64_t* xb, uint32_t epoch, int* status, hipStream_t st);
+template <typename T>
+void launch_bwd_chain(int s, int w, int64_t r, const SnArgs& a, const T* valL, const T* Z, T* offacc, T* Xf,
+                      int nrhs, uint64_t* xb, uint32_t epoch, int* status, hipStream_t st);
 void launch_spmv(int n, const int* rp, const int* ci, const double* v, const int* perm_f, const int* iperm_f,
                  const double* x, double* y, int nrhs, hipStream_t st);
 void launch_gather_rhs(int n, int nrhs, const double* B, int64_t ldb, const int* perm_f, double* out,

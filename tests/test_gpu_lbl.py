@@ -178,3 +178,39 @@ def test_cpp_dropin_lbl_main(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "Solution: 4,4" in out.stdout
+
+
+def _launches(s, name):
+    import ctypes
+    tot, cnt = ctypes.c_double(), ctypes.c_int64()
+    s.lib.lblg_kernel_time(s.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
+    return cnt.value
+
+
+@pytest.mark.parametrize("chain_w,group_w,grid,m,prec", [(129, 2048, 12, 300, 0), (129, 96, 14, 500, 0),
+                                                         (129, 2048, 14, 500, 1), (100000, 2048, 14, 500, 0)])
+def test_chain_sweeps(oracle, chain_w, group_w, grid, m, prec, monkeypatch):
+    # supernodes wider than chain_w are swept by one cooperative launch each
+    # (units of 128 rows chained by per-piece flags), the others piece by piece
+    monkeypatch.setenv("LBLG_CHAIN_W", str(chain_w))
+    monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
+    n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 6, seed=3, nrhs=3)
+    print(f"case chain n={n} chain_w={chain_w} group_w={group_w} prec={prec}", flush=True)
+    reg = 1e-9
+    s = engine(n, Ap, Ai, Ax, reg=reg, precision=prec)
+    assert s.lib.lblg_set_kernel_timing(s.h, 1) == 0
+    if prec == 0:
+        F = oracle_factor(oracle, s, n, Ap, Ai, Ax, reg)
+        s.req_ref_iter = 0
+        X0 = s.solve_only(B)
+        for c in range(B.shape[1]):
+            xo = F.solve(B[:, c])
+            assert np.linalg.norm(X0[:, c] - xo) <= 1e-6 * max(1.0, np.linalg.norm(xo))
+    s.target_bwd_err = 1e-10
+    s.max_refine_rounds = 30
+    X, bwd = s.solve_only(B, return_bwd=True)
+    for c in range(B.shape[1]):
+        assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
+    chains = _launches(s, "fwd_chain")
+    assert (chains > 0) == (chain_w < 1000), chains
+    assert _launches(s, "bwd_chain") == chains
