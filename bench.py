@@ -151,14 +151,145 @@ def mixed_leg(s, lib, step, bwd, dX, x64, args, torch):
             "forward_error_vs_fp64_per_rhs": fwd, "kernel_ms_per_step": kt}
 
 
+def bench_batch(args):
+    """--config batch1024: the batched leg alone, as its own JSON line."""
+    print(json.dumps(batch_leg(args)[1]))
+
+
+def batch_leg(args):
+    """Batched form: the KKT systems [H A'; A 0] of a batch of QPs of uneven
+    size (2-D Laplacian r x c, r, c ~ U{16..160}, + 1e-2 I; 1..64 equality
+    rows of 4 N(0,1) entries) factored and solved to a 1e-10 backward error
+    (checked per system) in one block-diagonal handle, FP64 and with the FP32
+    factor, against the plain path solving the same systems one by one."""
+    from nasoq_amd import _lib, lbl
+    from nasoq_amd import problems as P
+    import torch
+
+    t = time.perf_counter()
+    inst = P.batch_qp_kkts(args.batch, seed=0)
+    sizes = np.array([x[0] for x in inst])
+    rhs = np.concatenate([x[4] for x in inst])
+    print(f"[bench] batch of {len(inst)} KKTs, N={sizes.sum()} (n {sizes.min()}..{sizes.max()}) generated in "
+          f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    b = lbl.BatchSolverSettings([(n, Ap, Ai, Ax) for n, Ap, Ai, Ax, _ in inst])
+    b.reg_diag = 1e-9
+    b.target_bwd_err = 1e-10  # the worst system's backward error (per-system check on the device)
+    b.max_refine_rounds = 20
+    t = time.perf_counter()
+    assert b.symbolic_analysis() == 1
+    fl, _ = b.flops()
+    print(f"[bench] batch symbolic {time.perf_counter() - t:.1f}s nsuper={len(b.sym_ints('super')) - 1} "
+          f"xsize={b.sym_scalar('xsize'):.3e} sum c^2={fl:.3e}", file=sys.stderr, flush=True)
+    lib = b.lib
+    N = b.n
+    dB = torch.from_numpy(rhs.copy()).to("cuda")
+    dX = torch.zeros_like(dB)
+    bwd = np.zeros(1)
+
+    def run(prec):
+        b.precision = prec
+        b.numerical_factorization()
+
+        def step():
+            _lib.check(lib.lblg_refactor_resident(b.h), "refactor")
+            _lib.check(lib.lblg_solve_device(b.h, 1, ctypes.c_void_p(dB.data_ptr()), N,
+                                             ctypes.c_void_p(dX.data_ptr()), N, _lib.dptr(bwd)), "solve")
+        for _ in range(args.warmup):
+            step()
+        _lib.check(lib.lblg_synchronize(b.h), "sync")
+        torch.cuda.synchronize()
+        rounds = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            rounds.append(int(lib.lblg_num_ref_iter(b.h)))
+        _lib.check(lib.lblg_synchronize(b.h), "sync")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        X = dX.cpu().numpy()
+        err = b.backward_errors(rhs, X)[:, 0]
+        assert err.max() <= 1e-10, f"batched (precision {prec}) worst per-system backward error {err.max()}"
+        print(f"[bench] batched precision={prec}: {dt / args.steps * 1e3:.1f} ms per batch, rounds={rounds}, "
+              f"worst per-system bwd {err.max():.2e}", file=sys.stderr, flush=True)
+        return dt / args.steps, err.max(), rounds, X
+
+    t64, e64, r64, X64 = run(0)
+    t32, e32, r32, X32 = run(1)
+    fwd32 = float(max(np.linalg.norm(X32[o:o + n] - X64[o:o + n]) / max(np.linalg.norm(X64[o:o + n]), 1e-300)
+                      for o, n in zip(b.offsets[:-1], sizes)))
+    b.close()
+
+    # the plain path on the same systems, one handle after another (analysis and
+    # first factor untimed, as in the batched legs)
+    t_one, worst_one = 0.0, 0.0
+    for k, (n, Ap, Ai, Ax, bk) in enumerate(inst):
+        s = lbl.SolverSettings(n, Ap, Ai, Ax)
+        s.reg_diag = 1e-9
+        s.target_bwd_err = 1e-10
+        s.max_refine_rounds = 20
+        assert s.symbolic_analysis() == 1
+        s.numerical_factorization()
+        o = int(np.sum(sizes[:k]))
+        pb = ctypes.c_void_p(dB.data_ptr() + 8 * o)
+        px = ctypes.c_void_p(dX.data_ptr() + 8 * o)
+        one = np.zeros(1)
+        _lib.check(s.lib.lblg_solve_device(s.h, 1, pb, n, px, n, _lib.dptr(one)), "solve")  # warm
+        _lib.check(s.lib.lblg_synchronize(s.h), "sync")
+        t0 = time.perf_counter()
+        _lib.check(s.lib.lblg_refactor_resident(s.h), "refactor")
+        _lib.check(s.lib.lblg_solve_device(s.h, 1, pb, n, px, n, _lib.dptr(one)), "solve")
+        _lib.check(s.lib.lblg_synchronize(s.h), "sync")
+        t_one += time.perf_counter() - t0
+        worst_one = max(worst_one, float(one[0]))
+        s.close()
+    print(f"[bench] one by one (plain FP64): {t_one * 1e3:.1f} ms for the batch, worst bwd {worst_one:.2e}",
+          file=sys.stderr, flush=True)
+    nb = len(inst)
+    out = {
+        "metric": METRIC,
+        "value": nb / t64,
+        "unit": "KKT solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": f"batched FP64 LBL factor+solve+refine of {nb} QP KKT systems [H A'; A 0], "
+                               "H = 2-D Laplacian r x c (r, c ~ U{16..160}) + 1e-2 I, 1..64 equality rows",
+                   "systems": nb, "N_total": int(sizes.sum()), "n_min": int(sizes.min()), "n_max": int(sizes.max()),
+                   "nrhs": 1, "backward_error": 1e-10, "parallelism": "single GPU", "sum_colcount_sq": fl},
+        "worst_backward_error": float(e64),
+        "refine_rounds": r64,
+        "batched_fp32_factor": {"value": nb / t32, "ms_per_step": t32 * 1e3, "worst_backward_error": float(e32),
+                                "refine_rounds": r32, "forward_error_vs_fp64_max": fwd32},
+        "one_by_one_plain_fp64": {"value": nb / t_one, "ms_for_batch": t_one * 1e3,
+                                  "worst_backward_error": worst_one},
+        "speedup_batched_vs_one_by_one": t_one / t64,
+        "speedup_batched_fp32_vs_one_by_one": t_one / t32,
+    }
+    summary = {"workload": out["config"]["workload"], "systems": nb, "N_total": int(sizes.sum()),
+               "value": nb / t64, "unit": "KKT solves/s", "ms_per_batch": t64 * 1e3,
+               "worst_backward_error": float(e64), "fp32_factor_value": nb / t32,
+               "fp32_factor_worst_backward_error": float(e32), "one_by_one_plain_fp64_value": nb / t_one,
+               "speedup_vs_one_by_one": t_one / t64}
+    return summary, out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS) + ["batch1024"])
+    ap.add_argument("--batch", type=int, default=1024, help="instances of the batch1024 configuration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor + FP64-refinement leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference traffic for HBM counters)")
     args = ap.parse_args()
@@ -182,6 +313,9 @@ def main():
     from nasoq_amd import problems as P
     import torch
 
+    if args.config == "batch1024":
+        bench_batch(args)
+        return
     grid, m, k = CONFIGS[args.config]
     t = time.perf_counter()
     n, Ap, Ai, Ax, B = P.kkt_config(grid, m, k, shift=1e-3, seed=0, nrhs=4)
@@ -297,6 +431,11 @@ def main():
     if not args.no_mixed:
         out["mixed"] = mixed_leg(s, lib, step, bwd, dX, x64, args, torch)
     s.close()
+    del dB, dX
+    torch.cuda.empty_cache()
+    if not args.no_batched and args.config == "kkt64":
+        print("[bench] batched leg ...", file=sys.stderr, flush=True)
+        out["batched"] = batch_leg(args)[0]
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.config, fl)

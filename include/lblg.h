@@ -197,6 +197,28 @@ int lblg_kernel_time(lblg_handle* h, const char* name, double* total_ms, int64_t
 int lblg_selftest_mfma(const double* A, const double* B, double* C, int precision);
 
 /* ---------------------------------------------------------------------------
+ * Batched systems (engine extension; the reference factors one matrix per
+ * SolverSettings): `count` independent symmetric matrices of uneven size, each
+ * lower CSC with the diagonal first, held by one handle as a block-diagonal
+ * matrix.  lblg_batch_analyze orders every block by its own nested dissection
+ * (or naturally), so each block's symbolic result is the one it would have
+ * alone; lblg_factor / lblg_refactor_resident then factor all blocks in the
+ * same level-scheduled launches, and lblg_solve / lblg_solve_device solve
+ * all blocks at once (right-hand sides concatenated in block order).  For a
+ * batch, the backward error the solve calls return and that target-driven
+ * refinement checks is the worst system's.
+ * ------------------------------------------------------------------------- */
+int lblg_batch_create(lblg_handle** h, int count, const int* n, const int* const* Ap, const int* const* Ai,
+                      const double* const* Ax);
+int lblg_batch_analyze(lblg_handle* h, const lblg_settings* s);
+/* count, and offsets[count + 1]: first row of every block (may be NULL) */
+int lblg_batch_info(const lblg_handle* h, int* count, int64_t* offsets);
+/* per block and column, the backward error of SolverSettings::backward_error
+ * (linear_solver_wrapper.cpp:1462-1498) on that block: err[c * count + k] */
+int lblg_batch_backward_errors(const lblg_handle* h, int nrhs, const double* B, int64_t ldb, const double* X,
+                               int64_t ldx, double* err);
+
+/* ---------------------------------------------------------------------------
  * NASOQ QP solver over the engine:  min 1/2 x'Hx + q'x  s.t.  Ax = a, Bx <= b
  * (nasoq::Nasoq, include/nasoq/nasoq.h:61-493, src/nasoq.cpp).  The active-set
  * loop runs on the host; every KKT factorization and solve is a device call

@@ -102,6 +102,11 @@ SIGNATURES = [
     ("lblg_factor_info", ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
     ("lblg_pivot_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
     ("lblg_panel_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
+    ("lblg_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _i32p, _vp, _vp, _vp]),
+    ("lblg_batch_analyze", ctypes.c_int, [_vp, _vp]),
+    ("lblg_batch_info", ctypes.c_int, [_vp, _i32p, _i64p]),
+    ("lblg_batch_backward_errors", ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int64, _f64p, ctypes.c_int64,
+                                                   _f64p]),
     ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p, ctypes.c_int]),
     ("lblg_set_kernel_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("lblg_kernel_time", ctypes.c_int, [_vp, ctypes.c_char_p, _f64p, _i64p]),

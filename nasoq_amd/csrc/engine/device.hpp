@@ -102,6 +102,10 @@ struct DeviceFactor {
   unsigned char* isdiag = nullptr;
   int64_t nnzf = 0;             // entries of the full CSR
   double* k0norm = nullptr;     // ||K0||_inf on the device
+  int nseg = 0;                 // systems of a block-diagonal batch (0: one system)
+  int64_t* seg = nullptr;       // their first rows (nseg + 1), in both orderings
+  double* segnorm = nullptr;    // ||K0||_inf of every system
+  double* segsums = nullptr;    // |r|, |x|, |b| sums per system and column
   int* sync = nullptr;          // group-barrier counters
   uint64_t* ll = nullptr;       // flag-in-word exchange buffer of the wide panel kernel
   uint32_t ll_epoch = 0;        // next epoch base (monotone for the handle's lifetime)

@@ -159,6 +159,12 @@ static void setup(Handle& h) {
   D.isdiag = dupload(D, isdiag);
   D.red = dalloc<double>(D, 1024);
   D.k0norm = dalloc<double>(D, 1);
+  if (h.batch_off.size() > 2) {  // a batch: backward errors are checked per system
+    D.nseg = (int)h.batch_off.size() - 1;
+    D.seg = dupload(D, h.batch_off);
+    D.segnorm = dalloc<double>(D, D.nseg);
+    D.segsums = dalloc<double>(D, (int64_t)D.nseg * 3 * MAX_RHS);
+  }
 
   // full CSR of K in the original ordering (structure) and its source entries
   {
@@ -506,6 +512,7 @@ static void factor_device_t(Handle& h) {
   launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
   // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
   launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
+  launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
   int stat[6] = {0, 0, 0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(stat), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -695,6 +702,25 @@ static void backward_errors(Handle& h, const double* b, const double* x, int nrh
   const int64_t nn = (int64_t)D.n * nrhs;
   spmv_ordered(D, x, D.T, nrhs);
   launch_sub(nn, b, D.T, D.R, D.stream);
+  if (D.nseg > 0) {
+    // a block-diagonal batch: every system's own backward error, the worst per column
+    // (the systems occupy the same contiguous row ranges in both orderings)
+    launch_seg_abs_sums(D.nseg, D.seg, nrhs, D.R, x, b, D.segsums, D.stream);
+    std::vector<double> sums((size_t)D.nseg * 3 * nrhs), nrm(D.nseg);
+    HIPCHK(hipMemcpyAsync(sums.data(), D.segsums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, D.stream));
+    HIPCHK(hipMemcpyAsync(nrm.data(), D.segnorm, sizeof(double) * nrm.size(), hipMemcpyDeviceToHost, D.stream));
+    HIPCHK(hipStreamSynchronize(D.stream));
+    for (int c = 0; c < nrhs; ++c) {
+      double worst = 0.0;
+      for (int k = 0; k < D.nseg; ++k) {
+        const double* sk = sums.data() + (size_t)k * 3 * nrhs;
+        const double den = nrm[k] * sk[nrhs + c] + sk[2 * nrhs + c];
+        worst = std::max(worst, den > 0 ? sk[c] / den : 0.0);
+      }
+      err[c] = worst;
+    }
+    return;
+  }
   launch_abs_sums(D.n, nrhs, D.R, x, b, D.part, D.nblk_red, D.dots, D.stream);
   std::vector<double> sums(3 * nrhs);
   HIPCHK(hipMemcpyAsync(sums.data(), D.dots, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, D.stream));

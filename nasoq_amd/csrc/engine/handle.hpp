@@ -32,6 +32,7 @@ struct Handle {
   int64_t wide_panels = 0;       // wide panels factored
   int64_t wide_panels_pivoted = 0;  // wide panels whose 1x1 no-interchange pass stopped early
   double factor_seconds = 0.0;
+  std::vector<int64_t> batch_off;  // block-diagonal batch: first column of each system (count + 1)
   std::unique_ptr<DeviceFactor> dev;
   Handle();
   ~Handle();

@@ -1603,6 +1603,54 @@ void launch_abs_sums(int n, int nrhs, const double* r, const double* x, const do
   k_dot_final<<<1, 64, 0, st>>>(part, nblk, 3, nrhs, out);
   HIP_LAUNCH_CHECK("abs_final");
 }
+// the same sums per segment of rows (the systems of a block-diagonal batch):
+// out[(3 * k + which) * nrhs + c], one workgroup per segment
+__global__ void k_seg_abs_sums(const int64_t* seg, int nrhs, const double* r, const double* x, const double* b,
+                               double* out) {
+  __shared__ double sh[4][3 * MAXRHS];
+  const int k = blockIdx.x, t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  double acc[3 * MAXRHS];
+  for (int c = 0; c < 3 * MAXRHS; ++c) acc[c] = 0.0;
+  for (int64_t i = seg[k] + t; i < seg[k + 1]; i += 256)
+    for (int c = 0; c < nrhs; ++c) {
+      acc[c] += fabs(r[i * nrhs + c]);
+      acc[nrhs + c] += fabs(x[i * nrhs + c]);
+      acc[2 * nrhs + c] += fabs(b[i * nrhs + c]);
+    }
+  for (int c = 0; c < 3 * nrhs; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (ln == 0) sh[wv][c] = v;
+  }
+  __syncthreads();
+  if (t < 3 * nrhs) out[(int64_t)k * 3 * nrhs + t] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
+}
+void launch_seg_abs_sums(int nseg, const int64_t* seg, int nrhs, const double* r, const double* x, const double* b,
+                         double* out, hipStream_t st) {
+  if (nseg <= 0) return;
+  k_seg_abs_sums<<<nseg, 256, 0, st>>>(seg, nrhs, r, x, b, out);
+  HIP_LAUNCH_CHECK("seg_abs_sums");
+}
+// max row sum of |K0| per segment (norm_sparse_int of each system)
+__global__ void k_seg_rowabs_max(const int64_t* seg, const int* rp, const double* Kv, double* out) {
+  __shared__ double sh[4];
+  const int k = blockIdx.x, t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  double m = 0.0;
+  for (int64_t i = seg[k] + t; i < seg[k + 1]; i += 256) {
+    double s = 0.0;
+    for (int q = rp[i]; q < rp[i + 1]; ++q) s += fabs(Kv[q]);
+    m = fmax(m, s);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if (ln == 0) sh[wv] = m;
+  __syncthreads();
+  if (t == 0) out[k] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+void launch_seg_rowabs_max(int nseg, const int64_t* seg, const int* rp, const double* Kv, double* out,
+                           hipStream_t st) {
+  if (nseg <= 0) return;
+  k_seg_rowabs_max<<<nseg, 256, 0, st>>>(seg, rp, Kv, out);
+  HIP_LAUNCH_CHECK("seg_rowabs_max");
+}
 
 }  // namespace lblg
 

@@ -116,6 +116,11 @@ void launch_gmres_update(int n, int nrhs, int MR, const double* V, const double*
                          hipStream_t st);
 void launch_abs_sums(int n, int nrhs, const double* r, const double* x, const double* b, double* part,
                      int nblk, double* out, hipStream_t st);
+// per-system reductions of a block-diagonal batch (segments of rows)
+void launch_seg_abs_sums(int nseg, const int64_t* seg, int nrhs, const double* r, const double* x, const double* b,
+                         double* out, hipStream_t st);
+void launch_seg_rowabs_max(int nseg, const int64_t* seg, const int* rp, const double* Kv, double* out,
+                           hipStream_t st);
 void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, double reg_factor,
                        int regularization, unsigned char* pert, hipStream_t st);
 void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const double* Kx,

@@ -120,6 +120,56 @@ class SolverSettings:
         return fl.value, lnz.value
 
 
+class BatchSolverSettings(SolverSettings):
+    """Many independent symmetric systems (lower CSC, diagonal first) in one
+    handle: a block-diagonal matrix whose blocks are ordered by their own
+    nested dissection.  Factor and solve as SolverSettings; right-hand sides
+    and solutions are the blocks' vectors concatenated in order."""
+
+    def __init__(self, blocks):
+        self.lib = _lib.load()
+        self.blocks = [(int(n), np.ascontiguousarray(Ap, dtype=np.int32), np.ascontiguousarray(Ai, dtype=np.int32),
+                        np.ascontiguousarray(Ax, dtype=np.float64)) for n, Ap, Ai, Ax in blocks]
+        cnt = len(self.blocks)
+        ns = np.array([b[0] for b in self.blocks], dtype=np.int32)
+        P = (ctypes.c_void_p * cnt)(*[b[1].ctypes.data for b in self.blocks])
+        I = (ctypes.c_void_p * cnt)(*[b[2].ctypes.data for b in self.blocks])
+        X = (ctypes.c_void_p * cnt)(*[b[3].ctypes.data for b in self.blocks])
+        self.h = ctypes.c_void_p()
+        check(self.lib.lblg_batch_create(ctypes.byref(self.h), cnt, iptr(ns), ctypes.cast(P, ctypes.c_void_p),
+                                         ctypes.cast(I, ctypes.c_void_p), ctypes.cast(X, ctypes.c_void_p)),
+              "lblg_batch_create")
+        c = ctypes.c_int()
+        self.offsets = np.zeros(cnt + 1, dtype=np.int64)
+        check(self.lib.lblg_batch_info(self.h, ctypes.byref(c), lptr(self.offsets)), "lblg_batch_info")
+        self.n = int(self.offsets[-1])
+        st = _lib.Settings()
+        self.lib.lblg_default_settings(ctypes.byref(st))
+        for f, _ in _lib.Settings._fields_:
+            setattr(self, f, list(getattr(st, f)) if f in ("n_relax", "z_relax") else getattr(st, f))
+        self.perm_in = None
+        self.extra_cols = None
+        self.num_ref_iter = 0
+
+    def symbolic_analysis(self, perm=None, extra_cols=None):
+        assert perm is None and extra_cols is None, "batches are ordered block by block"
+        st = self._settings()
+        ok = self.lib.lblg_batch_analyze(self.h, ctypes.byref(st)) == 0
+        if ok:
+            self.ordering = ORDER_GIVEN  # the handle records the composed block ordering
+        return 1 if ok else 0
+
+    def backward_errors(self, B, X):
+        """(count, nrhs) backward errors of every block and column."""
+        B = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(self.n, -1))
+        X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(self.n, -1))
+        cnt = len(self.blocks)
+        err = np.zeros(cnt * B.shape[1])
+        check(self.lib.lblg_batch_backward_errors(self.h, B.shape[1], dptr(B), self.n, dptr(X), self.n, dptr(err)),
+              "lblg_batch_backward_errors")
+        return err.reshape(B.shape[1], cnt).T
+
+
 def nd_order(n, Ap, Ai):
     """The engine's nested-dissection ordering (perm[new] = old)."""
     lib = _lib.load()

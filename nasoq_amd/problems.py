@@ -23,8 +23,7 @@ def lower_csc(K):
     Ai = L.indices.astype(np.int32)
     Ax = L.data.astype(np.float64)
     # restore the diagonal values (may be exact zeros)
-    for j in range(n):
-        assert Ai[Ap[j]] == j
+    assert np.array_equal(Ai[Ap[:-1]], np.arange(n))
     Ax[Ap[:-1]] = d
     return n, Ap, Ai, Ax
 
@@ -125,3 +124,25 @@ def block_indefinite(nblk, bs, seed=0, coupling=0.3, nrhs=1):
         K[b * bs:(b + 1) * bs, (b + 1) * bs:(b + 2) * bs] = C.T
     B = rng.uniform(-1.0, 1.0, size=(n, nrhs))
     return lower_csc(K.tocsc()) + (B,)
+
+
+def batch_qp_kkts(count=1024, seed=0, nmin=16, nmax=160, meq_max=64, nnz_row=4):
+    """The KKT systems [H A'; A 0] of a batch of QPs of uneven size: H the
+    5-point Laplacian of an r x c grid (r, c ~ U{nmin..nmax}) + 1e-2 I,
+    A ~ U{1..meq_max} rows of nnz_row N(0,1) entries, right-hand side
+    [-q; A x0] with q ~ N(0,1) and x0 ~ U[-1,1].  Returns a list of
+    (n, Ap, Ai, Ax, b) with the lower CSC of each KKT (explicit zero
+    diagonal on the constraint block)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        rr, cc = int(rng.integers(nmin, nmax + 1)), int(rng.integers(nmin, nmax + 1))
+        H = laplacian_2d(rr, cc, 1e-2)
+        nh = H.shape[0]
+        m = int(rng.integers(1, meq_max + 1))
+        A = random_rows(rng, m, nh, nnz_row)
+        q = rng.standard_normal(nh)
+        x0 = rng.uniform(-1.0, 1.0, nh)
+        n, Ap, Ai, Ax = lower_csc(kkt(H, A))
+        out.append((n, Ap, Ai, Ax, np.concatenate([-q, A @ x0])))
+    return out
