@@ -291,6 +291,20 @@ int lblg_qp_solve(lblg_qp* qp, int* status);
 int lblg_qp_solve_warm(lblg_qp* qp, const double* x0, const double* z0, const int* active, int n_active,
                        const double* y0, int* converged);
 
+/* Warm start from a predicted active set (engine extension for the learned
+ * warm start; no reference counterpart): the predicted constraints are made
+ * equalities, those whose multipliers come out negative are dropped until
+ * the KKT point is dual feasible, then the dual active-set loop runs from
+ * there (as after set_warm_start, nasoq.cpp:413-436) to the same optimum as
+ * lblg_qp_solve.  *status: LBLG_QP_*; *prune_solves: KKT solves spent on the
+ * predicted set (the loop's own iterations are in lblg_qp_info.num_iter). */
+int lblg_qp_solve_predicted(lblg_qp* qp, const int* active, int n_active, int* status, int* prune_solves);
+
+/* New linear terms q (n), a (me), b (mi) for the next solve; a NULL pointer
+ * keeps that vector.  The super-KKT pattern and its analysis are kept (engine
+ * extension: the reference builds a new Nasoq for new data, nasoq.h:125-134). */
+int lblg_qp_set_linear(lblg_qp* qp, const double* q, const double* a, const double* b);
+
 /* primal (n), dual_ineq (mi), dual_eq (me), active set (n_active ints); any may be NULL. */
 int lblg_qp_get(const lblg_qp* qp, double* primal, double* dual_ineq, double* dual_eq, int* active);
 int lblg_qp_get_info(const lblg_qp* qp, lblg_qp_info* info);
@@ -321,6 +335,31 @@ int lblg_kkt_edit(lblg_qp* qp, int add, int count, const int* rows);
 int lblg_kkt_factor(lblg_qp* qp);
 /* Solve the super-KKT (solve_only, mode 1): rhs and x have n + me + mi entries. */
 int lblg_kkt_solve(lblg_qp* qp, const double* rhs, double* x, double* bwd_err);
+
+/* ---------------------------------------------------------------------------
+ * Active-set predictor of the learned warm start (engine extension; no
+ * reference counterpart): an MLP in -> h1 -> h2 -> out with ReLU hidden
+ * layers and sigmoid outputs, BF16 weights and FP32 accumulation on the
+ * GPU's BF16 MFMA, trained with binary cross-entropy and Adam.  Parameters
+ * are one FP32 vector: W1 (h1 x in, row-major), b1, W2 (h2 x h1), b2,
+ * W3 (out x h2), b3.  X rows hold `in` features, Y / P rows `out` labels /
+ * probabilities; all buffers are host memory.
+ * ------------------------------------------------------------------------- */
+typedef struct lblg_mlp lblg_mlp;
+int lblg_mlp_create(lblg_mlp** m, int in, int h1, int h2, int out, uint64_t seed, int max_batch);
+int lblg_mlp_destroy(lblg_mlp* m);
+int64_t lblg_mlp_num_params(const lblg_mlp* m);
+int lblg_mlp_get_params(lblg_mlp* m, float* params);
+/* also resets the Adam state */
+int lblg_mlp_set_params(lblg_mlp* m, const float* params);
+/* gradient of the BCE averaged over all batch x out entries (no update), and that loss */
+int lblg_mlp_gradients(lblg_mlp* m, const float* X, const float* Y, int batch, float* grad, float* loss);
+/* `epochs` passes of Adam over shuffled full minibatches; *last_loss: mean loss of the last pass */
+int lblg_mlp_train(lblg_mlp* m, const float* X, const float* Y, int nsamples, int batch, int epochs, float lr,
+                   uint64_t seed, float* last_loss);
+int lblg_mlp_predict(lblg_mlp* m, const float* X, int nsamples, float* P);
+/* one 16x16x16 BF16 MFMA, C = A B (row-major, inputs rounded to BF16): lane layout check */
+int lblg_selftest_mfma_bf16(const float* A, const float* B, float* C);
 
 #ifdef __cplusplus
 }

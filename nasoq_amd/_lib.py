@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "liblblg.so")
 _i32p = ctypes.POINTER(ctypes.c_int)
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -118,6 +119,19 @@ SIGNATURES = [
     ("lblg_qp_analyze", ctypes.c_int, [_vp]),
     ("lblg_qp_solve", ctypes.c_int, [_vp, _i32p]),
     ("lblg_qp_solve_warm", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, ctypes.c_int, _f64p, _i32p]),
+    ("lblg_qp_solve_predicted", ctypes.c_int, [_vp, _i32p, ctypes.c_int, _i32p, _i32p]),
+    ("lblg_qp_set_linear", ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
+    ("lblg_mlp_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_uint64, ctypes.c_int]),
+    ("lblg_mlp_destroy", ctypes.c_int, [_vp]),
+    ("lblg_mlp_num_params", ctypes.c_int64, [_vp]),
+    ("lblg_mlp_get_params", ctypes.c_int, [_vp, _f32p]),
+    ("lblg_mlp_set_params", ctypes.c_int, [_vp, _f32p]),
+    ("lblg_mlp_gradients", ctypes.c_int, [_vp, _f32p, _f32p, ctypes.c_int, _f32p, _f32p]),
+    ("lblg_mlp_train", ctypes.c_int, [_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                      ctypes.c_uint64, _f32p]),
+    ("lblg_mlp_predict", ctypes.c_int, [_vp, _f32p, ctypes.c_int, _f32p]),
+    ("lblg_selftest_mfma_bf16", ctypes.c_int, [_f32p, _f32p, _f32p]),
     ("lblg_qp_get", ctypes.c_int, [_vp, _f64p, _f64p, _f64p, _i32p]),
     ("lblg_qp_get_info", ctypes.c_int, [_vp, ctypes.POINTER(QPInfo)]),
     ("lblg_qp_super_kkt", ctypes.c_int64, [_vp, _i32p, _i32p, _f64p]),
@@ -159,6 +173,10 @@ def lptr(a):
 
 def dptr(a):
     return a.ctypes.data_as(_f64p) if a is not None else None
+
+
+def fptr(a):
+    return a.ctypes.data_as(_f32p) if a is not None else None
 
 
 def check(rc, what):

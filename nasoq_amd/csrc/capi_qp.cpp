@@ -123,6 +123,15 @@ extern "C" int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* st) {
   return LBLG_OK;
 }
 
+extern "C" int lblg_qp_set_linear(lblg_qp* qp, const double* q, const double* a, const double* b) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s) return LBLG_ERR_ARG;
+  if (q) s->q.assign(q, q + s->n);
+  if (a && s->me) s->a.assign(a, a + s->me);
+  if (b && s->mi) s->b.assign(b, b + s->mi);
+  return LBLG_OK;
+}
+
 extern "C" int lblg_qp_analyze(lblg_qp* qp) {
   QpSolver* s = reinterpret_cast<QpSolver*>(qp);
   if (!s) return LBLG_ERR_ARG;
@@ -152,6 +161,17 @@ extern "C" int lblg_qp_solve_warm(lblg_qp* qp, const double* x0, const double* z
     if (s->mi) z.assign(z0, z0 + s->mi);
     const int c = s->solve_warm(x0, z.data(), active, n_active, y0);
     if (converged) *converged = c;
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_qp_solve_predicted(lblg_qp* qp, const int* active, int n_active, int* status, int* prune_solves) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s || n_active < 0 || (n_active > 0 && !active)) return LBLG_ERR_ARG;
+  return guarded([&] {
+    const int st = s->solve_predicted(active, n_active);
+    if (status) *status = st;
+    if (prune_solves) *prune_solves = s->predicted_prunes;
     return LBLG_OK;
   });
 }

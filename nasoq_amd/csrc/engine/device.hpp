@@ -126,12 +126,17 @@ struct DeviceFactor {
   double* gm = nullptr;         // GMRES small state
   double* part = nullptr;       // reduction partials
   double* dots = nullptr;       // reduction results
-  const double** ptrs = nullptr;
   int nblk_red = 512;
   int mr_cap = 0;
   // schedules
   std::vector<FactorLevel> flevels;
   std::vector<SolveLevel> slevels;
+  // supernodes by level for the one-workgroup sweeps of small factors
+  int* ss_order = nullptr;
+  int* ss_levptr = nullptr;
+  int ss_nlev = 0;
+  int64_t ss_nnzL = 0, ss_max_nnz = 0;
+  double *stage_b = nullptr, *stage_x = nullptr;  // host-pointer solves: staged blocks (n x nrhs_cap)
   uint64_t* chain_xb = nullptr;     // flag-in-word exchange of the chain sweeps (epoch-stamped words)
   uint32_t chain_epoch = 0;
   int* piece_sn = nullptr;      // piece -> supernode

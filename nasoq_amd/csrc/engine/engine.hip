@@ -357,6 +357,28 @@ static void setup(Handle& h) {
     if (!chain[s]) pbylev[plev[p]].push_back(p);
     else if (p == first_piece[s]) D.slevels[plev[p]].chains.push_back(s);
   }
+  {  // supernode levels (parents come later in the numbering) for the one-workgroup sweeps
+    std::vector<int> lev(ns, 0);
+    int nl = 0;
+    for (int s = 0; s < ns; ++s) {
+      nl = std::max(nl, lev[s] + 1);
+      const int par = S.sparent[s];
+      if (par >= 0) {
+        if (par <= s) throw std::runtime_error("supernode tree: parent numbered before child");
+        lev[par] = std::max(lev[par], lev[s] + 1);
+      }
+    }
+    std::vector<int> ptr(nl + 1, 0), order(ns);
+    for (int s = 0; s < ns; ++s) ++ptr[lev[s] + 1];
+    for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int s = 0; s < ns; ++s) order[pos[lev[s]]++] = s;
+    D.ss_order = dupload(D, order);
+    D.ss_levptr = dupload(D, ptr);
+    D.ss_nlev = nl;
+    D.ss_nnzL = S.p[n];
+    D.ss_max_nnz = env_int("LBLG_SMALL_SOLVE_NNZ", 1 << 20);
+  }
   {  // one published block of SOLVE_W x MAX_RHS values per piece, two words per value
     const int64_t words = (int64_t)chain_pmax * SOLVE_W * MAX_RHS * 2;
     D.chain_xb = reinterpret_cast<uint64_t*>(dalloc<double>(D, words));
@@ -556,7 +578,9 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
   D.nblk_red = 512;
   D.part = dalloc<double>(D, (int64_t)D.nblk_red * 3 * cap * (mrc + 2));
   D.dots = dalloc<double>(D, 3 * cap * (mrc + 2));
-  D.ptrs = dalloc<const double*>(D, 2 * (mrc + 2));
+  D.acc2 = dalloc<double>(D, nn);
+  D.stage_b = dalloc<double>(D, nn);
+  D.stage_x = dalloc<double>(D, nn);
   D.nrhs_cap = cap;
   D.mr_cap = mrc;
 }
@@ -574,6 +598,13 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   T* Y = static_cast<T*>(D.Y);
   T* Z = static_cast<T*>(D.Z);
   T* acc = static_cast<T*>(D.acc);
+  if (D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && sizeof(T) * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS) {
+    timed(D, "solve_small", [&] {
+      launch_solve_small<T>(D.ss_order, D.ss_levptr, D.ss_nlev, a, static_cast<const T*>(D.valL),
+                            static_cast<const T*>(D.dval), in, out, nrhs, st);
+    });
+    return;
+  }
   T* O;
   if constexpr (std::is_same<T, double>::value) {
     launch_copy(X, in, nn, st);
@@ -639,15 +670,8 @@ static void spmv_ordered(DeviceFactor& D, const double* x, double* y, int nrhs) 
   timed(D, "spmv", [&] { launch_spmv(D.n, D.Krp, D.Kci, D.Kv, D.perm_f, D.iperm_f, x, y, nrhs, D.stream); });
 }
 
-static void dots(DeviceFactor& D, int nrhs, const std::vector<std::pair<const double*, const double*>>& pairs,
-                 double* out) {
-  std::vector<const double*> ptr(2 * pairs.size());
-  for (size_t i = 0; i < pairs.size(); ++i) {
-    ptr[i] = pairs[i].first;
-    ptr[pairs.size() + i] = pairs[i].second;
-  }
-  HIPCHK(hipMemcpyAsync(D.ptrs, ptr.data(), sizeof(double*) * ptr.size(), hipMemcpyHostToDevice, D.stream));
-  launch_dots(D.n, nrhs, D.ptrs, D.ptrs + pairs.size(), (int)pairs.size(), D.part, D.nblk_red, out, D.stream);
+static void dots(DeviceFactor& D, int nrhs, const double* a, const double* b, double* out) {
+  launch_dot(D.n, nrhs, a, b, D.part, D.nblk_red, out, D.stream);
 }
 
 // one GMRES(mr) cycle per column, pmgmres_ldlt_auto (mgmres.cpp:1015-1211)
@@ -665,28 +689,28 @@ static void gmres_cycle(DeviceFactor& D, const double* b, double* x, int nrhs, i
   spmv_ordered(D, x, D.T, nrhs);
   launch_sub(nn, b, D.T, D.R, st);
   apply_solve(D, D.R, D.Xs, nrhs);
-  dots(D, nrhs, {{D.Xs, D.Xs}}, D.dots);
+  dots(D, nrhs, D.Xs, D.Xs, D.dots);
   launch_gmres_scalar(0, nrhs, MR, D.gm, D.dots, 0, 0, itr, tol_abs, tol_rel, st);
   launch_fill(D.V, nn * (MR + 1), 0.0, st);
   launch_axpby(n, nrhs, coef0, (int)SS, D.Xs, Vk(0), mask, (int)SS, 0, 1, st);
   for (int k = 0; k < mr; ++k) {
     spmv_ordered(D, Vk(k), D.T, nrhs);
     apply_solve(D, D.T, Vk(k + 1), nrhs);
-    dots(D, nrhs, {{Vk(k + 1), Vk(k + 1)}}, D.dots);
+    dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
     launch_gmres_scalar(1, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
     for (int j = 0; j <= k; ++j) {
-      dots(D, nrhs, {{Vk(k + 1), Vk(j)}}, D.dots);
+      dots(D, nrhs, Vk(k + 1), Vk(j), D.dots);
       launch_gmres_scalar(2, nrhs, MR, D.gm, D.dots, j, k, itr, tol_abs, tol_rel, st);
       launch_axpby(n, nrhs, coef0, (int)SS, Vk(j), Vk(k + 1), mask, (int)SS, 0, 0, st);
     }
-    dots(D, nrhs, {{Vk(k + 1), Vk(k + 1)}}, D.dots);
+    dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
     launch_gmres_scalar(3, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
     for (int j = 0; j <= k; ++j) {  // conditional re-orthogonalisation (flag on device)
-      dots(D, nrhs, {{Vk(k + 1), Vk(j)}}, D.dots);
+      dots(D, nrhs, Vk(k + 1), Vk(j), D.dots);
       launch_gmres_scalar(4, nrhs, MR, D.gm, D.dots, j, k, itr, tol_abs, tol_rel, st);
       launch_axpby(n, nrhs, coef0, (int)SS, Vk(j), Vk(k + 1), reo, (int)SS, 0, 0, st);
     }
-    dots(D, nrhs, {{Vk(k + 1), Vk(k + 1)}}, D.dots);
+    dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
     launch_gmres_scalar(5, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
     launch_axpby(n, nrhs, coef0, (int)SS, Vk(k + 1), Vk(k + 1), mask, (int)SS, 0, 1, st);
     launch_gmres_scalar(6, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
@@ -742,22 +766,15 @@ void solve_ordered(Handle& h, double* xo, int nrhs, double* bwd_out) {
   apply_solve(D, D.B, xo, nrhs);
   h.num_ref_iter = 0;
   if (s.req_ref_iter > 0 || s.target_bwd_err > 0) {
-    launch_fill(D.gm, gmres_state_size(D.mr_cap) * nrhs, 0.0, st);
-    // mark every column running
-    std::vector<double> on(nrhs, 1.0);
-    for (int c = 0; c < nrhs; ++c)
-      HIPCHK(hipMemcpyAsync(D.gm + c * gmres_state_size(D.mr_cap) + gmres_off_S(D.mr_cap) + 3, on.data(),
-                            sizeof(double), hipMemcpyHostToDevice, st));
+    // zero state, every column marked running
+    launch_gmres_reset(D.gm, gmres_state_size(D.mr_cap), nrhs, gmres_off_S(D.mr_cap) + 3, st);
     if (s.target_bwd_err > 0) {
       std::vector<double> err(nrhs);
       backward_errors(h, D.B, xo, nrhs, err.data());
       int rounds = 0;
       while (*std::max_element(err.begin(), err.end()) > s.target_bwd_err && rounds < s.max_refine_rounds) {
         // each round: one reference GMRES cycle (itr_max = 1 with mr inner steps)
-        launch_fill(D.gm, gmres_state_size(D.mr_cap) * nrhs, 0.0, st);
-        for (int c = 0; c < nrhs; ++c)
-          HIPCHK(hipMemcpyAsync(D.gm + c * gmres_state_size(D.mr_cap) + gmres_off_S(D.mr_cap) + 3, on.data(),
-                                sizeof(double), hipMemcpyHostToDevice, st));
+        launch_gmres_reset(D.gm, gmres_state_size(D.mr_cap), nrhs, gmres_off_S(D.mr_cap) + 3, st);
         gmres_cycle(D, D.B, xo, nrhs, mr, 0, 0.0, 0.0);
         ++rounds;
         backward_errors(h, D.B, xo, nrhs, err.data());
@@ -789,34 +806,31 @@ void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t
   hipStream_t st = D.stream;
   const double* Bd = B;
   double* Xd = X;
-  double* tmpB = nullptr;
-  double* tmpX = nullptr;
-  if (!device_ptrs) {
-    tmpB = dalloc<double>(D, (int64_t)D.n * nrhs);
-    tmpX = dalloc<double>(D, (int64_t)D.n * nrhs);
-    for (int c = 0; c < nrhs; ++c)
-      HIPCHK(hipMemcpyAsync(tmpB + (int64_t)c * D.n, B + (int64_t)c * ldb, sizeof(double) * D.n,
-                            hipMemcpyHostToDevice, st));
-    Bd = tmpB;
-    Xd = tmpX;
+  if (!device_ptrs) {  // staged through the handle's blocks (sized with the rhs capacity)
+    if (ldb == D.n) {
+      HIPCHK(hipMemcpyAsync(D.stage_b, B, sizeof(double) * D.n * nrhs, hipMemcpyHostToDevice, st));
+    } else {
+      for (int c = 0; c < nrhs; ++c)
+        HIPCHK(hipMemcpyAsync(D.stage_b + (int64_t)c * D.n, B + (int64_t)c * ldb, sizeof(double) * D.n,
+                              hipMemcpyHostToDevice, st));
+    }
+    Bd = D.stage_b;
+    Xd = D.stage_x;
   }
   launch_gather_rhs(D.n, nrhs, Bd, device_ptrs ? ldb : D.n, D.perm_f, D.B, st);
-  double* xo = D.acc2 ? D.acc2 : (D.acc2 = dalloc<double>(D, (int64_t)D.n * D.nrhs_cap));
+  double* xo = D.acc2;
   solve_ordered(h, xo, nrhs, bwd_out);
   launch_scatter_sol(D.n, nrhs, xo, D.perm_f, Xd, device_ptrs ? ldx : D.n, st);
-  check_solve_status(D);
   if (!device_ptrs) {
-    for (int c = 0; c < nrhs; ++c)
-      HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, tmpX + (int64_t)c * D.n, sizeof(double) * D.n,
-                            hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    // release the staging buffers
-    for (double* p : {tmpB, tmpX}) {
-      auto it = std::find(D.owned.begin(), D.owned.end(), (void*)p);
-      if (it != D.owned.end()) D.owned.erase(it);
-      hipFree(p);
+    if (ldx == D.n) {
+      HIPCHK(hipMemcpyAsync(X, D.stage_x, sizeof(double) * D.n * nrhs, hipMemcpyDeviceToHost, st));
+    } else {
+      for (int c = 0; c < nrhs; ++c)
+        HIPCHK(hipMemcpyAsync(X + (int64_t)c * ldx, D.stage_x + (int64_t)c * D.n, sizeof(double) * D.n,
+                              hipMemcpyDeviceToHost, st));
     }
   }
+  check_solve_status(D);
 }
 
 }  // namespace lblg

@@ -15,6 +15,7 @@
 // steps of pmgmres_ldlt_auto (src/gmres/mgmres.cpp:1015-1211).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
@@ -64,6 +65,17 @@ void launch_fill_int(int* p, int64_t count, int v, hipStream_t st) {
   if (count <= 0) return;
   k_fill_int<<<grid_for(count, 256), 256, 0, st>>>(p, count, v);
   HIP_LAUNCH_CHECK("fill_int");
+}
+// GMRES state of nrhs columns (stride ss): zero, except word flag_off of each
+// column, which is set to 1 (the column's "running" flag)
+__global__ void k_gmres_reset(double* gm, int64_t ss, int nrhs, int64_t flag_off) {
+  const int64_t count = ss * nrhs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    gm[i] = (i % ss == flag_off) ? 1.0 : 0.0;
+}
+void launch_gmres_reset(double* gm, int64_t ss, int nrhs, int64_t flag_off, hipStream_t st) {
+  k_gmres_reset<<<grid_for(ss * nrhs, 256), 256, 0, st>>>(gm, ss, nrhs, flag_off);
+  HIP_LAUNCH_CHECK("gmres_reset");
 }
 void launch_iota(int* p, int count, hipStream_t st) {
   if (count <= 0) return;
@@ -1358,41 +1370,44 @@ __device__ inline double* gs_col(double* gm, int c, int MR) {
 #define GS_S(st, MR) (GS_D(st, MR) + (MR + 2))  // scalars: [0]rho [1]rho_tol [2]av [3]active [4]inner_done [5]k_last [6]used [7]norm_tmp
 
 // partial dot products of nd pairs (a_k, b_k) for every column; part[blk][k][c]
-__global__ void k_dot_partial(int n, int nrhs, const double* const* as, const double* const* bs, int nd,
-                              double* part) {
-  __shared__ double sh[4][MAXRHS * 4];
+// dot products per column of a (n x nrhs) and b: one partial per workgroup
+__global__ void k_dot_partial(int n, int nrhs, const double* a, const double* b, double* part) {
+  __shared__ double sh[4][MAXRHS];
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  for (int d = 0; d < nd; ++d) {
-    const double* a = as[d];
-    const double* b = bs[d];
-    double acc[MAXRHS];
-    for (int c = 0; c < nrhs; ++c) acc[c] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x)
-      for (int c = 0; c < nrhs; ++c) acc[c] += a[(int64_t)i * nrhs + c] * b[(int64_t)i * nrhs + c];
-    for (int c = 0; c < nrhs; ++c) {
-      const double v = wave_sum(acc[c]);
-      if (ln == 0) sh[wv][c] = v;
-    }
-    __syncthreads();
-    if (t < nrhs) part[((int64_t)blockIdx.x * nd + d) * nrhs + t] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
-    __syncthreads();
+  double acc[MAXRHS];
+  for (int c = 0; c < nrhs; ++c) acc[c] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < nrhs; ++c) acc[c] += a[(int64_t)i * nrhs + c] * b[(int64_t)i * nrhs + c];
+  for (int c = 0; c < nrhs; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (ln == 0) sh[wv][c] = v;
   }
+  __syncthreads();
+  if (t < nrhs) part[(int64_t)blockIdx.x * nrhs + t] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
 }
-// ordered sums of partials: out[d*nrhs + c]
-__global__ void k_dot_final(const double* part, int nblk, int nd, int nrhs, double* out) {
-  const int t = threadIdx.x;
-  if (t < nd * nrhs) {
-    const int d = t / nrhs, c = t % nrhs;
+// out[v] = sum over blocks of part[b * nv + v], in a fixed order (strided
+// per-thread sums, then a fixed tree), so repeated calls agree bit for bit
+__global__ void k_red_final(const double* part, int nblk, int nv, double* out) {
+  __shared__ double sh[4];
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  for (int v = 0; v < nv; ++v) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[((int64_t)b * nd + d) * nrhs + c];
-    out[t] = s;
+    for (int b = t; b < nblk; b += 256) s += part[(int64_t)b * nv + v];
+    s = wave_sum(s);
+    if (ln == 0) sh[wv] = s;
+    __syncthreads();
+    if (t == 0) out[v] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
   }
 }
-void launch_dots(int n, int nrhs, const double* const* as_dev, const double* const* bs_dev, int nd,
-                 double* part, int nblk, double* out, hipStream_t st) {
-  k_dot_partial<<<nblk, 256, 0, st>>>(n, nrhs, as_dev, bs_dev, nd, part);
+// workgroups for a reduction over n rows: enough to cover n, at most cap
+static int red_blocks(int n, int cap) { return (int)std::max<int64_t>(1, std::min<int64_t>(cap, (n + 255) / 256)); }
+void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,
+                hipStream_t st) {
+  nblk = red_blocks(n, nblk);
+  k_dot_partial<<<nblk, 256, 0, st>>>(n, nrhs, a, b, part);
   HIP_LAUNCH_CHECK("dot_partial");
-  k_dot_final<<<1, 64, 0, st>>>(part, nblk, nd, nrhs, out);
+  k_red_final<<<1, 256, 0, st>>>(part, nblk, nrhs, out);
   HIP_LAUNCH_CHECK("dot_final");
 }
 
@@ -1598,9 +1613,10 @@ __global__ void k_abs_sums(int n, int nrhs, const double* r, const double* x, co
 }
 void launch_abs_sums(int n, int nrhs, const double* r, const double* x, const double* b, double* part,
                      int nblk, double* out, hipStream_t st) {
+  nblk = red_blocks(n, nblk);
   k_abs_sums<<<nblk, 256, 0, st>>>(n, nrhs, r, x, b, part);
   HIP_LAUNCH_CHECK("abs_sums");
-  k_dot_final<<<1, 64, 0, st>>>(part, nblk, 3, nrhs, out);
+  k_red_final<<<1, 256, 0, st>>>(part, nblk, 3 * nrhs, out);
   HIP_LAUNCH_CHECK("abs_final");
 }
 // the same sums per segment of rows (the systems of a block-diagonal batch):

@@ -25,6 +25,7 @@ template <typename T>
 void launch_fill(T* p, int64_t count, double v, hipStream_t st);
 void launch_fill_int(int* p, int64_t count, int v, hipStream_t st);
 void launch_iota(int* p, int count, hipStream_t st);
+void launch_gmres_reset(double* gm, int64_t ss, int nrhs, int64_t flag_off, hipStream_t st);
 void launch_copy(double* dst, const double* src, int64_t count, hipStream_t st);
 template <typename TO, typename TI>
 void launch_convert(TO* dst, const TI* src, int64_t count, hipStream_t st);
@@ -105,8 +106,18 @@ void launch_gather_rhs(int n, int nrhs, const double* B, int64_t ldb, const int*
                        hipStream_t st);
 void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, double* X, int64_t ldx,
                         hipStream_t st);
-void launch_dots(int n, int nrhs, const double* const* as_dev, const double* const* bs_dev, int nd,
-                 double* part, int nblk, double* out, hipStream_t st);
+// L, D and L^T sweeps of a small factor in one workgroup with the ordered
+// block (n x nrhs, rows interleaved) resident in LDS; supernodes level by
+// level in order[levptr[l] .. levptr[l + 1]).  Requires n * nrhs *
+// sizeof(T) <= SOLVE_SMALL_LDS.
+constexpr size_t SOLVE_SMALL_LDS = 128 * 1024;
+template <typename T>
+void launch_solve_small(const int* order, const int* levptr, int nlev, const SnArgs& a, const T* valL,
+                        const T* dval, const double* in, double* out, int nrhs, hipStream_t st);
+// per-column dot products out[c] = sum_i a(i, c) b(i, c) (rows interleaved by
+// column), reduced in a fixed order through at most nblk partials
+void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,
+                hipStream_t st);
 void launch_sub(int64_t count, const double* b, const double* y, double* r, hipStream_t st);
 void launch_axpby(int n, int nrhs, const double* coefs, int cstride, const double* x, double* y,
                   const double* mask, int mstride, int mask_off, int mode, hipStream_t st);

@@ -1,0 +1,230 @@
+// L, D and L^T sweeps of a small factor in ONE workgroup.
+//
+// The level-scheduled sweeps (k_fwd_tri / k_fwd_gemv / k_bwd_gemv /
+// k_bwd_tri) take four launches per level of the supernodal tree; for the
+// KKT systems of an active-set iteration (a few hundred to a few thousand
+// rows, tens of levels) the launches, not the arithmetic, set the time, and
+// GMRES refinement repeats the sweeps several times per solve.  Here one
+// workgroup of 16 waves keeps the whole ordered right-hand-side block in
+// LDS and walks the same level schedule with a barrier between levels:
+// wave k of a level takes supernodes k, k + 16, ...; within a supernode the
+// 64 lanes hold 64 columns of the diagonal block (unit-lower triangle by
+// lane broadcasts, column by column), and the rows below are updated with
+// lanes over rows (forward: LDS atomics into the ancestors' entries, which
+// supernodes of one level may share) or gathered (backward: lanes over rows
+// with a wave sum per column for narrow chunks, lanes over columns
+// otherwise).  The arithmetic is the reference's blocked solve
+// (src/triangularSolve/BLAS.cpp, blocked L / D / L^T solves over the
+// supernodal factor) in the factor's type T.
+#include <hip/hip_runtime.h>
+
+#include "engine/device_util.hpp"
+#include "engine/kernels.hpp"
+
+namespace lblg {
+
+namespace {
+
+constexpr int SS_THREADS = 1024;
+constexpr int SS_WAVES = SS_THREADS / 64;
+
+// value of lane j (uniform j) in every lane
+__device__ inline double ss_lane(double v, int j) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), j);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline float ss_lane(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+template <typename T>
+__device__ inline T ss_wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// forward sweep of supernode s by one wave: x(s) = L_ss^{-1} x(s), then
+// x(label q) -= L(q, s) x(s) for the rows below
+template <typename T, int NR>
+__device__ void ss_forward(int s, int ln, const SnArgs& a, const T* valL, T* x) {
+  const int c0 = a.super[s], w = a.super[s + 1] - c0;
+  const int64_t pis = a.pi[s];
+  const int64_t r = a.pi[s + 1] - pis;
+  const T* P = valL + a.px[s];
+  for (int cb = 0; cb < w; cb += 64) {
+    const int cw = min(64, w - cb);
+    T v[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
+    for (int j = 0; j + 1 < cw; ++j) {
+      const T l = (ln > j && ln < cw) ? P[(cb + ln) + (int64_t)(cb + j) * r] : (T)0;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) v[c] -= l * ss_lane(v[c], j);
+    }
+    if (ln < cw) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+    }
+    for (int64_t q0 = cb + cw; q0 < r; q0 += 64) {
+      const int64_t q = q0 + ln;
+      const bool ok = q < r;
+      T acc[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = (T)0;
+      for (int j = 0; j < cw; ++j) {
+        const T l = ok ? P[q + (int64_t)(cb + j) * r] : (T)0;
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] += l * ss_lane(v[c], j);
+      }
+      if (ok) {
+        if (q < w) {  // a later chunk of this supernode: its rows are this wave's alone
+#pragma unroll
+          for (int c = 0; c < NR; ++c) x[(c0 + (int)q) * NR + c] -= acc[c];
+        } else {      // an ancestor's row, shared with other supernodes of the level
+          const int lab = a.rows[pis + q];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) atomicAdd(&x[lab * NR + c], -acc[c]);
+        }
+      }
+    }
+  }
+}
+
+// backward sweep of supernode s by one wave: x(s) = L_ss^{-T} (x(s) - L(q, s)^T x(label q))
+template <typename T, int NR>
+__device__ void ss_backward(int s, int ln, const SnArgs& a, const T* valL, T* x) {
+  const int c0 = a.super[s], w = a.super[s + 1] - c0;
+  const int64_t pis = a.pi[s];
+  const int64_t r = a.pi[s + 1] - pis;
+  const T* P = valL + a.px[s];
+  for (int cb = ((w - 1) / 64) * 64; cb >= 0; cb -= 64) {
+    const int cw = min(64, w - cb);
+    const int64_t qb = cb + cw, rr = r - qb;  // rows below the chunk
+    T v[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
+    if (rr > 0) {
+      if ((int64_t)cw * ((rr + 63) / 64 + 6 * NR) < rr) {
+        // narrow chunk, many rows: lanes over rows, one wave sum per column
+        for (int j = 0; j < cw; ++j) {
+          const T* col = P + (int64_t)(cb + j) * r;
+          T acc[NR];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] = (T)0;
+          for (int64_t q = qb + ln; q < r; q += 64) {
+            const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
+            const T l = col[q];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[c] += l * x[lab * NR + c];
+          }
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const T tot = ss_wave_sum(acc[c]);
+            if (ln == j) v[c] -= tot;
+          }
+        }
+      } else if (ln < cw) {
+        // lanes over columns, rows in order
+        const T* col = P + (int64_t)(cb + ln) * r;
+        for (int64_t q = qb; q < r; ++q) {
+          const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
+          const T l = col[q];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) v[c] -= l * x[lab * NR + c];
+        }
+      }
+    }
+    for (int i = cw - 1; i > 0; --i) {
+      const T l = ln < i ? P[(cb + i) + (int64_t)(cb + ln) * r] : (T)0;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) v[c] -= l * ss_lane(v[c], i);
+    }
+    if (ln < cw) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+    }
+  }
+}
+
+// out = (L D L^T)^{-1} in for the ordered, row-interleaved blocks in / out
+// (n x NR); supernodes listed level by level in order[levptr[l] .. levptr[l+1])
+template <typename T, int NR>
+__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, const int* levptr, int nlev, SnArgs a,
+                                                            const T* valL, const T* dval, const double* in,
+                                                            double* out) {
+  extern __shared__ double smem_d[];
+  T* x = reinterpret_cast<T*>(smem_d);
+  const int n = a.n, t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  for (int i = t; i < n * NR; i += SS_THREADS) x[i] = (T)in[i];
+  __syncthreads();
+  for (int l = 0; l < nlev; ++l) {
+    for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_forward<T, NR>(order[k], ln, a, valL, x);
+    __syncthreads();
+  }
+  // D: 1x1 pivots, and 2x2 pivots (off-diagonal in dval[n + i]) by their first row
+  for (int i = t; i < n; i += SS_THREADS) {
+    const T od = dval[n + i];
+    if (od != (T)0) {
+      const T d1 = dval[i], d2 = dval[i + 1];
+      const T inv = (T)1 / (d1 * d2 - od * od);
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const T x1 = x[i * NR + c], x2 = x[(i + 1) * NR + c];
+        x[i * NR + c] = (x1 * d2 - x2 * od) * inv;
+        x[(i + 1) * NR + c] = (x2 * d1 - x1 * od) * inv;
+      }
+    } else if (i == 0 || dval[n + i - 1] == (T)0) {
+      const T inv = (T)1 / dval[i];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) x[i * NR + c] *= inv;
+    }
+  }
+  __syncthreads();
+  for (int l = nlev - 1; l >= 0; --l) {
+    for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_backward<T, NR>(order[k], ln, a, valL, x);
+    __syncthreads();
+  }
+  for (int i = t; i < n * NR; i += SS_THREADS) out[i] = (double)x[i];
+}
+
+template <typename T, int NR>
+void solve_small_nr(const int* order, const int* levptr, int nlev, const SnArgs& a, const T* valL, const T* dval,
+                    const double* in, double* out, hipStream_t st) {
+  const size_t lds = sizeof(T) * (size_t)a.n * NR;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void*)k_solve_small<T, NR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOLVE_SMALL_LDS) != hipSuccess)
+      throw std::runtime_error("solve_small: LDS attribute");
+    attr = true;
+  }
+  k_solve_small<T, NR><<<1, SS_THREADS, lds, st>>>(order, levptr, nlev, a, valL, dval, in, out);
+  HIP_LAUNCH_CHECK("solve_small");
+}
+
+}  // namespace
+
+template <typename T>
+void launch_solve_small(const int* order, const int* levptr, int nlev, const SnArgs& a, const T* valL,
+                        const T* dval, const double* in, double* out, int nrhs, hipStream_t st) {
+  if (nrhs < 1 || nrhs > MAX_RHS || sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS)
+    throw std::invalid_argument("solve_small: block does not fit the workgroup's LDS");
+  switch (nrhs) {
+    case 1: solve_small_nr<T, 1>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 2: solve_small_nr<T, 2>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 3: solve_small_nr<T, 3>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 4: solve_small_nr<T, 4>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 5: solve_small_nr<T, 5>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 6: solve_small_nr<T, 6>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 7: solve_small_nr<T, 7>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    default: solve_small_nr<T, 8>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+  }
+}
+
+template void launch_solve_small<double>(const int*, const int*, int, const SnArgs&, const double*, const double*,
+                                         const double*, double*, int, hipStream_t);
+template void launch_solve_small<float>(const int*, const int*, int, const SnArgs&, const float*, const float*,
+                                        const double*, double*, int, hipStream_t);
+
+}  // namespace lblg

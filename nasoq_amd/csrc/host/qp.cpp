@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -183,9 +184,6 @@ void QpSolver::symbolic() {
   zero_thresh = diag_perturb;
   build_super_matrix();
   const int N = SM.ncol;
-  if (h) lblg_destroy(h);
-  h = nullptr;
-  check(lblg_create(&h, N, SM.p.data(), SM.i.data(), SM.x.data()), "qp: create");
   lblg_settings st;
   lblg_default_settings(&st);
   st.reg_diag = diag_perturb;
@@ -193,11 +191,19 @@ void QpSolver::symbolic() {
   st.max_inner_iter = inner_iter_ref;
   st.tol_abs = st.tol_rel = stop_tol;
   st.ordering = ordering;
+  // the super-KKT pattern depends on the problem only: a later solve of the
+  // same problem (new q or b, or the next call) keeps the analysis and the
+  // device factor, and the KKT values are refreshed at the next factorization
+  if (h && analysed && std::memcmp(&st, &analysed_settings, sizeof(st)) == 0) return;
+  if (h) lblg_destroy(h);
+  h = nullptr;
+  check(lblg_create(&h, N, SM.p.data(), SM.i.data(), SM.x.data()), "qp: create");
   // inequality rows are kept as single-column supernodes (extra_cols, SYM_REMOV)
   std::vector<int> extra(N, 0);
   for (int j = n + me; j < N; ++j) extra[j] = 1;
   check(lblg_analyze(h, &st, nullptr, extra.data()), "qp: analyze");
   analysed = true;
+  analysed_settings = st;
 }
 
 // SolverSettings::symbolic_analysis in mode 1 (linear_solver_wrapper.cpp:396-536)
@@ -523,6 +529,65 @@ int QpSolver::solve_warm(const double* x0, const double* dual_in, const int* as,
   lagrangian_residual_norm();
   status = conv;
   return conv;
+}
+
+// Warm start from a predicted active set (engine extension for the learned
+// warm start): the KKT system with the predicted constraints as equalities
+// gives x and their multipliers; constraints with negative multipliers are
+// dropped and the system solved again until the multipliers are
+// non-negative.  That state (x, z >= 0, Hx + q + A'y + B_W'z = 0) is the one
+// the dual active-set loop keeps (Nasoq::numeric_QP, nasoq.cpp:478-610), so
+// the loop continues from it as after set_warm_start (:413-436) and ends at
+// the same optimum as a cold start; only the iteration count changes.
+int QpSolver::solve_predicted(const int* as, int nas) {
+  status = QP_NOT_CONVERGED;
+  num_iter = 0;
+  predicted_prunes = 0;
+  auto_reg_en = variant == QP_AUTO ? 2 : (variant == QP_FIXED ? 1 : 0);
+  symbolic();
+  std::vector<unsigned char> in(mi, 0);
+  std::vector<int> act;
+  for (int k = 0; k < nas; ++k) {
+    const int c = as[k];
+    if (c < 0 || c >= mi) throw std::invalid_argument("qp: predicted constraint out of range");
+    if (in[c]) continue;
+    in[c] = 1;
+    act.push_back(c);
+    edit_kkt(true, c);
+  }
+  while (true) {
+    std::fill(rhs.begin(), rhs.end(), 0.0);
+    for (int i = 0; i < n; ++i) rhs[i] = -q[i];
+    for (int i = 0; i < me; ++i) rhs[n + i] = a[i];
+    for (int c : act) rhs[n + me + c] = b[c];
+    solve_kkt(true);
+    ++predicted_prunes;
+    std::vector<int> keep;
+    bool dropped = false;
+    for (int c : act) {
+      if (sol[n + me + c] < 0.0) {
+        edit_kkt(false, c);
+        in[c] = 0;
+        dropped = true;
+      } else {
+        keep.push_back(c);
+      }
+    }
+    act.swap(keep);
+    if (!dropped) break;
+  }
+  primal.assign(sol.begin(), sol.begin() + n);
+  dual_eq.assign(sol.begin() + n, sol.begin() + n + me);
+  dual.assign(mi, 0.0);
+  for (int c : act) dual[c] = sol[n + me + c];
+  used.assign(in.begin(), in.end());
+  active_set = act;
+  n_active = (int)act.size();
+  warm_start = 1;
+  const int conv = numeric();
+  compute_objective();
+  status = check_status(conv);
+  return status;
 }
 
 }  // namespace lblg

@@ -50,6 +50,7 @@ struct QpSolver {
   std::vector<double> rhs, sol;
   lblg_handle* h = nullptr;
   bool analysed = false;
+  lblg_settings analysed_settings{};  // settings of the current analysis of h
 
   QpSolver() = default;
   QpSolver(const QpSolver&) = delete;
@@ -63,6 +64,8 @@ struct QpSolver {
   int numeric();
   int solve();
   int solve_warm(const double* x0, const double* dual_in, const int* as, int nas, const double* dual_eq_in);
+  int solve_predicted(const int* as, int nas);
+  int predicted_prunes = 0;  // KKT solves spent making a predicted active set dual feasible
   int check_status(int converged);
   // pieces of the active-set iteration
   void solve_kkt(bool refactor);

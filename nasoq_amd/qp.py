@@ -109,6 +109,34 @@ class Nasoq:
         self._pull()
         return conv.value
 
+    def solve_predicted(self, active_set):
+        """Warm start from a predicted active set (kept where its multipliers
+        are non-negative); returns (status, KKT solves spent on the prediction)."""
+        self._push()
+        act = np.ascontiguousarray(active_set, dtype=np.int32)
+        st, pr = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.lblg_qp_solve_predicted(self.h, iptr(act) if len(act) else None, len(act), ctypes.byref(st),
+                                               ctypes.byref(pr)), "lblg_qp_solve_predicted")
+        self._pull()
+        return st.value, pr.value
+
+    def set_linear(self, q=None, a=None, b=None):
+        """New linear terms for the next solve (None keeps a vector); the
+        super-KKT pattern, its analysis and the device buffers are kept."""
+        if q is not None:
+            self._q = np.ascontiguousarray(q, dtype=np.float64).copy()
+            assert self._q.shape == (self.n,)
+        if a is not None and self.me:
+            self._a = np.ascontiguousarray(a, dtype=np.float64).copy()
+            assert self._a.shape == (self.me,)
+        if b is not None and self.mi:
+            self._b = np.ascontiguousarray(b, dtype=np.float64).copy()
+            assert self._b.shape == (self.mi,)
+        check(self.lib.lblg_qp_set_linear(self.h, dptr(self._q) if q is not None else None,
+                                          dptr(self._a) if a is not None and self.me else None,
+                                          dptr(self._b) if b is not None and self.mi else None),
+              "lblg_qp_set_linear")
+
     def super_kkt(self):
         """(N, p, i, x) of the current super-KKT (lower CSC)."""
         nnz = self.lib.lblg_qp_super_kkt(self.h, None, None, None)

@@ -86,10 +86,17 @@ def test_reference_example_lbl_main():
     assert np.allclose(s.solve_only(np.array([8.0, 8.0])), [4, 4], atol=1e-14)
 
 
+# sweeps: "small" = one workgroup with the block in LDS (the default for these
+# sizes), "levels" = the level-scheduled launches
+SWEEPS = {"small": str(1 << 30), "levels": "0"}
+
+
+@pytest.mark.parametrize("sweep", ["small", "levels"])
 @pytest.mark.parametrize("seed", range(6))
-def test_small_kkt_parity(oracle, seed):
+def test_small_kkt_parity(oracle, seed, sweep, monkeypatch):
+    monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
     n, Ap, Ai, Ax, B = P.kkt_small(50 + 30 * seed, 12 + 6 * seed, seed=seed, nrhs=2)
-    print(f"case small_kkt seed={seed} n={n}", flush=True)
+    print(f"case small_kkt seed={seed} n={n} sweep={sweep}", flush=True)
     reg = 1e-9
     s = engine(n, Ap, Ai, Ax, reg=reg)
     F = check_factor(oracle, s, n, Ap, Ai, Ax, reg)
@@ -129,11 +136,13 @@ def test_wide_supernodes(oracle, group_w, grid, m, monkeypatch):
 @pytest.mark.parametrize("nb,bs,order,group_w", [(8, 10, None, 2048), (20, 12, "nat", 2048), (6, 40, "nat", 2048),
                                                   (4, 200, "nat", 2048), (4, 200, "nat", 96),
                                                   (3, 600, "nat", 256)])
-def test_two_by_two_pivots(oracle, nb, bs, order, group_w, monkeypatch):
+@pytest.mark.parametrize("sweep", ["small", "levels"])
+def test_two_by_two_pivots(oracle, nb, bs, order, group_w, sweep, monkeypatch):
     # dense zero-diagonal blocks force Bunch-Kaufman 2x2 pivots in every panel
     # path: w <= 64 (LDS kernel), 64 < w <= group_w (single-workgroup panel),
     # w > group_w (cooperative multi-workgroup panel)
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
+    monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
     n, Ap, Ai, Ax, B = P.block_indefinite(nb, bs, seed=1, nrhs=2)
     perm = np.arange(n, dtype=np.int32) if order == "nat" else None
     print(f"case 2x2 n={n} nb={nb} bs={bs} order={order} group_w={group_w}", flush=True)
@@ -194,6 +203,7 @@ def test_chain_sweeps(oracle, chain_w, group_w, grid, m, prec, monkeypatch):
     # (units of 128 rows chained by per-piece flags), the others piece by piece
     monkeypatch.setenv("LBLG_CHAIN_W", str(chain_w))
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
+    monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS["levels"])
     n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 6, seed=3, nrhs=3)
     print(f"case chain n={n} chain_w={chain_w} group_w={group_w} prec={prec}", flush=True)
     reg = 1e-9
@@ -214,3 +224,44 @@ def test_chain_sweeps(oracle, chain_w, group_w, grid, m, prec, monkeypatch):
     chains = _launches(s, "fwd_chain")
     assert (chains > 0) == (chain_w < 1000), chains
     assert _launches(s, "bwd_chain") == chains
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_small_sweeps_match_level_sweeps(oracle, prec, monkeypatch):
+    # the one-workgroup sweeps against the level-scheduled launches on the
+    # same factor, for 1..8 right-hand sides: supernodes wider than 64
+    # columns (chunked triangles), and 2x2 pivots in natural order.  The
+    # unrefined solutions' backward errors for the factored matrix K~ (K with
+    # the perturbed diagonal; tiny pivots make them growth-sized) must be no
+    # worse than the level sweeps', and refinement must reach 1e-10 on K.
+    cases = [(P.kkt_config(10, 150, 6, seed=2, nrhs=8), None),
+             (P.block_indefinite(4, 150, seed=3, nrhs=8), "nat")]
+    for (n, Ap, Ai, Ax, B), order in cases:
+        perm = np.arange(n, dtype=np.int32) if order == "nat" else None
+        reg = 1e-9 if prec == 0 else max(1e-9, np.ldexp(np.abs(Ax).max(), -12))
+        K = P.lower_to_full(n, Ap, Ai, Ax).toarray()
+        d = np.diag(K).copy()
+        K[np.arange(n), np.arange(n)] = np.where(np.abs(d) < reg, reg, d)
+        knorm = np.abs(K).sum(1).max()
+        eta = {}
+        for sweep in ("small", "levels"):
+            monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
+            print(f"case sweeps n={n} prec={prec} sweep={sweep}", flush=True)
+            s = engine(n, Ap, Ai, Ax, reg=1e-9, perm=perm, precision=prec)
+            assert s.lib.lblg_set_kernel_timing(s.h, 1) == 0
+            s.req_ref_iter = 0
+            e = []
+            for k in range(1, 9):
+                X = s.solve_only(B[:, :k].copy())
+                for c in range(k):
+                    x, b = X[:, c], B[:, c]
+                    e.append(np.abs(b - K @ x).max() / (knorm * np.abs(x).max() + np.abs(b).max()))
+            eta[sweep] = np.array(e)
+            assert (_launches(s, "solve_small") > 0) == (sweep == "small")
+            s.target_bwd_err = 1e-10
+            s.max_refine_rounds = 30
+            X = s.solve_only(B)
+            for c in range(B.shape[1]):
+                assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
+        print(f"eta small {eta['small'].max():.2e} levels {eta['levels'].max():.2e}", flush=True)
+        assert eta["small"].max() <= 4 * eta["levels"].max() + 1e-15
