@@ -280,16 +280,74 @@ def batch_leg(args):
     return summary, out
 
 
+def learned_leg(args):
+    """Learned warm start on a parametric QP family: min 1/2 x'Hx + (U theta)'x
+    s.t. Bx <= d with H the 5-point Laplacian of a g x g grid + 1e-2 I,
+    round(0.4 n) inequality rows of 6 N(0,1) entries, d ~ U[0, 0.5] and
+    U ~ N(0, 1/n) of n x 32.  Plain NASOQ (every KKT factorization on the GPU)
+    labels the training parameters with its active sets; the MLP
+    32 -> 1024 -> 1024 -> m (BF16 MFMA, FP32 accumulation) is trained on them
+    by the library's HIP kernels; held-out members are solved cold and warm
+    (prediction -> Nasoq.solve_predicted, fallback to a cold start unless
+    NASOQ's check says OPTIMAL) and the two solutions compared."""
+    from nasoq_amd import learned
+
+    fam = learned.ParametricQpFamily(grid=args.learned_grid, seed=0)
+    S = fam.solver()
+    train = fam.thetas(args.learned_train, 1)
+    log = lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True)
+    log(f"family n={fam.n} m={fam.mi}, labelling {len(train)} members with plain NASOQ")
+    Y, iters, t_label = learned.generate(fam, train, solver=S, log=log)
+    net = learned.ActiveSetPredictor(fam.n_theta, fam.mi, hidden=(1024, 1024), seed=0, max_batch=128)
+    t0 = time.perf_counter()
+    loss = net.fit(train, Y, epochs=args.learned_epochs, batch=64, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    test = fam.thetas(args.learned_test, 2)
+    t0 = time.perf_counter()
+    net.predict_proba(test)
+    t_pred = time.perf_counter() - t0
+    res = learned.evaluate(fam, net, test, solver=S)
+    net.close()
+    S.close()
+    ci, wi = res["cold_iter"].sum(), res["warm_iter"].sum()
+    out = {"workload": f"parametric QP family, H = 2-D Laplacian {args.learned_grid}x{args.learned_grid} + 1e-2 I "
+                       f"(n={fam.n}), {fam.mi} inequality rows of 6 N(0,1), d ~ U[0,0.5], q = U theta (32 params)",
+           "mlp": f"32 -> 1024 -> 1024 -> {fam.mi}, BF16 weights, FP32 accumulation",
+           "train_solves": len(train), "label_seconds": t_label, "train_epochs": args.learned_epochs,
+           "train_seconds": t_train, "final_train_loss": loss,
+           "train_samples_per_s": len(train) * args.learned_epochs / t_train,
+           "predict_ms_per_member": t_pred / len(test) * 1e3,
+           "test_members": len(test), "cold_iterations_mean": float(res["cold_iter"].mean()),
+           "warm_iterations_mean": float(res["warm_iter"].mean()), "iteration_ratio": float(ci / max(wi, 1)),
+           "cold_factorizations_mean": float(res["cold_fact"].mean()),
+           "warm_factorizations_mean": float(res["warm_fact"].mean()),
+           "cold_solves_per_s": len(test) / res["cold_seconds"], "warm_solves_per_s": len(test) / res["warm_seconds"],
+           "fallbacks": int(res["fallback"].sum()), "max_rel_solution_difference": float(res["dx"].max()),
+           "max_kkt_residual_warm": float(res["kkt"].max()),
+           "predicted_active_mean": float(res["pred_size"].mean()), "true_active_mean": float(res["true_size"].mean()),
+           "correctly_predicted_mean": float(res["hits"].mean())}
+    log(f"iterations cold {out['cold_iterations_mean']:.1f} warm {out['warm_iterations_mean']:.1f} "
+        f"(x{out['iteration_ratio']:.2f}), solves/s cold {out['cold_solves_per_s']:.1f} warm "
+        f"{out['warm_solves_per_s']:.1f}, fallbacks {out['fallbacks']}, max dx {out['max_rel_solution_difference']:.1e}")
+    assert out["max_kkt_residual_warm"] <= 1e-6 and out["max_rel_solution_difference"] <= 1e-6
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS) + ["batch1024"])
+    ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS) + ["batch1024", "learned"])
     ap.add_argument("--batch", type=int, default=1024, help="instances of the batch1024 configuration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor + FP64-refinement leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--learned-grid", type=int, default=12, help="grid side of the learned leg's QP family")
+    ap.add_argument("--learned-train", type=int, default=400, help="training members (plain NASOQ solves)")
+    ap.add_argument("--learned-test", type=int, default=40, help="held-out members solved cold and warm")
+    ap.add_argument("--learned-epochs", type=int, default=60)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference traffic for HBM counters)")
     args = ap.parse_args()
@@ -315,6 +373,9 @@ def main():
 
     if args.config == "batch1024":
         bench_batch(args)
+        return
+    if args.config == "learned":
+        print(json.dumps(learned_leg(args)))
         return
     grid, m, k = CONFIGS[args.config]
     t = time.perf_counter()
@@ -436,6 +497,9 @@ def main():
     if not args.no_batched and args.config == "kkt64":
         print("[bench] batched leg ...", file=sys.stderr, flush=True)
         out["batched"] = batch_leg(args)[0]
+    if not args.no_learned and args.config == "kkt64":
+        print("[bench] learned warm-start leg ...", file=sys.stderr, flush=True)
+        out["learned"] = learned_leg(args)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.config, fl)

@@ -134,6 +134,7 @@ struct DeviceFactor {
   // supernodes by level for the one-workgroup sweeps of small factors
   int* ss_order = nullptr;
   int* ss_levptr = nullptr;
+  int* ss_mode = nullptr;  // per level: 1 = whole-workgroup sweeps
   int ss_nlev = 0;
   int64_t ss_nnzL = 0, ss_max_nnz = 0;
   double *stage_b = nullptr, *stage_x = nullptr;  // host-pointer solves: staged blocks (n x nrhs_cap)

@@ -373,8 +373,39 @@ static void setup(Handle& h) {
     for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
     std::vector<int> pos(ptr.begin(), ptr.end() - 1);
     for (int s = 0; s < ns; ++s) order[pos[lev[s]]++] = s;
+    // mode of each level: per-wave supernodes, or the whole workgroup on one
+    // supernode at a time.  Cost model in dependent global-load steps: a
+    // per-wave triangle costs a step per column, a 64-row block below it a
+    // quarter step per column (independent loads); the workgroup mode stages
+    // the triangle (one step), substitutes from LDS (a fifth of a step per
+    // column), splits the row blocks over 16 waves and pays three barriers
+    // per 64-column chunk.
+    std::vector<int> mode(nl, 0);
+    for (int l = 0; l < nl; ++l) {
+      double wave_max = 0.0, wave_sum = 0.0, wg = 0.0;
+      for (int k = ptr[l]; k < ptr[l + 1]; ++k) {
+        const int sn = order[k];
+        const int w = S.super[sn + 1] - S.super[sn];
+        const int64_t r = S.pi[sn + 1] - S.pi[sn];
+        double cw_cost = 0.0, cg_cost = 0.0;
+        for (int cb = 0; cb < w; cb += 64) {
+          const int cw = std::min(64, w - cb);
+          const int64_t rb = r - cb - cw;
+          cw_cost += (cw - 1) + 0.25 * cw * (double)((rb + 63) / 64);
+          cg_cost += 1.0 + 0.2 * (cw - 1) + 0.25 * cw * (double)((rb + 1023) / 1024) + 1.5;
+        }
+        wave_max = std::max(wave_max, cw_cost);
+        wave_sum += cw_cost;
+        wg += cg_cost;
+      }
+      const double per_wave = std::max(wave_max, wave_sum / 16.0) + 0.5;
+      mode[l] = wg < per_wave ? 1 : 0;
+    }
+    const int force = env_int("LBLG_SMALL_SOLVE_MODE", -1);  // 0 / 1: every level in that mode
+    if (force == 0 || force == 1) std::fill(mode.begin(), mode.end(), force);
     D.ss_order = dupload(D, order);
     D.ss_levptr = dupload(D, ptr);
+    D.ss_mode = dupload(D, mode);
     D.ss_nlev = nl;
     D.ss_nnzL = S.p[n];
     D.ss_max_nnz = env_int("LBLG_SMALL_SOLVE_NNZ", 1 << 20);
@@ -600,7 +631,7 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   T* acc = static_cast<T*>(D.acc);
   if (D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && sizeof(T) * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS) {
     timed(D, "solve_small", [&] {
-      launch_solve_small<T>(D.ss_order, D.ss_levptr, D.ss_nlev, a, static_cast<const T*>(D.valL),
+      launch_solve_small<T>(D.ss_order, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
                             static_cast<const T*>(D.dval), in, out, nrhs, st);
     });
     return;

@@ -108,12 +108,13 @@ void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, do
                         hipStream_t st);
 // L, D and L^T sweeps of a small factor in one workgroup with the ordered
 // block (n x nrhs, rows interleaved) resident in LDS; supernodes level by
-// level in order[levptr[l] .. levptr[l + 1]).  Requires n * nrhs *
+// level in order[levptr[l] .. levptr[l + 1]), mode[l] = 1 where the whole
+// workgroup takes the level's supernodes one at a time.  Requires n * nrhs *
 // sizeof(T) <= SOLVE_SMALL_LDS.
-constexpr size_t SOLVE_SMALL_LDS = 128 * 1024;
+constexpr size_t SOLVE_SMALL_LDS = 104 * 1024;
 template <typename T>
-void launch_solve_small(const int* order, const int* levptr, int nlev, const SnArgs& a, const T* valL,
-                        const T* dval, const double* in, double* out, int nrhs, hipStream_t st);
+void launch_solve_small(const int* order, const int* levptr, const int* mode, int nlev, const SnArgs& a,
+                        const T* valL, const T* dval, const double* in, double* out, int nrhs, hipStream_t st);
 // per-column dot products out[c] = sum_i a(i, c) b(i, c) (rows interleaved by
 // column), reduced in a fixed order through at most nblk partials
 void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,

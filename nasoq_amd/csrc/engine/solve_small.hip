@@ -13,7 +13,11 @@
 // lanes over rows (forward: LDS atomics into the ancestors' entries, which
 // supernodes of one level may share) or gathered (backward: lanes over rows
 // with a wave sum per column for narrow chunks, lanes over columns
-// otherwise).  The arithmetic is the reference's blocked solve
+// otherwise).  Levels holding few wide supernodes (near the root) are
+// swept by the whole workgroup, one supernode at a time: each 64-column
+// chunk's triangle is staged in LDS and substituted by one wave, and the
+// rows below are split over all 16 waves (the host picks the mode of each
+// level with a cost model of both).  The arithmetic is the reference's blocked solve
 // (src/triangularSolve/BLAS.cpp, blocked L / D / L^T solves over the
 // supernodal factor) in the factor's type T.
 #include <hip/hip_runtime.h>
@@ -147,20 +151,142 @@ __device__ void ss_backward(int s, int ln, const SnArgs& a, const T* valL, T* x)
   }
 }
 
+constexpr int SS_TP = 65;  // pitch of the staged 64 x 64 triangle
+
+// stage the unit-lower triangle of chunk cb of a supernode: tile[j * SS_TP + i] = L(cb + i, cb + j), i > j
+template <typename T>
+__device__ void ss_stage(const T* P, int64_t r, int cb, int cw, T* tile) {
+  for (int idx = threadIdx.x; idx < cw * cw; idx += SS_THREADS) {
+    const int i = idx % cw, j = idx / cw;
+    if (i > j) tile[j * SS_TP + i] = P[(cb + i) + (int64_t)(cb + j) * r];
+  }
+}
+
+// forward sweep of supernode s by the whole workgroup (uniform control flow)
+template <typename T, int NR>
+__device__ void ss_forward_wg(int s, const SnArgs& a, const T* valL, T* x, T* tile) {
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int c0 = a.super[s], w = a.super[s + 1] - c0;
+  const int64_t pis = a.pi[s];
+  const int64_t r = a.pi[s + 1] - pis;
+  const T* P = valL + a.px[s];
+  for (int cb = 0; cb < w; cb += 64) {
+    const int cw = min(64, w - cb);
+    ss_stage(P, r, cb, cw, tile);
+    __syncthreads();
+    if (wv == 0) {
+      T v[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
+      for (int j = 0; j + 1 < cw; ++j) {
+        const T l = (ln > j && ln < cw) ? tile[j * SS_TP + ln] : (T)0;
+#pragma unroll
+        for (int c = 0; c < NR; ++c) v[c] -= l * ss_lane(v[c], j);
+      }
+      if (ln < cw) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+      }
+    }
+    __syncthreads();
+    // rows below the chunk in 64-row blocks over the waves; each row has one
+    // owner (the level's supernodes are taken one at a time in this mode)
+    for (int64_t q0 = cb + cw + (int64_t)wv * 64; q0 < r; q0 += SS_THREADS) {
+      const int64_t q = q0 + ln;
+      if (q < r) {
+        T acc[NR];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = (T)0;
+#pragma unroll 8
+        for (int j = 0; j < cw; ++j) {
+          const T l = P[q + (int64_t)(cb + j) * r];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += l * x[(c0 + cb + j) * NR + c];
+        }
+        const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) x[lab * NR + c] -= acc[c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// backward sweep of supernode s by the whole workgroup (uniform control flow)
+template <typename T, int NR>
+__device__ void ss_backward_wg(int s, const SnArgs& a, const T* valL, T* x, T* tile, T* red) {
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int c0 = a.super[s], w = a.super[s + 1] - c0;
+  const int64_t pis = a.pi[s];
+  const int64_t r = a.pi[s + 1] - pis;
+  const T* P = valL + a.px[s];
+  for (int cb = ((w - 1) / 64) * 64; cb >= 0; cb -= 64) {
+    const int cw = min(64, w - cb);
+    const int64_t qb = cb + cw, rr = r - qb;
+    ss_stage(P, r, cb, cw, tile);
+    for (int i = t; i < 64 * NR; i += SS_THREADS) red[i] = (T)0;
+    __syncthreads();
+    // red(j) = sum_q L(q, cb + j) x(label q): lanes over columns, each wave a
+    // contiguous range of rows, partial sums met by LDS atomics
+    if (rr > 0 && ln < cw) {
+      const int64_t seg = (rr + SS_WAVES - 1) / SS_WAVES;
+      const int64_t qs = qb + wv * seg, qe = min(r, qs + seg);
+      const T* col = P + (int64_t)(cb + ln) * r;
+      T acc[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = (T)0;
+      for (int64_t q = qs; q < qe; ++q) {
+        const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
+        const T l = col[q];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] += l * x[lab * NR + c];
+      }
+      if (qs < qe) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) atomicAdd(&red[ln * NR + c], acc[c]);
+      }
+    }
+    __syncthreads();
+    if (wv == 0) {
+      T v[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] - red[ln * NR + c] : (T)0;
+      for (int i = cw - 1; i > 0; --i) {
+        const T l = ln < i ? tile[ln * SS_TP + i] : (T)0;  // L(cb + i, cb + ln)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) v[c] -= l * ss_lane(v[c], i);
+      }
+      if (ln < cw) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // out = (L D L^T)^{-1} in for the ordered, row-interleaved blocks in / out
 // (n x NR); supernodes listed level by level in order[levptr[l] .. levptr[l+1])
+// (n x NR); supernodes listed level by level in order[levptr[l] .. levptr[l+1]),
+// mode[l] = 1 for levels swept by the whole workgroup
 template <typename T, int NR>
-__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, const int* levptr, int nlev, SnArgs a,
-                                                            const T* valL, const T* dval, const double* in,
-                                                            double* out) {
+__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, const int* levptr, const int* mode,
+                                                            int nlev, SnArgs a, const T* valL, const T* dval,
+                                                            const double* in, double* out) {
   extern __shared__ double smem_d[];
   T* x = reinterpret_cast<T*>(smem_d);
   const int n = a.n, t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  T* tile = x + ((n * NR + 1) & ~1);  // 64 x SS_TP staged triangle
+  T* red = tile + 64 * SS_TP;         // 64 x NR partial sums
   for (int i = t; i < n * NR; i += SS_THREADS) x[i] = (T)in[i];
   __syncthreads();
   for (int l = 0; l < nlev; ++l) {
-    for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_forward<T, NR>(order[k], ln, a, valL, x);
-    __syncthreads();
+    if (mode[l]) {
+      for (int k = levptr[l]; k < levptr[l + 1]; ++k) ss_forward_wg<T, NR>(order[k], a, valL, x, tile);
+    } else {
+      for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_forward<T, NR>(order[k], ln, a, valL, x);
+      __syncthreads();
+    }
   }
   // D: 1x1 pivots, and 2x2 pivots (off-diagonal in dval[n + i]) by their first row
   for (int i = t; i < n; i += SS_THREADS) {
@@ -182,49 +308,59 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, co
   }
   __syncthreads();
   for (int l = nlev - 1; l >= 0; --l) {
-    for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_backward<T, NR>(order[k], ln, a, valL, x);
-    __syncthreads();
+    if (mode[l]) {
+      for (int k = levptr[l + 1] - 1; k >= levptr[l]; --k) ss_backward_wg<T, NR>(order[k], a, valL, x, tile, red);
+    } else {
+      for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_backward<T, NR>(order[k], ln, a, valL, x);
+      __syncthreads();
+    }
   }
   for (int i = t; i < n * NR; i += SS_THREADS) out[i] = (double)x[i];
 }
 
+// LDS of the kernel: the block (n x NR, padded to an even count), the staged
+// triangle and the partial sums
+template <typename T>
+size_t ss_lds(int n, int nr) {
+  return sizeof(T) * ((((size_t)n * nr + 1) & ~(size_t)1) + 64 * SS_TP + 64 * (size_t)nr);
+}
+
 template <typename T, int NR>
-void solve_small_nr(const int* order, const int* levptr, int nlev, const SnArgs& a, const T* valL, const T* dval,
-                    const double* in, double* out, hipStream_t st) {
-  const size_t lds = sizeof(T) * (size_t)a.n * NR;
+void solve_small_nr(const int* order, const int* levptr, const int* mode, int nlev, const SnArgs& a, const T* valL,
+                    const T* dval, const double* in, double* out, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute((const void*)k_solve_small<T, NR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)SOLVE_SMALL_LDS) != hipSuccess)
+                            (int)(SOLVE_SMALL_LDS + ss_lds<T>(0, MAX_RHS))) != hipSuccess)
       throw std::runtime_error("solve_small: LDS attribute");
     attr = true;
   }
-  k_solve_small<T, NR><<<1, SS_THREADS, lds, st>>>(order, levptr, nlev, a, valL, dval, in, out);
+  k_solve_small<T, NR><<<1, SS_THREADS, ss_lds<T>(a.n, NR), st>>>(order, levptr, mode, nlev, a, valL, dval, in, out);
   HIP_LAUNCH_CHECK("solve_small");
 }
 
 }  // namespace
 
 template <typename T>
-void launch_solve_small(const int* order, const int* levptr, int nlev, const SnArgs& a, const T* valL,
-                        const T* dval, const double* in, double* out, int nrhs, hipStream_t st) {
+void launch_solve_small(const int* order, const int* levptr, const int* mode, int nlev, const SnArgs& a,
+                        const T* valL, const T* dval, const double* in, double* out, int nrhs, hipStream_t st) {
   if (nrhs < 1 || nrhs > MAX_RHS || sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS)
     throw std::invalid_argument("solve_small: block does not fit the workgroup's LDS");
   switch (nrhs) {
-    case 1: solve_small_nr<T, 1>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    case 2: solve_small_nr<T, 2>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    case 3: solve_small_nr<T, 3>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    case 4: solve_small_nr<T, 4>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    case 5: solve_small_nr<T, 5>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    case 6: solve_small_nr<T, 6>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    case 7: solve_small_nr<T, 7>(order, levptr, nlev, a, valL, dval, in, out, st); break;
-    default: solve_small_nr<T, 8>(order, levptr, nlev, a, valL, dval, in, out, st); break;
+    case 1: solve_small_nr<T, 1>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 2: solve_small_nr<T, 2>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 3: solve_small_nr<T, 3>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 4: solve_small_nr<T, 4>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 5: solve_small_nr<T, 5>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 6: solve_small_nr<T, 6>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 7: solve_small_nr<T, 7>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    default: solve_small_nr<T, 8>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
   }
 }
 
-template void launch_solve_small<double>(const int*, const int*, int, const SnArgs&, const double*, const double*,
-                                         const double*, double*, int, hipStream_t);
-template void launch_solve_small<float>(const int*, const int*, int, const SnArgs&, const float*, const float*,
-                                        const double*, double*, int, hipStream_t);
+template void launch_solve_small<double>(const int*, const int*, const int*, int, const SnArgs&, const double*,
+                                         const double*, const double*, double*, int, hipStream_t);
+template void launch_solve_small<float>(const int*, const int*, const int*, int, const SnArgs&, const float*,
+                                        const float*, const double*, double*, int, hipStream_t);
 
 }  // namespace lblg

@@ -226,14 +226,19 @@ def test_chain_sweeps(oracle, chain_w, group_w, grid, m, prec, monkeypatch):
     assert _launches(s, "bwd_chain") == chains
 
 
+@pytest.mark.parametrize("ss_mode", ["auto", "0", "1"])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_small_sweeps_match_level_sweeps(oracle, prec, monkeypatch):
+def test_small_sweeps_match_level_sweeps(oracle, prec, ss_mode, monkeypatch):
     # the one-workgroup sweeps against the level-scheduled launches on the
     # same factor, for 1..8 right-hand sides: supernodes wider than 64
     # columns (chunked triangles), and 2x2 pivots in natural order.  The
     # unrefined solutions' backward errors for the factored matrix K~ (K with
     # the perturbed diagonal; tiny pivots make them growth-sized) must be no
     # worse than the level sweeps', and refinement must reach 1e-10 on K.
+    # ss_mode: levels swept per wave (0), by the whole workgroup (1), or as
+    # the cost model picks (auto).
+    if ss_mode != "auto":
+        monkeypatch.setenv("LBLG_SMALL_SOLVE_MODE", ss_mode)
     cases = [(P.kkt_config(10, 150, 6, seed=2, nrhs=8), None),
              (P.block_indefinite(4, 150, seed=3, nrhs=8), "nat")]
     for (n, Ap, Ai, Ax, B), order in cases:
