@@ -187,9 +187,17 @@ def batch_leg(args):
     dX = torch.zeros_like(dB)
     bwd = np.zeros(1)
 
+    schedules = {}
+
     def run(prec):
         b.precision = prec
         b.numerical_factorization()
+        schedules[prec] = b.tuning()
+        if args.autotune:
+            t = time.perf_counter()
+            schedules[prec] = b.autotune(nrhs=1, reps=1)
+            print(f"[bench] batched autotune (precision {prec}) {time.perf_counter() - t:.1f}s: {schedules[prec]}",
+                  file=sys.stderr, flush=True)
 
         def step():
             _lib.check(lib.lblg_refactor_resident(b.h), "refactor")
@@ -269,6 +277,7 @@ def batch_leg(args):
                                 "refine_rounds": r32, "forward_error_vs_fp64_max": fwd32},
         "one_by_one_plain_fp64": {"value": nb / t_one, "ms_for_batch": t_one * 1e3,
                                   "worst_backward_error": worst_one},
+        "schedule": dict(schedules[0], autotuned=bool(args.autotune)),
         "speedup_batched_vs_one_by_one": t_one / t64,
         "speedup_batched_fp32_vs_one_by_one": t_one / t32,
     }
@@ -344,6 +353,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor + FP64-refinement leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the device schedule with lblg_autotune before timing (plain and batched legs)")
     ap.add_argument("--learned-grid", type=int, default=12, help="grid side of the learned leg's QP family")
     ap.add_argument("--learned-train", type=int, default=400, help="training members (plain NASOQ solves)")
     ap.add_argument("--learned-test", type=int, default=40, help="held-out members solved cold and warm")
@@ -400,6 +411,11 @@ def main():
     s.numerical_factorization()
     t_first = time.perf_counter() - t
     print(f"[bench] first factor (incl. device setup) {t_first:.2f}s", file=sys.stderr, flush=True)
+    schedule = s.tuning()
+    if args.autotune:
+        t = time.perf_counter()
+        schedule = s.autotune(nrhs=nrhs, reps=2)
+        print(f"[bench] autotune {time.perf_counter() - t:.1f}s: {schedule}", file=sys.stderr, flush=True)
 
     lib = s.lib
     dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")  # nrhs x n rows = column-major n x nrhs
@@ -477,6 +493,8 @@ def main():
         "refine_rounds": rounds,
         "kernel_ms_per_step": {k: v[0] for k, v in kt.items()},
         "roofline": roof,
+        "schedule": dict(schedule, autotuned=bool(args.autotune)),
+        "graph_captures_launches": list(s.graph_stats()),
     }
     dval = np.zeros(2 * n)
     _lib.check(lib.lblg_get_factor(s.h, None, _lib.dptr(dval), None, None), "get_factor")

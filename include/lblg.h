@@ -149,6 +149,40 @@ int lblg_solve_device(lblg_handle* h, int nrhs, const double* dB, int64_t ldb, d
 /* GMRES cycles run by the last solve (the reference's num_ref_iter). */
 int lblg_num_ref_iter(const lblg_handle* h);
 
+/* Device schedule of a handle (engine extension, no reference counterpart:
+ * the reference's schedule is fixed by its cost model).  Defaults come from
+ * LBLG_GROUP_W, LBLG_CHAIN_W, LBLG_SMALL_SOLVE_NNZ and LBLG_SCHUR_BIG_M. */
+typedef struct lblg_tuning {
+  int group_w;          /* widest supernode with a single-workgroup panel (wider: cooperative panel) */
+  int chain_w;          /* narrowest supernode swept by one cooperative launch */
+  int small_solve_nnz;  /* largest factor (entries of L) swept in one workgroup; 0 = never */
+  int schur_big_m;      /* off-diagonal rows from which Schur updates use 128-row MFMA tiles */
+  int wide_sub;         /* cooperative panels per blocked trailing update (rank wide_sub * 64), 1..16 */
+  double factor_ms;     /* lblg_autotune: factor time of the chosen schedule */
+  double solve_ms;      /* lblg_autotune: solve time of the chosen schedule (settings' refinement) */
+  int schedules_tried;  /* lblg_autotune: schedules timed */
+} lblg_tuning;
+int lblg_get_tuning(lblg_handle* h, lblg_tuning* t);
+/* takes effect at the next lblg_factor (the device schedule is rebuilt) */
+int lblg_set_tuning(lblg_handle* h, const lblg_tuning* t);
+/* Autotuner: times candidate schedules on the handle's current values (panel
+ * group width, Schur tile threshold and the rank of the wide panels'
+ * trailing updates by the factor; chain width and the
+ * one-workgroup sweep limit by a solve of nrhs fixed right-hand sides with
+ * the handle's settings), keeps the fastest and leaves the handle factored
+ * with it.  Reuses the symbolic analysis.  reps: timed repetitions per
+ * schedule (best of). */
+int lblg_autotune(lblg_handle* h, int nrhs, int reps, lblg_tuning* t);
+
+/* Launch sequences replayed from captured HIP graphs (engine extension, no
+ * reference counterpart): the factor, the solve with its fixed GMRES cycles,
+ * and each GMRES cycle of target-driven refinement are captured on their
+ * second call with the same configuration and replayed from then on, except
+ * where cooperative wide panels or chain sweeps take part (their launches
+ * carry per-launch epochs).  LBLG_GRAPH=0 turns
+ * capture off.  Counts of captures and of graph launches so far. */
+int lblg_graph_stats(const lblg_handle* h, int64_t* captures, int64_t* launches);
+
 /* Wait for the handle's stream. */
 int lblg_synchronize(lblg_handle* h);
 

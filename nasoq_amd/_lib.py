@@ -77,6 +77,13 @@ class QPInfo(ctypes.Structure):
 
 
 # (name, restype, argtypes) for every exported entry point of include/lblg.h
+class Tuning(ctypes.Structure):
+    _fields_ = [("group_w", ctypes.c_int), ("chain_w", ctypes.c_int), ("small_solve_nnz", ctypes.c_int),
+                ("schur_big_m", ctypes.c_int), ("wide_sub", ctypes.c_int), ("factor_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double),
+                ("schedules_tried", ctypes.c_int)]
+
+
 SIGNATURES = [
     ("lblg_default_settings", None, [ctypes.POINTER(Settings)]),
     ("lblg_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _i32p, _i32p, _f64p]),
@@ -110,6 +117,10 @@ SIGNATURES = [
                                                    _f64p]),
     ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p, ctypes.c_int]),
     ("lblg_set_kernel_timing", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("lblg_graph_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
+    ("lblg_get_tuning", ctypes.c_int, [_vp, ctypes.POINTER(Tuning)]),
+    ("lblg_set_tuning", ctypes.c_int, [_vp, ctypes.POINTER(Tuning)]),
+    ("lblg_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tuning)]),
     ("lblg_kernel_time", ctypes.c_int, [_vp, ctypes.c_char_p, _f64p, _i64p]),
     ("lblg_qp_default_settings", None, [ctypes.POINTER(QPSettings)]),
     ("lblg_qp_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,

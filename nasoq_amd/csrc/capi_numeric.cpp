@@ -116,6 +116,59 @@ extern "C" int lblg_num_ref_iter(const lblg_handle* hh) {
   return h ? h->num_ref_iter : LBLG_ERR_ARG;
 }
 
+extern "C" int lblg_get_tuning(lblg_handle* hh, lblg_tuning* t) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !t) return LBLG_ERR_ARG;
+  lblg::resolve_tuning(*h);
+  t->group_w = h->tune.group_w;
+  t->chain_w = h->tune.chain_w;
+  t->small_solve_nnz = h->tune.small_nnz;
+  t->schur_big_m = h->tune.big_m;
+  t->wide_sub = h->tune.wide_sub;
+  return LBLG_OK;
+}
+
+extern "C" int lblg_set_tuning(lblg_handle* hh, const lblg_tuning* t) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !t || t->group_w < 64 || t->chain_w < 1 || t->small_solve_nnz < 0 || t->schur_big_m < 1 ||
+      t->wide_sub < 1 || t->wide_sub > 16)
+    return LBLG_ERR_ARG;
+  h->tune.group_w = t->group_w;
+  h->tune.chain_w = t->chain_w;
+  h->tune.small_nnz = t->small_solve_nnz;
+  h->tune.big_m = t->schur_big_m;
+  h->tune.wide_sub = t->wide_sub;
+  h->dev.reset();  // the device schedule is rebuilt at the next factorization
+  h->factored = false;
+  return LBLG_OK;
+}
+
+extern "C" int lblg_autotune(lblg_handle* hh, int nrhs, int reps, lblg_tuning* t) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || nrhs < 1 || nrhs > lblg::MAX_RHS || reps < 1) return LBLG_ERR_ARG;
+  if (!h->analyzed) return LBLG_ERR_STATE;
+  return guarded([&] {
+    double fms = 0.0, sms = 0.0;
+    int tried = 0;
+    lblg::autotune(*h, nrhs, reps, fms, sms, tried);
+    if (t) {
+      lblg_get_tuning(hh, t);
+      t->factor_ms = fms;
+      t->solve_ms = sms;
+      t->schedules_tried = tried;
+    }
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_graph_stats(const lblg_handle* hh, int64_t* captures, int64_t* launches) {
+  const Handle* h = reinterpret_cast<const Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  if (captures) *captures = h->dev ? h->dev->graph_captures : 0;
+  if (launches) *launches = h->dev ? h->dev->graph_launches : 0;
+  return LBLG_OK;
+}
+
 extern "C" int lblg_synchronize(lblg_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h || !h->dev) return LBLG_ERR_STATE;

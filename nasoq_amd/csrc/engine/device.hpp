@@ -132,7 +132,8 @@ struct DeviceFactor {
   std::vector<FactorLevel> flevels;
   std::vector<SolveLevel> slevels;
   // supernodes by level for the one-workgroup sweeps of small factors
-  int* ss_order = nullptr;
+  int4* ss_info = nullptr;      // (first column, width, rows, first row index) in level order
+  int64_t* ss_ipx = nullptr;   // first value of each, in level order
   int* ss_levptr = nullptr;
   int* ss_mode = nullptr;  // per level: 1 = whole-workgroup sweeps
   int ss_nlev = 0;
@@ -144,6 +145,19 @@ struct DeviceFactor {
   int* piece_c0 = nullptr;      // piece -> first local column within the supernode
   int npieces = 0;
   std::vector<void*> owned;     // device allocations freed at destruction
+  // captured launch sequences (hipGraph) of the factor and of the
+  // fixed-refinement solve; a sequence is captured on its second call with
+  // the same key and replayed from then on
+  struct GraphCache {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    int seen = 0;
+  };
+  GraphCache gfactor, gsolve, gcycle;
+  bool graphs = true;       // LBLG_GRAPH=0 turns capture off
+  bool has_coop = false;    // wide panels (cooperative, epoch-stamped): factor not captured
+  bool has_chains = false;  // chain sweeps (epoch-stamped): level sweeps not captured
+  int64_t graph_launches = 0, graph_captures = 0;
   // optional per-kernel timing with events on the engine stream
   bool timing = false;
   struct Pending {

@@ -11,7 +11,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -32,6 +34,8 @@ namespace lblg {
 
 DeviceFactor::~DeviceFactor() {
   resolve_timing();
+  for (GraphCache* g : {&gfactor, &gsolve, &gcycle})
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
   for (void* p : owned) hipFree(p);
   if (stream) hipStreamDestroy(stream);
 }
@@ -68,6 +72,64 @@ static void timed(DeviceFactor& D, const char* name, F launch) {
   if (D.pending.size() > 4096) D.resolve_timing();
 }
 
+// Run a launch sequence through the graph cache G: eagerly on the first call
+// with this key, captured (stream capture, then instantiate) and launched on
+// the second, replayed afterwards.  The sequence must only enqueue kernels on
+// D.stream whose arguments are fixed by the key.
+template <typename F>
+static void run_graph(DeviceFactor& D, DeviceFactor::GraphCache& G, const std::string& key, F body) {
+  if (!D.graphs || D.timing) {
+    body();
+    return;
+  }
+  if (G.key != key) {
+    if (G.exec) HIPCHK(hipGraphExecDestroy(G.exec));
+    G.exec = nullptr;
+    G.key = key;
+    G.seen = 0;
+  }
+  if (G.exec) {
+    HIPCHK(hipGraphLaunch(G.exec, D.stream));
+    ++D.graph_launches;
+    return;
+  }
+  if (G.seen++ == 0) {
+    body();
+    return;
+  }
+  HIPCHK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeRelaxed));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(D.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(D.stream, &g));
+  const hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(e);
+  ++D.graph_captures;
+  HIPCHK(hipGraphLaunch(G.exec, D.stream));
+  ++D.graph_launches;
+}
+
+static std::string graph_key(std::initializer_list<double> v, std::initializer_list<const void*> p) {
+  std::string k;
+  char buf[40];
+  for (double x : v) {
+    std::snprintf(buf, sizeof(buf), "%a;", x);
+    k += buf;
+  }
+  for (const void* q : p) {
+    std::snprintf(buf, sizeof(buf), "%p;", q);
+    k += buf;
+  }
+  return k;
+}
+
 template <typename T>
 static T* dalloc(DeviceFactor& D, int64_t count) {
   if (count <= 0) count = 1;
@@ -101,7 +163,19 @@ static int env_int(const char* name, int dflt) {
 // ---------------------------------------------------------------------------
 // one-time device setup for an analysed handle
 // ---------------------------------------------------------------------------
+// schedule knobs not set on the handle take the environment's or built-in default
+void resolve_tuning(Handle& h) {
+  Handle::Tuning& t = h.tune;
+  if (t.group_w < 0) t.group_w = env_int("LBLG_GROUP_W", 2048);
+  if (t.chain_w < 0) t.chain_w = env_int("LBLG_CHAIN_W", 512);
+  if (t.small_nnz < 0) t.small_nnz = env_int("LBLG_SMALL_SOLVE_NNZ", 1 << 20);
+  if (t.big_m < 0) t.big_m = env_int("LBLG_SCHUR_BIG_M", 256);
+  if (t.wide_sub < 0) t.wide_sub = env_int("LBLG_WIDE_SUB", WIDE_SUB);
+  t.wide_sub = std::max(1, std::min(t.wide_sub, 16));
+}
+
 static void setup(Handle& h) {
+  resolve_tuning(h);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device");
   h.dev.reset(new DeviceFactor);
@@ -209,7 +283,7 @@ static void setup(Handle& h) {
   }
   std::vector<std::vector<int>> bylev(nlev);
   for (int s = 0; s < ns; ++s) bylev[lev[s]].push_back(s);
-  const int group_w = env_int("LBLG_GROUP_W", 2048);
+  const int group_w = h.tune.group_w;
   int64_t ld_max = 0;
   int64_t wmax_total = 0;
   std::vector<int64_t> woff(ns, 0);
@@ -238,9 +312,10 @@ static void setup(Handle& h) {
           // far rows: the diagonal block below the panel and the off-diagonal rows
           const int G = std::max(1, (int)((r - nb + WIDE_ROWS - 1) / WIDE_ROWS));
           if (G > 256) throw std::runtime_error("supernode too tall for the cooperative panel kernel");
-          const int64_t wcols = (int64_t)WIDE_SUB * nb;
+          const int64_t wcols = (int64_t)h.tune.wide_sub * nb;
           wsum += (int64_t)w * wcols + r * nb + (int64_t)WIDE_MOVED * (w + r + wcols) + 64;
           F.large.push_back(s);
+          D.has_coop = true;
           F.large_G.push_back(G);
           F.large_nb.push_back(nb);
         }
@@ -252,7 +327,7 @@ static void setup(Handle& h) {
           offblk.push_back(blk);
           blk += (int)((r - w + 63) / 64);
         }
-        if (r - w >= 256) {
+        if (r - w >= h.tune.big_m) {
           schb.push_back(s);
           tilesb.push_back(tlb);
           colsb.push_back(ncolb);
@@ -322,7 +397,7 @@ static void setup(Handle& h) {
   }
   // wide supernodes are swept by one cooperative launch ("chain"): all their
   // pieces take one level
-  const int chain_w = env_int("LBLG_CHAIN_W", 512);
+  const int chain_w = h.tune.chain_w;
   std::vector<char> chain(ns, 0);
   int chain_pmax = 1;
   for (int s = 0; s < ns; ++s) {
@@ -355,7 +430,10 @@ static void setup(Handle& h) {
   for (int p = 0; p < np; ++p) {
     const int s = psn[p];
     if (!chain[s]) pbylev[plev[p]].push_back(p);
-    else if (p == first_piece[s]) D.slevels[plev[p]].chains.push_back(s);
+    else if (p == first_piece[s]) {
+      D.slevels[plev[p]].chains.push_back(s);
+      D.has_chains = true;
+    }
   }
   {  // supernode levels (parents come later in the numbering) for the one-workgroup sweeps
     std::vector<int> lev(ns, 0);
@@ -403,12 +481,27 @@ static void setup(Handle& h) {
     }
     const int force = env_int("LBLG_SMALL_SOLVE_MODE", -1);  // 0 / 1: every level in that mode
     if (force == 0 || force == 1) std::fill(mode.begin(), mode.end(), force);
-    D.ss_order = dupload(D, order);
+    std::vector<int> info(4 * (size_t)ns);
+    std::vector<int64_t> ipx(ns);
+    bool fits = nl <= SOLVE_SMALL_LEVELS && S.pi[ns] < (int64_t)INT32_MAX;
+    for (int k = 0; k < ns; ++k) {
+      const int sn = order[k];
+      const int64_t r = S.pi[sn + 1] - S.pi[sn];
+      info[4 * k] = S.super[sn];
+      info[4 * k + 1] = S.super[sn + 1] - S.super[sn];
+      info[4 * k + 2] = (int)r;
+      info[4 * k + 3] = (int)S.pi[sn];
+      ipx[k] = px[sn];
+    }
+    D.ss_info = reinterpret_cast<int4*>(dupload(D, info));
+    D.ss_ipx = dupload(D, ipx);
     D.ss_levptr = dupload(D, ptr);
     D.ss_mode = dupload(D, mode);
+    if (!fits) nl = 0;  // too many levels for the LDS copy: the level sweeps only
     D.ss_nlev = nl;
     D.ss_nnzL = S.p[n];
-    D.ss_max_nnz = env_int("LBLG_SMALL_SOLVE_NNZ", 1 << 20);
+    D.ss_max_nnz = h.tune.small_nnz;
+    D.graphs = env_int("LBLG_GRAPH", 1) != 0;
   }
   {  // one published block of SOLVE_W x MAX_RHS values per piece, two words per value
     const int64_t words = (int64_t)chain_pmax * SOLVE_W * MAX_RHS * 2;
@@ -472,100 +565,111 @@ static void factor_device_t(Handle& h) {
   const double reg_factor = D.prec == 1 ? std::max(reg, std::ldexp(h.max_abs_entry, -12)) : reg;
   SnArgs a = snargs(D);
   a.rows = D.rows0;  // pre-pivot labels during the factorization
-  launch_fill(valL, S.p[n], 0.0, st);
-  launch_fill(dval, 2 * (int64_t)n, 0.0, st);
-  launch_iota(D.perm_piv, n, st);
-  launch_fill_int(D.status, 16, 0, st);
-  launch_fill_int(D.kpos, D.nsuper, 0, st);
-  launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, reg_factor, h.set.regularization, D.pert, st);
-  launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg_factor, valL, st);
-  for (size_t l = 0; l < D.flevels.size(); ++l) {
-    FactorLevel& F = D.flevels[l];
-    timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, valL, dval, D.perm_piv, D.status, st); });
-    // medium supernodes: one workgroup each, panel by panel
-    if (!F.medium.empty()) {
-      int wmax = 0;
-      for (int s : F.medium) wmax = std::max(wmax, S.super[s + 1] - S.super[s]);
-      const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
-      for (int p = 0; p < rounds; ++p) {
-        timed(D, "bk_panel", [&] {
-          launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, valL, dval, D.perm_piv, Wb, D.woff,
-                          D.kpos, D.kstart, D.sync, D.status, false, st);
-        });
-        const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
-        const int nt = (wmax - lo + 63) / 64;
-        if (nt > 0)
-          timed(D, "trailing", [&] {
-            launch_trailing(F.d_medium, (int)F.medium.size(), nt * (nt + 1) / 2, lo, a, valL, Wb, D.woff,
-                            D.kpos, D.kstart, st);
+  // the whole launch sequence depends only on the symbolic structure, the
+  // buffers and (reg, reg_factor, regularization): captured once and replayed
+  // when no cooperative wide panel (epoch-stamped) takes part
+  auto seq = [&]() {
+    launch_fill(valL, S.p[n], 0.0, st);
+    launch_fill(dval, 2 * (int64_t)n, 0.0, st);
+    launch_iota(D.perm_piv, n, st);
+    launch_fill_int(D.status, 16, 0, st);
+    launch_fill_int(D.kpos, D.nsuper, 0, st);
+    launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, reg_factor, h.set.regularization, D.pert, st);
+    launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg_factor, valL, st);
+    for (size_t l = 0; l < D.flevels.size(); ++l) {
+      FactorLevel& F = D.flevels[l];
+      timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, valL, dval, D.perm_piv, D.status, st); });
+      // medium supernodes: one workgroup each, panel by panel
+      if (!F.medium.empty()) {
+        int wmax = 0;
+        for (int s : F.medium) wmax = std::max(wmax, S.super[s + 1] - S.super[s]);
+        const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
+        for (int p = 0; p < rounds; ++p) {
+          timed(D, "bk_panel", [&] {
+            launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, valL, dval, D.perm_piv, Wb, D.woff,
+                            D.kpos, D.kstart, D.sync, D.status, false, st);
           });
-      }
-    }
-    // large supernodes: a cooperative group of workgroups each, one at a time
-    for (size_t q = 0; q < F.large.size(); ++q) {
-      const int s = F.large[q];
-      const int w = S.super[s + 1] - S.super[s];
-      const int64_t r = S.pi[s + 1] - S.pi[s];
-      const int G = F.large_G[q], nb = F.large_nb[q];
-      const int rounds = (w + nb - 2) / (nb - 1) + 1;
-      const int wcols = WIDE_SUB * nb;
-      for (int p = 0; p < rounds; ++p) {
-        // panels run in blocks of WIDE_SUB: before panel p of a block, its columns
-        // receive the block's earlier panels; the rest of the trailing block is
-        // updated once at the end of the block (rank up to WIDE_SUB * nb)
-        const int sub = p % WIDE_SUB;
-        const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
-        if (sub > 0) {
-          const int Tr = (int)((r - lo + 127) / 128);
-          if (Tr > 0)
+          const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
+          const int nt = (wmax - lo + 63) / 64;
+          if (nt > 0)
             timed(D, "trailing", [&] {
-              launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
+              launch_trailing(F.d_medium, (int)F.medium.size(), nt * (nt + 1) / 2, lo, a, valL, Wb, D.woff,
+                              D.kpos, D.kstart, st);
             });
         }
-        launch_fill_int(D.sync, 1, 0, st);
-        // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
-        // panel eliminates >= nb - 1 columns); the last panel may be narrower
-        const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
-        const int Gp = std::min(G, std::max(1, (far + WIDE_ROWS - 1) / WIDE_ROWS));
-        const uint32_t eb = D.ll_epoch;
-        D.ll_epoch += 3u * (uint32_t)nb + 8u;
-        timed(D, "bk_panel_group", [&] {
-          launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
-                         D.kstart, D.sync, D.ll, eb, D.status, st);
+      }
+      // large supernodes: a cooperative group of workgroups each, one at a time
+      for (size_t q = 0; q < F.large.size(); ++q) {
+        const int s = F.large[q];
+        const int w = S.super[s + 1] - S.super[s];
+        const int64_t r = S.pi[s + 1] - S.pi[s];
+        const int G = F.large_G[q], nb = F.large_nb[q];
+        const int rounds = (w + nb - 2) / (nb - 1) + 1;
+        const int wsub = h.tune.wide_sub;
+        const int wcols = wsub * nb;
+        for (int p = 0; p < rounds; ++p) {
+          // panels run in blocks of wsub: before panel p of a block, its columns
+          // receive the block's earlier panels; the rest of the trailing block is
+          // updated once at the end of the block (rank up to wsub * nb)
+          const int sub = p % wsub;
+          const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
+          if (sub > 0) {
+            const int Tr = (int)((r - lo + 127) / 128);
+            if (Tr > 0)
+              timed(D, "trailing", [&] {
+                launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
+              });
+          }
+          launch_fill_int(D.sync, 1, 0, st);
+          // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
+          // panel eliminates >= nb - 1 columns); the last panel may be narrower
+          const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
+          const int Gp = std::min(G, std::max(1, (far + WIDE_ROWS - 1) / WIDE_ROWS));
+          const uint32_t eb = D.ll_epoch;
+          D.ll_epoch += 3u * (uint32_t)nb + 8u;
+          timed(D, "bk_panel_group", [&] {
+            launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
+                           D.kstart, D.sync, D.ll, eb, D.status, st);
+          });
+          if (sub == wsub - 1 || p == rounds - 1) {
+            const int lo2 = std::min(w, (p + 1) * (nb - 1));
+            const int Tj = (w - lo2 + 127) / 128, Ti = (int)((r - lo2 + 127) / 128);
+            const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
+            if (Tj > 0)
+              timed(D, "trailing", [&] {
+                launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
+              });
+          }
+        }
+      }
+      timed(D, "offdiag", [&] {
+        launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, valL, dval,
+                       D.perm_piv, st);
+      });
+      timed(D, "schur", [&] {
+        launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, valL, dval, st);
+      });
+      if (F.schur_big.count > 0) {
+        timed(D, "scale_ld", [&] {
+          launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, valL, dval,
+                          LDb, F.d_schur_big_ldoff, st);
         });
-        if (sub == WIDE_SUB - 1 || p == rounds - 1) {
-          const int lo2 = std::min(w, (p + 1) * (nb - 1));
-          const int Tj = (w - lo2 + 127) / 128, Ti = (int)((r - lo2 + 127) / 128);
-          const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
-          if (Tj > 0)
-            timed(D, "trailing", [&] {
-              launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
-            });
-        }
+        timed(D, "schur_big", [&] {
+          launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, valL, LDb,
+                           F.d_schur_big_ldoff, st);
+        });
       }
     }
-    timed(D, "offdiag", [&] {
-      launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, valL, dval,
-                     D.perm_piv, st);
-    });
-    timed(D, "schur", [&] {
-      launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, valL, dval, st);
-    });
-    if (F.schur_big.count > 0) {
-      timed(D, "scale_ld", [&] {
-        launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, valL, dval,
-                        LDb, F.d_schur_big_ldoff, st);
-      });
-      timed(D, "schur_big", [&] {
-        launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, valL, LDb,
-                         F.d_schur_big_ldoff, st);
-      });
-    }
+    launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
+    // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
+    launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
+    launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
+  };
+  if (D.has_coop) {
+    seq();
+  } else {
+    run_graph(D, D.gfactor, graph_key({(double)D.prec, reg, reg_factor, (double)h.set.regularization}, {}), seq);
   }
-  launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
-  // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
-  launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
-  launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
   int stat[6] = {0, 0, 0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(stat), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -631,7 +735,7 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   T* acc = static_cast<T*>(D.acc);
   if (D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && sizeof(T) * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS) {
     timed(D, "solve_small", [&] {
-      launch_solve_small<T>(D.ss_order, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
+      launch_solve_small<T>(D.ss_info, D.ss_ipx, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
                             static_cast<const T*>(D.dval), in, out, nrhs, st);
     });
     return;
@@ -787,38 +891,38 @@ static void backward_errors(Handle& h, const double* b, const double* x, int nrh
   }
 }
 
-// solve for an ordered-interleaved right-hand side block already on the
-// device (D.B), result in D.Xs-independent buffer xo
-void solve_ordered(Handle& h, double* xo, int nrhs, double* bwd_out) {
+// whether a solve of nrhs columns takes the one-workgroup sweeps
+static bool small_sweeps(const DeviceFactor& D, int nrhs) {
+  const size_t vs = D.prec == 1 ? sizeof(float) : sizeof(double);
+  return D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && vs * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS;
+}
+
+// refinement of the ordered solution xo of the ordered block D.B to a target
+// backward error: GMRES cycles until every column meets it (host decision
+// after each cycle; the cycle itself goes through the graph cache)
+static void refine_to_target(Handle& h, double* xo, int nrhs, bool capturable, double* bwd_out) {
   DeviceFactor& D = *h.dev;
   const lblg_settings& s = h.set;
   const int mr = std::max(s.max_inner_iter, 1);
   hipStream_t st = D.stream;
-  apply_solve(D, D.B, xo, nrhs);
-  h.num_ref_iter = 0;
-  if (s.req_ref_iter > 0 || s.target_bwd_err > 0) {
-    // zero state, every column marked running
+  std::vector<double> err(nrhs);
+  backward_errors(h, D.B, xo, nrhs, err.data());
+  int rounds = 0;
+  auto cycle = [&]() {
+    // each round: one reference GMRES cycle (itr_max = 1 with mr inner steps)
     launch_gmres_reset(D.gm, gmres_state_size(D.mr_cap), nrhs, gmres_off_S(D.mr_cap) + 3, st);
-    if (s.target_bwd_err > 0) {
-      std::vector<double> err(nrhs);
-      backward_errors(h, D.B, xo, nrhs, err.data());
-      int rounds = 0;
-      while (*std::max_element(err.begin(), err.end()) > s.target_bwd_err && rounds < s.max_refine_rounds) {
-        // each round: one reference GMRES cycle (itr_max = 1 with mr inner steps)
-        launch_gmres_reset(D.gm, gmres_state_size(D.mr_cap), nrhs, gmres_off_S(D.mr_cap) + 3, st);
-        gmres_cycle(D, D.B, xo, nrhs, mr, 0, 0.0, 0.0);
-        ++rounds;
-        backward_errors(h, D.B, xo, nrhs, err.data());
-      }
-      h.num_ref_iter = rounds;
-      if (bwd_out) std::copy(err.begin(), err.end(), bwd_out);
-      return;
-    }
-    for (int itr = 0; itr < s.req_ref_iter; ++itr)
-      gmres_cycle(D, D.B, xo, nrhs, mr, itr, s.tol_abs, s.tol_rel);
-    h.num_ref_iter = s.req_ref_iter;
+    gmres_cycle(D, D.B, xo, nrhs, mr, 0, 0.0, 0.0);
+  };
+  while (*std::max_element(err.begin(), err.end()) > s.target_bwd_err && rounds < s.max_refine_rounds) {
+    if (capturable)
+      run_graph(D, D.gcycle, graph_key({(double)nrhs, (double)mr, 1.0}, {xo, D.X, D.V, D.gm, D.part}), cycle);
+    else
+      cycle();
+    ++rounds;
+    backward_errors(h, D.B, xo, nrhs, err.data());
   }
-  if (bwd_out) backward_errors(h, D.B, xo, nrhs, bwd_out);
+  h.num_ref_iter = rounds;
+  if (bwd_out) std::copy(err.begin(), err.end(), bwd_out);
 }
 
 // a chain sweep whose bounded wait ran out leaves status bit 16
@@ -832,7 +936,8 @@ static void check_solve_status(DeviceFactor& D) {
 void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx, bool device_ptrs,
            double* bwd_out) {
   DeviceFactor& D = *h.dev;
-  const int mr = std::max(h.set.max_inner_iter, 1);
+  const lblg_settings& s = h.set;
+  const int mr = std::max(s.max_inner_iter, 1);
   ensure_solve_buffers(D, nrhs, mr);
   hipStream_t st = D.stream;
   const double* Bd = B;
@@ -848,10 +953,39 @@ void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t
     Bd = D.stage_b;
     Xd = D.stage_x;
   }
-  launch_gather_rhs(D.n, nrhs, Bd, device_ptrs ? ldb : D.n, D.perm_f, D.B, st);
+  const int64_t lb = device_ptrs ? ldb : D.n, lx = device_ptrs ? ldx : D.n;
   double* xo = D.acc2;
-  solve_ordered(h, xo, nrhs, bwd_out);
-  launch_scatter_sol(D.n, nrhs, xo, D.perm_f, Xd, device_ptrs ? ldx : D.n, st);
+  const bool to_target = s.target_bwd_err > 0;
+  const int fixed_cycles = to_target ? 0 : s.req_ref_iter;
+  // every launch below is fixed by (nrhs, refinement budget and tolerances,
+  // buffers); the level sweeps of chained supernodes are epoch-stamped and
+  // keep the eager path
+  const bool capturable = !D.has_chains || small_sweeps(D, nrhs);
+  auto head = [&]() {
+    launch_gather_rhs(D.n, nrhs, Bd, lb, D.perm_f, D.B, st);
+    apply_solve(D, D.B, xo, nrhs);
+    if (fixed_cycles > 0) {
+      // zero state, every column marked running
+      launch_gmres_reset(D.gm, gmres_state_size(D.mr_cap), nrhs, gmres_off_S(D.mr_cap) + 3, st);
+      for (int itr = 0; itr < fixed_cycles; ++itr) gmres_cycle(D, D.B, xo, nrhs, mr, itr, s.tol_abs, s.tol_rel);
+    }
+    if (!to_target) launch_scatter_sol(D.n, nrhs, xo, D.perm_f, Xd, lx, st);
+  };
+  if (capturable)
+    run_graph(D, D.gsolve,
+              graph_key({(double)nrhs, (double)mr, (double)fixed_cycles, s.tol_abs, s.tol_rel, (double)lb,
+                         (double)lx, to_target ? 1.0 : 0.0},
+                        {Bd, Xd, xo, D.X, D.V, D.gm, D.part}),
+              head);
+  else
+    head();
+  h.num_ref_iter = fixed_cycles;
+  if (to_target) {
+    refine_to_target(h, xo, nrhs, capturable, bwd_out);
+    launch_scatter_sol(D.n, nrhs, xo, D.perm_f, Xd, lx, st);
+  } else if (bwd_out) {
+    backward_errors(h, D.B, xo, nrhs, bwd_out);
+  }
   if (!device_ptrs) {
     if (ldx == D.n) {
       HIPCHK(hipMemcpyAsync(X, D.stage_x, sizeof(double) * D.n * nrhs, hipMemcpyDeviceToHost, st));
@@ -862,6 +996,116 @@ void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t
     }
   }
   check_solve_status(D);
+}
+
+// ---------------------------------------------------------------------------
+// autotuner of the device schedule
+// ---------------------------------------------------------------------------
+// factor and solve times (ms, best of reps after two warm runs, which also
+// capture the graphs) of the schedule in h.tune
+static void time_schedule(Handle& h, int nrhs, int reps, const std::vector<double>& B, std::vector<double>& X,
+                          double& fms, double& sms) {
+  h.dev.reset();
+  factor(h);  // device setup and first factor, untimed
+  for (int w = 0; w < 2; ++w) {
+    factor_device(h);
+    solve(h, nrhs, B.data(), h.n, X.data(), h.n, false, nullptr);
+  }
+  fms = sms = 1e300;
+  for (int r = 0; r < reps; ++r) {
+    auto t0 = std::chrono::steady_clock::now();
+    factor_device(h);  // reads back the status word: ends synchronised
+    auto t1 = std::chrono::steady_clock::now();
+    solve(h, nrhs, B.data(), h.n, X.data(), h.n, false, nullptr);  // synchronised on return
+    auto t2 = std::chrono::steady_clock::now();
+    fms = std::min(fms, std::chrono::duration<double, std::milli>(t1 - t0).count());
+    sms = std::min(sms, std::chrono::duration<double, std::milli>(t2 - t1).count());
+  }
+}
+
+void autotune(Handle& h, int nrhs, int reps, double& factor_ms, double& solve_ms, int& tried) {
+  resolve_tuning(h);
+  const Symbolic& S = h.sym;
+  int wmax = 0;
+  for (int s = 0; s < S.nsuper; ++s) wmax = std::max(wmax, S.super[s + 1] - S.super[s]);
+  const int64_t nnzL = S.p[h.n];
+  std::vector<double> B((size_t)h.n * nrhs), X((size_t)h.n * nrhs);
+  uint64_t seed = 0x9e3779b97f4a7c15ull;
+  for (double& v : B) {  // fixed uniform(-1, 1) right-hand sides
+    seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+    v = (double)(seed >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+  tried = 0;
+  auto attempt = [&](const Handle::Tuning& t, double& f, double& sl) {
+    h.tune = t;
+    try {
+      time_schedule(h, nrhs, reps, B, X, f, sl);
+      ++tried;
+      return true;
+    } catch (const std::exception&) {  // a schedule the matrix does not admit
+      h.dev.reset();
+      return false;
+    }
+  };
+  // factor schedule: panel group width (only where it changes the partition)
+  // and the Schur tile threshold, timed by the factor
+  Handle::Tuning best = h.tune;
+  double bf = 1e300, bs = 1e300;
+  std::vector<int> gws = {best.group_w};
+  for (int g : {1024, 2048, 4096})
+    if (g != best.group_w && g < wmax) gws.push_back(g);
+  for (int g : gws)
+    for (int m : {128, 256, 512}) {
+      Handle::Tuning t = best;
+      t.group_w = g;
+      t.big_m = m;
+      double f, sl;
+      if (attempt(t, f, sl) && f < bf) {
+        bf = f;
+        bs = sl;
+        best = t;
+      }
+    }
+  if (wmax > best.group_w) {  // cooperative panels: the rank of their blocked trailing updates
+    const int cur = best.wide_sub;
+    for (int ws : {2, 4, 8}) {
+      if (ws == cur) continue;
+      Handle::Tuning t = best;
+      t.wide_sub = ws;
+      double f, sl;
+      if (attempt(t, f, sl) && f < bf) {
+        bf = f;
+        bs = sl;
+        best = t;
+      }
+    }
+  }
+  // solve schedule: chain width and the one-workgroup sweep limit, timed by the solve
+  const Handle::Tuning fbest = best;
+  std::vector<int> cws = {fbest.chain_w};
+  for (int c : {256, 512, 1024, 1 << 30})
+    if (c != fbest.chain_w && c < wmax) cws.push_back(c);
+  std::vector<int> sns = {fbest.small_nnz};
+  for (int v : {0, 1 << 20, 1 << 22})
+    if (v != fbest.small_nnz && (v == 0 || nnzL <= v)) sns.push_back(v);
+  for (int c : cws)
+    for (int v : sns) {
+      if (c == fbest.chain_w && v == fbest.small_nnz) continue;  // timed above
+      Handle::Tuning t = fbest;
+      t.chain_w = c;
+      t.small_nnz = v;
+      double f, sl;
+      if (attempt(t, f, sl) && sl < bs) {
+        bs = sl;
+        best.chain_w = c;
+        best.small_nnz = v;
+      }
+    }
+  h.tune = best;
+  double f, sl;
+  time_schedule(h, nrhs, reps, B, X, f, sl);  // leave the handle factored with the chosen schedule
+  factor_ms = f;
+  solve_ms = sl;
 }
 
 }  // namespace lblg

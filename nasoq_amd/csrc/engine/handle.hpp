@@ -33,6 +33,14 @@ struct Handle {
   int64_t wide_panels_pivoted = 0;  // wide panels whose 1x1 no-interchange pass stopped early
   double factor_seconds = 0.0;
   std::vector<int64_t> batch_off;  // block-diagonal batch: first column of each system (count + 1)
+  // device schedule knobs (-1: the environment's or built-in default at setup)
+  struct Tuning {
+    int group_w = -1;    // widest supernode with a single-workgroup panel
+    int chain_w = -1;    // narrowest supernode swept by one cooperative launch
+    int small_nnz = -1;  // largest factor (entries of L) swept by one workgroup; 0 = never
+    int big_m = -1;      // off-diagonal rows from which Schur updates take 128-row tiles
+    int wide_sub = -1;   // cooperative panels per trailing update (rank wide_sub * 64)
+  } tune;
   std::unique_ptr<DeviceFactor> dev;
   Handle();
   ~Handle();

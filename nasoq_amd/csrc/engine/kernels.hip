@@ -697,9 +697,14 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
                                                  const int* super, const int64_t* pi, const int64_t* px,
                                                  T* valL, const T* dval, const int* perm_piv,
                                                  int n) {
-  __shared__ T X[64 * 65];   // X[row + col*65] for a 64 x 64 column block
-  __shared__ T Lt[64 * 65];  // L(cb rows, cb cols)
-  __shared__ T As[KC * LDT], Bs[KC * LDT];
+  __shared__ T X[64 * 65];  // X[row + col*65] for a 64 x 64 column block
+  // the GEMM's operand chunks, then the column block's triangle L(cb rows, cb
+  // cols) for the substitution: never live together, so one region (two
+  // workgroups per CU)
+  __shared__ T Sh[(2 * KC * LDT > 64 * 65) ? 2 * KC * LDT : 64 * 65];
+  T* const As = Sh;
+  T* const Bs = Sh + KC * LDT;
+  T* const Lt = Sh;
   // locate (supernode, row block)
   int lo = 0, hi = count - 1;
   while (lo < hi) {
@@ -727,10 +732,6 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
       }
       X[i + j * 65] = v;
     }
-    for (int idx = t; idx < cw * cw; idx += 256) {
-      const int i = idx % cw, j = idx / cw;
-      Lt[i + j * 65] = (i > j) ? P[(cb0 + i) + (int64_t)(cb0 + j) * r] : 0.0;
-    }
     __syncthreads();
     // X(:,cb) -= X(:, <cb) * L(cb, <cb)^T with MFMA (only for wide supernodes)
     if (cb0 > 0) {
@@ -756,14 +757,28 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
       tile_foreach(acc, [&](int ri, int ci, T v) {
         if (ci < cw) X[ri + ci * 65] -= v;
       });
-      __syncthreads();
     }
-    // triangular solve inside the column block: thread per row
-    if (t < nr) {
-      for (int j = 0; j < cw; ++j) {
-        T v = X[t + j * 65];
-        for (int l = 0; l < j; ++l) v -= X[t + l * 65] * Lt[j + l * 65];
-        X[t + j * 65] = v;
+    // the column block's unit-lower triangle (after the GEMM: shares its buffers)
+    for (int idx = t; idx < cw * cw; idx += 256) {
+      const int i = idx % cw, j = idx / cw;
+      Lt[i + j * 65] = (i > j) ? P[(cb0 + i) + (int64_t)(cb0 + j) * r] : 0.0;
+    }
+    __syncthreads();
+    // substitution inside the column block: four adjacent lanes per row split
+    // each column's sum (l = sub, sub + 4, ...) and meet by shuffles; the
+    // solved entry is written by the first lane of the row before any lane
+    // of the row reads it (same wave, LDS in program order)
+    {
+      const int row = t >> 2, sub = t & 3;
+      if (row < nr) {
+        for (int j = 0; j < cw; ++j) {
+          T v = 0;
+          for (int l = sub; l < j; l += 4) v += X[row + l * 65] * Lt[j + l * 65];
+          v += __shfl_xor(v, 1, 64);
+          v += __shfl_xor(v, 2, 64);
+          const T xj = X[row + j * 65] - v;
+          if (sub == 0) X[row + j * 65] = xj;
+        }
       }
     }
     __syncthreads();

@@ -112,9 +112,13 @@ void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, do
 // workgroup takes the level's supernodes one at a time.  Requires n * nrhs *
 // sizeof(T) <= SOLVE_SMALL_LDS.
 constexpr size_t SOLVE_SMALL_LDS = 104 * 1024;
+constexpr int SOLVE_SMALL_LEVELS = 2048;  // level pointers and modes kept in LDS
+// info[k] = (first column, width, rows, first row index), ipx[k] = first value,
+// for the k-th supernode of the level order
 template <typename T>
-void launch_solve_small(const int* order, const int* levptr, const int* mode, int nlev, const SnArgs& a,
-                        const T* valL, const T* dval, const double* in, double* out, int nrhs, hipStream_t st);
+void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
+                        const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
+                        hipStream_t st);
 // per-column dot products out[c] = sum_i a(i, c) b(i, c) (rows interleaved by
 // column), reduced in a fixed order through at most nblk partials
 void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,

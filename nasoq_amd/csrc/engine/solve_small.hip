@@ -51,11 +51,10 @@ __device__ inline T ss_wave_sum(T v) {
 // forward sweep of supernode s by one wave: x(s) = L_ss^{-1} x(s), then
 // x(label q) -= L(q, s) x(s) for the rows below
 template <typename T, int NR>
-__device__ void ss_forward(int s, int ln, const SnArgs& a, const T* valL, T* x) {
-  const int c0 = a.super[s], w = a.super[s + 1] - c0;
-  const int64_t pis = a.pi[s];
-  const int64_t r = a.pi[s + 1] - pis;
-  const T* P = valL + a.px[s];
+__device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, T* x) {
+  const int c0 = I.x, w = I.y;
+  const int64_t r = I.z, pis = I.w;
+  const T* P = valL + px;
   for (int cb = 0; cb < w; cb += 64) {
     const int cw = min(64, w - cb);
     T v[NR];
@@ -97,11 +96,10 @@ __device__ void ss_forward(int s, int ln, const SnArgs& a, const T* valL, T* x) 
 
 // backward sweep of supernode s by one wave: x(s) = L_ss^{-T} (x(s) - L(q, s)^T x(label q))
 template <typename T, int NR>
-__device__ void ss_backward(int s, int ln, const SnArgs& a, const T* valL, T* x) {
-  const int c0 = a.super[s], w = a.super[s + 1] - c0;
-  const int64_t pis = a.pi[s];
-  const int64_t r = a.pi[s + 1] - pis;
-  const T* P = valL + a.px[s];
+__device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, T* x) {
+  const int c0 = I.x, w = I.y;
+  const int64_t r = I.z, pis = I.w;
+  const T* P = valL + px;
   for (int cb = ((w - 1) / 64) * 64; cb >= 0; cb -= 64) {
     const int cw = min(64, w - cb);
     const int64_t qb = cb + cw, rr = r - qb;  // rows below the chunk
@@ -164,12 +162,11 @@ __device__ void ss_stage(const T* P, int64_t r, int cb, int cw, T* tile) {
 
 // forward sweep of supernode s by the whole workgroup (uniform control flow)
 template <typename T, int NR>
-__device__ void ss_forward_wg(int s, const SnArgs& a, const T* valL, T* x, T* tile) {
+__device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x, T* tile) {
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  const int c0 = a.super[s], w = a.super[s + 1] - c0;
-  const int64_t pis = a.pi[s];
-  const int64_t r = a.pi[s + 1] - pis;
-  const T* P = valL + a.px[s];
+  const int c0 = I.x, w = I.y;
+  const int64_t r = I.z, pis = I.w;
+  const T* P = valL + px;
   for (int cb = 0; cb < w; cb += 64) {
     const int cw = min(64, w - cb);
     ss_stage(P, r, cb, cw, tile);
@@ -214,12 +211,11 @@ __device__ void ss_forward_wg(int s, const SnArgs& a, const T* valL, T* x, T* ti
 
 // backward sweep of supernode s by the whole workgroup (uniform control flow)
 template <typename T, int NR>
-__device__ void ss_backward_wg(int s, const SnArgs& a, const T* valL, T* x, T* tile, T* red) {
+__device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x, T* tile, T* red) {
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  const int c0 = a.super[s], w = a.super[s + 1] - c0;
-  const int64_t pis = a.pi[s];
-  const int64_t r = a.pi[s + 1] - pis;
-  const T* P = valL + a.px[s];
+  const int c0 = I.x, w = I.y;
+  const int64_t r = I.z, pis = I.w;
+  const T* P = valL + px;
   for (int cb = ((w - 1) / 64) * 64; cb >= 0; cb -= 64) {
     const int cw = min(64, w - cb);
     const int64_t qb = cb + cw, rr = r - qb;
@@ -266,25 +262,53 @@ __device__ void ss_backward_wg(int s, const SnArgs& a, const T* valL, T* x, T* t
 }
 
 // out = (L D L^T)^{-1} in for the ordered, row-interleaved blocks in / out
-// (n x NR); supernodes listed level by level in order[levptr[l] .. levptr[l+1])
-// (n x NR); supernodes listed level by level in order[levptr[l] .. levptr[l+1]),
-// mode[l] = 1 for levels swept by the whole workgroup
+// (n x NR).  Supernodes are listed level by level: info[k] = (first column,
+// width, rows, first row index) and ipx[k] = first value of supernode k of the
+// order, level l holding k in [levptr[l], levptr[l+1]); mode[l] = 1 for
+// levels swept by the whole workgroup.  levptr and mode are copied to LDS;
+// each wave loads the description of its first supernode of the next level
+// before working on the current one, so a level costs one dependent round of
+// value loads instead of four.
 template <typename T, int NR>
-__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, const int* levptr, const int* mode,
-                                                            int nlev, SnArgs a, const T* valL, const T* dval,
-                                                            const double* in, double* out) {
+__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, const int64_t* ipx, const int* levptr,
+                                                            const int* mode, int nlev, SnArgs a, const T* valL,
+                                                            const T* dval, const double* in, double* out) {
   extern __shared__ double smem_d[];
   T* x = reinterpret_cast<T*>(smem_d);
   const int n = a.n, t = threadIdx.x, wv = t >> 6, ln = t & 63;
   T* tile = x + ((n * NR + 1) & ~1);  // 64 x SS_TP staged triangle
   T* red = tile + 64 * SS_TP;         // 64 x NR partial sums
+  int* lp = reinterpret_cast<int*>(red + 64 * NR);  // nlev + 1 level pointers
+  int* lm = lp + nlev + 1;                          // nlev modes
   for (int i = t; i < n * NR; i += SS_THREADS) x[i] = (T)in[i];
+  for (int i = t; i <= nlev; i += SS_THREADS) lp[i] = levptr[i];
+  for (int i = t; i < nlev; i += SS_THREADS) lm[i] = mode[i];
   __syncthreads();
+  int4 nI = make_int4(0, 0, 0, 0);
+  int64_t npx = 0;
+  bool nok = lp[0] + wv < lp[1];
+  if (nok) {
+    nI = info[lp[0] + wv];
+    npx = ipx[lp[0] + wv];
+  }
   for (int l = 0; l < nlev; ++l) {
-    if (mode[l]) {
-      for (int k = levptr[l]; k < levptr[l + 1]; ++k) ss_forward_wg<T, NR>(order[k], a, valL, x, tile);
+    const int4 I = nI;
+    const int64_t px = npx;
+    const bool ok = nok;
+    if (l + 1 < nlev) {  // the next level's first item of this wave, in flight meanwhile
+      const int k2 = lp[l + 1] + wv;
+      nok = k2 < lp[l + 2];
+      if (nok) {
+        nI = info[k2];
+        npx = ipx[k2];
+      }
+    }
+    if (lm[l]) {
+      for (int k = lp[l]; k < lp[l + 1]; ++k) ss_forward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile);
     } else {
-      for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_forward<T, NR>(order[k], ln, a, valL, x);
+      if (ok) ss_forward<T, NR>(I, px, ln, a, valL, x);
+      for (int k = lp[l] + wv + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
+        ss_forward<T, NR>(info[k], ipx[k], ln, a, valL, x);
       __syncthreads();
     }
   }
@@ -306,12 +330,30 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, co
       for (int c = 0; c < NR; ++c) x[i * NR + c] *= inv;
     }
   }
+  nok = lp[nlev - 1] + wv < lp[nlev];
+  if (nok) {
+    nI = info[lp[nlev - 1] + wv];
+    npx = ipx[lp[nlev - 1] + wv];
+  }
   __syncthreads();
   for (int l = nlev - 1; l >= 0; --l) {
-    if (mode[l]) {
-      for (int k = levptr[l + 1] - 1; k >= levptr[l]; --k) ss_backward_wg<T, NR>(order[k], a, valL, x, tile, red);
+    const int4 I = nI;
+    const int64_t px = npx;
+    const bool ok = nok;
+    if (l > 0) {
+      const int k2 = lp[l - 1] + wv;
+      nok = k2 < lp[l];
+      if (nok) {
+        nI = info[k2];
+        npx = ipx[k2];
+      }
+    }
+    if (lm[l]) {
+      for (int k = lp[l + 1] - 1; k >= lp[l]; --k) ss_backward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile, red);
     } else {
-      for (int k = levptr[l] + wv; k < levptr[l + 1]; k += SS_WAVES) ss_backward<T, NR>(order[k], ln, a, valL, x);
+      if (ok) ss_backward<T, NR>(I, px, ln, a, valL, x);
+      for (int k = lp[l] + wv + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
+        ss_backward<T, NR>(info[k], ipx[k], ln, a, valL, x);
       __syncthreads();
     }
   }
@@ -321,46 +363,50 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int* order, co
 // LDS of the kernel: the block (n x NR, padded to an even count), the staged
 // triangle and the partial sums
 template <typename T>
-size_t ss_lds(int n, int nr) {
-  return sizeof(T) * ((((size_t)n * nr + 1) & ~(size_t)1) + 64 * SS_TP + 64 * (size_t)nr);
+size_t ss_lds(int n, int nr, int nlev) {
+  return sizeof(T) * ((((size_t)n * nr + 1) & ~(size_t)1) + 64 * SS_TP + 64 * (size_t)nr) +
+         sizeof(int) * (2 * (size_t)nlev + 1);
 }
 
 template <typename T, int NR>
-void solve_small_nr(const int* order, const int* levptr, const int* mode, int nlev, const SnArgs& a, const T* valL,
-                    const T* dval, const double* in, double* out, hipStream_t st) {
+void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
+                    const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute((const void*)k_solve_small<T, NR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(SOLVE_SMALL_LDS + ss_lds<T>(0, MAX_RHS))) != hipSuccess)
+                            (int)(SOLVE_SMALL_LDS + ss_lds<T>(0, MAX_RHS, SOLVE_SMALL_LEVELS))) != hipSuccess)
       throw std::runtime_error("solve_small: LDS attribute");
     attr = true;
   }
-  k_solve_small<T, NR><<<1, SS_THREADS, ss_lds<T>(a.n, NR), st>>>(order, levptr, mode, nlev, a, valL, dval, in, out);
+  k_solve_small<T, NR><<<1, SS_THREADS, ss_lds<T>(a.n, NR, nlev), st>>>(info, ipx, levptr, mode, nlev, a, valL, dval,
+                                                                        in, out);
   HIP_LAUNCH_CHECK("solve_small");
 }
 
 }  // namespace
 
 template <typename T>
-void launch_solve_small(const int* order, const int* levptr, const int* mode, int nlev, const SnArgs& a,
-                        const T* valL, const T* dval, const double* in, double* out, int nrhs, hipStream_t st) {
-  if (nrhs < 1 || nrhs > MAX_RHS || sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS)
-    throw std::invalid_argument("solve_small: block does not fit the workgroup's LDS");
+void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
+                        const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
+                        hipStream_t st) {
+  if (nrhs < 1 || nrhs > MAX_RHS || sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS || nlev < 1 ||
+      nlev > SOLVE_SMALL_LEVELS)
+    throw std::invalid_argument("solve_small: block or level schedule does not fit the workgroup's LDS");
   switch (nrhs) {
-    case 1: solve_small_nr<T, 1>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 2: solve_small_nr<T, 2>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 3: solve_small_nr<T, 3>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 4: solve_small_nr<T, 4>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 5: solve_small_nr<T, 5>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 6: solve_small_nr<T, 6>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 7: solve_small_nr<T, 7>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    default: solve_small_nr<T, 8>(order, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 1: solve_small_nr<T, 1>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 2: solve_small_nr<T, 2>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 3: solve_small_nr<T, 3>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 4: solve_small_nr<T, 4>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 5: solve_small_nr<T, 5>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 6: solve_small_nr<T, 6>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 7: solve_small_nr<T, 7>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    default: solve_small_nr<T, 8>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
   }
 }
 
-template void launch_solve_small<double>(const int*, const int*, const int*, int, const SnArgs&, const double*,
-                                         const double*, const double*, double*, int, hipStream_t);
-template void launch_solve_small<float>(const int*, const int*, const int*, int, const SnArgs&, const float*,
-                                        const float*, const double*, double*, int, hipStream_t);
+template void launch_solve_small<double>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
+                                         const double*, const double*, const double*, double*, int, hipStream_t);
+template void launch_solve_small<float>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
+                                        const float*, const float*, const double*, double*, int, hipStream_t);
 
 }  // namespace lblg

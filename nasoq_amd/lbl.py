@@ -209,6 +209,40 @@ def _numeric_methods():
         x = X[:, 0].copy() if one else X
         return (x, bwd) if return_bwd else x
 
+    def set_values(self, Ax):
+        """New values for the same pattern (the next numerical_factorization uses them)."""
+        self.Ax = np.ascontiguousarray(Ax, dtype=np.float64).copy()
+        check(self.lib.lblg_set_values(self.h, dptr(self.Ax)), "lblg_set_values")
+
+    def tuning(self):
+        """The handle's device schedule knobs (dict)."""
+        t = _lib.Tuning()
+        check(self.lib.lblg_get_tuning(self.h, ctypes.byref(t)), "lblg_get_tuning")
+        return {f: getattr(t, f) for f in ("group_w", "chain_w", "small_solve_nnz", "schur_big_m", "wide_sub")}
+
+    def set_tuning(self, **knobs):
+        """Change schedule knobs; the next numerical_factorization rebuilds the device schedule."""
+        t = _lib.Tuning()
+        check(self.lib.lblg_get_tuning(self.h, ctypes.byref(t)), "lblg_get_tuning")
+        for k, v in knobs.items():
+            setattr(t, k, int(v))
+        check(self.lib.lblg_set_tuning(self.h, ctypes.byref(t)), "lblg_set_tuning")
+
+    def autotune(self, nrhs=1, reps=2):
+        """Time candidate device schedules on the current values and keep the
+        fastest (the handle is left factored with it); returns the knobs and
+        the chosen schedule's factor / solve times."""
+        check(self._push_settings(self._settings()), "lblg_set_settings")
+        t = _lib.Tuning()
+        check(self.lib.lblg_autotune(self.h, nrhs, reps, ctypes.byref(t)), "lblg_autotune")
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
+    def graph_stats(self):
+        """(captures, graph launches) of the handle's captured launch sequences."""
+        c, l = ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.lblg_graph_stats(self.h, ctypes.byref(c), ctypes.byref(l)), "lblg_graph_stats")
+        return c.value, l.value
+
     def inertia(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(self.lib.lblg_inertia(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "inertia")
@@ -231,8 +265,8 @@ def _numeric_methods():
         check(self.lib.lblg_factor_info(self.h, ctypes.byref(z), ctypes.byref(nrm), ctypes.byref(sec)), "info")
         return {"zero_pivot": z.value, "k0_norm_inf": nrm.value, "factor_seconds": sec.value}
 
-    for f in (numerical_factorization, _push_settings, refactor_resident, solve_only, inertia, get_factor,
-              factor_info):
+    for f in (numerical_factorization, _push_settings, refactor_resident, solve_only, set_values, graph_stats,
+              tuning, set_tuning, autotune, inertia, get_factor, factor_info):
         setattr(SolverSettings, f.__name__, f)
 
 

@@ -60,3 +60,19 @@ def test_invalid_inputs_are_rejected():
     X = np.zeros(2)
     assert lib.lblg_solve(h, 1, _lib.dptr(B), 2, _lib.dptr(X), 2, None) in (-1, -2)
     lib.lblg_destroy(h)
+
+
+def test_tuning_knobs_on_the_host():
+    # the schedule knobs resolve and validate without a device
+    from nasoq_amd import lbl
+    from nasoq_amd import problems as P
+    n, Ap, Ai, Ax, _ = P.kkt_small(40, 10, seed=1)
+    s = lbl.SolverSettings(n, Ap, Ai, Ax)
+    t = s.tuning()
+    assert t["group_w"] >= 64 and t["chain_w"] >= 1 and t["small_solve_nnz"] >= 0 and t["schur_big_m"] >= 1
+    s.set_tuning(group_w=1024, schur_big_m=128)
+    assert s.tuning()["group_w"] == 1024 and s.tuning()["schur_big_m"] == 128
+    import pytest as _pt
+    with _pt.raises(RuntimeError):
+        s.set_tuning(group_w=8)
+    s.close()
