@@ -166,11 +166,13 @@ static int env_int(const char* name, int dflt) {
 // schedule knobs not set on the handle take the environment's or built-in default
 void resolve_tuning(Handle& h) {
   Handle::Tuning& t = h.tune;
-  if (t.group_w < 0) t.group_w = env_int("LBLG_GROUP_W", 2048);
-  if (t.chain_w < 0) t.chain_w = env_int("LBLG_CHAIN_W", 512);
+  // defaults: the autotuner's choice on the 64^3 KKT (3-D Laplacian + 16,384
+  // constraint rows), the largest workload measured
+  if (t.group_w < 0) t.group_w = env_int("LBLG_GROUP_W", 1024);
+  if (t.chain_w < 0) t.chain_w = env_int("LBLG_CHAIN_W", 1024);
   if (t.small_nnz < 0) t.small_nnz = env_int("LBLG_SMALL_SOLVE_NNZ", 1 << 20);
   if (t.big_m < 0) t.big_m = env_int("LBLG_SCHUR_BIG_M", 256);
-  if (t.wide_sub < 0) t.wide_sub = env_int("LBLG_WIDE_SUB", WIDE_SUB);
+  if (t.wide_sub < 0) t.wide_sub = env_int("LBLG_WIDE_SUB", 8);
   t.wide_sub = std::max(1, std::min(t.wide_sub, 16));
 }
 

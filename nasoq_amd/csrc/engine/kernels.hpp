@@ -46,7 +46,6 @@ void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL
 constexpr int WIDE_NB = 64;         // panel width
 constexpr int WIDE_ROWS = 192;      // far rows per workgroup
 constexpr int WIDE_MOVED = 128;     // moved indices per panel (scratch rows)
-constexpr int WIDE_SUB = 4;         // panels per trailing update (W holds WIDE_SUB * WIDE_NB columns)
 constexpr int64_t LL_WORDS = 2 * 256 * 4 + 2 * 256 * 8 + 2 * 256 * 2 * 64;
 template <typename T>
 void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
