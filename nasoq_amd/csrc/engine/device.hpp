@@ -155,6 +155,8 @@ struct DeviceFactor {
   };
   GraphCache gfactor, gsolve, gcycle;
   bool graphs = true;       // LBLG_GRAPH=0 turns capture off
+  bool schur_direct = true;      // 128-tile Schur updates with operands loaded lane by lane (no LDS)
+  bool trailing_direct = true;   // the same for the wide panels' trailing updates
   bool has_coop = false;    // wide panels (cooperative, epoch-stamped): factor not captured
   bool has_chains = false;  // chain sweeps (epoch-stamped): level sweeps not captured
   int64_t graph_launches = 0, graph_captures = 0;

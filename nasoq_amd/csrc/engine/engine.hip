@@ -504,6 +504,8 @@ static void setup(Handle& h) {
     D.ss_nnzL = S.p[n];
     D.ss_max_nnz = h.tune.small_nnz;
     D.graphs = env_int("LBLG_GRAPH", 1) != 0;
+    D.schur_direct = env_int("LBLG_SCHUR_DIRECT", 1) != 0;
+    D.trailing_direct = env_int("LBLG_TRAILING_DIRECT", 1) != 0;
   }
   {  // one published block of SOLVE_W x MAX_RHS values per piece, two words per value
     const int64_t words = (int64_t)chain_pmax * SOLVE_W * MAX_RHS * 2;
@@ -619,7 +621,8 @@ static void factor_device_t(Handle& h) {
             const int Tr = (int)((r - lo + 127) / 128);
             if (Tr > 0)
               timed(D, "trailing", [&] {
-                launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
+                launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
+                                    D.trailing_direct, st);
               });
           }
           launch_fill_int(D.sync, 1, 0, st);
@@ -639,7 +642,8 @@ static void factor_device_t(Handle& h) {
             const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
             if (Tj > 0)
               timed(D, "trailing", [&] {
-                launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
+                launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
+                                    D.trailing_direct, st);
               });
           }
         }
@@ -658,7 +662,7 @@ static void factor_device_t(Handle& h) {
         });
         timed(D, "schur_big", [&] {
           launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, valL, LDb,
-                           F.d_schur_big_ldoff, st);
+                           F.d_schur_big_ldoff, D.schur_direct, st);
         });
       }
     }

@@ -9,11 +9,14 @@
 //                   (dlasyf's trailing update inside LAPACKE_dsytrf, :155)
 //
 // A workgroup (4 waves) owns a 128 x 128 tile; each wave a 64 x 64 quadrant
-// = 4 x 4 blocks of the 16x16x4 MFMA of the value type.  Operands are staged through LDS
-// in chunks of 16 columns of k, k-major with a row pitch of 144 doubles
-// (rows of one k land in distinct bank halves for the four k of a step), and
-// the next chunk is prefetched into registers while the current one is
-// multiplied.  As in the 64 x 64 engine, the second operand is passed first
+// = 4 x 4 blocks of the 16x16x4 MFMA of the value type.  By default every
+// lane loads its own MFMA operands straight from memory (the L2 serves the
+// reuse between the waves of a tile), two k-steps ahead, with no LDS staging
+// and no barrier in the k loop (big_gemm_direct).  The staged form
+// (big_gemm, LBLG_SCHUR_DIRECT=0 / LBLG_TRAILING_DIRECT=0) moves chunks of 16
+// columns of k through LDS, k-major with a row pitch of 144 doubles (rows of
+// one k land in distinct bank halves for the four k of a step), prefetching
+// the next chunk into registers while the current one is multiplied.  As in the 64 x 64 engine, the second operand is passed first
 // so the accumulator holds the transposed block and lanes 0..15 cover 16
 // consecutive rows in the epilogue.
 #include <hip/hip_runtime.h>
@@ -91,6 +94,64 @@ __device__ inline void big_gemm(BigAcc<T>& acc, int K, T* As, T* Bs, LA la, LB l
   }
 }
 
+// acc = sum_k A(x, k) B(y, k) with A(x, k) = Ab[x + k lda], B(y, k) = Bb[y + k ldb],
+// k < K, without LDS: every lane loads its own MFMA operands (rows 16 x + lane % 16
+// of its quadrant, k = 4 step + lane / 16) straight from memory, two k-steps
+// ahead of the products.  Rows beyond xa and columns beyond xb are clamped to
+// the last valid one (the caller discards those results); k beyond K is zero.
+template <typename T>
+__device__ inline void big_gemm_direct(BigAcc<T>& acc, int K, const T* Ab, int64_t lda, int xa, const T* Bb,
+                                       int64_t ldb, int xb) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
+  const int li = ln & 15, lk = ln >> 4;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) acc.c[a][b] = typename Mfma<T>::acc{0, 0, 0, 0};
+  const T* pa[4];
+  const T* pb[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    pa[x] = Ab + min(r0 + 16 * x + li, xa - 1) + (int64_t)lk * lda;
+    pb[x] = Bb + min(q0 + 16 * x + li, xb - 1) + (int64_t)lk * ldb;
+  }
+  const int64_t sa = 4 * lda, sb = 4 * ldb;  // one k-step
+  const int nfull = K / 4;
+  T a0[4], b0[4], a1[4], b1[4];
+  auto load = [&](T* a, T* b, int ks) {
+    const int64_t oa = ks * sa, ob = ks * sb;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      a[x] = pa[x][oa];
+      b[x] = pb[x][ob];
+    }
+  };
+  auto mul = [&](const T* a, const T* b) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) acc.c[x][y] = Mfma<T>::mma(b[y], a[x], acc.c[x][y]);
+  };
+  if (nfull > 0) load(a0, b0, 0);
+  if (nfull > 1) load(a1, b1, 1);
+  int ks = 0;
+  for (; ks + 1 < nfull; ks += 2) {
+    mul(a0, b0);
+    if (ks + 2 < nfull) load(a0, b0, ks + 2);
+    mul(a1, b1);
+    if (ks + 3 < nfull) load(a1, b1, ks + 3);
+  }
+  if (ks < nfull) mul(a0, b0);
+  if (4 * nfull < K) {  // last partial k-step
+    const bool kok = 4 * nfull + lk < K;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      a0[x] = kok ? pa[x][nfull * sa] : (T)0;
+      b0[x] = kok ? pb[x][nfull * sb] : (T)0;
+    }
+    mul(a0, b0);
+  }
+}
+
 template <typename T, typename F>
 __device__ inline void big_foreach(const BigAcc<T>& acc, F f) {
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -119,12 +180,14 @@ __device__ inline int64_t big_row_pos(const int* rows, int64_t b, int64_t e, int
   return lo - b;
 }
 
-template <typename T>
+// DIRECT: operands loaded by each lane from memory (big_gemm_direct) instead
+// of staged through LDS (big_gemm)
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_t* tile_prefix, int count,
                                                    const int* super, const int64_t* pi, const int64_t* px,
                                                    const int* rows, const int* col2sup, T* valL,
                                                    const T* LD, const int64_t* ldoff) {
-  __shared__ T As[BKC * BLD], Bs[BKC * BLD];
+  __shared__ T As[DIRECT ? 1 : BKC * BLD], Bs[DIRECT ? 1 : BKC * BLD];
   __shared__ int rlab[BT], clab[BT], ctg[BT], dtcol[BT];
   __shared__ int tpos[BRUNS][BT];
   __shared__ int64_t cbase[BT];
@@ -175,12 +238,19 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
     const int tg = ctg[t];
     cbase[t] = tg >= 0 ? px[tg] + (int64_t)(clab[t] - super[tg]) * (pi[tg + 1] - pi[tg]) : 0;
   }
+  // the epilogue reads every thread's tpos / cbase entries (the direct GEMM has
+  // no barrier of its own)
+  __syncthreads();
   const T* LDs = LD + ldoff[lo];
   BigAcc<T> acc;
-  big_gemm(
-      acc, w, As, Bs,
-      [&](int k, int x) -> T { return (k < w && i0 + x < m) ? P[w + i0 + x + (int64_t)k * r] : (T)0; },
-      [&](int k, int x) -> T { return (k < w && j0 + x < m) ? LDs[(int64_t)k * m + j0 + x] : (T)0; });
+  if constexpr (DIRECT) {
+    big_gemm_direct(acc, w, P + w + i0, r, m - i0, LDs + j0, (int64_t)m, m - j0);
+  } else {
+    big_gemm(
+        acc, w, As, Bs,
+        [&](int k, int x) -> T { return (k < w && i0 + x < m) ? P[w + i0 + x + (int64_t)k * r] : (T)0; },
+        [&](int k, int x) -> T { return (k < w && j0 + x < m) ? LDs[(int64_t)k * m + j0 + x] : (T)0; });
+  }
   // scatter-add: no global reads on the common path (target bases and row
   // positions come from LDS), so the no-return atomics stream out
   big_foreach(acc, [&](int ri, int ci, T v) {
@@ -208,12 +278,12 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
 //              written to the panel buffer Pp[(j - k1) r + i] (the matrix itself
 //              stays without the block's updates until the block ends); one
 //              column of tiles starting at k1
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, int strip, int wcols, const int* super,
                                                       const int64_t* pi, const int64_t* px, T* valL,
                                                       T* Wbase, const int64_t* woff,
                                                       const int* kpos, const int* kstart) {
-  __shared__ T As[BKC * BLD], Bs[BKC * BLD];
+  __shared__ T As[DIRECT ? 1 : BKC * BLD], Bs[DIRECT ? 1 : BKC * BLD];
   const int s = list[0];
   const int w = super[s + 1] - super[s];
   const int64_t r = pi[s + 1] - pi[s];
@@ -244,10 +314,15 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
   T* Pp = Wbase + woff[s] + (int64_t)w * wcols;  // r x strip, pitch r
   const int kb = k1 - k0;
   BigAcc<T> acc;
-  big_gemm(
-      acc, kb, As, Bs,
-      [&](int k, int x) -> T { return (k < kb && i0 + x < r) ? A[(int64_t)(k0 + k) * r + i0 + x] : (T)0; },
-      [&](int k, int x) -> T { return (k < kb && j0 + x < w) ? Wp[(int64_t)k * w + j0 + x] : (T)0; });
+  if constexpr (DIRECT) {
+    big_gemm_direct(acc, kb, A + (int64_t)k0 * r + i0, r, (int)min<int64_t>(r - i0, BT), Wp + j0, (int64_t)w,
+                    w - j0);
+  } else {
+    big_gemm(
+        acc, kb, As, Bs,
+        [&](int k, int x) -> T { return (k < kb && i0 + x < r) ? A[(int64_t)(k0 + k) * r + i0 + x] : (T)0; },
+        [&](int k, int x) -> T { return (k < kb && j0 + x < w) ? Wp[(int64_t)k * w + j0 + x] : (T)0; });
+  }
   // read-modify-write in batches of 16 values per lane: the loads use a
   // safe address for masked elements so that they issue back to back
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -327,21 +402,29 @@ void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncol
 
 template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                      T* valL, const T* LD, const int64_t* ldoff, hipStream_t st) {
+                      T* valL, const T* LD, const int64_t* ldoff, bool direct, hipStream_t st) {
   if (count <= 0 || ntiles <= 0) return;
   if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur_big: too many tiles");
-  k_schur_big<T><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                                a.col2sup, valL, LD, ldoff);
+  if (direct)
+    k_schur_big<T, true><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                                          a.col2sup, valL, LD, ldoff);
+  else
+    k_schur_big<T, false><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                                           a.col2sup, valL, LD, ldoff);
   HIP_LAUNCH_CHECK("schur_big");
 }
 
 template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
-                         T* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
+                         T* W, const int64_t* woff, const int* kpos, const int* kstart, bool direct, hipStream_t st) {
   if (ntiles <= 0) return;
   if (strip < 0 || strip > BT) throw std::runtime_error("trailing_big: strip wider than a tile");
-  k_trailing_big<T><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff, kpos,
-                                            kstart);
+  if (direct)
+    k_trailing_big<T, true><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff,
+                                                    kpos, kstart);
+  else
+    k_trailing_big<T, false><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff,
+                                                     kpos, kstart);
   HIP_LAUNCH_CHECK("trailing_big");
 }
 
@@ -349,9 +432,9 @@ void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wco
   template void launch_scale_ld<T>(const int*, int, const int*, int, const SnArgs&, const T*, const T*, T*,  \
                                    const int64_t*, hipStream_t);                                            \
   template void launch_schur_big<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,   \
-                                    const int64_t*, hipStream_t);                                           \
+                                    const int64_t*, bool, hipStream_t);                                     \
   template void launch_trailing_big<T>(const int*, int, int, int, int, const SnArgs&, T*, T*, const int64_t*, \
-                                       const int*, const int*, hipStream_t);
+                                       const int*, const int*, bool, hipStream_t);
 LBLG_INSTANTIATE(double)
 LBLG_INSTANTIATE(float)
 #undef LBLG_INSTANTIATE

@@ -61,10 +61,10 @@ void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncol
                      const T* valL, const T* dval, T* LD, const int64_t* ldoff, hipStream_t st);
 template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                      T* valL, const T* LD, const int64_t* ldoff, hipStream_t st);
+                      T* valL, const T* LD, const int64_t* ldoff, bool direct, hipStream_t st);
 template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
-                         T* W, const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
+                         T* W, const int64_t* woff, const int* kpos, const int* kstart, bool direct, hipStream_t st);
 template <typename T>
 void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
                     T* valL, const T* dval, const int* perm_piv, hipStream_t st);
