@@ -289,6 +289,35 @@ def batch_leg(args):
     return summary, out
 
 
+def qp_leg(args):
+    """The plain NASOQ configuration on the GPU: min 1/2 x'Hx + q'x, Ax = b,
+    x >= -0.5 with H the 40 x 40 Laplacian + 1e-2 I and 100 equality rows
+    (eps 1e-8); one cold solve after an untimed one (analysis and device
+    setup), wall time and active-set iterations."""
+    from nasoq_amd import problems as P
+    from nasoq_amd import qp
+    import scipy.sparse as sp
+
+    H, q, A, b, C, d = P.qp_config()
+    m = qp.Nasoq(sp.tril(H, format="csc"), q, A, b, C, d)
+    m.eps_abs = 1e-8
+    m.variant = qp.FIXED
+    assert m.solve() == qp.OPTIMAL
+    f0 = int(m.factorizations)
+    t0 = time.perf_counter()
+    st = m.solve()
+    dt = time.perf_counter() - t0
+    assert st == qp.OPTIMAL
+    out = {"workload": "QP: H = 2-D Laplacian 40x40 + 1e-2 I (n=1600), 100 equality rows of 5 N(0,1), "
+                       "x >= -0.5, eps 1e-8", "status": "optimal", "seconds": dt, "active_set_iterations": m.num_iter,
+           "ms_per_iteration": dt / max(m.num_iter, 1) * 1e3, "active_constraints": len(m.active_set),
+           "objective": m.objective, "kkt_factorizations": int(m.factorizations) - f0}
+    m.close()
+    print(f"[bench] qp configuration: {dt:.2f} s, {out['active_set_iterations']} iterations "
+          f"({out['ms_per_iteration']:.2f} ms each), {out['active_constraints']} active", file=sys.stderr, flush=True)
+    return out
+
+
 def learned_leg(args):
     """Learned warm start on a parametric QP family: min 1/2 x'Hx + (U theta)'x
     s.t. Bx <= d with H the 5-point Laplacian of a g x g grid + 1e-2 I,
@@ -353,6 +382,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor + FP64-refinement leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--no-qp", action="store_true", help="skip the plain NASOQ QP configuration leg")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the device schedule with lblg_autotune before timing (plain and batched legs)")
     ap.add_argument("--learned-grid", type=int, default=12, help="grid side of the learned leg's QP family")
@@ -518,6 +548,9 @@ def main():
     if not args.no_learned and args.config == "kkt64":
         print("[bench] learned warm-start leg ...", file=sys.stderr, flush=True)
         out["learned"] = learned_leg(args)
+    if not args.no_qp and args.config == "kkt64":
+        print("[bench] plain NASOQ QP configuration ...", file=sys.stderr, flush=True)
+        out["qp_configuration"] = qp_leg(args)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.config, fl)

@@ -146,3 +146,21 @@ def batch_qp_kkts(count=1024, seed=0, nmin=16, nmax=160, meq_max=64, nnz_row=4):
         n, Ap, Ai, Ax = lower_csc(kkt(H, A))
         out.append((n, Ap, Ai, Ax, np.concatenate([-q, A @ x0])))
     return out
+
+
+def qp_config(grid=40, meq=100, nnz_row=5, seed=0):
+    """The convex QP min 1/2 x'Hx + q'x s.t. Ax = b, Cx <= d of the plain
+    NASOQ configuration: H the 5-point Laplacian of a grid x grid mesh
+    + 1e-2 I, q ~ N(0,1), A meq x n with nnz_row N(0,1) entries at uniform
+    random columns, b = A x0 with x0 ~ U[-1,1], C = -I and d = 0.5 (x >= -0.5);
+    all from one seeded generator.  Returns (H (full), q, A, b, C, d)."""
+    rng = np.random.default_rng(seed)
+    H = laplacian_2d(grid, grid, 1e-2)
+    n = H.shape[0]
+    q = rng.standard_normal(n)
+    A = random_rows(rng, meq, n, nnz_row).tocsc()
+    x0 = rng.uniform(-1.0, 1.0, n)
+    b = A @ x0
+    C = -sp.identity(n, format="csc")
+    d = np.full(n, 0.5)
+    return H, q, A, b, C, d

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+from nasoq_amd import problems as P
 from nasoq_amd import qp, smp
 
 from qp_cases import GOLDEN, nasoq_main_problem, out_correct, quadprog_1
@@ -147,3 +148,33 @@ def test_cpp_dropin_somod(tmp_path, zero):
     assert len(got) == 4
     for g, w in zip(got, want):
         assert np.allclose(g, w, atol=1e-12), (g, w)
+
+
+def test_plain_nasoq_configuration():
+    # the reference's CPU-runnable configuration (40 x 40 Laplacian + 1e-2 I,
+    # 100 equality rows of 5 N(0,1), x >= -0.5), eps 1e-8: NASOQ on the GPU
+    # must return a point that satisfies the KKT conditions, and that point
+    # must be the solution of the KKT system of its own active set
+    H, q, A, b, C, d = P.qp_config()
+    n, me = H.shape[0], A.shape[0]
+    print(f"case qp_config n={n} me={me} mi={C.shape[0]}", flush=True)
+    m = qp.Nasoq(sp.tril(H, format="csc"), q, A, b, C, d)
+    m.eps_abs = 1e-8
+    m.variant = qp.FIXED
+    assert m.solve() == qp.OPTIMAL
+    _kkt_check(m, sp.tril(H, format="csc"), q, A, b, C, d, 1e-6)
+    act = np.array(m.active_set, dtype=int)
+    assert len(act) > 0
+    # active bounds x_i = -0.5: eliminate them and solve the reduced equality-constrained KKT densely
+    free = np.setdiff1d(np.arange(n), act)
+    Hd = H.toarray()
+    Ad = A.toarray()
+    xa = np.full(len(act), -0.5)
+    Kr = np.block([[Hd[np.ix_(free, free)], Ad[:, free].T], [Ad[:, free], np.zeros((me, me))]])
+    rhs = np.concatenate([-q[free] - Hd[np.ix_(free, act)] @ xa, b - Ad[:, act] @ xa])
+    sol = np.linalg.solve(Kr, rhs)
+    x = np.zeros(n)
+    x[free] = sol[:len(free)]
+    x[act] = xa
+    assert np.linalg.norm(m.primal_vars - x) <= 1e-7 * max(1.0, np.linalg.norm(x))
+    m.close()
