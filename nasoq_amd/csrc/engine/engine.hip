@@ -452,7 +452,20 @@ static void setup(Handle& h) {
     for (int s = 0; s < ns; ++s) ++ptr[lev[s] + 1];
     for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
     std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-    for (int s = 0; s < ns; ++s) order[pos[lev[s]]++] = s;
+    // "thin" supernodes (one column, at most SS_THIN_ROWS rows: the singleton
+    // constraint rows of the QP's KKT) are swept one per lane; they come first
+    // within their level
+    auto thin = [&](int sn) {
+      return S.super[sn + 1] - S.super[sn] == 1 && S.pi[sn + 1] - S.pi[sn] <= SS_THIN_ROWS;
+    };
+    std::vector<int> nthin(nl, 0);
+    for (int s = 0; s < ns; ++s)
+      if (thin(s)) {
+        order[pos[lev[s]]++] = s;
+        ++nthin[lev[s]];
+      }
+    for (int s = 0; s < ns; ++s)
+      if (!thin(s)) order[pos[lev[s]]++] = s;
     // mode of each level: per-wave supernodes, or the whole workgroup on one
     // supernode at a time.  Cost model in dependent global-load steps: a
     // per-wave triangle costs a step per column, a 64-row block below it a
@@ -460,10 +473,15 @@ static void setup(Handle& h) {
     // the triangle (one step), substitutes from LDS (a fifth of a step per
     // column), splits the row blocks over 16 waves and pays three barriers
     // per 64-column chunk.
+    // one lane per thin supernode pays where a level holds many of them; a few
+    // (chain tails) stay with the prefetched per-wave sweep
+    const bool thin_on = env_int("LBLG_SMALL_SOLVE_THIN", 1) != 0;
+    std::vector<int> lthin(nl, 0);
+    for (int l = 0; l < nl; ++l) lthin[l] = (thin_on && nthin[l] >= SS_THIN_MIN) ? nthin[l] : 0;
     std::vector<int> mode(nl, 0);
     for (int l = 0; l < nl; ++l) {
       double wave_max = 0.0, wave_sum = 0.0, wg = 0.0;
-      for (int k = ptr[l]; k < ptr[l + 1]; ++k) {
+      for (int k = ptr[l] + lthin[l]; k < ptr[l + 1]; ++k) {
         const int sn = order[k];
         const int w = S.super[sn + 1] - S.super[sn];
         const int64_t r = S.pi[sn + 1] - S.pi[sn];
@@ -479,10 +497,11 @@ static void setup(Handle& h) {
         wg += cg_cost;
       }
       const double per_wave = std::max(wave_max, wave_sum / 16.0) + 0.5;
-      mode[l] = wg < per_wave ? 1 : 0;
+      mode[l] = (ptr[l + 1] - ptr[l] > lthin[l] && wg < per_wave) ? 1 : 0;
     }
     const int force = env_int("LBLG_SMALL_SOLVE_MODE", -1);  // 0 / 1: every level in that mode
     if (force == 0 || force == 1) std::fill(mode.begin(), mode.end(), force);
+    for (int l = 0; l < nl; ++l) mode[l] |= lthin[l] << 1;  // bit 0: mode, bits 1..: thin count
     std::vector<int> info(4 * (size_t)ns);
     std::vector<int64_t> ipx(ns);
     bool fits = nl <= SOLVE_SMALL_LEVELS && S.pi[ns] < (int64_t)INT32_MAX;

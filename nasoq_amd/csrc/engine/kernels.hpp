@@ -112,8 +112,11 @@ void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, do
 // sizeof(T) <= SOLVE_SMALL_LDS.
 constexpr size_t SOLVE_SMALL_LDS = 104 * 1024;
 constexpr int SOLVE_SMALL_LEVELS = 2048;  // level pointers and modes kept in LDS
+constexpr int SS_THIN_ROWS = 32;          // one-column supernodes up to this many rows: one lane each,
+constexpr int SS_THIN_MIN = 64;           // in levels holding at least this many of them
 // info[k] = (first column, width, rows, first row index), ipx[k] = first value,
-// for the k-th supernode of the level order
+// for the k-th supernode of the level order; mode[l]: bit 0 = whole-workgroup
+// level, bits 1.. = number of thin supernodes, which lead the level's list
 template <typename T>
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                         const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,

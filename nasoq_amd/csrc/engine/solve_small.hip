@@ -149,6 +149,42 @@ __device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, c
   }
 }
 
+// thin supernode (one column, a few rows) by one lane: forward
+// x(label q) -= L(q) x(c0) for its rows below the diagonal
+template <typename T, int NR>
+__device__ void ss_thin_forward(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x) {
+  const int c0 = I.x, r = I.z;
+  const int64_t pis = I.w;
+  const T* P = valL + px;
+  T y[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) y[c] = x[c0 * NR + c];
+  for (int q = 1; q < r; ++q) {
+    const int lab = a.rows[pis + q];
+    const T l = P[q];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) atomicAdd(&x[lab * NR + c], -l * y[c]);
+  }
+}
+// and backward: x(c0) -= sum_q L(q) x(label q)
+template <typename T, int NR>
+__device__ void ss_thin_backward(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x) {
+  const int c0 = I.x, r = I.z;
+  const int64_t pis = I.w;
+  const T* P = valL + px;
+  T v[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) v[c] = x[c0 * NR + c];
+  for (int q = 1; q < r; ++q) {
+    const int lab = a.rows[pis + q];
+    const T l = P[q];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) v[c] -= l * x[lab * NR + c];
+  }
+#pragma unroll
+  for (int c = 0; c < NR; ++c) x[c0 * NR + c] = v[c];
+}
+
 constexpr int SS_TP = 65;  // pitch of the staged 64 x 64 triangle
 
 // stage the unit-lower triangle of chunk cb of a supernode: tile[j * SS_TP + i] = L(cb + i, cb + j), i > j
@@ -284,30 +320,36 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
   for (int i = t; i <= nlev; i += SS_THREADS) lp[i] = levptr[i];
   for (int i = t; i < nlev; i += SS_THREADS) lm[i] = mode[i];
   __syncthreads();
+  // first wave-swept item (after the level's thin supernodes) of this wave
+  auto first = [&](int l) { return lp[l] + (lm[l] >> 1) + wv; };
   int4 nI = make_int4(0, 0, 0, 0);
   int64_t npx = 0;
-  bool nok = lp[0] + wv < lp[1];
+  bool nok = first(0) < lp[1];
   if (nok) {
-    nI = info[lp[0] + wv];
-    npx = ipx[lp[0] + wv];
+    nI = info[first(0)];
+    npx = ipx[first(0)];
   }
   for (int l = 0; l < nlev; ++l) {
     const int4 I = nI;
     const int64_t px = npx;
     const bool ok = nok;
     if (l + 1 < nlev) {  // the next level's first item of this wave, in flight meanwhile
-      const int k2 = lp[l + 1] + wv;
+      const int k2 = first(l + 1);
       nok = k2 < lp[l + 2];
       if (nok) {
         nI = info[k2];
         npx = ipx[k2];
       }
     }
-    if (lm[l]) {
-      for (int k = lp[l]; k < lp[l + 1]; ++k) ss_forward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile);
+    const int lt = lp[l] + (lm[l] >> 1);  // end of the thin supernodes
+    for (int k = lp[l] + t; k < lt; k += SS_THREADS) ss_thin_forward<T, NR>(info[k], ipx[k], a, valL, x);
+    if (lm[l] & 1) {
+      __syncthreads();  // the thin sweeps' atomics before the workgroup's plain updates
+      for (int k = lt; k < lp[l + 1]; ++k) ss_forward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile);
+      __syncthreads();  // also where the level holds thin supernodes only
     } else {
       if (ok) ss_forward<T, NR>(I, px, ln, a, valL, x);
-      for (int k = lp[l] + wv + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
+      for (int k = first(l) + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
         ss_forward<T, NR>(info[k], ipx[k], ln, a, valL, x);
       __syncthreads();
     }
@@ -330,10 +372,10 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
       for (int c = 0; c < NR; ++c) x[i * NR + c] *= inv;
     }
   }
-  nok = lp[nlev - 1] + wv < lp[nlev];
+  nok = first(nlev - 1) < lp[nlev];
   if (nok) {
-    nI = info[lp[nlev - 1] + wv];
-    npx = ipx[lp[nlev - 1] + wv];
+    nI = info[first(nlev - 1)];
+    npx = ipx[first(nlev - 1)];
   }
   __syncthreads();
   for (int l = nlev - 1; l >= 0; --l) {
@@ -341,18 +383,21 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
     const int64_t px = npx;
     const bool ok = nok;
     if (l > 0) {
-      const int k2 = lp[l - 1] + wv;
+      const int k2 = first(l - 1);
       nok = k2 < lp[l];
       if (nok) {
         nI = info[k2];
         npx = ipx[k2];
       }
     }
-    if (lm[l]) {
-      for (int k = lp[l + 1] - 1; k >= lp[l]; --k) ss_backward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile, red);
+    const int lt = lp[l] + (lm[l] >> 1);
+    for (int k = lp[l] + t; k < lt; k += SS_THREADS) ss_thin_backward<T, NR>(info[k], ipx[k], a, valL, x);
+    if (lm[l] & 1) {
+      for (int k = lp[l + 1] - 1; k >= lt; --k) ss_backward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile, red);
+      __syncthreads();
     } else {
       if (ok) ss_backward<T, NR>(I, px, ln, a, valL, x);
-      for (int k = lp[l] + wv + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
+      for (int k = first(l) + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
         ss_backward<T, NR>(info[k], ipx[k], ln, a, valL, x);
       __syncthreads();
     }

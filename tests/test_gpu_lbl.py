@@ -8,6 +8,7 @@ bound; solutions agree with the oracle's and the refined backward error
 import os
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 from nasoq_amd import lbl
@@ -18,14 +19,31 @@ from conftest import dense_from_factor
 pytestmark = pytest.mark.gpu
 
 
-def engine(n, Ap, Ai, Ax, reg=1e-9, perm=None, **kw):
+def engine(n, Ap, Ai, Ax, reg=1e-9, perm=None, extra_cols=None, **kw):
     s = lbl.SolverSettings(n, Ap, Ai, Ax)
     s.reg_diag = reg
     for k, v in kw.items():
         setattr(s, k, v)
-    assert s.symbolic_analysis(perm=perm) == 1
+    assert s.symbolic_analysis(perm=perm, extra_cols=extra_cols) == 1
     assert s.numerical_factorization() == 1
     return s
+
+
+QP_ROWS = 20 * 20 + 20  # variables and equality rows of the small QP below
+
+
+def qp_super_kkt(nrhs=1):
+    """The super-KKT pattern [H A' C'; A . .; C . .] of a QP of the plain
+    NASOQ shape (20 x 20 Laplacian, 20 equality rows, x >= -0.5), made
+    quasi-definite (-1e-2 on the 420 constraint diagonals; more constraints
+    than variables would leave the zero block singular)."""
+    H, q, A, b, C, d = P.qp_config(grid=20, meq=20)
+    nc = A.shape[0] + C.shape[0]
+    K = sp.bmat([[H, A.T, C.T], [A, None, None], [C, None, None]], format="csc")
+    K = (K - 1e-2 * sp.diags(np.r_[np.zeros(H.shape[0]), np.ones(nc)])).tocsc()
+    n, Ap, Ai, Ax = P.lower_csc(K)
+    B = np.random.default_rng(11).uniform(-1, 1, (n, nrhs))
+    return n, Ap, Ai, Ax, B
 
 
 def oracle_factor(O, s, n, Ap, Ai, Ax, reg):
@@ -240,9 +258,14 @@ def test_small_sweeps_match_level_sweeps(oracle, prec, ss_mode, monkeypatch):
     if ss_mode != "auto":
         monkeypatch.setenv("LBLG_SMALL_SOLVE_MODE", ss_mode)
     cases = [(P.kkt_config(10, 150, 6, seed=2, nrhs=8), None),
-             (P.block_indefinite(4, 150, seed=3, nrhs=8), "nat")]
+             (P.block_indefinite(4, 150, seed=3, nrhs=8), "nat"),
+             (qp_super_kkt(nrhs=8), "extra")]
     for (n, Ap, Ai, Ax, B), order in cases:
         perm = np.arange(n, dtype=np.int32) if order == "nat" else None
+        extra = None
+        if order == "extra":  # the QP's inequality rows as singleton supernodes ("thin" sweeps)
+            extra = np.zeros(n, dtype=np.int32)
+            extra[QP_ROWS:] = 1
         reg = 1e-9 if prec == 0 else max(1e-9, np.ldexp(np.abs(Ax).max(), -12))
         K = P.lower_to_full(n, Ap, Ai, Ax).toarray()
         d = np.diag(K).copy()
@@ -252,7 +275,7 @@ def test_small_sweeps_match_level_sweeps(oracle, prec, ss_mode, monkeypatch):
         for sweep in ("small", "levels"):
             monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
             print(f"case sweeps n={n} prec={prec} sweep={sweep}", flush=True)
-            s = engine(n, Ap, Ai, Ax, reg=1e-9, perm=perm, precision=prec)
+            s = engine(n, Ap, Ai, Ax, reg=1e-9, perm=perm, precision=prec, extra_cols=extra)
             assert s.lib.lblg_set_kernel_timing(s.h, 1) == 0
             s.req_ref_iter = 0
             e = []
