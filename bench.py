@@ -312,9 +312,20 @@ def qp_leg(args):
                        "x >= -0.5, eps 1e-8", "status": "optimal", "seconds": dt, "active_set_iterations": m.num_iter,
            "ms_per_iteration": dt / max(m.num_iter, 1) * 1e3, "active_constraints": len(m.active_set),
            "objective": m.objective, "kkt_factorizations": int(m.factorizations) - f0}
+    # the same solve with the KKT schedule picked by the autotuner
+    t0 = time.perf_counter()
+    sched = m.autotune()
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st = m.solve()
+    dt2 = time.perf_counter() - t0
+    assert st == qp.OPTIMAL
+    out["autotuned"] = {"seconds": dt2, "ms_per_iteration": dt2 / max(m.num_iter, 1) * 1e3, "tune_seconds": t_tune,
+                        "schedule": sched}
     m.close()
     print(f"[bench] qp configuration: {dt:.2f} s, {out['active_set_iterations']} iterations "
-          f"({out['ms_per_iteration']:.2f} ms each), {out['active_constraints']} active", file=sys.stderr, flush=True)
+          f"({out['ms_per_iteration']:.2f} ms each), {out['active_constraints']} active; autotuned schedule "
+          f"{dt2:.2f} s (tuning {t_tune:.1f} s)", file=sys.stderr, flush=True)
     return out
 
 

@@ -316,6 +316,11 @@ int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* s);
  * settings, the super-KKT and its symbolic analysis (host only). */
 int lblg_qp_analyze(lblg_qp* qp);
 
+/* Device schedule of the QP's KKT handle chosen by the autotuner
+ * (lblg_autotune) on the super-KKT with every inequality inactive; kept for
+ * the solves that follow (engine extension, no reference counterpart). */
+int lblg_qp_autotune(lblg_qp* qp, int reps, lblg_tuning* t);
+
 /* Nasoq::solve() (nasoq.cpp:656-710); *status receives the LBLG_QP_* status. */
 int lblg_qp_solve(lblg_qp* qp, int* status);
 

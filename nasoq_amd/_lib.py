@@ -128,6 +128,7 @@ SIGNATURES = [
     ("lblg_qp_destroy", ctypes.c_int, [_vp]),
     ("lblg_qp_set_settings", ctypes.c_int, [_vp, ctypes.POINTER(QPSettings)]),
     ("lblg_qp_analyze", ctypes.c_int, [_vp]),
+    ("lblg_qp_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Tuning)]),
     ("lblg_qp_solve", ctypes.c_int, [_vp, _i32p]),
     ("lblg_qp_solve_warm", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, ctypes.c_int, _f64p, _i32p]),
     ("lblg_qp_solve_predicted", ctypes.c_int, [_vp, _i32p, ctypes.c_int, _i32p, _i32p]),

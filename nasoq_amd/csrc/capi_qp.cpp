@@ -142,6 +142,19 @@ extern "C" int lblg_qp_analyze(lblg_qp* qp) {
   });
 }
 
+extern "C" int lblg_qp_autotune(lblg_qp* qp, int reps, lblg_tuning* t) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  if (!s || reps < 1) return LBLG_ERR_ARG;
+  return guarded([&] {
+    s->auto_reg_en = s->variant == LBLG_QP_AUTO ? 2 : (s->variant == LBLG_QP_FIXED ? 1 : 0);
+    s->symbolic();  // the super-KKT and its analysis, kept for the solves that follow
+    int rc = lblg_set_values(s->h, s->SM.x.data());
+    if (rc == LBLG_OK) rc = lblg_autotune(s->h, 1, reps, t);
+    if (rc != LBLG_OK) throw std::runtime_error("qp: autotune of the KKT schedule failed");
+    return LBLG_OK;
+  });
+}
+
 extern "C" int lblg_qp_solve(lblg_qp* qp, int* status) {
   QpSolver* s = reinterpret_cast<QpSolver*>(qp);
   if (!s) return LBLG_ERR_ARG;

@@ -1072,44 +1072,47 @@ void autotune(Handle& h, int nrhs, int reps, double& factor_ms, double& solve_ms
       return false;
     }
   };
-  // factor schedule: panel group width (only where it changes the partition)
-  // and the Schur tile threshold, timed by the factor
+  // factor schedule, one knob at a time (coordinate descent), timed by the
+  // factor: panel group width (only values below the widest supernode change
+  // the partition), then the Schur tile threshold, then the rank of the wide
+  // panels' trailing updates where cooperative panels remain
   Handle::Tuning best = h.tune;
   double bf = 1e300, bs = 1e300;
-  std::vector<int> gws = {best.group_w};
-  for (int g : {1024, 2048, 4096})
-    if (g != best.group_w && g < wmax) gws.push_back(g);
-  for (int g : gws)
-    for (int m : {128, 256, 512}) {
+  auto consider = [&](const Handle::Tuning& t) {
+    double f, sl;
+    if (attempt(t, f, sl) && f < bf) {
+      bf = f;
+      bs = sl;
+      best = t;
+    }
+  };
+  consider(best);
+  for (int g : {128, 256, 512, 1024, 2048, 4096})
+    if (g != best.group_w && g < wmax) {
       Handle::Tuning t = best;
       t.group_w = g;
-      t.big_m = m;
-      double f, sl;
-      if (attempt(t, f, sl) && f < bf) {
-        bf = f;
-        bs = sl;
-        best = t;
-      }
+      consider(t);
     }
-  if (wmax > best.group_w) {  // cooperative panels: the rank of their blocked trailing updates
-    const int cur = best.wide_sub;
-    for (int ws : {2, 4, 8}) {
-      if (ws == cur) continue;
+  for (int m : {128, 256, 512})
+    if (m != best.big_m) {
       Handle::Tuning t = best;
-      t.wide_sub = ws;
-      double f, sl;
-      if (attempt(t, f, sl) && f < bf) {
-        bf = f;
-        bs = sl;
-        best = t;
-      }
+      t.big_m = m;
+      consider(t);
     }
+  if (wmax > best.group_w) {
+    const int cur = best.wide_sub;
+    for (int ws : {2, 4, 8})
+      if (ws != cur) {
+        Handle::Tuning t = best;
+        t.wide_sub = ws;
+        consider(t);
+      }
   }
   // solve schedule: chain width and the one-workgroup sweep limit, timed by the solve
   const Handle::Tuning fbest = best;
   std::vector<int> cws = {fbest.chain_w};
-  for (int c : {256, 512, 1024, 1 << 30})
-    if (c != fbest.chain_w && c < wmax) cws.push_back(c);
+  for (int c : {256, 512, 1024, 1 << 30})  // 1 << 30: no chain sweeps (only tried where some exist now)
+    if (c != fbest.chain_w && (c < wmax || (c == (1 << 30) && wmax >= fbest.chain_w))) cws.push_back(c);
   std::vector<int> sns = {fbest.small_nnz};
   for (int v : {0, 1 << 20, 1 << 22})
     if (v != fbest.small_nnz && (v == 0 || nnzL <= v)) sns.push_back(v);

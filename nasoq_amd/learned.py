@@ -47,12 +47,15 @@ class ParametricQpFamily:
     def thetas(self, count, seed):
         return np.random.default_rng(seed).standard_normal((count, self.n_theta))
 
-    def solver(self, eps_abs=1e-8):
+    def solver(self, eps_abs=1e-8, autotune=False):
         """A Nasoq instance of the family (analysis and device buffers kept
-        across members; set_linear swaps q)."""
+        across members; set_linear swaps q); autotune: pick the KKT's device
+        schedule once for the whole family."""
         m = qp.Nasoq(self.H, np.zeros(self.n), B=self.B, b=self.d)
         m.eps_abs = eps_abs
         m.variant = qp.FIXED
+        if autotune:
+            m.autotune()
         return m
 
     def kkt_residuals(self, x, z, theta):

@@ -120,6 +120,14 @@ class Nasoq:
         self._pull()
         return st.value, pr.value
 
+    def autotune(self, reps=1):
+        """Pick the device schedule of the QP's KKT handle by timing candidates
+        (lblg_qp_autotune); kept for the solves that follow."""
+        self._push()
+        t = _lib.Tuning()
+        check(self.lib.lblg_qp_autotune(self.h, reps, ctypes.byref(t)), "lblg_qp_autotune")
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
     def set_linear(self, q=None, a=None, b=None):
         """New linear terms for the next solve (None keeps a vector); the
         super-KKT pattern, its analysis and the device buffers are kept."""

@@ -178,3 +178,23 @@ def test_plain_nasoq_configuration():
     x[act] = xa
     assert np.linalg.norm(m.primal_vars - x) <= 1e-7 * max(1.0, np.linalg.norm(x))
     m.close()
+
+
+def test_qp_autotuned_schedule_keeps_the_solution():
+    # a KKT device schedule picked by the autotuner gives the same optimum
+    H, q, A, b, C, d = P.qp_config(grid=24, meq=30, seed=3)
+    Hl = sp.tril(H, format="csc")
+    sols = []
+    for tune in (False, True):
+        m = qp.Nasoq(Hl, q, A, b, C, d)
+        m.eps_abs = 1e-8
+        m.variant = qp.FIXED
+        if tune:
+            t = m.autotune()
+            print(f"case qp autotune {t}", flush=True)
+            assert t["schedules_tried"] >= 2
+        assert m.solve() == qp.OPTIMAL
+        _kkt_check(m, Hl, q, A, b, C, d, 1e-6)
+        sols.append(m.primal_vars.copy())
+        m.close()
+    assert np.linalg.norm(sols[0] - sols[1]) <= 1e-7 * max(1.0, np.linalg.norm(sols[0]))
