@@ -1038,6 +1038,42 @@ __device__ inline int row_label(const int* rows, int64_t pis, int c0, int w, int
 template <typename T>
 __device__ inline T bcast(T v, int lane) { return __shfl(v, lane, 64); }
 
+// Stage the strict lower triangle of a piece's pw x pw diagonal block (pw <=
+// SOLVE_W) into LDS: entry L(d0 + i, d0 + j), i > j, at Lt[j * lp + i]
+// (colmajor) or Lt[i * lp + j] (the transpose).  Thread (i, h) (i = row, h =
+// which half of the 256) loads the entries j = h, h + 2, ... < i of its row,
+// 16 loads in flight before the first LDS store; a wave reads 64 consecutive
+// rows of one column per load.  Narrow pieces cost a few loads per thread.
+template <typename T>
+__device__ __forceinline__ void stage_piece_triangle(T* Lt, int lp, const T* P, int64_t r, int d0, int pw,
+                                                     bool colmajor) {
+  constexpr int B = 16;
+  const int i = threadIdx.x & (SOLVE_W - 1), h = threadIdx.x / SOLVE_W;
+  if (i >= pw) return;
+  const int nj = (i - h + 1) / 2;  // entries j = h + 2 k < i
+  const T* Pi = P + (int64_t)(d0 + i) + (int64_t)(d0 + h) * r;
+  for (int k0 = 0; k0 < nj; k0 += B) {
+    T v[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) v[k] = (k0 + k < nj) ? Pi[(int64_t)(2 * (k0 + k)) * r] : (T)0;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int j = h + 2 * (k0 + k);
+      if (k0 + k < nj) Lt[colmajor ? j * lp + i : i * lp + j] = v[k];
+    }
+  }
+}
+// value of lane `lane` (wave-uniform) of a VGPR, through v_readlane
+__device__ inline double readlane_t(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline float readlane_t(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // forward triangle of a piece: Y(p) = L_pp^{-1} X(p)
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* piece_sn, const int* piece_c0,
@@ -1052,10 +1088,7 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
   const int64_t r = pi[s + 1] - pi[s];
   const T* P = valL + px[s];
   const int t = threadIdx.x;
-  for (int idx = t; idx < pw * pw; idx += 256) {
-    const int i = idx % pw, j = idx / pw;
-    if (i > j) Lt[j * lp + i] = P[(c0l + i) + (int64_t)(c0l + j) * r];
-  }
+  stage_piece_triangle(Lt, lp, P, r, c0l, pw, true);
   __syncthreads();
   if (t >= 64) return;
   const int64_t base = (int64_t)(c0 + c0l) * nrhs;
@@ -1065,16 +1098,37 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
     x0[c] = (c < nrhs && t < pw) ? X[base + (int64_t)t * nrhs + c] : (T)0;
     x1[c] = (c < nrhs && t + 64 < pw) ? X[base + (int64_t)(t + 64) * nrhs + c] : (T)0;
   }
-  for (int j = 0; j < pw - 1; ++j) {
-    const T* col = Lt + j * lp;
-    const T l0 = (t > j && t < pw) ? col[t] : (T)0;
-    const T l1 = (t + 64 > j && t + 64 < pw) ? col[t + 64] : (T)0;
+  // column j holds rows j + 1 .. pw - 1; the entries of step j + 1 are read
+  // during step j (addresses clamped into the column, masked afterwards)
+  const int n0 = min(64, pw - 1);
+  T a0 = 0, a1 = 0;
+  if (n0 > 0) {
+    a0 = Lt[min(max(t, 1), pw - 1)];
+    a1 = Lt[min(t + 64, pw - 1)];
+  }
+  for (int j = 0; j < n0; ++j) {  // x_j in lane j of x0
+    const T l0 = (t > j && t < pw) ? a0 : (T)0;
+    const T l1 = (t + 64 < pw) ? a1 : (T)0;
+    if (j + 1 < n0) {
+      a0 = Lt[(j + 1) * lp + min(max(t, j + 2), pw - 1)];
+      a1 = Lt[(j + 1) * lp + min(t + 64, pw - 1)];
+    }
 #pragma unroll
     for (int c = 0; c < MAXRHS; ++c) {
       if (c >= nrhs) break;
-      const T xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
+      const T xj = readlane_t(x0[c], j);
       x0[c] -= l0 * xj;
       x1[c] -= l1 * xj;
+    }
+  }
+  if (64 < pw - 1) a1 = Lt[64 * lp + min(max(t + 64, 65), pw - 1)];
+  for (int j = 64; j < pw - 1; ++j) {  // x_j in lane j - 64 of x1
+    const T l1 = (t + 64 > j && t + 64 < pw) ? a1 : (T)0;
+    if (j + 1 < pw - 1) a1 = Lt[(j + 1) * lp + min(max(t + 64, j + 2), pw - 1)];
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c) {
+      if (c >= nrhs) break;
+      x1[c] -= l1 * readlane_t(x1[c], j - 64);
     }
   }
 #pragma unroll
@@ -1182,15 +1236,24 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
   const int t = threadIdx.x;
   const int64_t qb = c0l + pw + (int64_t)GROWS * rb;
   const int nq = (int)min((int64_t)GROWS, r - qb);
+  // the GROWS x pw block: thread (qq, g) loads columns g, g + 4, ... of row
+  // qq, 16 loads in flight before its first LDS store
+  const int qq0 = t % GROWS, g0 = t / GROWS;
+  const int njb = (pw - g0 + 3) / 4;
+  const T* Pq = P + qb + qq0 + (int64_t)(c0l + g0) * r;
   if (t < nq) {
     const int lab = row_label(rows, pi[s], c0, w, qb + t);
     for (int c = 0; c < nrhs; ++c) xq[t * nrhs + c] = Xf[(int64_t)lab * nrhs + c];
   } else if (t < GROWS) {
     for (int c = 0; c < nrhs; ++c) xq[t * nrhs + c] = 0.0;  // rows past the panel: 0 * stale LDS could be NaN
   }
-  for (int idx = t; idx < GROWS * pw; idx += 256) {
-    const int qq = idx % GROWS, j = idx / GROWS;
-    Lb[j * (GROWS + 1) + qq] = qq < nq ? P[qb + qq + (int64_t)(c0l + j) * r] : (T)0;
+  for (int k0 = 0; k0 < njb; k0 += 16) {
+    T v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = (qq0 < nq && k0 + k < njb) ? Pq[(int64_t)(4 * (k0 + k)) * r] : (T)0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k0 + k < njb) Lb[(g0 + 4 * (k0 + k)) * (GROWS + 1) + qq0] = v[k];
   }
   __syncthreads();
   const int j = t & (SOLVE_W - 1), half = t >> 7;  // 2 x 128 threads: column j, rows of one half
@@ -1221,7 +1284,7 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
                                                  const T* valL, const T* Z, const T* acc,
                                                  T* Xf, int nrhs, int lp) {
   extern __shared__ double smem_d[];
-  T* Lt = reinterpret_cast<T*>(smem_d);  // Lt[i * lp + j] = L(j, i), j > i (column i of L)
+  T* Lt = reinterpret_cast<T*>(smem_d);  // Lt[i * lp + j] = L(i, j), i > j (row i of L)
   const int p = pieces[blockIdx.x];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
@@ -1229,10 +1292,7 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
   const int64_t r = pi[s + 1] - pi[s];
   const T* P = valL + px[s];
   const int t = threadIdx.x;
-  for (int idx = t; idx < pw * pw; idx += 256) {
-    const int jj = idx % pw, i = idx / pw;
-    if (jj > i) Lt[i * lp + jj] = P[(c0l + jj) + (int64_t)(c0l + i) * r];
-  }
+  stage_piece_triangle(Lt, lp, P, r, c0l, pw, false);
   __syncthreads();
   if (t >= 64) return;
   const int64_t base = (int64_t)(c0 + c0l) * nrhs;
@@ -1243,16 +1303,38 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
     x1[c] = (c < nrhs && t + 64 < pw) ? Z[base + (int64_t)(t + 64) * nrhs + c] - acc[base + (int64_t)(t + 64) * nrhs + c]
                                       : (T)0;
   }
-  for (int j = pw - 1; j > 0; --j) {
-    // rows i < j: x_i -= L(j, i) x_j
-    const T l0 = (t < j) ? Lt[t * lp + j] : (T)0;
-    const T l1 = (t + 64 < j) ? Lt[(t + 64) * lp + j] : (T)0;
+  // rows i < j: x_i -= L(j, i) x_j, from the last column back; row j holds
+  // L(j, 0 .. j - 1) and the entries of the next step are read during the
+  // current one (addresses clamped into the row, masked afterwards)
+  T a0 = 0, a1 = 0;
+  if (pw - 1 >= 64) {
+    a0 = Lt[(pw - 1) * lp + t];
+    a1 = Lt[(pw - 1) * lp + min(t + 64, pw - 2)];
+  }
+  for (int j = pw - 1; j >= 64; --j) {  // x_j in lane j - 64 of x1
+    const T l0 = a0;
+    const T l1 = (t + 64 < j) ? a1 : (T)0;
+    if (j - 1 >= 64) {
+      a0 = Lt[(j - 1) * lp + t];
+      a1 = Lt[(j - 1) * lp + min(t + 64, j - 2)];
+    }
 #pragma unroll
     for (int c = 0; c < MAXRHS; ++c) {
       if (c >= nrhs) break;
-      const T xj = bcast(j < 64 ? x0[c] : x1[c], j & 63);
+      const T xj = readlane_t(x1[c], j - 64);
       x0[c] -= l0 * xj;
       x1[c] -= l1 * xj;
+    }
+  }
+  const int m0 = min(pw, 64) - 1;
+  if (m0 > 0) a0 = Lt[m0 * lp + min(t, m0 - 1)];
+  for (int j = m0; j > 0; --j) {  // x_j in lane j of x0
+    const T l0 = (t < j) ? a0 : (T)0;
+    if (j - 1 > 0) a0 = Lt[(j - 1) * lp + min(t, j - 2)];
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c) {
+      if (c >= nrhs) break;
+      x0[c] -= l0 * readlane_t(x0[c], j);
     }
   }
 #pragma unroll

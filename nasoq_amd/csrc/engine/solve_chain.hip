@@ -91,13 +91,44 @@ __device__ inline void xwait_block(const uint64_t* b, int64_t last_word, uint32_
   }
   __syncthreads();
 }
-__device__ inline double xget(const uint64_t* b, int64_t idx, uint32_t ep, int* status, int& abort, double*) {
-  const uint64_t lo = xword(b, 2 * idx, ep, status, abort);
-  const uint64_t hi = xword(b, 2 * idx + 1, ep, status, abort);
-  return __longlong_as_double((long long)((hi << 32) | lo));
+// Read the published values idx0 .. idx0 + n - 1 into dst[0 .. n) and zero
+// dst[n .. lim), lim <= 256 * NV (thread t handles k = t, t + 256, ...).
+// Every word of a thread is loaded before any is checked, so the round trips
+// overlap; a word that does not carry the epoch yet is polled again
+// (bounded, as in xword).
+template <typename T>
+__device__ inline T xassemble(const uint64_t* x);
+template <>
+__device__ inline double xassemble<double>(const uint64_t* x) {
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)x[1] << 32) | (uint32_t)x[0]));
 }
-__device__ inline float xget(const uint64_t* b, int64_t idx, uint32_t ep, int* status, int& abort, float*) {
-  return __uint_as_float(xword(b, idx, ep, status, abort));
+template <>
+__device__ inline float xassemble<float>(const uint64_t* x) {
+  return __uint_as_float((uint32_t)x[0]);
+}
+template <typename T, int NV>
+__device__ inline void xget_block(const uint64_t* b, int64_t idx0, int n, int lim, T* dst, uint32_t ep, int* status,
+                                  int& abort) {
+  constexpr int W = XWords<T>::WPV;
+  uint64_t x[NV][W];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int k = threadIdx.x + 256 * v;
+#pragma unroll
+    for (int h = 0; h < W; ++h)
+      x[v][h] = k < n ? __hip_atomic_load(b + (idx0 + k) * W + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int k = threadIdx.x + 256 * v;
+    if (k >= lim) continue;
+    if (k < n) {
+#pragma unroll
+      for (int h = 0; h < W; ++h)
+        if ((uint32_t)(x[v][h] >> 32) != ep) x[v][h] = xword(b, (idx0 + k) * W + h, ep, status, abort);
+    }
+    dst[k] = k < n ? xassemble<T>(x[v]) : (T)0;
+  }
 }
 
 // stage the strict lower triangle of the pw x pw diagonal block at (d0, d0)
@@ -192,10 +223,23 @@ __global__ __launch_bounds__(256) void k_fwd_chain(int s, int G, const int* supe
           x0[c] = t < pw ? xs[k][t * nrhs + c] : (T)0;
           x1[c] = t + 64 < pw ? xs[k][(t + 64) * nrhs + c] : (T)0;
         }
-        for (int j = 0; j < 64 && j < pw - 1; ++j) {  // x_j in lane j of x0
-          const T* col = tri + j * CU - j * (j + 1) / 2 - j - 1;  // col[i] = L(i, j), i > j
-          const T l0 = (t > j && t < pw) ? col[t] : (T)0;
-          const T l1 = (t + 64 < pw) ? col[t + 64] : (T)0;
+        // column j of the packed triangle holds rows j + 1 .. CU - 1 at
+        // tri + cofs(j) + row; the entries of step j + 1 are read during step
+        // j (addresses clamped into the column, masked afterwards)
+        auto cofs = [](int j) { return j * CU - j * (j + 1) / 2 - j - 1; };
+        const int n0 = min(64, pw - 1);
+        T a0 = 0, a1 = 0;
+        if (n0 > 0) {
+          a0 = tri[cofs(0) + max(t, 1)];
+          a1 = tri[cofs(0) + t + 64];
+        }
+        for (int j = 0; j < n0; ++j) {  // x_j in lane j of x0
+          const T l0 = (t > j && t < pw) ? a0 : (T)0;
+          const T l1 = (t + 64 < pw) ? a1 : (T)0;
+          if (j + 1 < n0) {
+            a0 = tri[cofs(j + 1) + max(t, j + 2)];
+            a1 = tri[cofs(j + 1) + t + 64];
+          }
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
             const T xj = lane_value(x0[c], j);
@@ -203,9 +247,10 @@ __global__ __launch_bounds__(256) void k_fwd_chain(int s, int G, const int* supe
             x1[c] -= l1 * xj;
           }
         }
+        if (64 < pw - 1) a1 = tri[cofs(64) + max(t + 64, 65)];
         for (int j = 64; j < pw - 1; ++j) {  // x_j in lane j - 64 of x1
-          const T* col = tri + j * CU - j * (j + 1) / 2 - j - 1;
-          const T l1 = (t + 64 > j && t + 64 < pw) ? col[t + 64] : (T)0;
+          const T l1 = (t + 64 > j && t + 64 < pw) ? a1 : (T)0;
+          if (j + 1 < pw - 1) a1 = tri[cofs(j + 1) + max(t + 64, j + 2)];
 #pragma unroll
           for (int c = 0; c < NR; ++c) x1[c] -= l1 * lane_value(x1[c], j - 64);
         }
@@ -239,8 +284,7 @@ __global__ __launch_bounds__(256) void k_fwd_chain(int s, int G, const int* supe
     CHAIN_LOAD_BLOCK(g + k * G)
     xwait_block(xb, (((int64_t)CU * p + pw - 1) * nrhs + nrhs - 1) * XWords<T>::WPV + XWords<T>::WPV - 1, epoch,
                 status, abort);
-    for (int idx = t; idx < CU * nrhs; idx += 256)
-      ys[idx] = idx < pw * nrhs ? xget(xb, (int64_t)CU * p * nrhs + idx, epoch, status, abort, (T*)nullptr) : (T)0;
+    xget_block<T, (CU * NR + 255) / 256>(xb, (int64_t)CU * p * nrhs, pw * nrhs, CU * nrhs, ys, epoch, status, abort);
     __syncthreads();
     for (; k < CUPW; ++k) {
       const int u = g + k * G;
@@ -307,9 +351,9 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
         lv[hh][k] = (row < nq && j < pw) ? L[q0 + row + (int64_t)(cu0 + j) * r] : (T)0;
       }
   };
-  // acc += (loaded block)^T x(rows): x of rows below the diagonal block from Xf at
-  // labels labs[row]; x of diagonal unit v (labs null) from the exchange buffer
-  auto gather_loaded = [&](int nq, int pw, const int* labs, int v) {
+  // acc += (loaded block)^T x_v: the block of L in rows of diagonal unit v,
+  // x_v from the exchange buffer
+  auto gather_loaded = [&](int nq, int pw, int v) {
     for (int hh = 0; hh < 2; ++hh) {
       if (64 * hh >= nq) break;
       const int nh = min(64, nq - 64 * hh);
@@ -318,14 +362,8 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
         const int idx = t + 256 * k, qq = idx & 63, j = idx >> 6;
         stage[qq * (CU + 1) + j] = lv[hh][k];
       }
-      for (int idx = t; idx < 64 * nrhs; idx += 256) {
-        const int qq = idx / nrhs, c = idx % nrhs;
-        T x = 0;
-        if (qq < nh)
-          x = labs ? Xf[(int64_t)labs[64 * hh + qq] * nrhs + c]
-                   : xget(xb, ((int64_t)CU * v + 64 * hh + qq) * nrhs + c, epoch, status, abort, (T*)nullptr);
-        xv[idx] = x;
-      }
+      xget_block<T, (64 * NR + 255) / 256>(xb, ((int64_t)CU * v + 64 * hh) * nrhs, nh * nrhs, 64 * nrhs, xv, epoch,
+                                           status, abort);
       __syncthreads();
       const int j = t & (CU - 1), half = t >> 7;
       T a[NR];
@@ -366,7 +404,7 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
       load_block((int64_t)CU * v, pv, CU * u, pw);
       xwait_block(xb, (((int64_t)CU * v + pv - 1) * nrhs + nrhs - 1) * XWords<T>::WPV + XWords<T>::WPV - 1, epoch,
                   status, abort);
-      gather_loaded(pv, pw, nullptr, v);
+      gather_loaded(pv, pw, v);
     }
     // x_u = L_uu^{-T} (z_u - acc): row-packed strict lower triangle, row i at i (i - 1) / 2
     if (staged != u) {
@@ -382,10 +420,21 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
         x0[c] = t < pw ? Z[base + (int64_t)t * nrhs + c] - acc[t * nrhs + c] : (T)0;
         x1[c] = t + 64 < pw ? Z[base + (int64_t)(t + 64) * nrhs + c] - acc[(t + 64) * nrhs + c] : (T)0;
       }
+      // row j of the packed triangle holds L(j, 0 .. j - 1) at tri + j (j - 1) / 2;
+      // the entries of the next step are read during the current one
+      // (addresses clamped into the row, masked afterwards)
+      T a0 = 0, a1 = 0;
+      if (pw - 1 >= 64) {
+        a0 = tri[(pw - 1) * (pw - 2) / 2 + t];
+        a1 = tri[(pw - 1) * (pw - 2) / 2 + min(t + 64, pw - 2)];
+      }
       for (int j = pw - 1; j >= 64; --j) {  // x_j in lane j - 64 of x1
-        const T* row = tri + j * (j - 1) / 2;  // row[i] = L(j, i), i < j
-        const T l0 = row[t];
-        const T l1 = (t + 64 < j) ? row[t + 64] : (T)0;
+        const T l0 = a0;
+        const T l1 = (t + 64 < j) ? a1 : (T)0;
+        if (j - 1 >= 64) {
+          a0 = tri[(j - 1) * (j - 2) / 2 + t];
+          a1 = tri[(j - 1) * (j - 2) / 2 + min(t + 64, j - 2)];
+        }
 #pragma unroll
         for (int c = 0; c < NR; ++c) {
           const T xj = lane_value(x1[c], j - 64);
@@ -393,9 +442,11 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
           x1[c] -= l1 * xj;
         }
       }
-      for (int j = min(pw, 64) - 1; j > 0; --j) {  // x_j in lane j of x0
-        const T* row = tri + j * (j - 1) / 2;
-        const T l0 = (t < j) ? row[t] : (T)0;
+      const int m0 = min(pw, 64) - 1;
+      if (m0 > 0) a0 = tri[m0 * (m0 - 1) / 2 + min(t, m0 - 1)];
+      for (int j = m0; j > 0; --j) {  // x_j in lane j of x0
+        const T l0 = (t < j) ? a0 : (T)0;
+        if (j - 1 > 0) a0 = tri[(j - 1) * (j - 2) / 2 + min(t, j - 2)];
 #pragma unroll
         for (int c = 0; c < NR; ++c) x0[c] -= l0 * lane_value(x0[c], j);
       }
