@@ -220,8 +220,13 @@ int lblg_panel_stats(const lblg_handle* h, int64_t* panels, int64_t* panels_pivo
  * accumulated milliseconds and launch count of a kernel family since timing
  * was enabled: bk_small, bk_panel, bk_panel_group, trailing, offdiag, schur
  * (64-row tiles), scale_ld, schur_big (128-row tiles), fwd_tri, fwd_gemv,
- * bwd_gemv, bwd_tri, spmv. */
+ * bwd_gemv, bwd_tri, spmv.  lblg_set_kernel_timing_only times the families
+ * named in a comma-separated list alone (two events per launch of them, none
+ * for the others), so a measured run keeps its launch rate;
+ * lblg_set_kernel_timing(h, 0) ends either mode.
+ * Captured launch sequences (HIP graphs) are not used while timing is on. */
 int lblg_set_kernel_timing(lblg_handle* h, int on);
+int lblg_set_kernel_timing_only(lblg_handle* h, const char* name);
 int lblg_kernel_time(lblg_handle* h, const char* name, double* total_ms, int64_t* launches);
 
 /* Device self-test of the 16x16x4 MFMA lane layout the engine assumes, for

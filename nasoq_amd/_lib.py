@@ -117,6 +117,7 @@ SIGNATURES = [
                                                    _f64p]),
     ("lblg_selftest_mfma", ctypes.c_int, [_f64p, _f64p, _f64p, ctypes.c_int]),
     ("lblg_set_kernel_timing", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("lblg_set_kernel_timing_only", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("lblg_graph_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
     ("lblg_get_tuning", ctypes.c_int, [_vp, ctypes.POINTER(Tuning)]),
     ("lblg_set_tuning", ctypes.c_int, [_vp, ctypes.POINTER(Tuning)]),

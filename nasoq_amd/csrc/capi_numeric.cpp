@@ -297,7 +297,26 @@ extern "C" int lblg_set_kernel_timing(lblg_handle* hh, int on) {
   if (!h || !h->dev) return LBLG_ERR_STATE;
   h->dev->resolve_timing();
   h->dev->timing = on != 0;
+  h->dev->timing_only.clear();
   if (on) h->dev->ktime.clear();
+  return LBLG_OK;
+}
+
+extern "C" int lblg_set_kernel_timing_only(lblg_handle* hh, const char* name) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !h->dev) return LBLG_ERR_STATE;
+  if (!name || !*name) return LBLG_ERR_ARG;
+  h->dev->resolve_timing();
+  h->dev->timing_only.clear();
+  for (const char* p = name; *p;) {  // comma-separated family names
+    const char* e = p;
+    while (*e && *e != ',') ++e;
+    if (e > p) h->dev->timing_only.insert(std::string(p, e));
+    p = *e ? e + 1 : e;
+  }
+  if (h->dev->timing_only.empty()) return LBLG_ERR_ARG;
+  h->dev->timing = true;
+  h->dev->ktime.clear();
   return LBLG_OK;
 }
 

@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -162,6 +163,7 @@ struct DeviceFactor {
   int64_t graph_launches = 0, graph_captures = 0;
   // optional per-kernel timing with events on the engine stream
   bool timing = false;
+  std::set<std::string> timing_only;  // when not empty, only these kernel families are timed
   struct Pending {
     std::string name;
     hipEvent_t a, b;

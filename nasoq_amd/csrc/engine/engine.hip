@@ -58,7 +58,7 @@ void DeviceFactor::resolve_timing() {
 // run `launch` bracketed by events on the engine stream when timing is on
 template <typename F>
 static void timed(DeviceFactor& D, const char* name, F launch) {
-  if (!D.timing) {
+  if (!D.timing || (!D.timing_only.empty() && !D.timing_only.count(name))) {
     launch();
     return;
   }

@@ -1090,52 +1090,56 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
   const int t = threadIdx.x;
   stage_piece_triangle(Lt, lp, P, r, c0l, pw, true);
   __syncthreads();
-  if (t >= 64) return;
+  // wave wv substitutes right-hand sides wv and wv + 4 (lane ln holds rows
+  // ln and ln + 64), so up to four SIMDs run the sequential steps at once
+  const int wv = t >> 6, ln = t & 63;
+  if (wv >= min(nrhs, 4)) return;
   const int64_t base = (int64_t)(c0 + c0l) * nrhs;
-  T x0[MAXRHS], x1[MAXRHS];
+  T x0[2], x1[2];
 #pragma unroll
-  for (int c = 0; c < MAXRHS; ++c) {
-    x0[c] = (c < nrhs && t < pw) ? X[base + (int64_t)t * nrhs + c] : (T)0;
-    x1[c] = (c < nrhs && t + 64 < pw) ? X[base + (int64_t)(t + 64) * nrhs + c] : (T)0;
+  for (int q = 0; q < 2; ++q) {
+    const int c = wv + 4 * q;
+    x0[q] = (c < nrhs && ln < pw) ? X[base + (int64_t)ln * nrhs + c] : (T)0;
+    x1[q] = (c < nrhs && ln + 64 < pw) ? X[base + (int64_t)(ln + 64) * nrhs + c] : (T)0;
   }
+  const bool two = wv + 4 < nrhs;
   // column j holds rows j + 1 .. pw - 1; the entries of step j + 1 are read
   // during step j (addresses clamped into the column, masked afterwards)
   const int n0 = min(64, pw - 1);
   T a0 = 0, a1 = 0;
   if (n0 > 0) {
-    a0 = Lt[min(max(t, 1), pw - 1)];
-    a1 = Lt[min(t + 64, pw - 1)];
+    a0 = Lt[min(max(ln, 1), pw - 1)];
+    a1 = Lt[min(ln + 64, pw - 1)];
   }
   for (int j = 0; j < n0; ++j) {  // x_j in lane j of x0
-    const T l0 = (t > j && t < pw) ? a0 : (T)0;
-    const T l1 = (t + 64 < pw) ? a1 : (T)0;
+    const T l0 = (ln > j && ln < pw) ? a0 : (T)0;
+    const T l1 = (ln + 64 < pw) ? a1 : (T)0;
     if (j + 1 < n0) {
-      a0 = Lt[(j + 1) * lp + min(max(t, j + 2), pw - 1)];
-      a1 = Lt[(j + 1) * lp + min(t + 64, pw - 1)];
+      a0 = Lt[(j + 1) * lp + min(max(ln, j + 2), pw - 1)];
+      a1 = Lt[(j + 1) * lp + min(ln + 64, pw - 1)];
     }
-#pragma unroll
-    for (int c = 0; c < MAXRHS; ++c) {
-      if (c >= nrhs) break;
-      const T xj = readlane_t(x0[c], j);
-      x0[c] -= l0 * xj;
-      x1[c] -= l1 * xj;
+    const T xa = readlane_t(x0[0], j);
+    x0[0] -= l0 * xa;
+    x1[0] -= l1 * xa;
+    if (two) {
+      const T xb = readlane_t(x0[1], j);
+      x0[1] -= l0 * xb;
+      x1[1] -= l1 * xb;
     }
   }
-  if (64 < pw - 1) a1 = Lt[64 * lp + min(max(t + 64, 65), pw - 1)];
+  if (64 < pw - 1) a1 = Lt[64 * lp + min(max(ln + 64, 65), pw - 1)];
   for (int j = 64; j < pw - 1; ++j) {  // x_j in lane j - 64 of x1
-    const T l1 = (t + 64 > j && t + 64 < pw) ? a1 : (T)0;
-    if (j + 1 < pw - 1) a1 = Lt[(j + 1) * lp + min(max(t + 64, j + 2), pw - 1)];
-#pragma unroll
-    for (int c = 0; c < MAXRHS; ++c) {
-      if (c >= nrhs) break;
-      x1[c] -= l1 * readlane_t(x1[c], j - 64);
-    }
+    const T l1 = (ln + 64 > j && ln + 64 < pw) ? a1 : (T)0;
+    if (j + 1 < pw - 1) a1 = Lt[(j + 1) * lp + min(max(ln + 64, j + 2), pw - 1)];
+    x1[0] -= l1 * readlane_t(x1[0], j - 64);
+    if (two) x1[1] -= l1 * readlane_t(x1[1], j - 64);
   }
 #pragma unroll
-  for (int c = 0; c < MAXRHS; ++c) {
+  for (int q = 0; q < 2; ++q) {
+    const int c = wv + 4 * q;
     if (c >= nrhs) break;
-    if (t < pw) Y[base + (int64_t)t * nrhs + c] = x0[c];
-    if (t + 64 < pw) Y[base + (int64_t)(t + 64) * nrhs + c] = x1[c];
+    if (ln < pw) Y[base + (int64_t)ln * nrhs + c] = x0[q];
+    if (ln + 64 < pw) Y[base + (int64_t)(ln + 64) * nrhs + c] = x1[q];
   }
 }
 // forward update of the rows below a piece: X(label q) -= L(q, p) Y(p)
@@ -1294,54 +1298,59 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
   const int t = threadIdx.x;
   stage_piece_triangle(Lt, lp, P, r, c0l, pw, false);
   __syncthreads();
-  if (t >= 64) return;
+  // wave wv substitutes right-hand sides wv and wv + 4 (lane ln holds rows
+  // ln and ln + 64)
+  const int wv = t >> 6, ln = t & 63;
+  if (wv >= min(nrhs, 4)) return;
   const int64_t base = (int64_t)(c0 + c0l) * nrhs;
-  T x0[MAXRHS], x1[MAXRHS];
+  T x0[2], x1[2];
 #pragma unroll
-  for (int c = 0; c < MAXRHS; ++c) {
-    x0[c] = (c < nrhs && t < pw) ? Z[base + (int64_t)t * nrhs + c] - acc[base + (int64_t)t * nrhs + c] : (T)0;
-    x1[c] = (c < nrhs && t + 64 < pw) ? Z[base + (int64_t)(t + 64) * nrhs + c] - acc[base + (int64_t)(t + 64) * nrhs + c]
-                                      : (T)0;
+  for (int q = 0; q < 2; ++q) {
+    const int c = wv + 4 * q;
+    x0[q] = (c < nrhs && ln < pw) ? Z[base + (int64_t)ln * nrhs + c] - acc[base + (int64_t)ln * nrhs + c] : (T)0;
+    x1[q] = (c < nrhs && ln + 64 < pw)
+                ? Z[base + (int64_t)(ln + 64) * nrhs + c] - acc[base + (int64_t)(ln + 64) * nrhs + c]
+                : (T)0;
   }
+  const bool two = wv + 4 < nrhs;
   // rows i < j: x_i -= L(j, i) x_j, from the last column back; row j holds
   // L(j, 0 .. j - 1) and the entries of the next step are read during the
   // current one (addresses clamped into the row, masked afterwards)
   T a0 = 0, a1 = 0;
   if (pw - 1 >= 64) {
-    a0 = Lt[(pw - 1) * lp + t];
-    a1 = Lt[(pw - 1) * lp + min(t + 64, pw - 2)];
+    a0 = Lt[(pw - 1) * lp + ln];
+    a1 = Lt[(pw - 1) * lp + min(ln + 64, pw - 2)];
   }
   for (int j = pw - 1; j >= 64; --j) {  // x_j in lane j - 64 of x1
     const T l0 = a0;
-    const T l1 = (t + 64 < j) ? a1 : (T)0;
+    const T l1 = (ln + 64 < j) ? a1 : (T)0;
     if (j - 1 >= 64) {
-      a0 = Lt[(j - 1) * lp + t];
-      a1 = Lt[(j - 1) * lp + min(t + 64, j - 2)];
+      a0 = Lt[(j - 1) * lp + ln];
+      a1 = Lt[(j - 1) * lp + min(ln + 64, j - 2)];
     }
-#pragma unroll
-    for (int c = 0; c < MAXRHS; ++c) {
-      if (c >= nrhs) break;
-      const T xj = readlane_t(x1[c], j - 64);
-      x0[c] -= l0 * xj;
-      x1[c] -= l1 * xj;
+    const T xa = readlane_t(x1[0], j - 64);
+    x0[0] -= l0 * xa;
+    x1[0] -= l1 * xa;
+    if (two) {
+      const T xb = readlane_t(x1[1], j - 64);
+      x0[1] -= l0 * xb;
+      x1[1] -= l1 * xb;
     }
   }
   const int m0 = min(pw, 64) - 1;
-  if (m0 > 0) a0 = Lt[m0 * lp + min(t, m0 - 1)];
+  if (m0 > 0) a0 = Lt[m0 * lp + min(ln, m0 - 1)];
   for (int j = m0; j > 0; --j) {  // x_j in lane j of x0
-    const T l0 = (t < j) ? a0 : (T)0;
-    if (j - 1 > 0) a0 = Lt[(j - 1) * lp + min(t, j - 2)];
-#pragma unroll
-    for (int c = 0; c < MAXRHS; ++c) {
-      if (c >= nrhs) break;
-      x0[c] -= l0 * readlane_t(x0[c], j);
-    }
+    const T l0 = (ln < j) ? a0 : (T)0;
+    if (j - 1 > 0) a0 = Lt[(j - 1) * lp + min(ln, j - 2)];
+    x0[0] -= l0 * readlane_t(x0[0], j);
+    if (two) x0[1] -= l0 * readlane_t(x0[1], j);
   }
 #pragma unroll
-  for (int c = 0; c < MAXRHS; ++c) {
+  for (int q = 0; q < 2; ++q) {
+    const int c = wv + 4 * q;
     if (c >= nrhs) break;
-    if (t < pw) Xf[base + (int64_t)t * nrhs + c] = x0[c];
-    if (t + 64 < pw) Xf[base + (int64_t)(t + 64) * nrhs + c] = x1[c];
+    if (ln < pw) Xf[base + (int64_t)ln * nrhs + c] = x0[q];
+    if (ln + 64 < pw) Xf[base + (int64_t)(ln + 64) * nrhs + c] = x1[q];
   }
 }
 

@@ -216,12 +216,17 @@ __global__ __launch_bounds__(256) void k_fwd_chain(int s, int G, const int* supe
         staged = p;
         __syncthreads();
       }
-      if (t < 64) {
-        T x0[NR], x1[NR];
+      // wave wv substitutes right-hand sides wv and wv + 4 (lane ln holds rows
+      // ln and ln + 64), so up to four SIMDs run the sequential steps at once
+      constexpr int NW = NR < 4 ? NR : 4, RPW = (NR + 3) / 4;
+      const int wv = t >> 6, ln = t & 63;
+      if (wv < NW) {
+        T x0[RPW], x1[RPW];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          x0[c] = t < pw ? xs[k][t * nrhs + c] : (T)0;
-          x1[c] = t + 64 < pw ? xs[k][(t + 64) * nrhs + c] : (T)0;
+        for (int q = 0; q < RPW; ++q) {
+          const int c = wv + 4 * q;
+          x0[q] = (c < NR && ln < pw) ? xs[k][ln * nrhs + c] : (T)0;
+          x1[q] = (c < NR && ln + 64 < pw) ? xs[k][(ln + 64) * nrhs + c] : (T)0;
         }
         // column j of the packed triangle holds rows j + 1 .. CU - 1 at
         // tri + cofs(j) + row; the entries of step j + 1 are read during step
@@ -230,40 +235,44 @@ __global__ __launch_bounds__(256) void k_fwd_chain(int s, int G, const int* supe
         const int n0 = min(64, pw - 1);
         T a0 = 0, a1 = 0;
         if (n0 > 0) {
-          a0 = tri[cofs(0) + max(t, 1)];
-          a1 = tri[cofs(0) + t + 64];
+          a0 = tri[cofs(0) + max(ln, 1)];
+          a1 = tri[cofs(0) + ln + 64];
         }
         for (int j = 0; j < n0; ++j) {  // x_j in lane j of x0
-          const T l0 = (t > j && t < pw) ? a0 : (T)0;
-          const T l1 = (t + 64 < pw) ? a1 : (T)0;
+          const T l0 = (ln > j && ln < pw) ? a0 : (T)0;
+          const T l1 = (ln + 64 < pw) ? a1 : (T)0;
           if (j + 1 < n0) {
-            a0 = tri[cofs(j + 1) + max(t, j + 2)];
-            a1 = tri[cofs(j + 1) + t + 64];
+            a0 = tri[cofs(j + 1) + max(ln, j + 2)];
+            a1 = tri[cofs(j + 1) + ln + 64];
           }
 #pragma unroll
-          for (int c = 0; c < NR; ++c) {
-            const T xj = lane_value(x0[c], j);
-            x0[c] -= l0 * xj;
-            x1[c] -= l1 * xj;
+          for (int q = 0; q < RPW; ++q) {
+            const T xj = lane_value(x0[q], j);
+            x0[q] -= l0 * xj;
+            x1[q] -= l1 * xj;
           }
         }
-        if (64 < pw - 1) a1 = tri[cofs(64) + max(t + 64, 65)];
+        if (64 < pw - 1) a1 = tri[cofs(64) + max(ln + 64, 65)];
         for (int j = 64; j < pw - 1; ++j) {  // x_j in lane j - 64 of x1
-          const T l1 = (t + 64 > j && t + 64 < pw) ? a1 : (T)0;
-          if (j + 1 < pw - 1) a1 = tri[cofs(j + 1) + max(t + 64, j + 2)];
+          const T l1 = (ln + 64 > j && ln + 64 < pw) ? a1 : (T)0;
+          if (j + 1 < pw - 1) a1 = tri[cofs(j + 1) + max(ln + 64, j + 2)];
 #pragma unroll
-          for (int c = 0; c < NR; ++c) x1[c] -= l1 * lane_value(x1[c], j - 64);
+          for (int q = 0; q < RPW; ++q) x1[q] -= l1 * lane_value(x1[q], j - 64);
         }
         // publish y_p first (the chain waits on it), then the copy for the D solve
 #pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          if (t < pw) XWords<T>::put(xb, ((int64_t)CU * p + t) * nrhs + c, x0[c], epoch);
-          if (t + 64 < pw) XWords<T>::put(xb, ((int64_t)CU * p + t + 64) * nrhs + c, x1[c], epoch);
+        for (int q = 0; q < RPW; ++q) {
+          const int c = wv + 4 * q;
+          if (c >= NR) continue;
+          if (ln < pw) XWords<T>::put(xb, ((int64_t)CU * p + ln) * nrhs + c, x0[q], epoch);
+          if (ln + 64 < pw) XWords<T>::put(xb, ((int64_t)CU * p + ln + 64) * nrhs + c, x1[q], epoch);
         }
 #pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          if (t < pw) Y[(int64_t)(c0 + CU * p + t) * nrhs + c] = x0[c];
-          if (t + 64 < pw) Y[(int64_t)(c0 + CU * p + t + 64) * nrhs + c] = x1[c];
+        for (int q = 0; q < RPW; ++q) {
+          const int c = wv + 4 * q;
+          if (c >= NR) continue;
+          if (ln < pw) Y[(int64_t)(c0 + CU * p + ln) * nrhs + c] = x0[q];
+          if (ln + 64 < pw) Y[(int64_t)(c0 + CU * p + ln + 64) * nrhs + c] = x1[q];
         }
       }
       __syncthreads();
@@ -412,53 +421,63 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
       staged = u;
     }
     __syncthreads();
-    if (t < 64) {
+    // wave wv substitutes right-hand sides wv and wv + 4 (lane ln holds rows
+    // ln and ln + 64)
+    constexpr int NW = NR < 4 ? NR : 4, RPW = (NR + 3) / 4;
+    const int wv = t >> 6, ln = t & 63;
+    if (wv < NW) {
       const int64_t base = (int64_t)(c0 + CU * u) * nrhs;
-      T x0[NR], x1[NR];
+      T x0[RPW], x1[RPW];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) {
-        x0[c] = t < pw ? Z[base + (int64_t)t * nrhs + c] - acc[t * nrhs + c] : (T)0;
-        x1[c] = t + 64 < pw ? Z[base + (int64_t)(t + 64) * nrhs + c] - acc[(t + 64) * nrhs + c] : (T)0;
+      for (int q = 0; q < RPW; ++q) {
+        const int c = wv + 4 * q;
+        x0[q] = (c < NR && ln < pw) ? Z[base + (int64_t)ln * nrhs + c] - acc[ln * nrhs + c] : (T)0;
+        x1[q] = (c < NR && ln + 64 < pw) ? Z[base + (int64_t)(ln + 64) * nrhs + c] - acc[(ln + 64) * nrhs + c]
+                                         : (T)0;
       }
       // row j of the packed triangle holds L(j, 0 .. j - 1) at tri + j (j - 1) / 2;
       // the entries of the next step are read during the current one
       // (addresses clamped into the row, masked afterwards)
       T a0 = 0, a1 = 0;
       if (pw - 1 >= 64) {
-        a0 = tri[(pw - 1) * (pw - 2) / 2 + t];
-        a1 = tri[(pw - 1) * (pw - 2) / 2 + min(t + 64, pw - 2)];
+        a0 = tri[(pw - 1) * (pw - 2) / 2 + ln];
+        a1 = tri[(pw - 1) * (pw - 2) / 2 + min(ln + 64, pw - 2)];
       }
       for (int j = pw - 1; j >= 64; --j) {  // x_j in lane j - 64 of x1
         const T l0 = a0;
-        const T l1 = (t + 64 < j) ? a1 : (T)0;
+        const T l1 = (ln + 64 < j) ? a1 : (T)0;
         if (j - 1 >= 64) {
-          a0 = tri[(j - 1) * (j - 2) / 2 + t];
-          a1 = tri[(j - 1) * (j - 2) / 2 + min(t + 64, j - 2)];
+          a0 = tri[(j - 1) * (j - 2) / 2 + ln];
+          a1 = tri[(j - 1) * (j - 2) / 2 + min(ln + 64, j - 2)];
         }
 #pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          const T xj = lane_value(x1[c], j - 64);
-          x0[c] -= l0 * xj;
-          x1[c] -= l1 * xj;
+        for (int q = 0; q < RPW; ++q) {
+          const T xj = lane_value(x1[q], j - 64);
+          x0[q] -= l0 * xj;
+          x1[q] -= l1 * xj;
         }
       }
       const int m0 = min(pw, 64) - 1;
-      if (m0 > 0) a0 = tri[m0 * (m0 - 1) / 2 + min(t, m0 - 1)];
+      if (m0 > 0) a0 = tri[m0 * (m0 - 1) / 2 + min(ln, m0 - 1)];
       for (int j = m0; j > 0; --j) {  // x_j in lane j of x0
-        const T l0 = (t < j) ? a0 : (T)0;
-        if (j - 1 > 0) a0 = tri[(j - 1) * (j - 2) / 2 + min(t, j - 2)];
+        const T l0 = (ln < j) ? a0 : (T)0;
+        if (j - 1 > 0) a0 = tri[(j - 1) * (j - 2) / 2 + min(ln, j - 2)];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) x0[c] -= l0 * lane_value(x0[c], j);
+        for (int q = 0; q < RPW; ++q) x0[q] -= l0 * lane_value(x0[q], j);
       }
 #pragma unroll
-      for (int c = 0; c < NR; ++c) {
-        if (t < pw) XWords<T>::put(xb, ((int64_t)CU * u + t) * nrhs + c, x0[c], epoch);
-        if (t + 64 < pw) XWords<T>::put(xb, ((int64_t)CU * u + t + 64) * nrhs + c, x1[c], epoch);
+      for (int q = 0; q < RPW; ++q) {
+        const int c = wv + 4 * q;
+        if (c >= NR) continue;
+        if (ln < pw) XWords<T>::put(xb, ((int64_t)CU * u + ln) * nrhs + c, x0[q], epoch);
+        if (ln + 64 < pw) XWords<T>::put(xb, ((int64_t)CU * u + ln + 64) * nrhs + c, x1[q], epoch);
       }
 #pragma unroll
-      for (int c = 0; c < NR; ++c) {
-        if (t < pw) Xf[base + (int64_t)t * nrhs + c] = x0[c];
-        if (t + 64 < pw) Xf[base + (int64_t)(t + 64) * nrhs + c] = x1[c];
+      for (int q = 0; q < RPW; ++q) {
+        const int c = wv + 4 * q;
+        if (c >= NR) continue;
+        if (ln < pw) Xf[base + (int64_t)ln * nrhs + c] = x0[q];
+        if (ln + 64 < pw) Xf[base + (int64_t)(ln + 64) * nrhs + c] = x1[q];
       }
     }
     __syncthreads();

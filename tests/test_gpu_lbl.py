@@ -214,6 +214,30 @@ def _launches(s, name):
     return cnt.value
 
 
+def test_kernel_timing_filter():
+    # lblg_set_kernel_timing_only brackets only the named families with events;
+    # lblg_set_kernel_timing(h, 0) ends it and the next full timing starts afresh
+    n, Ap, Ai, Ax, B = P.kkt_config(12, 300, 6, seed=5, nrhs=2)
+    print(f"case timing filter n={n}", flush=True)
+    s = engine(n, Ap, Ai, Ax)
+    assert s.lib.lblg_set_kernel_timing_only(s.h, b"spmv,fwd_tri") == 0
+    s.target_bwd_err = 1e-10
+    s.max_refine_rounds = 10
+    s.numerical_factorization()
+    s.solve_only(B)
+    assert _launches(s, "spmv") >= 1  # the residual of every backward error
+    for name in ("bk_small", "bk_panel", "bk_panel_group", "schur", "schur_big", "bwd_tri", "bwd_gemv"):
+        assert _launches(s, name) == 0, name
+    assert s.lib.lblg_set_kernel_timing_only(s.h, b"") != 0
+    assert s.lib.lblg_set_kernel_timing_only(s.h, b",") != 0
+    assert s.lib.lblg_set_kernel_timing(s.h, 0) == 0
+    assert s.lib.lblg_set_kernel_timing(s.h, 1) == 0
+    s.numerical_factorization()
+    assert _launches(s, "spmv") == 0
+    assert _launches(s, "bk_small") + _launches(s, "bk_panel") + _launches(s, "bk_panel_group") > 0
+    s.close()
+
+
 @pytest.mark.parametrize("chain_w,group_w,grid,m,prec", [(129, 2048, 12, 300, 0), (129, 96, 14, 500, 0),
                                                          (129, 2048, 14, 500, 1), (100000, 2048, 14, 500, 0)])
 def test_chain_sweeps(oracle, chain_w, group_w, grid, m, prec, monkeypatch):
