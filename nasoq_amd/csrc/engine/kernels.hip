@@ -1035,6 +1035,7 @@ __device__ inline int row_label(const int* rows, int64_t pis, int c0, int w, int
 // blocks below a piece: GROWS rows per workgroup, the four waves split the
 // piece's columns, partial sums meet in LDS.
 #define GROWS SOLVE_ROWS
+#define TSTEP 16  // substitution steps per batch of LDS reads
 template <typename T>
 __device__ inline T bcast(T v, int lane) { return __shfl(v, lane, 64); }
 
@@ -1103,36 +1104,50 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
     x1[q] = (c < nrhs && ln + 64 < pw) ? X[base + (int64_t)(ln + 64) * nrhs + c] : (T)0;
   }
   const bool two = wv + 4 < nrhs;
-  // column j holds rows j + 1 .. pw - 1; the entries of step j + 1 are read
-  // during step j (addresses clamped into the column, masked afterwards)
+  // column j holds rows j + 1 .. pw - 1.  The steps go in chunks of TSTEP:
+  // a lane reads its entries of the chunk's columns together (addresses
+  // clamped into the columns, masked afterwards), then substitutes from
+  // registers, so an LDS round trip is paid per chunk rather than per step.
   const int n0 = min(64, pw - 1);
-  T a0 = 0, a1 = 0;
-  if (n0 > 0) {
-    a0 = Lt[min(max(ln, 1), pw - 1)];
-    a1 = Lt[min(ln + 64, pw - 1)];
-  }
-  for (int j = 0; j < n0; ++j) {  // x_j in lane j of x0
-    const T l0 = (ln > j && ln < pw) ? a0 : (T)0;
-    const T l1 = (ln + 64 < pw) ? a1 : (T)0;
-    if (j + 1 < n0) {
-      a0 = Lt[(j + 1) * lp + min(max(ln, j + 2), pw - 1)];
-      a1 = Lt[(j + 1) * lp + min(ln + 64, pw - 1)];
+  for (int jc = 0; jc < n0; jc += TSTEP) {  // x_j in lane j of x0
+    T e0[TSTEP], e1[TSTEP];
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = min(jc + k, n0 - 1);
+      e0[k] = Lt[j * lp + min(max(ln, j + 1), pw - 1)];
+      e1[k] = Lt[j * lp + min(ln + 64, pw - 1)];
     }
-    const T xa = readlane_t(x0[0], j);
-    x0[0] -= l0 * xa;
-    x1[0] -= l1 * xa;
-    if (two) {
-      const T xb = readlane_t(x0[1], j);
-      x0[1] -= l0 * xb;
-      x1[1] -= l1 * xb;
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = jc + k;
+      if (j >= n0) break;
+      const T l0 = (ln > j && ln < pw) ? e0[k] : (T)0;
+      const T l1 = (ln + 64 < pw) ? e1[k] : (T)0;
+      const T xa = readlane_t(x0[0], j);
+      x0[0] -= l0 * xa;
+      x1[0] -= l1 * xa;
+      if (two) {
+        const T xb = readlane_t(x0[1], j);
+        x0[1] -= l0 * xb;
+        x1[1] -= l1 * xb;
+      }
     }
   }
-  if (64 < pw - 1) a1 = Lt[64 * lp + min(max(ln + 64, 65), pw - 1)];
-  for (int j = 64; j < pw - 1; ++j) {  // x_j in lane j - 64 of x1
-    const T l1 = (ln + 64 > j && ln + 64 < pw) ? a1 : (T)0;
-    if (j + 1 < pw - 1) a1 = Lt[(j + 1) * lp + min(max(ln + 64, j + 2), pw - 1)];
-    x1[0] -= l1 * readlane_t(x1[0], j - 64);
-    if (two) x1[1] -= l1 * readlane_t(x1[1], j - 64);
+  for (int jc = 64; jc < pw - 1; jc += TSTEP) {  // x_j in lane j - 64 of x1
+    T e1[TSTEP];
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = min(jc + k, pw - 2);
+      e1[k] = Lt[j * lp + min(max(ln + 64, j + 1), pw - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = jc + k;
+      if (j >= pw - 1) break;
+      const T l1 = (ln + 64 > j && ln + 64 < pw) ? e1[k] : (T)0;
+      x1[0] -= l1 * readlane_t(x1[0], j - 64);
+      if (two) x1[1] -= l1 * readlane_t(x1[1], j - 64);
+    }
   }
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
@@ -1314,36 +1329,48 @@ __global__ __launch_bounds__(256) void k_bwd_tri(const int* pieces, const int* p
   }
   const bool two = wv + 4 < nrhs;
   // rows i < j: x_i -= L(j, i) x_j, from the last column back; row j holds
-  // L(j, 0 .. j - 1) and the entries of the next step are read during the
-  // current one (addresses clamped into the row, masked afterwards)
-  T a0 = 0, a1 = 0;
-  if (pw - 1 >= 64) {
-    a0 = Lt[(pw - 1) * lp + ln];
-    a1 = Lt[(pw - 1) * lp + min(ln + 64, pw - 2)];
-  }
-  for (int j = pw - 1; j >= 64; --j) {  // x_j in lane j - 64 of x1
-    const T l0 = a0;
-    const T l1 = (ln + 64 < j) ? a1 : (T)0;
-    if (j - 1 >= 64) {
-      a0 = Lt[(j - 1) * lp + ln];
-      a1 = Lt[(j - 1) * lp + min(ln + 64, j - 2)];
+  // L(j, 0 .. j - 1).  The steps go in chunks of TSTEP rows read together
+  // (addresses clamped into the rows, masked afterwards), then substituted
+  // from registers.
+  for (int jc = pw - 1; jc >= 64; jc -= TSTEP) {  // x_j in lane j - 64 of x1
+    T e0[TSTEP], e1[TSTEP];
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = max(jc - k, 64);
+      e0[k] = Lt[j * lp + ln];
+      e1[k] = Lt[j * lp + min(ln + 64, j - 1)];
     }
-    const T xa = readlane_t(x1[0], j - 64);
-    x0[0] -= l0 * xa;
-    x1[0] -= l1 * xa;
-    if (two) {
-      const T xb = readlane_t(x1[1], j - 64);
-      x0[1] -= l0 * xb;
-      x1[1] -= l1 * xb;
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = jc - k;
+      if (j < 64) break;
+      const T l0 = e0[k];
+      const T l1 = (ln + 64 < j) ? e1[k] : (T)0;
+      const T xa = readlane_t(x1[0], j - 64);
+      x0[0] -= l0 * xa;
+      x1[0] -= l1 * xa;
+      if (two) {
+        const T xb = readlane_t(x1[1], j - 64);
+        x0[1] -= l0 * xb;
+        x1[1] -= l1 * xb;
+      }
     }
   }
-  const int m0 = min(pw, 64) - 1;
-  if (m0 > 0) a0 = Lt[m0 * lp + min(ln, m0 - 1)];
-  for (int j = m0; j > 0; --j) {  // x_j in lane j of x0
-    const T l0 = (ln < j) ? a0 : (T)0;
-    if (j - 1 > 0) a0 = Lt[(j - 1) * lp + min(ln, j - 2)];
-    x0[0] -= l0 * readlane_t(x0[0], j);
-    if (two) x0[1] -= l0 * readlane_t(x0[1], j);
+  for (int jc = min(pw, 64) - 1; jc > 0; jc -= TSTEP) {  // x_j in lane j of x0
+    T e0[TSTEP];
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = max(jc - k, 1);
+      e0[k] = Lt[j * lp + min(ln, j - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < TSTEP; ++k) {
+      const int j = jc - k;
+      if (j <= 0) break;
+      const T l0 = (ln < j) ? e0[k] : (T)0;
+      x0[0] -= l0 * readlane_t(x0[0], j);
+      if (two) x0[1] -= l0 * readlane_t(x0[1], j);
+    }
   }
 #pragma unroll
   for (int q = 0; q < 2; ++q) {

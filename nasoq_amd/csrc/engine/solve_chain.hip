@@ -39,6 +39,7 @@ namespace lblg {
 #define CUPW 4                // units per workgroup (at most)
 #define CTRI (CU * (CU + 1) / 2)
 #define CSPIN (1L << 24)
+#define CSTEP 16              // substitution steps per batch of LDS reads
 
 // exchange words: value idx of a published block occupies WPV words
 template <typename T>
@@ -229,35 +230,49 @@ __global__ __launch_bounds__(256) void k_fwd_chain(int s, int G, const int* supe
           x1[q] = (c < NR && ln + 64 < pw) ? xs[k][(ln + 64) * nrhs + c] : (T)0;
         }
         // column j of the packed triangle holds rows j + 1 .. CU - 1 at
-        // tri + cofs(j) + row; the entries of step j + 1 are read during step
-        // j (addresses clamped into the column, masked afterwards)
+        // tri + cofs(j) + row.  The steps go in chunks of CSTEP: a lane reads
+        // its entries of the chunk's columns together (addresses clamped into
+        // the columns, masked afterwards), then substitutes from registers,
+        // so an LDS round trip is paid per chunk rather than per step.
         auto cofs = [](int j) { return j * CU - j * (j + 1) / 2 - j - 1; };
         const int n0 = min(64, pw - 1);
-        T a0 = 0, a1 = 0;
-        if (n0 > 0) {
-          a0 = tri[cofs(0) + max(ln, 1)];
-          a1 = tri[cofs(0) + ln + 64];
-        }
-        for (int j = 0; j < n0; ++j) {  // x_j in lane j of x0
-          const T l0 = (ln > j && ln < pw) ? a0 : (T)0;
-          const T l1 = (ln + 64 < pw) ? a1 : (T)0;
-          if (j + 1 < n0) {
-            a0 = tri[cofs(j + 1) + max(ln, j + 2)];
-            a1 = tri[cofs(j + 1) + ln + 64];
+        for (int jc = 0; jc < n0; jc += CSTEP) {  // x_j in lane j of x0
+          T e0[CSTEP], e1[CSTEP];
+#pragma unroll
+          for (int k = 0; k < CSTEP; ++k) {
+            const int j = min(jc + k, n0 - 1);
+            e0[k] = tri[cofs(j) + max(ln, j + 1)];
+            e1[k] = tri[cofs(j) + ln + 64];
           }
 #pragma unroll
-          for (int q = 0; q < RPW; ++q) {
-            const T xj = lane_value(x0[q], j);
-            x0[q] -= l0 * xj;
-            x1[q] -= l1 * xj;
+          for (int k = 0; k < CSTEP; ++k) {
+            const int j = jc + k;
+            if (j >= n0) break;
+            const T l0 = (ln > j && ln < pw) ? e0[k] : (T)0;
+            const T l1 = (ln + 64 < pw) ? e1[k] : (T)0;
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+              const T xj = lane_value(x0[q], j);
+              x0[q] -= l0 * xj;
+              x1[q] -= l1 * xj;
+            }
           }
         }
-        if (64 < pw - 1) a1 = tri[cofs(64) + max(ln + 64, 65)];
-        for (int j = 64; j < pw - 1; ++j) {  // x_j in lane j - 64 of x1
-          const T l1 = (ln + 64 > j && ln + 64 < pw) ? a1 : (T)0;
-          if (j + 1 < pw - 1) a1 = tri[cofs(j + 1) + max(ln + 64, j + 2)];
+        for (int jc = 64; jc < pw - 1; jc += CSTEP) {  // x_j in lane j - 64 of x1
+          T e1[CSTEP];
 #pragma unroll
-          for (int q = 0; q < RPW; ++q) x1[q] -= l1 * lane_value(x1[q], j - 64);
+          for (int k = 0; k < CSTEP; ++k) {
+            const int j = min(jc + k, pw - 2);
+            e1[k] = tri[cofs(j) + max(ln + 64, j + 1)];
+          }
+#pragma unroll
+          for (int k = 0; k < CSTEP; ++k) {
+            const int j = jc + k;
+            if (j >= pw - 1) break;
+            const T l1 = (ln + 64 > j && ln + 64 < pw) ? e1[k] : (T)0;
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) x1[q] -= l1 * lane_value(x1[q], j - 64);
+          }
         }
         // publish y_p first (the chain waits on it), then the copy for the D solve
 #pragma unroll
@@ -436,34 +451,45 @@ __global__ __launch_bounds__(256) void k_bwd_chain(int s, int G, const int* supe
                                          : (T)0;
       }
       // row j of the packed triangle holds L(j, 0 .. j - 1) at tri + j (j - 1) / 2;
-      // the entries of the next step are read during the current one
-      // (addresses clamped into the row, masked afterwards)
-      T a0 = 0, a1 = 0;
-      if (pw - 1 >= 64) {
-        a0 = tri[(pw - 1) * (pw - 2) / 2 + ln];
-        a1 = tri[(pw - 1) * (pw - 2) / 2 + min(ln + 64, pw - 2)];
-      }
-      for (int j = pw - 1; j >= 64; --j) {  // x_j in lane j - 64 of x1
-        const T l0 = a0;
-        const T l1 = (ln + 64 < j) ? a1 : (T)0;
-        if (j - 1 >= 64) {
-          a0 = tri[(j - 1) * (j - 2) / 2 + ln];
-          a1 = tri[(j - 1) * (j - 2) / 2 + min(ln + 64, j - 2)];
+      // the steps go in chunks of CSTEP rows read together (addresses clamped
+      // into the rows, masked afterwards), then substituted from registers
+      for (int jc = pw - 1; jc >= 64; jc -= CSTEP) {  // x_j in lane j - 64 of x1
+        T e0[CSTEP], e1[CSTEP];
+#pragma unroll
+        for (int k = 0; k < CSTEP; ++k) {
+          const int j = max(jc - k, 64);
+          e0[k] = tri[j * (j - 1) / 2 + ln];
+          e1[k] = tri[j * (j - 1) / 2 + min(ln + 64, j - 1)];
         }
 #pragma unroll
-        for (int q = 0; q < RPW; ++q) {
-          const T xj = lane_value(x1[q], j - 64);
-          x0[q] -= l0 * xj;
-          x1[q] -= l1 * xj;
+        for (int k = 0; k < CSTEP; ++k) {
+          const int j = jc - k;
+          if (j < 64) break;
+          const T l0 = e0[k];
+          const T l1 = (ln + 64 < j) ? e1[k] : (T)0;
+#pragma unroll
+          for (int q = 0; q < RPW; ++q) {
+            const T xj = lane_value(x1[q], j - 64);
+            x0[q] -= l0 * xj;
+            x1[q] -= l1 * xj;
+          }
         }
       }
-      const int m0 = min(pw, 64) - 1;
-      if (m0 > 0) a0 = tri[m0 * (m0 - 1) / 2 + min(ln, m0 - 1)];
-      for (int j = m0; j > 0; --j) {  // x_j in lane j of x0
-        const T l0 = (ln < j) ? a0 : (T)0;
-        if (j - 1 > 0) a0 = tri[(j - 1) * (j - 2) / 2 + min(ln, j - 2)];
+      for (int jc = min(pw, 64) - 1; jc > 0; jc -= CSTEP) {  // x_j in lane j of x0
+        T e0[CSTEP];
 #pragma unroll
-        for (int q = 0; q < RPW; ++q) x0[q] -= l0 * lane_value(x0[q], j);
+        for (int k = 0; k < CSTEP; ++k) {
+          const int j = max(jc - k, 1);
+          e0[k] = tri[j * (j - 1) / 2 + min(ln, j - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < CSTEP; ++k) {
+          const int j = jc - k;
+          if (j <= 0) break;
+          const T l0 = (ln < j) ? e0[k] : (T)0;
+#pragma unroll
+          for (int q = 0; q < RPW; ++q) x0[q] -= l0 * lane_value(x0[q], j);
+        }
       }
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
