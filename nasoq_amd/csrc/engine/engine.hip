@@ -585,7 +585,8 @@ static void factor_device_t(Handle& h) {
   // threshold sqrt(eps_f32) max_ij |K_ij| (2^-12; pivots of reg_diag = 1e-9
   // would give growth ~1e9, beyond FP32); the refinement still runs on the
   // reference's K0, so only the preconditioner changes.
-  const double reg_factor = D.prec == 1 ? std::max(reg, std::ldexp(h.max_abs_entry, -12)) : reg;
+  const int pexp = env_int("LBLG_F32_PIVOT_EXP", -12);
+  const double reg_factor = D.prec == 1 ? std::max(reg, std::ldexp(h.max_abs_entry, pexp)) : reg;
   SnArgs a = snargs(D);
   a.rows = D.rows0;  // pre-pivot labels during the factorization
   // the whole launch sequence depends only on the symbolic structure, the
