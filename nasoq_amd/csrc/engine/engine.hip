@@ -808,8 +808,8 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
     }
     if (L.npieces > 0) {
       timed(D, "bwd_gemv", [&] {
-        launch_bwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)O, acc,
-                        nrhs, st);
+        launch_bwd_gemv(L.d_piece, L.npieces, L.maxpw, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL,
+                        (const T*)O, acc, nrhs, st);
       });
       timed(D, "bwd_tri", [&] {
         launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)Z, (const T*)acc,

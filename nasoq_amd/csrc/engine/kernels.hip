@@ -1231,14 +1231,17 @@ __global__ void k_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs) {
 }
 // backward gather: acc(p) += L(q, p)^T X(label q) over the rows below a piece.
 // The GROWS x pw block is staged in LDS (coalesced along the rows), then
-// each thread sums a column over half of the rows.
+// each thread sums a column over half of the rows.  The block's LDS is sized
+// by the launch for the level's widest piece, so levels of narrow pieces
+// keep many workgroups per CU.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* blk_prefix, int np,
                                                   const int* piece_sn, const int* piece_c0, const int* super,
                                                   const int64_t* pi, const int64_t* px, const int* rows,
                                                   const T* valL, const T* Xf, T* acc,
                                                   int nrhs) {
-  __shared__ T Lb[SOLVE_W * (GROWS + 1)];  // Lb[j * (GROWS + 1) + qq]
+  extern __shared__ double smem_d[];
+  T* Lb = reinterpret_cast<T*>(smem_d);  // Lb[j * (GROWS + 1) + qq], j < the level's widest piece
   __shared__ T xq[GROWS * MAXRHS];
   __shared__ T part[SOLVE_W * MAXRHS];
   int lo = 0, hi = np - 1;
@@ -1281,7 +1284,7 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
   for (int c = 0; c < MAXRHS; ++c) a[c] = 0.0;
   if (j < pw) {
     const int q0 = half * (GROWS / 2);
-    for (int qq = q0; qq < q0 + GROWS / 2; ++qq) {
+    for (int qq = q0; qq < min(q0 + GROWS / 2, nq); ++qq) {  // the block's rows only
       const T l = Lb[j * (GROWS + 1) + qq];
 #pragma unroll
       for (int c = 0; c < MAXRHS; ++c)
@@ -1414,12 +1417,21 @@ void launch_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs, hipStream_t
   HIP_LAUNCH_CHECK("dsolve");
 }
 template <typename T>
-void launch_bwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix, int nblk, const int* piece_sn,
                      const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
                      int nrhs, hipStream_t st) {
   if (np <= 0 || nblk <= 0) return;
-  k_bwd_gemv<T><<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
-                                   valL, Xf, acc, nrhs);
+  if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("bwd_gemv: piece width out of range");
+  static bool attr = false;
+  if (!attr) {  // a level of 128-column pieces needs more than the default dynamic LDS
+    if (hipFuncSetAttribute((const void*)k_bwd_gemv<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(T) * SOLVE_W * (GROWS + 1))) != hipSuccess)
+      throw std::runtime_error("bwd_gemv: cannot raise the dynamic LDS limit");
+    attr = true;
+  }
+  k_bwd_gemv<T><<<nblk, 256, sizeof(T) * maxpw * (GROWS + 1), st>>>(pieces, blk_prefix, np, piece_sn, piece_c0,
+                                                                    a.super, a.pi, a.px, a.rows, valL, Xf, acc,
+                                                                    nrhs);
   HIP_LAUNCH_CHECK("bwd_gemv");
 }
 template <typename T>
@@ -1882,7 +1894,8 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
   template void launch_fwd_gemv<T>(const int*, int, const int*, int, const int*, const int*, const SnArgs&,    \
                                    const T*, const T*, T*, int, hipStream_t);                                  \
   template void launch_dsolve<T>(int, const T*, const T*, T*, int, hipStream_t);                               \
-  template void launch_bwd_gemv<T>(const int*, int, const int*, int, const int*, const int*, const SnArgs&,    \
+  template void launch_bwd_gemv<T>(const int*, int, int, const int*, int, const int*, const int*,              \
+                                   const SnArgs&,                                                              \
                                    const T*, const T*, T*, int, hipStream_t);                                  \
   template void launch_bwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \
                                   const T*, const T*, T*, int, hipStream_t);
