@@ -33,6 +33,7 @@ struct FactorLevel {
   std::vector<int> large;          // w > GROUP_W: a cooperative group each
   SnList offdiag;                  // supernodes with rows below the diagonal block
   int offdiag_blocks = 0;          // total 64-row blocks of those
+  int offdiag_wmax = 0;            // widest of those
   int* d_offdiag_blk = nullptr;    // prefix of 64-row blocks per offdiag supernode
   SnList schur;                    // supernodes with an update (m > 0)
   int64_t schur_tiles = 0;

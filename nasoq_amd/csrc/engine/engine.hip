@@ -351,6 +351,8 @@ static void setup(Handle& h) {
     F.small.d = dupload(D, small);
     F.offdiag.count = (int)off.size();
     F.offdiag.d = dupload(D, off);
+    F.offdiag_wmax = 0;
+    for (int s : off) F.offdiag_wmax = std::max(F.offdiag_wmax, S.super[s + 1] - S.super[s]);
     F.offdiag_blocks = blk;
     F.d_offdiag_blk = dupload(D, offblk);
     F.schur.count = (int)sch.size();
@@ -669,8 +671,8 @@ static void factor_device_t(Handle& h) {
         }
       }
       timed(D, "offdiag", [&] {
-        launch_offdiag(F.offdiag.d, F.offdiag.count, F.d_offdiag_blk, F.offdiag_blocks, a, valL, dval,
-                       D.perm_piv, st);
+        launch_offdiag(F.offdiag.d, F.offdiag.count, F.offdiag_wmax, F.d_offdiag_blk, F.offdiag_blocks, a, valL,
+                       dval, D.perm_piv, st);
       });
       timed(D, "schur", [&] {
         launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, valL, dval, st);

@@ -692,16 +692,25 @@ void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const
 // gather != 0: columns are permuted here (narrow supernodes); otherwise the
 // panel kernel has already permuted them.
 // --------------------------------------------------------------------------
+// The LDS is sized by the launch for the list's widest supernode wmax: X
+// holds 64 rows (pitch 65) x min(64, wmax) columns; the second region holds
+// the GEMM's operand chunks (only when wmax > 64), then the column block's
+// triangle.  Lists of narrow supernodes thus keep many workgroups per CU.
+__host__ __device__ inline int offdiag_lds_elems(int wmax) {
+  const int xw = wmax < 64 ? wmax : 64;
+  const int sh = wmax > 64 ? (2 * KC * LDT > 64 * 65 ? 2 * KC * LDT : 64 * 65) : 65 * xw;
+  return 65 * xw + sh;
+}
 template <typename T>
 __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk_prefix, int count,
                                                  const int* super, const int64_t* pi, const int64_t* px,
                                                  T* valL, const T* dval, const int* perm_piv,
-                                                 int n) {
-  __shared__ T X[64 * 65];  // X[row + col*65] for a 64 x 64 column block
+                                                 int n, int wmax) {
+  extern __shared__ double smem_d[];
+  T* const X = reinterpret_cast<T*>(smem_d);  // X[row + col*65], col < min(64, wmax)
   // the GEMM's operand chunks, then the column block's triangle L(cb rows, cb
-  // cols) for the substitution: never live together, so one region (two
-  // workgroups per CU)
-  __shared__ T Sh[(2 * KC * LDT > 64 * 65) ? 2 * KC * LDT : 64 * 65];
+  // cols) for the substitution: never live together, so one region
+  T* const Sh = X + 65 * (wmax < 64 ? wmax : 64);
   T* const As = Sh;
   T* const Bs = Sh + KC * LDT;
   T* const Lt = Sh;
@@ -722,15 +731,24 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   const bool gather = w <= SMALL_W;
   for (int cb0 = 0; cb0 < w; cb0 += 64) {
     const int cw = min(64, w - cb0);
-    // load B block (with column gather for narrow supernodes)
-    for (int idx = t; idx < 64 * cw; idx += 256) {
-      const int i = idx % 64, j = idx / 64;
-      T v = 0;
-      if (i < nr) {
-        const int srcc = gather ? (perm_piv[c0 + cb0 + j] - c0) : (cb0 + j);
-        v = P[q0 + i + (int64_t)srcc * r];
+    // load B block (with column gather for narrow supernodes); a thread's
+    // loads of a batch issue before its LDS stores
+    for (int idx0 = 0; idx0 < 64 * cw; idx0 += 256 * 8) {
+      T v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int idx = idx0 + t + 256 * k, i = idx & 63, j = idx >> 6;
+        v[k] = 0;
+        if (idx < 64 * cw && i < nr) {
+          const int srcc = gather ? (perm_piv[c0 + cb0 + j] - c0) : (cb0 + j);
+          v[k] = P[q0 + i + (int64_t)srcc * r];
+        }
       }
-      X[i + j * 65] = v;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int idx = idx0 + t + 256 * k, i = idx & 63, j = idx >> 6;
+        if (idx < 64 * cw) X[i + j * 65] = v[k];
+      }
     }
     __syncthreads();
     // X(:,cb) -= X(:, <cb) * L(cb, <cb)^T with MFMA (only for wide supernodes)
@@ -759,9 +777,18 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
       });
     }
     // the column block's unit-lower triangle (after the GEMM: shares its buffers)
-    for (int idx = t; idx < cw * cw; idx += 256) {
-      const int i = idx % cw, j = idx / cw;
-      Lt[i + j * 65] = (i > j) ? P[(cb0 + i) + (int64_t)(cb0 + j) * r] : 0.0;
+    for (int idx0 = 0; idx0 < cw * cw; idx0 += 256 * 8) {
+      T v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int idx = idx0 + t + 256 * k, i = idx % cw, j = idx / cw;
+        v[k] = (idx < cw * cw && i > j) ? P[(cb0 + i) + (int64_t)(cb0 + j) * r] : (T)0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int idx = idx0 + t + 256 * k, i = idx % cw, j = idx / cw;
+        if (idx < cw * cw) Lt[i + j * 65] = v[k];
+      }
     }
     __syncthreads();
     // substitution inside the column block: four adjacent lanes per row split
@@ -826,11 +853,19 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   }
 }
 template <typename T>
-void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
+void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, int nblocks, const SnArgs& a,
                     T* valL, const T* dval, const int* perm_piv, hipStream_t st) {
   if (count <= 0 || nblocks <= 0) return;
-  k_offdiag<T><<<nblocks, 256, 0, st>>>(list, blk_prefix, count, a.super, a.pi, a.px, valL, dval, perm_piv,
-                                     a.n);
+  if (wmax < 1) throw std::runtime_error("offdiag: empty supernode width");
+  static bool attr = false;
+  if (!attr) {  // lists with supernodes wider than 64 need more than the default dynamic LDS
+    if (hipFuncSetAttribute((const void*)k_offdiag<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(T) * offdiag_lds_elems(65))) != hipSuccess)
+      throw std::runtime_error("offdiag: cannot raise the dynamic LDS limit");
+    attr = true;
+  }
+  k_offdiag<T><<<nblocks, 256, sizeof(T) * offdiag_lds_elems(wmax), st>>>(list, blk_prefix, count, a.super, a.pi,
+                                                                          a.px, valL, dval, perm_piv, a.n, wmax);
   HIP_LAUNCH_CHECK("offdiag");
 }
 
@@ -1885,8 +1920,8 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
                                    int*, int*, int*, bool, hipStream_t);                                       \
   template void launch_trailing<T>(const int*, int, int, int, const SnArgs&, T*, const T*, const int64_t*,      \
                                    const int*, const int*, hipStream_t);                                       \
-  template void launch_offdiag<T>(const int*, int, const int*, int, const SnArgs&, T*, const T*, const int*,   \
-                                  hipStream_t);                                                                \
+  template void launch_offdiag<T>(const int*, int, int, const int*, int, const SnArgs&, T*, const T*,        \
+                                  const int*, hipStream_t);                                                                \
   template void launch_schur<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,         \
                                 hipStream_t);                                                                  \
   template void launch_fwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \

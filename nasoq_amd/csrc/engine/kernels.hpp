@@ -66,7 +66,7 @@ template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
                          T* W, const int64_t* woff, const int* kpos, const int* kstart, bool direct, hipStream_t st);
 template <typename T>
-void launch_offdiag(const int* list, int count, const int* blk_prefix, int nblocks, const SnArgs& a,
+void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, int nblocks, const SnArgs& a,
                     T* valL, const T* dval, const int* perm_piv, hipStream_t st);
 template <typename T>
 void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
