@@ -34,6 +34,7 @@ struct FactorLevel {
   SnList offdiag;                  // supernodes with rows below the diagonal block
   int offdiag_blocks = 0;          // total 64-row blocks of those
   int offdiag_wmax = 0;            // widest of those
+  int small_wmax = 0;              // widest supernode of the small list
   int* d_offdiag_blk = nullptr;    // prefix of 64-row blocks per offdiag supernode
   SnList schur;                    // supernodes with an update (m > 0)
   int64_t schur_tiles = 0;

@@ -348,6 +348,8 @@ static void setup(Handle& h) {
     }
     wmax_total = std::max(wmax_total, wsum);
     F.small.count = (int)small.size();
+    F.small_wmax = 0;
+    for (int s : small) F.small_wmax = std::max(F.small_wmax, S.super[s + 1] - S.super[s]);
     F.small.d = dupload(D, small);
     F.offdiag.count = (int)off.size();
     F.offdiag.d = dupload(D, off);
@@ -604,7 +606,7 @@ static void factor_device_t(Handle& h) {
     launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg_factor, valL, st);
     for (size_t l = 0; l < D.flevels.size(); ++l) {
       FactorLevel& F = D.flevels[l];
-      timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, a, valL, dval, D.perm_piv, D.status, st); });
+      timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, F.small_wmax, a, valL, dval, D.perm_piv, D.status, st); });
       // medium supernodes: one workgroup each, panel by panel
       if (!F.medium.empty()) {
         int wmax = 0;

@@ -134,7 +134,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* super, const int64_t* pi,
                                                   const int64_t* px, T* valL, T* dval,
                                                   int* perm_piv, int n, int* status) {
-  __shared__ T a[64 * LDA_S];
+  // the block, a[i + j * LDA_S] for j < the list's widest supernode (the
+  // launch sizes the LDS, so lists of narrow supernodes keep many
+  // workgroups per CU)
+  extern __shared__ double smem_d[];
+  T* const a = reinterpret_cast<T*>(smem_d);
   __shared__ T wk[64], wk1[64];
   __shared__ int lperm[64], kind[64];
   __shared__ int sh_kp, sh_kstep, sh_zero;
@@ -143,9 +147,18 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
   const int64_t r = pi[s + 1] - pi[s];
   T* P = valL + px[s];
   const int t = threadIdx.x;
-  for (int idx = t; idx < w * w; idx += 256) {
-    const int i = idx % w, j = idx / w;
-    a[i + j * LDA_S] = (i >= j) ? P[i + j * r] : 0.0;
+  for (int idx0 = 0; idx0 < w * w; idx0 += 256 * 8) {  // a thread's loads of a batch issue before its stores
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = idx0 + t + 256 * q, i = idx % w, j = idx / w;
+      v[q] = (idx < w * w && i >= j) ? P[i + j * r] : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = idx0 + t + 256 * q, i = idx % w, j = idx / w;
+      if (idx < w * w) a[i + j * LDA_S] = v[q];
+    }
   }
   if (t < 64) {
     lperm[t] = t;
@@ -307,10 +320,12 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
   }
 }
 template <typename T>
-void launch_bk_small(const int* list, int count, const SnArgs& a, T* valL, T* dval,
+void launch_bk_small(const int* list, int count, int wmax, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, int* status, hipStream_t st) {
   if (count <= 0) return;
-  k_bk_small<T><<<count, 256, 0, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n, status);
+  if (wmax < 1 || wmax > 64) throw std::runtime_error("bk_small: supernode width out of range");
+  k_bk_small<T><<<count, 256, sizeof(T) * LDA_S * wmax, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n,
+                                                               status);
   HIP_LAUNCH_CHECK("bk_small");
 }
 
@@ -1915,7 +1930,7 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
   template void launch_fill<T>(T*, int64_t, double, hipStream_t);                                              \
   template void launch_assemble<T>(const int64_t*, const double*, const unsigned char*, int, double, T*,       \
                                    hipStream_t);                                                               \
-  template void launch_bk_small<T>(const int*, int, const SnArgs&, T*, T*, int*, int*, hipStream_t);           \
+  template void launch_bk_small<T>(const int*, int, int, const SnArgs&, T*, T*, int*, int*, hipStream_t);      \
   template void launch_bk_panel<T>(const int*, int, int, const SnArgs&, T*, T*, int*, T*, const int64_t*, int*, \
                                    int*, int*, int*, bool, hipStream_t);                                       \
   template void launch_trailing<T>(const int*, int, int, int, const SnArgs&, T*, const T*, const int64_t*,      \

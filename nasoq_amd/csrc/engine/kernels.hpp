@@ -36,7 +36,7 @@ template <typename T>
 void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
                      T* valL, hipStream_t st);
 template <typename T>
-void launch_bk_small(const int* list, int count, const SnArgs& a, T* valL, T* dval,
+void launch_bk_small(const int* list, int count, int wmax, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, int* status, hipStream_t st);
 template <typename T>
 void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL, T* dval,
