@@ -57,8 +57,10 @@ struct SolveLevel {
   std::vector<int> chains;  // wide supernodes swept by one cooperative launch each
   int npieces = 0;
   int maxpw = 1;         // widest piece of the level
+  int nnarrow = 0;       // pieces of at most TRI_NARROW columns: the first nnarrow of d_piece
+  int maxpw_narrow = 1;  // widest of those
   int nblocks = 0;       // total row blocks (SOLVE_ROWS rows) of all pieces
-  int* d_piece = nullptr;   // piece ids
+  int* d_piece = nullptr;   // piece ids, narrowest first
   int* d_blk = nullptr;     // prefix of row blocks per piece
 };
 

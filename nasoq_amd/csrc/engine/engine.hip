@@ -538,6 +538,10 @@ static void setup(Handle& h) {
   D.chain_epoch = 0;
   for (int l = 0; l < nplev; ++l) {
     SolveLevel& L = D.slevels[l];
+    // narrowest pieces first: the triangle kernels launch the narrow ones
+    // apart, with LDS sized for them (many workgroups per CU)
+    auto pwidth = [&](int p) { return std::min(SOLVE_W, S.super[psn[p] + 1] - S.super[psn[p]] - pc0[p]); };
+    std::stable_sort(pbylev[l].begin(), pbylev[l].end(), [&](int x, int y) { return pwidth(x) < pwidth(y); });
     std::vector<int> pre;
     int nb = 0;
     for (int p : pbylev[l]) {
@@ -548,6 +552,10 @@ static void setup(Handle& h) {
       pre.push_back(nb);
       nb += (int)((r - pc0[p] - pw + SOLVE_ROWS - 1) / SOLVE_ROWS);
       L.maxpw = std::max(L.maxpw, pw);
+      if (pw <= TRI_NARROW) {
+        ++L.nnarrow;
+        L.maxpw_narrow = std::max(L.maxpw_narrow, pw);
+      }
     }
     L.npieces = (int)pbylev[l].size();
     L.nblocks = nb;
@@ -782,7 +790,12 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
     const SolveLevel& L = D.slevels[l];
     if (L.npieces > 0) {
       timed(D, "fwd_tri", [&] {
-        launch_fwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)X, Y, nrhs, st);
+        if (L.nnarrow > 0)
+          launch_fwd_tri(L.d_piece, L.nnarrow, L.maxpw_narrow, D.piece_sn, D.piece_c0, a, valL, (const T*)X, Y, nrhs,
+                         st);
+        if (L.npieces > L.nnarrow)
+          launch_fwd_tri(L.d_piece + L.nnarrow, L.npieces - L.nnarrow, L.maxpw, D.piece_sn, D.piece_c0, a, valL,
+                         (const T*)X, Y, nrhs, st);
       });
       timed(D, "fwd_gemv", [&] {
         launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)Y, X,
@@ -816,8 +829,12 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
                         (const T*)O, acc, nrhs, st);
       });
       timed(D, "bwd_tri", [&] {
-        launch_bwd_tri(L.d_piece, L.npieces, L.maxpw, D.piece_sn, D.piece_c0, a, valL, (const T*)Z, (const T*)acc,
-                       O, nrhs, st);
+        if (L.nnarrow > 0)
+          launch_bwd_tri(L.d_piece, L.nnarrow, L.maxpw_narrow, D.piece_sn, D.piece_c0, a, valL, (const T*)Z,
+                         (const T*)acc, O, nrhs, st);
+        if (L.npieces > L.nnarrow)
+          launch_bwd_tri(L.d_piece + L.nnarrow, L.npieces - L.nnarrow, L.maxpw, D.piece_sn, D.piece_c0, a, valL,
+                         (const T*)Z, (const T*)acc, O, nrhs, st);
       });
     }
   }

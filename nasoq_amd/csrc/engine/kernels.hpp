@@ -10,6 +10,7 @@ namespace lblg {
 constexpr int SMALL_W = 64;    // widest supernode factored entirely in LDS
 constexpr int PANEL_NB = 32;   // Bunch-Kaufman panel width for wider supernodes
 constexpr int SOLVE_W = 128;   // column width of a solve piece
+constexpr int TRI_NARROW = 32; // triangle launches of a level: pieces up to this width, then the wider ones
 constexpr int MAX_RHS = 8;     // right-hand sides per solve call
 
 struct SnArgs {  // symbolic arrays on the device
