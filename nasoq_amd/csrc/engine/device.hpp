@@ -59,6 +59,7 @@ struct SolveLevel {
   int maxpw = 1;         // widest piece of the level
   int nnarrow = 0;       // pieces of at most TRI_NARROW columns: the first nnarrow of d_piece
   int maxpw_narrow = 1;  // widest of those
+  int nblocks_narrow = 0;  // their row blocks (the first of the level's)
   int nblocks = 0;       // total row blocks (SOLVE_ROWS rows) of all pieces
   int* d_piece = nullptr;   // piece ids, narrowest first
   int* d_blk = nullptr;     // prefix of row blocks per piece

@@ -555,6 +555,7 @@ static void setup(Handle& h) {
       if (pw <= TRI_NARROW) {
         ++L.nnarrow;
         L.maxpw_narrow = std::max(L.maxpw_narrow, pw);
+        L.nblocks_narrow = nb;  // row blocks of the narrow pieces (they come first)
       }
     }
     L.npieces = (int)pbylev[l].size();
@@ -825,8 +826,14 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
     }
     if (L.npieces > 0) {
       timed(D, "bwd_gemv", [&] {
-        launch_bwd_gemv(L.d_piece, L.npieces, L.maxpw, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL,
-                        (const T*)O, acc, nrhs, st);
+        // narrow pieces apart, with LDS sized for them (as the triangles)
+        if (L.nnarrow > 0 && L.nblocks_narrow > 0)
+          launch_bwd_gemv(L.d_piece, L.nnarrow, L.maxpw_narrow, L.d_blk, L.nblocks_narrow, 0, D.piece_sn,
+                          D.piece_c0, a, valL, (const T*)O, acc, nrhs, st);
+        if (L.npieces > L.nnarrow && L.nblocks > L.nblocks_narrow)
+          launch_bwd_gemv(L.d_piece + L.nnarrow, L.npieces - L.nnarrow, L.maxpw, L.d_blk + L.nnarrow,
+                          L.nblocks - L.nblocks_narrow, L.nblocks_narrow, D.piece_sn, D.piece_c0, a, valL,
+                          (const T*)O, acc, nrhs, st);
       });
       timed(D, "bwd_tri", [&] {
         if (L.nnarrow > 0)

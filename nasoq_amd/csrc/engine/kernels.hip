@@ -1289,17 +1289,19 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* 
                                                   const int* piece_sn, const int* piece_c0, const int* super,
                                                   const int64_t* pi, const int64_t* px, const int* rows,
                                                   const T* valL, const T* Xf, T* acc,
-                                                  int nrhs) {
+                                                  int nrhs, int blk_base) {
   extern __shared__ double smem_d[];
-  T* Lb = reinterpret_cast<T*>(smem_d);  // Lb[j * (GROWS + 1) + qq], j < the level's widest piece
+  T* Lb = reinterpret_cast<T*>(smem_d);  // Lb[j * (GROWS + 1) + qq], j < the launch's widest piece
   __shared__ T xq[GROWS * MAXRHS];
   __shared__ T part[SOLVE_W * MAXRHS];
+  // the launch covers pieces[0, np) whose row blocks start at blk_base
+  const int b = (int)blockIdx.x + blk_base;
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (blk_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    if (blk_prefix[mid] <= b) lo = mid; else hi = mid - 1;
   }
-  const int p = pieces[lo], rb = blockIdx.x - blk_prefix[lo];
+  const int p = pieces[lo], rb = b - blk_prefix[lo];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int pw = min(SOLVE_W, w - c0l);
@@ -1467,8 +1469,8 @@ void launch_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs, hipStream_t
   HIP_LAUNCH_CHECK("dsolve");
 }
 template <typename T>
-void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix, int nblk, const int* piece_sn,
-                     const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
+void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix, int nblk, int blk_base,
+                     const int* piece_sn, const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
                      int nrhs, hipStream_t st) {
   if (np <= 0 || nblk <= 0) return;
   if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("bwd_gemv: piece width out of range");
@@ -1481,7 +1483,7 @@ void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix
   }
   k_bwd_gemv<T><<<nblk, 256, sizeof(T) * maxpw * (GROWS + 1), st>>>(pieces, blk_prefix, np, piece_sn, piece_c0,
                                                                     a.super, a.pi, a.px, a.rows, valL, Xf, acc,
-                                                                    nrhs);
+                                                                    nrhs, blk_base);
   HIP_LAUNCH_CHECK("bwd_gemv");
 }
 template <typename T>
@@ -1944,7 +1946,7 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
   template void launch_fwd_gemv<T>(const int*, int, const int*, int, const int*, const int*, const SnArgs&,    \
                                    const T*, const T*, T*, int, hipStream_t);                                  \
   template void launch_dsolve<T>(int, const T*, const T*, T*, int, hipStream_t);                               \
-  template void launch_bwd_gemv<T>(const int*, int, int, const int*, int, const int*, const int*,              \
+  template void launch_bwd_gemv<T>(const int*, int, int, const int*, int, int, const int*, const int*,         \
                                    const SnArgs&,                                                              \
                                    const T*, const T*, T*, int, hipStream_t);                                  \
   template void launch_bwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \
