@@ -962,24 +962,39 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   tile_zero(acc);
   for (int kc0 = 0; kc0 < w; kc0 += KC) {
     const int kc = min(KC, w - kc0);
-    // raw L columns of the chunk, plus one halo column on each side (2x2 blocks)
-    for (int idx = t; idx < KC * 64; idx += 256) {
-      const int kk = idx / 64, x = idx % 64;
-      const int la = kc0 + kk;
-      const int i = i0 + x, j = j0 + x;
-      As[kk * LDT + x] = (kk < kc && i < m) ? P[w + i + (int64_t)la * r] : (T)0;
-      Bs[kk * LDT + x] = (kk < kc && j < m) ? P[w + j + (int64_t)la * r] : (T)0;
+    // raw L columns of the chunk, plus one halo column on each side (2x2
+    // blocks) and the chunk's D entries; a thread's loads all issue before
+    // its LDS stores
+    constexpr int SQ = KC * 64 / 256;
+    T av[SQ], bw[SQ], hv = 0, dv = 0, ov = 0;
+#pragma unroll
+    for (int q = 0; q < SQ; ++q) {
+      const int idx = t + 256 * q, kk = idx / 64, x = idx % 64;
+      const int64_t la = kc0 + kk;
+      av[q] = (kk < kc && i0 + x < m) ? P[w + i0 + x + la * r] : (T)0;
+      bw[q] = (kk < kc && j0 + x < m) ? P[w + j0 + x + la * r] : (T)0;
     }
     if (t < 128) {
       const int side = t >> 6, x = t & 63;
       const int l = side == 0 ? kc0 - 1 : kc0 + kc;
       const int j = j0 + x;
-      halo[side][x] = (l >= 0 && l < w && j < m) ? P[w + j + (int64_t)l * r] : (T)0;
+      hv = (l >= 0 && l < w && j < m) ? P[w + j + (int64_t)l * r] : (T)0;
     }
     if (t < KC + 2) {
       const int l = kc0 - 1 + t;
-      dd[t] = (l >= 0 && l < w) ? dval[c0 + l] : (T)0;
-      oo[t] = (l >= 0 && l < w) ? dval[n + c0 + l] : (T)0;
+      dv = (l >= 0 && l < w) ? dval[c0 + l] : (T)0;
+      ov = (l >= 0 && l < w) ? dval[n + c0 + l] : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < SQ; ++q) {
+      const int idx = t + 256 * q, kk = idx / 64, x = idx % 64;
+      As[kk * LDT + x] = av[q];
+      Bs[kk * LDT + x] = bw[q];
+    }
+    if (t < 128) halo[t >> 6][t & 63] = hv;
+    if (t < KC + 2) {
+      dd[t] = dv;
+      oo[t] = ov;
     }
     __syncthreads();
     // B = (L D) columns: 1x1 scaling or the matching row of a 2x2 block,
