@@ -63,6 +63,7 @@ struct SolveLevel {
   int nblocks = 0;       // total row blocks (SOLVE_ROWS rows) of all pieces
   int* d_piece = nullptr;   // piece ids, narrowest first
   int* d_blk = nullptr;     // prefix of row blocks per piece
+  int* d_blkpos = nullptr;  // row block -> position of its piece in d_piece
 };
 
 struct DeviceFactor {

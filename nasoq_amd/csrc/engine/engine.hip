@@ -562,6 +562,12 @@ static void setup(Handle& h) {
     L.nblocks = nb;
     L.d_piece = dupload(D, pbylev[l]);
     L.d_blk = dupload(D, pre);
+    std::vector<int> bpos(nb);  // row block -> position of its piece in the level's list
+    for (int k = 0; k < L.npieces; ++k) {
+      const int e = k + 1 < L.npieces ? pre[k + 1] : nb;
+      for (int b = pre[k]; b < e; ++b) bpos[b] = k;
+    }
+    L.d_blkpos = dupload(D, bpos);
   }
   D.piece_sn = dupload(D, psn);
   D.piece_c0 = dupload(D, pc0);
@@ -799,7 +805,7 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
                          (const T*)X, Y, nrhs, st);
       });
       timed(D, "fwd_gemv", [&] {
-        launch_fwd_gemv(L.d_piece, L.npieces, L.d_blk, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)Y, X,
+        launch_fwd_gemv(L.d_piece, L.d_blk, L.d_blkpos, L.nblocks, D.piece_sn, D.piece_c0, a, valL, (const T*)Y, X,
                         nrhs, st);
       });
     }
@@ -827,13 +833,12 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
     if (L.npieces > 0) {
       timed(D, "bwd_gemv", [&] {
         // narrow pieces apart, with LDS sized for them (as the triangles)
-        if (L.nnarrow > 0 && L.nblocks_narrow > 0)
-          launch_bwd_gemv(L.d_piece, L.nnarrow, L.maxpw_narrow, L.d_blk, L.nblocks_narrow, 0, D.piece_sn,
+        if (L.nblocks_narrow > 0)
+          launch_bwd_gemv(L.d_piece, L.maxpw_narrow, L.d_blk, L.d_blkpos, L.nblocks_narrow, 0, D.piece_sn,
                           D.piece_c0, a, valL, (const T*)O, acc, nrhs, st);
-        if (L.npieces > L.nnarrow && L.nblocks > L.nblocks_narrow)
-          launch_bwd_gemv(L.d_piece + L.nnarrow, L.npieces - L.nnarrow, L.maxpw, L.d_blk + L.nnarrow,
-                          L.nblocks - L.nblocks_narrow, L.nblocks_narrow, D.piece_sn, D.piece_c0, a, valL,
-                          (const T*)O, acc, nrhs, st);
+        if (L.nblocks > L.nblocks_narrow)
+          launch_bwd_gemv(L.d_piece, L.maxpw, L.d_blk, L.d_blkpos, L.nblocks - L.nblocks_narrow, L.nblocks_narrow,
+                          D.piece_sn, D.piece_c0, a, valL, (const T*)O, acc, nrhs, st);
       });
       timed(D, "bwd_tri", [&] {
         if (L.nnarrow > 0)

@@ -1224,18 +1224,15 @@ __global__ __launch_bounds__(256) void k_fwd_tri(const int* pieces, const int* p
 }
 // forward update of the rows below a piece: X(label q) -= L(q, p) Y(p)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* blk_prefix, int np,
-                                                  const int* piece_sn, const int* piece_c0, const int* super,
-                                                  const int64_t* pi, const int64_t* px, const int* rows,
-                                                  const T* valL, const T* Y, T* X, int nrhs) {
+__global__ __launch_bounds__(256) void k_fwd_gemv(const int* pieces, const int* blk_prefix, const int* blkpos,
+                                                  int blk_base, const int* piece_sn, const int* piece_c0,
+                                                  const int* super, const int64_t* pi, const int64_t* px,
+                                                  const int* rows, const T* valL, const T* Y, T* X, int nrhs) {
   __shared__ T ys[SOLVE_W * MAXRHS];
   __shared__ T part[4][GROWS * MAXRHS];
-  int lo = 0, hi = np - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (blk_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
-  }
-  const int p = pieces[lo], rb = blockIdx.x - blk_prefix[lo];
+  // row block b of the level: piece at position blkpos[b] of the level's list
+  const int b = (int)blockIdx.x + blk_base, lo = blkpos[b];
+  const int p = pieces[lo], rb = b - blk_prefix[lo];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int pw = min(SOLVE_W, w - c0l);
@@ -1300,22 +1297,18 @@ __global__ void k_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs) {
 // by the launch for the level's widest piece, so levels of narrow pieces
 // keep many workgroups per CU.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* blk_prefix, int np,
-                                                  const int* piece_sn, const int* piece_c0, const int* super,
-                                                  const int64_t* pi, const int64_t* px, const int* rows,
-                                                  const T* valL, const T* Xf, T* acc,
-                                                  int nrhs, int blk_base) {
+__global__ __launch_bounds__(256) void k_bwd_gemv(const int* pieces, const int* blk_prefix, const int* blkpos,
+                                                  int blk_base, const int* piece_sn, const int* piece_c0,
+                                                  const int* super, const int64_t* pi, const int64_t* px,
+                                                  const int* rows, const T* valL, const T* Xf, T* acc,
+                                                  int nrhs) {
   extern __shared__ double smem_d[];
   T* Lb = reinterpret_cast<T*>(smem_d);  // Lb[j * (GROWS + 1) + qq], j < the launch's widest piece
   __shared__ T xq[GROWS * MAXRHS];
   __shared__ T part[SOLVE_W * MAXRHS];
-  // the launch covers pieces[0, np) whose row blocks start at blk_base
-  const int b = (int)blockIdx.x + blk_base;
-  int lo = 0, hi = np - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (blk_prefix[mid] <= b) lo = mid; else hi = mid - 1;
-  }
+  // row block b of the level (the launch covers blocks from blk_base): piece
+  // at position blkpos[b] of the level's list
+  const int b = (int)blockIdx.x + blk_base, lo = blkpos[b];
   const int p = pieces[lo], rb = b - blk_prefix[lo];
   const int s = piece_sn[p], c0l = piece_c0[p];
   const int c0 = super[s], w = super[s + 1] - c0;
@@ -1470,12 +1463,12 @@ void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, c
   HIP_LAUNCH_CHECK("fwd_tri");
 }
 template <typename T>
-void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+void launch_fwd_gemv(const int* pieces, const int* blk_prefix, const int* blkpos, int nblk, const int* piece_sn,
                      const int* piece_c0, const SnArgs& a, const T* valL, const T* Y, T* X,
                      int nrhs, hipStream_t st) {
-  if (np <= 0 || nblk <= 0) return;
-  k_fwd_gemv<T><<<nblk, 256, 0, st>>>(pieces, blk_prefix, np, piece_sn, piece_c0, a.super, a.pi, a.px, a.rows,
-                                   valL, Y, X, nrhs);
+  if (nblk <= 0) return;
+  k_fwd_gemv<T><<<nblk, 256, 0, st>>>(pieces, blk_prefix, blkpos, 0, piece_sn, piece_c0, a.super, a.pi, a.px,
+                                   a.rows, valL, Y, X, nrhs);
   HIP_LAUNCH_CHECK("fwd_gemv");
 }
 template <typename T>
@@ -1484,10 +1477,10 @@ void launch_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs, hipStream_t
   HIP_LAUNCH_CHECK("dsolve");
 }
 template <typename T>
-void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix, int nblk, int blk_base,
+void launch_bwd_gemv(const int* pieces, int maxpw, const int* blk_prefix, const int* blkpos, int nblk, int blk_base,
                      const int* piece_sn, const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
                      int nrhs, hipStream_t st) {
-  if (np <= 0 || nblk <= 0) return;
+  if (nblk <= 0) return;
   if (maxpw < 1 || maxpw > SOLVE_W) throw std::runtime_error("bwd_gemv: piece width out of range");
   static bool attr = false;
   if (!attr) {  // a level of 128-column pieces needs more than the default dynamic LDS
@@ -1496,9 +1489,9 @@ void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix
       throw std::runtime_error("bwd_gemv: cannot raise the dynamic LDS limit");
     attr = true;
   }
-  k_bwd_gemv<T><<<nblk, 256, sizeof(T) * maxpw * (GROWS + 1), st>>>(pieces, blk_prefix, np, piece_sn, piece_c0,
-                                                                    a.super, a.pi, a.px, a.rows, valL, Xf, acc,
-                                                                    nrhs, blk_base);
+  k_bwd_gemv<T><<<nblk, 256, sizeof(T) * maxpw * (GROWS + 1), st>>>(pieces, blk_prefix, blkpos, blk_base,
+                                                                    piece_sn, piece_c0, a.super, a.pi, a.px,
+                                                                    a.rows, valL, Xf, acc, nrhs);
   HIP_LAUNCH_CHECK("bwd_gemv");
 }
 template <typename T>
@@ -1958,10 +1951,11 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
                                 hipStream_t);                                                                  \
   template void launch_fwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \
                                   const T*, T*, int, hipStream_t);                                             \
-  template void launch_fwd_gemv<T>(const int*, int, const int*, int, const int*, const int*, const SnArgs&,    \
+  template void launch_fwd_gemv<T>(const int*, const int*, const int*, int, const int*, const int*,            \
+                                   const SnArgs&,                                                              \
                                    const T*, const T*, T*, int, hipStream_t);                                  \
   template void launch_dsolve<T>(int, const T*, const T*, T*, int, hipStream_t);                               \
-  template void launch_bwd_gemv<T>(const int*, int, int, const int*, int, int, const int*, const int*,         \
+  template void launch_bwd_gemv<T>(const int*, int, const int*, const int*, int, int, const int*, const int*, \
                                    const SnArgs&,                                                              \
                                    const T*, const T*, T*, int, hipStream_t);                                  \
   template void launch_bwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \

@@ -79,13 +79,13 @@ template <typename T>
 void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, const int* piece_c0, const SnArgs& a,
                     const T* valL, const T* X, T* Y, int nrhs, hipStream_t st);
 template <typename T>
-void launch_fwd_gemv(const int* pieces, int np, const int* blk_prefix, int nblk, const int* piece_sn,
+void launch_fwd_gemv(const int* pieces, const int* blk_prefix, const int* blkpos, int nblk, const int* piece_sn,
                      const int* piece_c0, const SnArgs& a, const T* valL, const T* Y, T* X,
                      int nrhs, hipStream_t st);
 template <typename T>
 void launch_dsolve(int n, const T* dval, const T* Y, T* Z, int nrhs, hipStream_t st);
 template <typename T>
-void launch_bwd_gemv(const int* pieces, int np, int maxpw, const int* blk_prefix, int nblk, int blk_base,
+void launch_bwd_gemv(const int* pieces, int maxpw, const int* blk_prefix, const int* blkpos, int nblk, int blk_base,
                      const int* piece_sn, const int* piece_c0, const SnArgs& a, const T* valL, const T* Xf, T* acc,
                      int nrhs, hipStream_t st);
 template <typename T>
