@@ -36,9 +36,11 @@ struct FactorLevel {
   int offdiag_wmax = 0;            // widest of those
   int small_wmax = 0;              // widest supernode of the small list
   int* d_offdiag_blk = nullptr;    // prefix of 64-row blocks per offdiag supernode
+  int* d_offdiag_pos = nullptr;    // 64-row block -> position of its supernode in the list
   SnList schur;                    // supernodes with an update (m > 0)
   int64_t schur_tiles = 0;
   int64_t* d_schur_tile = nullptr; // prefix of 64x64 lower tiles per schur supernode
+  int* d_schur_pos = nullptr;      // tile -> position of its supernode in the list
   SnList schur_big;                // Schur updates with >= 256 off-diagonal rows: 128x128 tiles
   int64_t schur_big_tiles = 0;
   int64_t* d_schur_big_tile = nullptr;

@@ -357,10 +357,27 @@ static void setup(Handle& h) {
     for (int s : off) F.offdiag_wmax = std::max(F.offdiag_wmax, S.super[s + 1] - S.super[s]);
     F.offdiag_blocks = blk;
     F.d_offdiag_blk = dupload(D, offblk);
+    {  // row block -> position of its supernode in the off-diagonal list
+      std::vector<int> pos(blk);
+      for (size_t k = 0; k < off.size(); ++k) {
+        const int e = k + 1 < off.size() ? offblk[k + 1] : blk;
+        for (int b = offblk[k]; b < e; ++b) pos[b] = (int)k;
+      }
+      F.d_offdiag_pos = dupload(D, pos);
+    }
     F.schur.count = (int)sch.size();
     F.schur.d = dupload(D, sch);
     F.schur_tiles = tl;
     F.d_schur_tile = dupload(D, tiles);
+    {  // tile -> position of its supernode in the Schur list
+      if (tl > 0x7fffffffLL) throw std::runtime_error("schur: too many tiles");
+      std::vector<int> pos((size_t)tl);
+      for (size_t k = 0; k < sch.size(); ++k) {
+        const int64_t e = k + 1 < sch.size() ? tiles[k + 1] : tl;
+        for (int64_t b = tiles[k]; b < e; ++b) pos[(size_t)b] = (int)k;
+      }
+      F.d_schur_pos = dupload(D, pos);
+    }
     F.schur_big.count = (int)schb.size();
     F.schur_big.d = dupload(D, schb);
     F.schur_big_tiles = tlb;
@@ -688,11 +705,11 @@ static void factor_device_t(Handle& h) {
         }
       }
       timed(D, "offdiag", [&] {
-        launch_offdiag(F.offdiag.d, F.offdiag.count, F.offdiag_wmax, F.d_offdiag_blk, F.offdiag_blocks, a, valL,
-                       dval, D.perm_piv, st);
+        launch_offdiag(F.offdiag.d, F.offdiag.count, F.offdiag_wmax, F.d_offdiag_blk, F.d_offdiag_pos,
+                       F.offdiag_blocks, a, valL, dval, D.perm_piv, st);
       });
       timed(D, "schur", [&] {
-        launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.schur_tiles, a, valL, dval, st);
+        launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.d_schur_pos, F.schur_tiles, a, valL, dval, st);
       });
       if (F.schur_big.count > 0) {
         timed(D, "scale_ld", [&] {

@@ -720,7 +720,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk_prefix, int count,
                                                  const int* super, const int64_t* pi, const int64_t* px,
                                                  T* valL, const T* dval, const int* perm_piv,
-                                                 int n, int wmax) {
+                                                 int n, int wmax, const int* blkpos) {
   extern __shared__ double smem_d[];
   T* const X = reinterpret_cast<T*>(smem_d);  // X[row + col*65], col < min(64, wmax)
   // the GEMM's operand chunks, then the column block's triangle L(cb rows, cb
@@ -729,12 +729,8 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   T* const As = Sh;
   T* const Bs = Sh + KC * LDT;
   T* const Lt = Sh;
-  // locate (supernode, row block)
-  int lo = 0, hi = count - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (blk_prefix[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
-  }
+  // (supernode, row block): the block's position in the list from blkpos
+  const int lo = blkpos[blockIdx.x];
   const int s = list[lo];
   const int rb = blockIdx.x - blk_prefix[lo];
   const int c0 = super[s], w = super[s + 1] - c0;
@@ -868,8 +864,8 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   }
 }
 template <typename T>
-void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, int nblocks, const SnArgs& a,
-                    T* valL, const T* dval, const int* perm_piv, hipStream_t st) {
+void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, const int* blkpos, int nblocks,
+                    const SnArgs& a, T* valL, const T* dval, const int* perm_piv, hipStream_t st) {
   if (count <= 0 || nblocks <= 0) return;
   if (wmax < 1) throw std::runtime_error("offdiag: empty supernode width");
   static bool attr = false;
@@ -880,7 +876,8 @@ void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix,
     attr = true;
   }
   k_offdiag<T><<<nblocks, 256, sizeof(T) * offdiag_lds_elems(wmax), st>>>(list, blk_prefix, count, a.super, a.pi,
-                                                                          a.px, valL, dval, perm_piv, a.n, wmax);
+                                                                          a.px, valL, dval, perm_piv, a.n, wmax,
+                                                                          blkpos);
   HIP_LAUNCH_CHECK("offdiag");
 }
 
@@ -904,7 +901,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* tile_prefix, int count,
                                                const int* super, const int64_t* pi, const int64_t* px,
                                                const int* rows, const int* col2sup, T* valL,
-                                               const T* dval, int n) {
+                                               const T* dval, int n, const int* tilepos) {
   __shared__ T As[KC * LDT], Bs[KC * LDT];
   __shared__ T halo[2][64];
   __shared__ T dd[KC + 2], oo[KC + 2];
@@ -912,12 +909,8 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   __shared__ int tpos[4][64];
   __shared__ int64_t cbase[64];
   __shared__ int dtl[4], ntg;
-  int lo = 0, hi = count - 1;
   const int64_t bid = blockIdx.x;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tile_prefix[mid] <= bid) lo = mid; else hi = mid - 1;
-  }
+  const int lo = tilepos[bid];  // the tile's supernode: its position in the list
   const int s = list[lo];
   const int64_t tix = bid - tile_prefix[lo];
   int ti = (int)((sqrt(8.0 * (double)tix + 1.0) - 1.0) * 0.5);
@@ -1040,12 +1033,12 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   });
 }
 template <typename T>
-void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                  T* valL, const T* dval, hipStream_t st) {
+void launch_schur(const int* list, int count, const int64_t* tile_prefix, const int* tilepos, int64_t ntiles,
+                  const SnArgs& a, T* valL, const T* dval, hipStream_t st) {
   if (count <= 0 || ntiles <= 0) return;
   if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur: too many tiles");
   k_schur<T><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                            a.col2sup, valL, dval, a.n);
+                                            a.col2sup, valL, dval, a.n, tilepos);
   HIP_LAUNCH_CHECK("schur");
 }
 
@@ -1945,10 +1938,11 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
                                    int*, int*, int*, bool, hipStream_t);                                       \
   template void launch_trailing<T>(const int*, int, int, int, const SnArgs&, T*, const T*, const int64_t*,      \
                                    const int*, const int*, hipStream_t);                                       \
-  template void launch_offdiag<T>(const int*, int, int, const int*, int, const SnArgs&, T*, const T*,        \
+  template void launch_offdiag<T>(const int*, int, int, const int*, const int*, int, const SnArgs&, T*,      \
+                                  const T*,                                                                    \
                                   const int*, hipStream_t);                                                                \
-  template void launch_schur<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,         \
-                                hipStream_t);                                                                  \
+  template void launch_schur<T>(const int*, int, const int64_t*, const int*, int64_t, const SnArgs&, T*,     \
+                                const T*, hipStream_t);                                                                  \
   template void launch_fwd_tri<T>(const int*, int, int, const int*, const int*, const SnArgs&, const T*,       \
                                   const T*, T*, int, hipStream_t);                                             \
   template void launch_fwd_gemv<T>(const int*, const int*, const int*, int, const int*, const int*,            \

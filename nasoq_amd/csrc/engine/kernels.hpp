@@ -67,11 +67,11 @@ template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
                          T* W, const int64_t* woff, const int* kpos, const int* kstart, bool direct, hipStream_t st);
 template <typename T>
-void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, int nblocks, const SnArgs& a,
-                    T* valL, const T* dval, const int* perm_piv, hipStream_t st);
+void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, const int* blkpos, int nblocks,
+                    const SnArgs& a, T* valL, const T* dval, const int* perm_piv, hipStream_t st);
 template <typename T>
-void launch_schur(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                  T* valL, const T* dval, hipStream_t st);
+void launch_schur(const int* list, int count, const int64_t* tile_prefix, const int* tilepos, int64_t ntiles,
+                  const SnArgs& a, T* valL, const T* dval, hipStream_t st);
 void launch_relabel(const SnArgs& a, int nsuper, const int* rows0, int* rows, const int* perm_piv,
                     int* newpos, const int* perm0, int* perm_f, int* iperm_f, hipStream_t st);
 constexpr int SOLVE_ROWS = 64;  // rows per workgroup in the sweeps below a piece
