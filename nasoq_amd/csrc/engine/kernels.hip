@@ -768,16 +768,20 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
       tile_zero(acc);
       for (int kc0 = 0; kc0 < cb0; kc0 += KC) {
         const int kc = min(KC, cb0 - kc0);
-        for (int idx = t; idx < KC * 64; idx += 256) {
-          const int kk = idx / 64, x = idx % 64;
-          T va = 0, vb = 0;
-          if (kk < kc) {
-            const int l = kc0 + kk;
-            if (x < nr) va = P[q0 + x + (int64_t)l * r];
-            if (x < cw) vb = P[(cb0 + x) + (int64_t)l * r];
-          }
-          As[kk * LDT + x] = va;
-          Bs[kk * LDT + x] = vb;
+        constexpr int GQ = KC * 64 / 256;  // a thread's loads all issue before its LDS stores
+        T va[GQ], vb[GQ];
+#pragma unroll
+        for (int q = 0; q < GQ; ++q) {
+          const int idx = t + 256 * q, kk = idx / 64, x = idx % 64;
+          const int64_t l = kc0 + kk;
+          va[q] = (kk < kc && x < nr) ? P[q0 + x + l * r] : (T)0;
+          vb[q] = (kk < kc && x < cw) ? P[(cb0 + x) + l * r] : (T)0;
+        }
+#pragma unroll
+        for (int q = 0; q < GQ; ++q) {
+          const int idx = t + 256 * q, kk = idx / 64, x = idx % 64;
+          As[kk * LDT + x] = va[q];
+          Bs[kk * LDT + x] = vb[q];
         }
         __syncthreads();
         tile_mma(acc, As, Bs, (kc + 3) & ~3);
