@@ -157,11 +157,32 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   // ---- load the panel rows
   // panel column j, row i >= j (with the block's earlier panels applied)
   auto pcol = [&](int i, int j) -> T { return sub ? Pp[(int64_t)j * r + i] : AG(i, k0 + j); };
-  if (hasrow)
-    for (int j = 0; j < nbe; ++j) LCR(j) = pcol(fr, j);
-  for (int idx = t; idx < nbe * nbe; idx += 256) {
-    const int i = idx % nbe, j = idx / nbe;
-    ADR(i, j) = (i >= j) ? pcol(k0 + i, j) : pcol(k0 + j, i);
+  // a thread's loads of a batch issue before its LDS stores (one round trip
+  // per batch rather than per entry)
+  if (hasrow) {
+#pragma unroll
+    for (int j0 = 0; j0 < WNB; j0 += 16) {
+      T v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = (j0 + q < nbe) ? pcol(fr, j0 + q) : (T)0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (j0 + q < nbe) LCR(j0 + q) = v[q];
+    }
+  }
+  {
+    constexpr int Q = WNB * WNB / 256;
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int idx = t + 256 * q, i = idx % nbe, j = idx / nbe;
+      v[q] = idx < nbe * nbe ? ((i >= j) ? pcol(k0 + i, j) : pcol(k0 + j, i)) : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int idx = t + 256 * q, i = idx % nbe, j = idx / nbe;
+      if (idx < nbe * nbe) ADR(i, j) = v[q];
+    }
   }
   if (t < WNB) {
     dphys[t] = k0 + t;
