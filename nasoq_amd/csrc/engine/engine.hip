@@ -785,7 +785,8 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
 // sweeps in the factor's type T; in / out are the FP64 ordered blocks of the
 // refinement (converted at entry and exit when T is float)
 template <typename T>
-static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nrhs) {
+static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nrhs, const double* active,
+                          int64_t astride) {
   hipStream_t st = D.stream;
   SnArgs a = snargs(D);
   const int64_t nn = (int64_t)D.n * nrhs;
@@ -798,7 +799,7 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   if (D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && sizeof(T) * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS) {
     timed(D, "solve_small", [&] {
       launch_solve_small<T>(D.ss_info, D.ss_ipx, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
-                            static_cast<const T*>(D.dval), in, out, nrhs, st);
+                            static_cast<const T*>(D.dval), in, out, nrhs, active, astride, st);
     });
     return;
   }
@@ -870,11 +871,13 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   if constexpr (!std::is_same<T, double>::value) launch_convert(out, (const T*)O, nn, st);
 }
 
-void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs) {
+// active: see launch_solve_small (only the one-workgroup sweeps use it; the
+// level sweeps always run)
+void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs, const double* active, int64_t astride) {
   if (D.prec == 1)
-    apply_solve_t<float>(D, in, out, nrhs);
+    apply_solve_t<float>(D, in, out, nrhs, active, astride);
   else
-    apply_solve_t<double>(D, in, out, nrhs);
+    apply_solve_t<double>(D, in, out, nrhs, active, astride);
 }
 
 static void spmv_ordered(DeviceFactor& D, const double* x, double* y, int nrhs) {
@@ -899,14 +902,14 @@ static void gmres_cycle(DeviceFactor& D, const double* b, double* x, int nrhs, i
   auto Vk = [&](int k) { return D.V + (int64_t)k * nn; };
   spmv_ordered(D, x, D.T, nrhs);
   launch_sub(nn, b, D.T, D.R, st);
-  apply_solve(D, D.R, D.Xs, nrhs);
+  apply_solve(D, D.R, D.Xs, nrhs, nullptr, 0);
   dots(D, nrhs, D.Xs, D.Xs, D.dots);
   launch_gmres_scalar(0, nrhs, MR, D.gm, D.dots, 0, 0, itr, tol_abs, tol_rel, st);
   launch_fill(D.V, nn * (MR + 1), 0.0, st);
   launch_axpby(n, nrhs, coef0, (int)SS, D.Xs, Vk(0), mask, (int)SS, 0, 1, st);
   for (int k = 0; k < mr; ++k) {
     spmv_ordered(D, Vk(k), D.T, nrhs);
-    apply_solve(D, D.T, Vk(k + 1), nrhs);
+    apply_solve(D, D.T, Vk(k + 1), nrhs, mask, SS);  // columns out of the step keep V(k + 1) = 0
     dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
     launch_gmres_scalar(1, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
     for (int j = 0; j <= k; ++j) {
@@ -1039,7 +1042,7 @@ void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t
   const bool capturable = !D.has_chains || small_sweeps(D, nrhs);
   auto head = [&]() {
     launch_gather_rhs(D.n, nrhs, Bd, lb, D.perm_f, D.B, st);
-    apply_solve(D, D.B, xo, nrhs);
+    apply_solve(D, D.B, xo, nrhs, nullptr, 0);
     if (fixed_cycles > 0) {
       // zero state, every column marked running
       launch_gmres_reset(D.gm, gmres_state_size(D.mr_cap), nrhs, gmres_off_S(D.mr_cap) + 3, st);

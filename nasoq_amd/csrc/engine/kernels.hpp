@@ -119,9 +119,11 @@ constexpr int SS_THIN_MIN = 64;           // in levels holding at least this man
 // for the k-th supernode of the level order; mode[l]: bit 0 = whole-workgroup
 // level, bits 1.. = number of thin supernodes, which lead the level's list
 template <typename T>
+// active (stride astride per column, may be null): a GMRES step's per-column
+// participation flags; when none is set the launch returns at once
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                         const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
-                        hipStream_t st);
+                        const double* active, int64_t astride, hipStream_t st);
 // per-column dot products out[c] = sum_i a(i, c) b(i, c) (rows interleaved by
 // column), reduced in a fixed order through at most nblk partials
 void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,

@@ -308,7 +308,14 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
 template <typename T, int NR>
 __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, const int64_t* ipx, const int* levptr,
                                                             const int* mode, int nlev, SnArgs a, const T* valL,
-                                                            const T* dval, const double* in, double* out) {
+                                                            const T* dval, const double* in, double* out,
+                                                            const double* active, int64_t astride) {
+  if (active) {  // inside a GMRES step: when no column takes part, no sweep is needed
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) any |= active[(int64_t)c * astride] != 0.0;
+    if (!any) return;
+  }
   extern __shared__ double smem_d[];
   T* x = reinterpret_cast<T*>(smem_d);
   const int n = a.n, t = threadIdx.x, wv = t >> 6, ln = t & 63;
@@ -415,7 +422,8 @@ size_t ss_lds(int n, int nr, int nlev) {
 
 template <typename T, int NR>
 void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
-                    const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, hipStream_t st) {
+                    const SnArgs& a, const T* valL, const T* dval, const double* in, double* out,
+                    const double* active, int64_t astride, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute((const void*)k_solve_small<T, NR>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -424,7 +432,7 @@ void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, con
     attr = true;
   }
   k_solve_small<T, NR><<<1, SS_THREADS, ss_lds<T>(a.n, NR, nlev), st>>>(info, ipx, levptr, mode, nlev, a, valL, dval,
-                                                                        in, out);
+                                                                        in, out, active, astride);
   HIP_LAUNCH_CHECK("solve_small");
 }
 
@@ -433,25 +441,27 @@ void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, con
 template <typename T>
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                         const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
-                        hipStream_t st) {
+                        const double* active, int64_t astride, hipStream_t st) {
   if (nrhs < 1 || nrhs > MAX_RHS || sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS || nlev < 1 ||
       nlev > SOLVE_SMALL_LEVELS)
     throw std::invalid_argument("solve_small: block or level schedule does not fit the workgroup's LDS");
   switch (nrhs) {
-    case 1: solve_small_nr<T, 1>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 2: solve_small_nr<T, 2>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 3: solve_small_nr<T, 3>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 4: solve_small_nr<T, 4>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 5: solve_small_nr<T, 5>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 6: solve_small_nr<T, 6>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    case 7: solve_small_nr<T, 7>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
-    default: solve_small_nr<T, 8>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, st); break;
+    case 1: solve_small_nr<T, 1>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 2: solve_small_nr<T, 2>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 3: solve_small_nr<T, 3>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 4: solve_small_nr<T, 4>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 5: solve_small_nr<T, 5>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 6: solve_small_nr<T, 6>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 7: solve_small_nr<T, 7>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    default: solve_small_nr<T, 8>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
   }
 }
 
 template void launch_solve_small<double>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
-                                         const double*, const double*, const double*, double*, int, hipStream_t);
+                                         const double*, const double*, const double*, double*, int, const double*,
+                                         int64_t, hipStream_t);
 template void launch_solve_small<float>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
-                                        const float*, const float*, const double*, double*, int, hipStream_t);
+                                        const float*, const float*, const double*, double*, int, const double*,
+                                        int64_t, hipStream_t);
 
 }  // namespace lblg
