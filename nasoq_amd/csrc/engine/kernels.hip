@@ -1604,8 +1604,40 @@ __global__ void k_red_final(const double* part, int nblk, int nv, double* out) {
 }
 // workgroups for a reduction over n rows: enough to cover n, at most cap
 static int red_blocks(int n, int cap) { return (int)std::max<int64_t>(1, std::min<int64_t>(cap, (n + 255) / 256)); }
+// short vectors: one workgroup of 1024 threads, strided per-thread sums and
+// a fixed tree (one launch instead of two; repeated calls agree bit for bit)
+__global__ __launch_bounds__(1024) void k_dot_one(int n, int nrhs, const double* a, const double* b, double* out) {
+  __shared__ double sh[16][MAXRHS];
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  double acc[MAXRHS];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) acc[c] = 0.0;
+  for (int i = t; i < n; i += 1024)
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c)
+      if (c < nrhs) acc[c] += a[(int64_t)i * nrhs + c] * b[(int64_t)i * nrhs + c];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) {
+    if (c >= nrhs) break;
+    const double v = wave_sum(acc[c]);
+    if (ln == 0) sh[wv][c] = v;
+  }
+  __syncthreads();
+  if (t < nrhs) {
+    double s = 0.0;
+    for (int q = 0; q < 16; ++q) s += sh[q][t];
+    out[t] = s;
+  }
+}
+#define DOT_ONE_MAX (1 << 16)  // rows x columns up to which one workgroup takes the dot products
+
 void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,
                 hipStream_t st) {
+  if ((int64_t)n * nrhs <= DOT_ONE_MAX) {
+    k_dot_one<<<1, 1024, 0, st>>>(n, nrhs, a, b, out);
+    HIP_LAUNCH_CHECK("dot_one");
+    return;
+  }
   nblk = red_blocks(n, nblk);
   k_dot_partial<<<nblk, 256, 0, st>>>(n, nrhs, a, b, part);
   HIP_LAUNCH_CHECK("dot_partial");
