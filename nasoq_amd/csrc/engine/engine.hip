@@ -884,9 +884,6 @@ static void spmv_ordered(DeviceFactor& D, const double* x, double* y, int nrhs) 
   timed(D, "spmv", [&] { launch_spmv(D.n, D.Krp, D.Kci, D.Kv, D.perm_f, D.iperm_f, x, y, nrhs, D.stream); });
 }
 
-static void dots(DeviceFactor& D, int nrhs, const double* a, const double* b, double* out) {
-  launch_dot(D.n, nrhs, a, b, D.part, D.nblk_red, out, D.stream);
-}
 
 // one GMRES(mr) cycle per column, pmgmres_ldlt_auto (mgmres.cpp:1015-1211)
 static void gmres_cycle(DeviceFactor& D, const double* b, double* x, int nrhs, int mr, int itr,
@@ -903,29 +900,29 @@ static void gmres_cycle(DeviceFactor& D, const double* b, double* x, int nrhs, i
   spmv_ordered(D, x, D.T, nrhs);
   launch_sub(nn, b, D.T, D.R, st);
   apply_solve(D, D.R, D.Xs, nrhs, nullptr, 0);
-  dots(D, nrhs, D.Xs, D.Xs, D.dots);
-  launch_gmres_scalar(0, nrhs, MR, D.gm, D.dots, 0, 0, itr, tol_abs, tol_rel, st);
+  launch_dot_scalar(n, nrhs, D.Xs, D.Xs, D.part, D.nblk_red, D.dots, 0, MR, D.gm, 0, 0,
+                    itr, tol_abs, tol_rel, st);
   launch_fill(D.V, nn * (MR + 1), 0.0, st);
   launch_axpby(n, nrhs, coef0, (int)SS, D.Xs, Vk(0), mask, (int)SS, 0, 1, st);
   for (int k = 0; k < mr; ++k) {
     spmv_ordered(D, Vk(k), D.T, nrhs);
     apply_solve(D, D.T, Vk(k + 1), nrhs, mask, SS);  // columns out of the step keep V(k + 1) = 0
-    dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
-    launch_gmres_scalar(1, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+    launch_dot_scalar(n, nrhs, Vk(k + 1), Vk(k + 1), D.part, D.nblk_red, D.dots, 1, MR, D.gm, 0, k,
+                      itr, tol_abs, tol_rel, st);
     for (int j = 0; j <= k; ++j) {
-      dots(D, nrhs, Vk(k + 1), Vk(j), D.dots);
-      launch_gmres_scalar(2, nrhs, MR, D.gm, D.dots, j, k, itr, tol_abs, tol_rel, st);
+      launch_dot_scalar(n, nrhs, Vk(k + 1), Vk(j), D.part, D.nblk_red, D.dots, 2, MR, D.gm, j, k,
+                        itr, tol_abs, tol_rel, st);
       launch_axpby(n, nrhs, coef0, (int)SS, Vk(j), Vk(k + 1), mask, (int)SS, 0, 0, st);
     }
-    dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
-    launch_gmres_scalar(3, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+    launch_dot_scalar(n, nrhs, Vk(k + 1), Vk(k + 1), D.part, D.nblk_red, D.dots, 3, MR, D.gm, 0, k,
+                      itr, tol_abs, tol_rel, st);
     for (int j = 0; j <= k; ++j) {  // conditional re-orthogonalisation (flag on device)
-      dots(D, nrhs, Vk(k + 1), Vk(j), D.dots);
-      launch_gmres_scalar(4, nrhs, MR, D.gm, D.dots, j, k, itr, tol_abs, tol_rel, st);
+      launch_dot_scalar(n, nrhs, Vk(k + 1), Vk(j), D.part, D.nblk_red, D.dots, 4, MR, D.gm, j, k,
+                        itr, tol_abs, tol_rel, st);
       launch_axpby(n, nrhs, coef0, (int)SS, Vk(j), Vk(k + 1), reo, (int)SS, 0, 0, st);
     }
-    dots(D, nrhs, Vk(k + 1), Vk(k + 1), D.dots);
-    launch_gmres_scalar(5, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
+    launch_dot_scalar(n, nrhs, Vk(k + 1), Vk(k + 1), D.part, D.nblk_red, D.dots, 5, MR, D.gm, 0, k,
+                      itr, tol_abs, tol_rel, st);
     launch_axpby(n, nrhs, coef0, (int)SS, Vk(k + 1), Vk(k + 1), mask, (int)SS, 0, 1, st);
     launch_gmres_scalar(6, nrhs, MR, D.gm, D.dots, 0, k, itr, tol_abs, tol_rel, st);
   }

@@ -1689,10 +1689,8 @@ void launch_axpby(int n, int nrhs, const double* coefs, int cstride, const doubl
 // op 5: h(k+1,k) = sqrt(d0) after reorth
 // op 6: Givens for column k, convergence check
 // op 7: back substitution -> y
-__global__ void k_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k,
-                               int itr, double tol_abs, double tol_rel) {
-  const int c = threadIdx.x;
-  if (c >= nrhs) return;
+__device__ void gmres_scalar_col(int op, int c, int MR, double* gm, double d0, int j, int k, int itr,
+                                 double tol_abs, double tol_rel) {
   double* st = gs_col(gm, c, MR);
   double* S = GS_S(st, MR);
   double* H = st;
@@ -1701,7 +1699,6 @@ __global__ void k_gmres_scalar(int op, int nrhs, int MR, double* gm, const doubl
   double* g = GS_G(st, MR);
   double* y = GS_Y(st, MR);
   double* D = GS_D(st, MR);  // D[0]: coefficient slot used by vector kernels
-  const double d0 = dots[c];
   const double delta = 1.0e-3;
   if (op == 0) {
     S[7] = 0.0;
@@ -1802,10 +1799,57 @@ __global__ void k_gmres_scalar(int op, int nrhs, int MR, double* gm, const doubl
     if (S[4] != 0.0) S[3] = 0.0;  // converged: stop outer loop for this column
   }
 }
+__global__ void k_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k,
+                               int itr, double tol_abs, double tol_rel) {
+  const int c = threadIdx.x;
+  if (c >= nrhs) return;
+  gmres_scalar_col(op, c, MR, gm, dots[c], j, k, itr, tol_abs, tol_rel);
+}
 void launch_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k, int itr,
                          double tol_abs, double tol_rel, hipStream_t st) {
   k_gmres_scalar<<<1, 64, 0, st>>>(op, nrhs, MR, gm, dots, j, k, itr, tol_abs, tol_rel);
   HIP_LAUNCH_CHECK("gmres_scalar");
+}
+
+// dot products of short vectors followed by the GMRES scalar step on them,
+// in one workgroup and one launch (launch_dot + launch_gmres_scalar for long
+// vectors); the sums are those of k_dot_one
+__global__ __launch_bounds__(1024) void k_dot_scalar_one(int n, int nrhs, const double* a, const double* b,
+                                                         double* out, int op, int MR, double* gm, int j, int k,
+                                                         int itr, double tol_abs, double tol_rel) {
+  __shared__ double sh[16][MAXRHS];
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  double acc[MAXRHS];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) acc[c] = 0.0;
+  for (int i = t; i < n; i += 1024)
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c)
+      if (c < nrhs) acc[c] += a[(int64_t)i * nrhs + c] * b[(int64_t)i * nrhs + c];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) {
+    if (c >= nrhs) break;
+    const double v = wave_sum(acc[c]);
+    if (ln == 0) sh[wv][c] = v;
+  }
+  __syncthreads();
+  if (t < nrhs) {
+    double s = 0.0;
+    for (int q = 0; q < 16; ++q) s += sh[q][t];
+    out[t] = s;
+    gmres_scalar_col(op, t, MR, gm, s, j, k, itr, tol_abs, tol_rel);
+  }
+}
+void launch_dot_scalar(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,
+                       int op, int MR, double* gm, int j, int k, int itr, double tol_abs, double tol_rel,
+                       hipStream_t st) {
+  if ((int64_t)n * nrhs <= DOT_ONE_MAX) {
+    k_dot_scalar_one<<<1, 1024, 0, st>>>(n, nrhs, a, b, out, op, MR, gm, j, k, itr, tol_abs, tol_rel);
+    HIP_LAUNCH_CHECK("dot_scalar");
+    return;
+  }
+  launch_dot(n, nrhs, a, b, part, nblk, out, st);
+  launch_gmres_scalar(op, nrhs, MR, gm, out, j, k, itr, tol_abs, tol_rel, st);
 }
 
 // x += V y (columns with active or just-converged state)

@@ -131,6 +131,10 @@ void launch_dot(int n, int nrhs, const double* a, const double* b, double* part,
 void launch_sub(int64_t count, const double* b, const double* y, double* r, hipStream_t st);
 void launch_axpby(int n, int nrhs, const double* coefs, int cstride, const double* x, double* y,
                   const double* mask, int mstride, int mask_off, int mode, hipStream_t st);
+// out = column dots of a and b, then GMRES scalar step op on them (one launch for short vectors)
+void launch_dot_scalar(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,
+                       int op, int MR, double* gm, int j, int k, int itr, double tol_abs, double tol_rel,
+                       hipStream_t st);
 void launch_gmres_scalar(int op, int nrhs, int MR, double* gm, const double* dots, int j, int k, int itr,
                          double tol_abs, double tol_rel, hipStream_t st);
 void launch_gmres_update(int n, int nrhs, int MR, const double* V, const double* gm, double* x,
