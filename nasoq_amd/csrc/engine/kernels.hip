@@ -1104,14 +1104,16 @@ __device__ inline T bcast(T v, int lane) { return __shfl(v, lane, 64); }
 // Stage the strict lower triangle of a piece's pw x pw diagonal block (pw <=
 // SOLVE_W) into LDS: entry L(d0 + i, d0 + j), i > j, at Lt[j * lp + i]
 // (colmajor) or Lt[i * lp + j] (the transpose).  Thread (i, h) (i = row, h =
-// which half of the 256) loads the entries j = h, h + 2, ... < i of its row,
+// which half of the workgroup; pw <= blockDim.x / 2) loads the entries
+// j = h, h + 2, ... < i of its row,
 // 16 loads in flight before the first LDS store; a wave reads 64 consecutive
 // rows of one column per load.  Narrow pieces cost a few loads per thread.
 template <typename T>
 __device__ __forceinline__ void stage_piece_triangle(T* Lt, int lp, const T* P, int64_t r, int d0, int pw,
                                                      bool colmajor) {
   constexpr int B = 16;
-  const int i = threadIdx.x & (SOLVE_W - 1), h = threadIdx.x / SOLVE_W;
+  const int half = blockDim.x / 2;  // rows per half: 128 (256 threads) or 32 (64 threads, pw <= 32)
+  const int i = threadIdx.x % half, h = threadIdx.x / half;
   if (i >= pw) return;
   const int nj = (i - h + 1) / 2;  // entries j = h + 2 k < i
   const T* Pi = P + (int64_t)(d0 + i) + (int64_t)(d0 + h) * r;
@@ -1455,8 +1457,11 @@ void launch_fwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, c
       throw std::runtime_error("fwd_tri: cannot raise the dynamic LDS limit");
     attr = true;
   }
-  k_fwd_tri<T><<<np, 256, sizeof(T) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X,
-                                                          Y, nrhs, lp);
+  // one right-hand side and pieces of at most 32 columns: one wave does all
+  // the work, so the launch uses one-wave workgroups (more of them per CU)
+  const int nt = (nrhs == 1 && maxpw <= 32) ? 64 : 256;
+  k_fwd_tri<T><<<np, nt, sizeof(T) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, X,
+                                                         Y, nrhs, lp);
   HIP_LAUNCH_CHECK("fwd_tri");
 }
 template <typename T>
@@ -1505,7 +1510,8 @@ void launch_bwd_tri(const int* pieces, int np, int maxpw, const int* piece_sn, c
       throw std::runtime_error("bwd_tri: cannot raise the dynamic LDS limit");
     attr = true;
   }
-  k_bwd_tri<T><<<np, 256, sizeof(T) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z,
+  const int nt = (nrhs == 1 && maxpw <= 32) ? 64 : 256;  // as in launch_fwd_tri
+  k_bwd_tri<T><<<np, nt, sizeof(T) * maxpw * lp, st>>>(pieces, piece_sn, piece_c0, a.super, a.pi, a.px, valL, Z,
                                                           acc, Xf, nrhs, lp);
   HIP_LAUNCH_CHECK("bwd_tri");
 }
