@@ -147,16 +147,17 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
   const int64_t r = pi[s + 1] - pi[s];
   T* P = valL + px[s];
   const int t = threadIdx.x;
-  for (int idx0 = 0; idx0 < w * w; idx0 += 256 * 8) {  // a thread's loads of a batch issue before its stores
+  const int NT = blockDim.x;  // 256, or 64 for lists of supernodes at most 16 wide
+  for (int idx0 = 0; idx0 < w * w; idx0 += NT * 8) {  // a thread's loads of a batch issue before its stores
     T v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int idx = idx0 + t + 256 * q, i = idx % w, j = idx / w;
+      const int idx = idx0 + t + NT * q, i = idx % w, j = idx / w;
       v[q] = (idx < w * w && i >= j) ? P[i + j * r] : (T)0;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int idx = idx0 + t + 256 * q, i = idx % w, j = idx / w;
+      const int idx = idx0 + t + NT * q, i = idx % w, j = idx / w;
       if (idx < w * w) a[i + j * LDA_S] = v[q];
     }
   }
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
     const int zero = sh_zero;
     if (zero && t == 0) atomicOr(status, 1);
     if (kp != kk) {
-      for (int q = t; q < 64 * 3; q += 256) {
+      for (int q = t; q < 64 * 3; q += NT) {
         const int part = q / 64, x = q % 64;
         if (part == 0) {
           const int i = kp + 1 + x;
@@ -259,10 +260,10 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
           const T d11 = (T)1 / akk;
           const int m = w - k - 1;
           if (!big) {
-            for (int i = k + 1 + t; i < w; i += 256) a[i + k * LDA_S] /= akk;
+            for (int i = k + 1 + t; i < w; i += NT) a[i + k * LDA_S] /= akk;
             __syncthreads();
           }
-          for (int idx = t; idx < m * m; idx += 256) {
+          for (int idx = t; idx < m * m; idx += NT) {
             const int i = k + 1 + idx % m, j = k + 1 + idx / m;
             if (i >= j) {
               const T f = big ? -d11 * a[j + k * LDA_S] : -akk * a[j + k * LDA_S];
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
           }
           __syncthreads();
           if (big)
-            for (int i = k + 1 + t; i < w; i += 256) a[i + k * LDA_S] *= d11;
+            for (int i = k + 1 + t; i < w; i += NT) a[i + k * LDA_S] *= d11;
         }
       } else if (k < w - 2) {
         T d21 = a[k + 1 + k * LDA_S];
@@ -279,18 +280,18 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
         const T d22 = a[k + k * LDA_S] / d21;
         const T tt = (T)1 / (d11 * d22 - (T)1);
         d21 = tt / d21;
-        for (int j = k + 2 + t; j < w; j += 256) {
+        for (int j = k + 2 + t; j < w; j += NT) {
           wk[j] = d21 * (d11 * a[j + k * LDA_S] - a[j + (k + 1) * LDA_S]);
           wk1[j] = d21 * (d22 * a[j + (k + 1) * LDA_S] - a[j + k * LDA_S]);
         }
         __syncthreads();
         const int m = w - k - 2;
-        for (int idx = t; idx < m * m; idx += 256) {
+        for (int idx = t; idx < m * m; idx += NT) {
           const int i = k + 2 + idx % m, j = k + 2 + idx / m;
           if (i >= j) a[i + j * LDA_S] -= a[i + k * LDA_S] * wk[j] + a[i + (k + 1) * LDA_S] * wk1[j];
         }
         __syncthreads();
-        for (int j = k + 2 + t; j < w; j += 256) {
+        for (int j = k + 2 + t; j < w; j += NT) {
           a[j + k * LDA_S] = wk[j];
           a[j + (k + 1) * LDA_S] = wk1[j];
         }
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(256) void k_bk_small(const int* list, const int* su
     a[l + l * LDA_S] = 1.0;
   }
   __syncthreads();
-  for (int idx = t; idx < w * w; idx += 256) {
+  for (int idx = t; idx < w * w; idx += NT) {
     const int i = idx % w, j = idx / w;
     if (i >= j) P[i + j * r] = a[i + j * LDA_S];
   }
@@ -324,8 +325,10 @@ void launch_bk_small(const int* list, int count, int wmax, const SnArgs& a, T* v
                      int* perm_piv, int* status, hipStream_t st) {
   if (count <= 0) return;
   if (wmax < 1 || wmax > 64) throw std::runtime_error("bk_small: supernode width out of range");
-  k_bk_small<T><<<count, 256, sizeof(T) * LDA_S * wmax, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n,
-                                                               status);
+  // lists of supernodes at most 16 wide: one-wave workgroups (the block's
+  // work fits one wave; more of them per CU)
+  k_bk_small<T><<<count, wmax <= 16 ? 64 : 256, sizeof(T) * LDA_S * wmax, st>>>(list, a.super, a.pi, a.px, valL,
+                                                                                 dval, perm_piv, a.n, status);
   HIP_LAUNCH_CHECK("bk_small");
 }
 
