@@ -742,16 +742,17 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
   const int64_t q0 = w + 64 * (int64_t)rb;
   const int nr = (int)min((int64_t)64, r - q0);
   const int t = threadIdx.x;
+  const int NT = blockDim.x;  // 256; 64 for lists of supernodes at most 16 wide (no GEMM part)
   const bool gather = w <= SMALL_W;
   for (int cb0 = 0; cb0 < w; cb0 += 64) {
     const int cw = min(64, w - cb0);
     // load B block (with column gather for narrow supernodes); a thread's
     // loads of a batch issue before its LDS stores
-    for (int idx0 = 0; idx0 < 64 * cw; idx0 += 256 * 8) {
+    for (int idx0 = 0; idx0 < 64 * cw; idx0 += NT * 8) {
       T v[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int idx = idx0 + t + 256 * k, i = idx & 63, j = idx >> 6;
+        const int idx = idx0 + t + NT * k, i = idx & 63, j = idx >> 6;
         v[k] = 0;
         if (idx < 64 * cw && i < nr) {
           const int srcc = gather ? (perm_piv[c0 + cb0 + j] - c0) : (cb0 + j);
@@ -760,7 +761,7 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int idx = idx0 + t + 256 * k, i = idx & 63, j = idx >> 6;
+        const int idx = idx0 + t + NT * k, i = idx & 63, j = idx >> 6;
         if (idx < 64 * cw) X[i + j * 65] = v[k];
       }
     }
@@ -795,39 +796,42 @@ __global__ __launch_bounds__(256) void k_offdiag(const int* list, const int* blk
       });
     }
     // the column block's unit-lower triangle (after the GEMM: shares its buffers)
-    for (int idx0 = 0; idx0 < cw * cw; idx0 += 256 * 8) {
+    for (int idx0 = 0; idx0 < cw * cw; idx0 += NT * 8) {
       T v[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int idx = idx0 + t + 256 * k, i = idx % cw, j = idx / cw;
+        const int idx = idx0 + t + NT * k, i = idx % cw, j = idx / cw;
         v[k] = (idx < cw * cw && i > j) ? P[(cb0 + i) + (int64_t)(cb0 + j) * r] : (T)0;
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int idx = idx0 + t + 256 * k, i = idx % cw, j = idx / cw;
+        const int idx = idx0 + t + NT * k, i = idx % cw, j = idx / cw;
         if (idx < cw * cw) Lt[i + j * 65] = v[k];
       }
     }
     __syncthreads();
-    // substitution inside the column block: four adjacent lanes per row split
-    // each column's sum (l = sub, sub + 4, ...) and meet by shuffles; the
+    // substitution inside the column block: four adjacent lanes per row (one
+    // in one-wave workgroups) split each column's sum (l = sub, sub + 4, ...)
+    // and meet by shuffles; the
     // solved entry is written by the first lane of the row before any lane
     // of the row reads it (same wave, LDS in program order)
     {
-      const int row = t >> 2, sub = t & 3;
+      const int lpr = NT / 64, row = t / lpr, sub = t % lpr;  // lanes per row: 4, or 1
       if (row < nr) {
         for (int j = 0; j < cw; ++j) {
           T v = 0;
-          for (int l = sub; l < j; l += 4) v += X[row + l * 65] * Lt[j + l * 65];
-          v += __shfl_xor(v, 1, 64);
-          v += __shfl_xor(v, 2, 64);
+          for (int l = sub; l < j; l += lpr) v += X[row + l * 65] * Lt[j + l * 65];
+          if (lpr == 4) {
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+          }
           const T xj = X[row + j * 65] - v;
           if (sub == 0) X[row + j * 65] = xj;
         }
       }
     }
     __syncthreads();
-    for (int idx = t; idx < 64 * cw; idx += 256) {
+    for (int idx = t; idx < 64 * cw; idx += NT) {
       const int i = idx % 64, j = idx / 64;
       if (i < nr) P[q0 + i + (int64_t)(cb0 + j) * r] = X[i + j * 65];
     }
@@ -882,7 +886,9 @@ void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix,
       throw std::runtime_error("offdiag: cannot raise the dynamic LDS limit");
     attr = true;
   }
-  k_offdiag<T><<<nblocks, 256, sizeof(T) * offdiag_lds_elems(wmax), st>>>(list, blk_prefix, count, a.super, a.pi,
+  // lists of supernodes at most 16 wide: one-wave workgroups, one lane per row
+  k_offdiag<T><<<nblocks, wmax <= 16 ? 64 : 256, sizeof(T) * offdiag_lds_elems(wmax), st>>>(list, blk_prefix, count,
+                                                                                            a.super, a.pi,
                                                                           a.px, valL, dval, perm_piv, a.n, wmax,
                                                                           blkpos);
   HIP_LAUNCH_CHECK("offdiag");
