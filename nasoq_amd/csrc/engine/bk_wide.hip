@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
                                                  const int64_t* pi, const int64_t* px, T* valL,
                                                  T* dval, int* perm_piv, T* Wbase,
                                                  const int64_t* woff, int* kpos, int* kstart, int* sync,
-                                                 uint64_t* ll, uint32_t ebase, int n, int* status) {
+                                                 int* sync_next, uint64_t* ll, uint32_t ebase, int n,
+                                                 int* status) {
   __shared__ T LC[WNB * WCH];       // own far rows, column-major [column][row]
   __shared__ T Ad[WNB * WLDA];      // diagonal rows: L (eliminated) | symmetric A (pending)
   __shared__ T wk[WNB], wi[WNB], td0[WNB], td1[WNB], dd[WNB], dsb[WNB], prow[WNB];
@@ -119,6 +120,10 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   __shared__ int s_nm, s_abort, s_nM, s_nMa, s_fail;
 
   const int g = blockIdx.x, t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  // consecutive launches alternate between two barrier counters: this one
+  // counts from zero, and zeroes the other for the next launch (the launch
+  // before used it and has ended; the next has not started)
+  if (g == 0 && t == 0) __hip_atomic_store(sync_next, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int s = list[0];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
@@ -728,19 +733,19 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 template <typename T>
 void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
                     int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
-                    uint64_t* ll, uint32_t ebase, int* status, hipStream_t st) {
+                    int* sync_next, uint64_t* ll, uint32_t ebase, int* status, hipStream_t st) {
   if (nb > WNB || nb < 4 || G < 1 || G > 256) throw std::runtime_error("bk_wide: launch shape out of range");
   int n = a.n;
   void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&sub, (void*)&wcols, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
                   (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
-                  (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&ll, (void*)&ebase, (void*)&n,
+                  (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&sync_next, (void*)&ll, (void*)&ebase, (void*)&n,
                   (void*)&status};
   hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_wide<T>, dim3(G), dim3(256), args, 0, st);
   if (e != hipSuccess) throw std::runtime_error(std::string("bk_wide: ") + hipGetErrorString(e));
 }
 template void launch_bk_wide<double>(const int*, int, int, int, int, const SnArgs&, double*, double*, int*, double*,
-                                     const int64_t*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
+                                     const int64_t*, int*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
 template void launch_bk_wide<float>(const int*, int, int, int, int, const SnArgs&, float*, float*, int*, float*,
-                                    const int64_t*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
+                                    const int64_t*, int*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
 
 }  // namespace lblg

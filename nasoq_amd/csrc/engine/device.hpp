@@ -116,6 +116,7 @@ struct DeviceFactor {
   double* segnorm = nullptr;    // ||K0||_inf of every system
   double* segsums = nullptr;    // |r|, |x|, |b| sums per system and column
   int* sync = nullptr;          // group-barrier counters
+  int wide_par = 0;             // which of sync[0], sync[1] the next wide panel counts on
   uint64_t* ll = nullptr;       // flag-in-word exchange buffer of the wide panel kernel
   uint32_t ll_epoch = 0;        // next epoch base (monotone for the handle's lifetime)
   double* red = nullptr;        // reduction scratch

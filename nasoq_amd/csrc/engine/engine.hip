@@ -395,7 +395,9 @@ static void setup(Handle& h) {
   D.woff = dupload(D, woff);
   D.kpos = dalloc<int>(D, ns);
   D.kstart = dalloc<int>(D, ns);
-  D.sync = dalloc<int>(D, std::max<int64_t>(ns, 1));
+  D.sync = dalloc<int>(D, std::max<int64_t>(ns, 2));
+  launch_fill_int(D.sync, (int)std::max<int64_t>(ns, 2), 0, D.stream);  // the wide panel's two counters start at 0
+  D.wide_par = 0;
   D.ll = reinterpret_cast<uint64_t*>(dalloc<double>(D, LL_WORDS));
   launch_fill(reinterpret_cast<double*>(D.ll), LL_WORDS, 0.0, D.stream);
   D.ll_epoch = 0;
@@ -681,7 +683,6 @@ static void factor_device_t(Handle& h) {
                                     D.trailing_direct, st);
               });
           }
-          launch_fill_int(D.sync, 1, 0, st);
           // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
           // panel eliminates >= nb - 1 columns); the last panel may be narrower
           const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
@@ -690,8 +691,9 @@ static void factor_device_t(Handle& h) {
           D.ll_epoch += 3u * (uint32_t)nb + 8u;
           timed(D, "bk_panel_group", [&] {
             launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
-                           D.kstart, D.sync, D.ll, eb, D.status, st);
+                           D.kstart, D.sync + D.wide_par, D.sync + (D.wide_par ^ 1), D.ll, eb, D.status, st);
           });
+          D.wide_par ^= 1;
           if (sub == wsub - 1 || p == rounds - 1) {
             const int lo2 = std::min(w, (p + 1) * (nb - 1));
             const int Tj = (w - lo2 + 127) / 128, Ti = (int)((r - lo2 + 127) / 128);

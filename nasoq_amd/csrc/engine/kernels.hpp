@@ -51,7 +51,7 @@ constexpr int64_t LL_WORDS = 2 * 256 * 4 + 2 * 256 * 8 + 2 * 256 * 2 * 64;
 template <typename T>
 void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
                     int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
-                    uint64_t* ll, uint32_t ebase, int* status, hipStream_t st);
+                    int* sync_next, uint64_t* ll, uint32_t ebase, int* status, hipStream_t st);
 template <typename T>
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, T* valL,
                      const T* W, const int64_t* woff, const int* kpos, const int* kstart,
