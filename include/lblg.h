@@ -149,6 +149,24 @@ int lblg_solve_device(lblg_handle* h, int nrhs, const double* dB, int64_t ldb, d
 /* GMRES cycles run by the last solve (the reference's num_ref_iter). */
 int lblg_num_ref_iter(const lblg_handle* h);
 
+/* SolverSettings::compute_norms + backward_error (linear_solver_wrapper.cpp:1462-1498,
+ * norm_dense / norm_sparse_int of src/common/Norm.cpp) for caller-given
+ * solutions X of B, recomputed on the device with the unperturbed K0 of the
+ * last factorization (set_diags(0)).  B and X are column-major host arrays in
+ * the caller's ordering, 1 <= nrhs <= 8.  norms[4 c + 0..3] receive res_l1,
+ * x_l1, rhs_l1 and bwd_err = res_l1 / (A_l1 x_l1 + rhs_l1) of column c; A_l1
+ * is lblg_factor_info's k0_norm_inf.  Not for batch handles (see
+ * lblg_batch_backward_errors). */
+int lblg_residual_norms(lblg_handle* h, int nrhs, const double* B, int64_t ldb, const double* X, int64_t ldx,
+                        double* norms);
+
+/* The same backward errors with B, X and d_err (nrhs doubles) in device
+ * memory, enqueued on the handle's stream without host synchronisation
+ * (lblg_synchronize or the stream orders the result).  Partial sums go to a
+ * buffer the handle allocates on the first call. */
+int lblg_backward_error_device(lblg_handle* h, int nrhs, const double* dB, int64_t ldb, const double* dX,
+                               int64_t ldx, double* d_err);
+
 /* Device schedule of a handle (engine extension, no reference counterpart:
  * the reference's schedule is fixed by its cost model).  Defaults come from
  * LBLG_GROUP_W, LBLG_CHAIN_W, LBLG_SMALL_SOLVE_NNZ and LBLG_SCHUR_BIG_M. */
@@ -371,6 +389,10 @@ int64_t lblg_qp_super_kkt(const lblg_qp* qp, int* p, int* i, double* x);
 int lblg_kkt_analyze(lblg_qp* qp, const lblg_settings* s);
 /* Order of the super-KKT (n + me + mi), or negative before lblg_kkt_analyze. */
 int64_t lblg_kkt_size(const lblg_qp* qp);
+/* The engine handle of the analysed super-KKT (NULL before lblg_kkt_analyze),
+ * owned by qp: the symbolic introspection, lblg_get_factor, lblg_inertia and
+ * lblg_residual_norms apply to it (the reference's L, valL, d_val of mode 1). */
+lblg_handle* lblg_kkt_handle(lblg_qp* qp);
 /* Activate (add = 1) or deactivate (add = 0) inequality rows (add_del_matrix_qp). */
 int lblg_kkt_edit(lblg_qp* qp, int add, int count, const int* rows);
 /* Factor the super-KKT with the current active rows (update_factorization /

@@ -103,6 +103,11 @@ SIGNATURES = [
     ("lblg_solve", ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int64, _f64p, ctypes.c_int64, _f64p]),
     ("lblg_solve_device", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _f64p]),
     ("lblg_num_ref_iter", ctypes.c_int, [_vp]),
+    ("lblg_residual_norms", ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int64, _f64p, ctypes.c_int64,
+                                            _f64p]),
+    ("lblg_backward_error_device", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64,
+                                                   _vp]),
+    ("lblg_kkt_handle", _vp, [_vp]),
     ("lblg_synchronize", ctypes.c_int, [_vp]),
     ("lblg_stream", _vp, [_vp]),
     ("lblg_get_factor", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, _i32p]),

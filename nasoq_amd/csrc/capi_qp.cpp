@@ -248,6 +248,13 @@ extern "C" int64_t lblg_kkt_size(const lblg_qp* qp) {
   return s->SM.ncol;
 }
 
+// the engine handle of the super-KKT (symbolic introspection, factor copies,
+// residual norms of the mode-1 drop-in)
+extern "C" lblg_handle* lblg_kkt_handle(lblg_qp* qp) {
+  QpSolver* s = reinterpret_cast<QpSolver*>(qp);
+  return s && s->analysed ? s->h : nullptr;
+}
+
 extern "C" int lblg_kkt_edit(lblg_qp* qp, int add, int count, const int* rows) {
   QpSolver* s = reinterpret_cast<QpSolver*>(qp);
   if (!s || count < 0 || (count > 0 && !rows)) return LBLG_ERR_ARG;

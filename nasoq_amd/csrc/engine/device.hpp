@@ -149,6 +149,10 @@ struct DeviceFactor {
   int ss_nlev = 0;
   int64_t ss_nnzL = 0, ss_max_nnz = 0;
   double *stage_b = nullptr, *stage_x = nullptr;  // host-pointer solves: staged blocks (n x nrhs_cap)
+  // residual norms of caller-given solutions (residual.hip), allocated at first use
+  double* res_part = nullptr;   // per-workgroup partial sums
+  double* res_out = nullptr;    // final sums and errors
+  double* res_bx = nullptr;     // staged B and X of the host-pointer call (n x MAX_RHS each)
   uint64_t* chain_xb = nullptr;     // flag-in-word exchange of the chain sweeps (epoch-stamped words)
   uint32_t chain_epoch = 0;
   int* piece_sn = nullptr;      // piece -> supernode

@@ -1033,6 +1033,9 @@ void solve(Handle& h, int nrhs, const double* B, int64_t ldb, double* X, int64_t
   }
   const int64_t lb = device_ptrs ? ldb : D.n, lx = device_ptrs ? ldx : D.n;
   double* xo = D.acc2;
+  // status word 0 reports this solve's chain-sweep time-outs only (the
+  // factor's flags were read back when it ended)
+  launch_fill_int(D.status, 1, 0, st);
   const bool to_target = s.target_bwd_err > 0;
   const int fixed_cycles = to_target ? 0 : s.req_ref_iter;
   // every launch below is fixed by (nrhs, refinement budget and tolerances,

@@ -260,13 +260,31 @@ def _numeric_methods():
         del xs
         return valL, d_val, rows[:ss], perm
 
+    def residual_norms(self, rhs, x):
+        """(res_l1, x_l1, rhs_l1, bwd_err) per column of the caller's x for rhs,
+        recomputed on the device with the unperturbed K0 of the last
+        factorization (SolverSettings::compute_norms / backward_error,
+        linear_solver_wrapper.cpp:1462-1498); an (nrhs, 4) array."""
+        B = np.asfortranarray(np.asarray(rhs, dtype=np.float64).reshape(self.n, -1))
+        X = np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(self.n, -1))
+        nrhs = B.shape[1]
+        out = np.zeros((nrhs, 4), dtype=np.float64)
+        check(self.lib.lblg_residual_norms(self.h, nrhs, dptr(B), self.n, dptr(X), self.n, dptr(out)),
+              "lblg_residual_norms")
+        return out
+
+    def backward_error(self, rhs, x):
+        """Backward error of x for rhs (one column: a float; several: an array)."""
+        e = self.residual_norms(rhs, x)[:, 3]
+        return float(e[0]) if np.asarray(rhs).ndim == 1 else e
+
     def factor_info(self):
         z, nrm, sec = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         check(self.lib.lblg_factor_info(self.h, ctypes.byref(z), ctypes.byref(nrm), ctypes.byref(sec)), "info")
         return {"zero_pivot": z.value, "k0_norm_inf": nrm.value, "factor_seconds": sec.value}
 
     for f in (numerical_factorization, _push_settings, refactor_resident, solve_only, set_values, graph_stats,
-              tuning, set_tuning, autotune, inertia, get_factor, factor_info):
+              tuning, set_tuning, autotune, inertia, get_factor, residual_norms, backward_error, factor_info):
         setattr(SolverSettings, f.__name__, f)
 
 

@@ -317,3 +317,35 @@ def test_small_sweeps_match_level_sweeps(oracle, prec, ss_mode, monkeypatch):
                 assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
         print(f"eta small {eta['small'].max():.2e} levels {eta['levels'].max():.2e}", flush=True)
         assert eta["small"].max() <= 4 * eta["levels"].max() + 1e-15
+
+
+def test_residual_norms_recomputed(oracle):
+    # SolverSettings::compute_norms / backward_error (linear_solver_wrapper.cpp:
+    # 1462-1498) of caller-given x, recomputed on the device: equal to the
+    # oracle's backward error for the refined solution and for an edited one,
+    # and the device-pointer entry point agrees bit for bit with the host one
+    import ctypes
+    import torch
+    n, Ap, Ai, Ax, B = P.kkt_config(8, 40, 6, seed=5, nrhs=3)
+    print(f"case residual_norms n={n}", flush=True)
+    s = engine(n, Ap, Ai, Ax)
+    X = s.solve_only(B)
+    Xe = X.copy()
+    Xe[::7, 1] += 1e-3  # an edited solution
+    for XX in (X, Xe):
+        nm = s.residual_norms(B, XX)
+        for c in range(B.shape[1]):
+            want = oracle.backward_error(n, Ap, Ai, Ax, XX[:, c], B[:, c])
+            assert abs(nm[c, 3] - want) <= 1e-12 * want + 1e-300, (c, nm[c, 3], want)
+            assert abs(nm[c, 1] - np.abs(XX[:, c]).sum()) <= 1e-13 * nm[c, 1]
+            assert abs(nm[c, 2] - np.abs(B[:, c]).sum()) <= 1e-13 * nm[c, 2]
+        dB = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()  # column-major n x nrhs
+        dX = torch.from_numpy(np.ascontiguousarray(XX.T)).cuda()
+        dE = torch.zeros(B.shape[1], dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        rc = s.lib.lblg_backward_error_device(s.h, B.shape[1], ctypes.c_void_p(dB.data_ptr()), n,
+                                              ctypes.c_void_p(dX.data_ptr()), n, ctypes.c_void_p(dE.data_ptr()))
+        assert rc == 0
+        assert s.lib.lblg_synchronize(s.h) == 0
+        assert np.array_equal(dE.cpu().numpy(), nm[:, 3])
+    s.close()

@@ -130,6 +130,16 @@ def test_cpp_dropin_nasoq_main(tmp_path):
     assert out.returncode == 0, out.stdout + out.stderr
 
 
+def test_cpp_dropin_readable_state(tmp_path):
+    # the reference's readable SolverSettings state (L, valL, d_val, A_ord,
+    # AT_ord, perm_piv, level sets) is consistent: L D L' rebuilt from it equals
+    # the perturbed A_ord, and backward_error() recomputes the residual of an
+    # edited x (linear_solver_wrapper.cpp:1432-1498)
+    import subprocess
+    out = subprocess.run([_build_cpp("lbl_state_main", tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), (out.returncode, out.stdout + out.stderr)
+
+
 @pytest.mark.parametrize("zero", [True, False])
 def test_cpp_dropin_somod(tmp_path, zero):
     # examples/lbl_somod_main.cpp: with zero right-hand sides for the added
