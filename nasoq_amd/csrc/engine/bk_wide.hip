@@ -12,8 +12,8 @@
 // and the supernode's off-diagonal rows [w, r), which get their L in the same
 // steps (so no separate off-diagonal solve is needed) but are never pivot
 // candidates.  G workgroups each own up to WCH far rows (one per thread of
-// waves 0-2) and keep those rows of the panel (L for eliminated columns, A
-// for pending ones) in LDS.  Every
+// every wave but the last) and keep those rows of the panel (L for
+// eliminated columns, A for pending ones) in LDS.  Every
 // workgroup also keeps the diagonal rows (nb x nb) and computes their
 // updated values itself, so the W row needed by each column step is always
 // local.
@@ -44,12 +44,9 @@
 
 namespace lblg {
 
-#define WCH 192                        // far rows per workgroup
 #define WNB 64                         // widest panel
 #define WMM (2 * WNB)                  // moved indices per panel
-#define WLDA (WNB + 1)                 // padded row length of the diagonal rows
 #define OSB 16                         // columns per sub-block of the optimistic pass
-#define MBITS_WORDS (256 * WCH / 32)   // moved-far-row bitmap
 #define LL_SPIN (1L << 22)
 
 // exchange buffer layout, in 64-bit words (see kernels.hpp: LL_WORDS)
@@ -62,13 +59,32 @@ static_assert(LL_PAY + 2 * 256 * 2 * WNB == LL_WORDS, "exchange buffer size");
 __device__ inline void ll_store(uint64_t* p, uint32_t v, uint32_t flag) {
   __hip_atomic_store(p, ((uint64_t)flag << 32) | (uint64_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The first wait of a factorization that runs out records where: status[8]
+// = site + 16 x (word index - ll), status[9] = the waiting workgroup,
+// status[10] = the flag it waited for, status[11] = the flag it last saw.
+struct LlDiag {
+  int* status;
+  int site;
+  int g;
+  const uint64_t* base;
+};
+__device__ inline void ll_note(const LlDiag& d, const uint64_t* p, uint32_t flag, uint32_t seen) {
+  if (atomicCAS(d.status + 8, 0, d.site + 16 * (int)(p - d.base)) == 0) {
+    d.status[9] = d.g;
+    d.status[10] = (int)flag;
+    d.status[11] = (int)seen;
+  }
+}
 // wait (bounded) for the word to carry this epoch's flag; abort != 0 skips the wait
-__device__ inline uint32_t ll_load(uint64_t* p, uint32_t flag, int& abort) {
+__device__ inline uint32_t ll_load(uint64_t* p, uint32_t flag, int& abort, const LlDiag& d) {
   long spins = 0;
   while (!abort) {
     const uint64_t x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((uint32_t)(x >> 32) == flag) return (uint32_t)x;
-    if (++spins > LL_SPIN) abort = 1;
+    if (++spins > LL_SPIN) {
+      abort = 1;
+      ll_note(d, p, flag, (uint32_t)(x >> 32));
+    }
   }
   return 0u;
 }
@@ -77,9 +93,9 @@ __device__ inline void ll_store_d(uint64_t* p, double v, uint32_t flag) {
   ll_store(p, (uint32_t)b, flag);
   ll_store(p + 1, (uint32_t)(b >> 32), flag);
 }
-__device__ inline double ll_load_d(uint64_t* p, uint32_t flag, int& abort) {
-  const uint64_t lo = ll_load(p, flag, abort);
-  const uint64_t hi = ll_load(p + 1, flag, abort);
+__device__ inline double ll_load_d(uint64_t* p, uint32_t flag, int& abort, const LlDiag& d) {
+  const uint64_t lo = ll_load(p, flag, abort, d);
+  const uint64_t hi = ll_load(p + 1, flag, abort, d);
   return __longlong_as_double((long long)((hi << 32) | lo));
 }
 
@@ -100,21 +116,32 @@ __device__ inline int far_phys(int a, int fstart, const unsigned* mbits, const i
   return a;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb, int sub, int wcols, const int* super,
+// NT threads per workgroup: waves 0 .. NT/64 - 2 own one far row per
+// thread (WCH = NT - 64 rows per workgroup), the last wave computes the
+// diagonal rows; NBM is the widest panel the launch takes.  256 threads with
+// 64-column panels keep 192 far rows per workgroup (G <= 256: 49,152 rows);
+// 512 threads keep 448 (114,688 rows) with 32-column FP64 or 64-column FP32
+// panels, whose LDS rows are the same 256 bytes.
+template <typename T, int NT, int NBM>
+__global__ __launch_bounds__(NT) void k_bk_wide(const int* list, int G, int nb, int sub, int wcols, const int* super,
                                                  const int64_t* pi, const int64_t* px, T* valL,
                                                  T* dval, int* perm_piv, T* Wbase,
                                                  const int64_t* woff, int* kpos, int* kstart, int* sync,
                                                  int* sync_next, uint64_t* ll, uint32_t ebase, int n,
                                                  int* status) {
-  __shared__ T LC[WNB * WCH];       // own far rows, column-major [column][row]
-  __shared__ T Ad[WNB * WLDA];      // diagonal rows: L (eliminated) | symmetric A (pending)
-  __shared__ T wk[WNB], wi[WNB], td0[WNB], td1[WNB], dd[WNB], dsb[WNB], prow[WNB];
-  __shared__ int dphys[WNB];
+  constexpr int NW = NT / 64;                   // waves; wave NW - 1 holds the diagonal rows
+  constexpr int WCH = NT - 64;                  // far rows per workgroup
+  constexpr int WLDA = NBM + 1;                 // padded row length of the diagonal rows
+  constexpr int MBITS_WORDS = 256 * WCH / 32;   // moved-far-row bitmap
+  static_assert(NBM <= WNB && NT % 64 == 0 && NT >= 256, "wide panel launch shape");
+  __shared__ T LC[NBM * WCH];       // own far rows, column-major [column][row]
+  __shared__ T Ad[NBM * WLDA];      // diagonal rows: L (eliminated) | symmetric A (pending)
+  __shared__ T wk[NBM], wi[NBM], td0[NBM], td1[NBM], dd[NBM], dsb[NBM], prow[NBM];
+  __shared__ int dphys[NBM];
   __shared__ int mlog[WMM], mphys[WMM], Ml[WMM], Mp[WMM];
   __shared__ unsigned mbits[MBITS_WORDS];
-  __shared__ double redv[4];
-  __shared__ int redi[4];
+  __shared__ double redv[NW];
+  __shared__ int redi[NW];
   __shared__ double bcd[4];
   __shared__ int bci[4];
   __shared__ int s_nm, s_abort, s_nM, s_nMa, s_fail;
@@ -146,7 +173,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   // take part in every update and get their L here, but are never pivots)
   const int fstart = k0 + nbe, nfar = (int)(r - fstart);
   const int chunk = nfar > 0 ? (nfar + G - 1) / G : 0;
-  if (chunk > WCH || nb > WNB || off < 0 || off + nb > wcols) {  // uniform over the grid: every workgroup returns
+  if (chunk > WCH || nb > NBM || off < 0 || off + nb > wcols) {  // uniform over the grid: every workgroup returns
     if (g == 0 && t == 0) atomicOr(status, 8);
     return;
   }
@@ -166,7 +193,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   // per batch rather than per entry)
   if (hasrow) {
 #pragma unroll
-    for (int j0 = 0; j0 < WNB; j0 += 16) {
+    for (int j0 = 0; j0 < NBM; j0 += 16) {
       T v[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = (j0 + q < nbe) ? pcol(fr, j0 + q) : (T)0;
@@ -176,25 +203,25 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
     }
   }
   {
-    constexpr int Q = WNB * WNB / 256;
+    constexpr int Q = NBM * NBM / NT;
     T v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int idx = t + 256 * q, i = idx % nbe, j = idx / nbe;
+      const int idx = t + NT * q, i = idx % nbe, j = idx / nbe;
       v[q] = idx < nbe * nbe ? ((i >= j) ? pcol(k0 + i, j) : pcol(k0 + j, i)) : (T)0;
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int idx = t + 256 * q, i = idx % nbe, j = idx / nbe;
+      const int idx = t + NT * q, i = idx % nbe, j = idx / nbe;
       if (idx < nbe * nbe) ADR(i, j) = v[q];
     }
   }
-  if (t < WNB) {
+  if (t < NBM) {
     dphys[t] = k0 + t;
     dd[t] = 0.0;
     dsb[t] = 0.0;
   }
-  for (int q = t; q < MBITS_WORDS; q += 256) mbits[q] = 0u;
+  for (int q = t; q < MBITS_WORDS; q += NT) mbits[q] = 0u;
   if (t == 0) {
     s_nm = 0;
     s_abort = 0;
@@ -227,7 +254,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
         __syncthreads();
         const int nft = (fcnt + 15) >> 4;      // far-row tiles (rows < WCH)
         const int ndt = (nbe - cb + 15) >> 4;  // diagonal-row tiles, rows [cb, 64)
-        for (int tile = wv; tile < nft + ndt; tile += 4) {
+        for (int tile = wv; tile < nft + ndt; tile += NW) {
           const bool isfar = tile < nft;
           const int i0 = isfar ? tile * 16 : cb + (tile - nft) * 16;
           typename Mfma<T>::acc acc = {0, 0, 0, 0};
@@ -269,7 +296,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
         const double d = __shfl(vd, kr, 64);
         const double absakk = fabs(d);
         if (!(absakk > 0.0)) myfail = min(myfail, kr);
-        if (wv == 3 && drow && ln > kr) {
+        if (wv == NW - 1 && drow && ln > kr) {
           const double x = fabs(vd);
           if (x == x && !(absakk >= BK_ALPHA * x)) myfail = min(myfail, kr);
         }
@@ -296,7 +323,9 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       int abort = 0;
       if (ln == 0) ll_store(ll + LL_RECA + (256 + (int64_t)g) * 4, (uint32_t)s_fail, fO);
       int jm = nbe;
-      for (int q = ln; q < G; q += 64) jm = min(jm, (int)ll_load(ll + LL_RECA + (256 + (int64_t)q) * 4, fO, abort));
+      const LlDiag dg{status, 1, g, ll};
+      for (int q = ln; q < G; q += 64)
+        jm = min(jm, (int)ll_load(ll + LL_RECA + (256 + (int64_t)q) * 4, fO, abort, dg));
       for (int o = 32; o > 0; o >>= 1) jm = min(jm, __shfl_xor(jm, o, 64));
       const bool any_abort = __ballot(abort) != 0;
       if (ln == 0) {
@@ -311,7 +340,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   // accepted columns, the original columns from J on
   if (J < nbe && hasrow)
     for (int j = J; j < nbe; ++j) LCR(j) = pcol(fr, j);
-  for (int idx = t; idx < nbe * nbe; idx += 256) {
+  for (int idx = t; idx < nbe * nbe; idx += NT) {
     const int i = idx % nbe, j = idx / nbe;
     if (min(i, j) >= J) ADR(i, j) = i >= j ? pcol(k0 + i, j) : pcol(k0 + j, i);
     else if (i > j) ADR(i, j) = ADR(j, i) * wk[j];
@@ -342,7 +371,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       for (int j = 0; j < kr; ++j) v -= LCR(j) * wk[j];
       v0 = v;
     }
-    if (wv == 3 && ln >= kr && ln < nbe) {
+    if (wv == NW - 1 && ln >= kr && ln < nbe) {
       double v = ADR(ln, kr);
       for (int j = 0; j < kr; ++j) v -= ADR(ln, j) * wk[j];
       td0[ln] = v;
@@ -355,7 +384,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       ai = fr;
     }
     __syncthreads();
-    if (wv == 3 && ln > kr && ln < nbe) {
+    if (wv == NW - 1 && ln > kr && ln < nbe) {
       av = fabs(td0[ln]);
       if (!(av == av)) av = -1.0;
       ai = k0 + ln;
@@ -367,7 +396,9 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       uint64_t* rec = ll + LL_RECA + ((int64_t)par * 256 + g) * 4;
       const bool cand = av >= 0.0;
       const bool own_far = cand && hasrow && ai == fr;
-      const bool own_diag = cand && wv == 3 && ai == k0 + ln;
+      // lanes of the last wave from nbe on hold no diagonal row (the far rows
+      // k0 + nbe .. k0 + 63 must not match them)
+      const bool own_diag = cand && wv == NW - 1 && ln < nbe && ai == k0 + ln;
       if (own_far || own_diag) {
         ll_store_d(rec, own_far ? v0 : td0[ln], fA);
         ll_store(rec + 2, (uint32_t)ai, fA);
@@ -385,8 +416,8 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       int bi = 0x7fffffff, bg = 0;
       for (int q = ln; q < G; q += 64) {
         uint64_t* rec = ll + LL_RECA + ((int64_t)par * 256 + q) * 4;
-        const double v = ll_load_d(rec, fA, abort);
-        const int idx = (int)ll_load(rec + 2, fA, abort);
+        const double v = ll_load_d(rec, fA, abort, LlDiag{status, 2, g, ll});
+        const int idx = (int)ll_load(rec + 2, fA, abort, LlDiag{status, 3, g, ll});
         double x = fabs(v);
         if (!(x == x) || idx == 0x7fffffff) x = -1.0;
         if (x > ba || (x == ba && idx < bi)) {
@@ -438,7 +469,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
         if (t < nbe) prow[t] = ADR(imr, t);
       } else if (wv == 0) {
         uint64_t* pay = ll + LL_PAY + ((int64_t)par * 256 + gwin) * 2 * WNB;
-        for (int j = ln; j < nbe; j += 64) prow[j] = ll_load_d(pay + 2 * j, fA, abort);
+        for (int j = ln; j < nbe; j += 64) prow[j] = ll_load_d(pay + 2 * j, fA, abort, LlDiag{status, 4, g, ll});
       }
       __syncthreads();
       if (t < kr) wi[t] = w_entry([&](int j) { return prow[j]; }, t, dd, dsb);
@@ -452,7 +483,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
         for (int j = 0; j < kr; ++j) v -= LCR(j) * wi[j];
         v1 = v;
       }
-      if (wv == 3 && ln >= kr && ln < nbe) {
+      if (wv == NW - 1 && ln >= kr && ln < nbe) {
         double v = im_diag ? ADR(ln, imr) : prow[ln];
         for (int j = 0; j < kr; ++j) v -= ADR(ln, j) * wi[j];
         td1[ln] = v;
@@ -465,7 +496,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
         bi2 = fr;
       }
       __syncthreads();
-      if (wv == 3 && ln >= kr && ln < nbe && k0 + ln != imax) {
+      if (wv == NW - 1 && ln >= kr && ln < nbe && k0 + ln != imax) {
         bv2 = fabs(td1[ln]);
         if (!(bv2 == bv2)) bv2 = -1.0;
         bi2 = k0 + ln;
@@ -479,12 +510,13 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
       if (wv == 0) {
         double rm = -1.0;
         for (int q = ln; q < G; q += 64) {
-          const double x = ll_load_d(ll + LL_RECB + ((int64_t)par * 256 + q) * 8, fB, abort);
+          const double x = ll_load_d(ll + LL_RECB + ((int64_t)par * 256 + q) * 8, fB, abort, LlDiag{status, 5, g, ll});
           rm = fmax(rm, x);
         }
         for (int off = 32; off > 0; off >>= 1) rm = fmax(rm, __shfl_xor(rm, off, 64));
         double ti = 0.0;
-        if (!im_diag && ln == 0) ti = ll_load_d(ll + LL_RECB + ((int64_t)par * 256 + gwin) * 8 + 4, fB, abort);
+        if (!im_diag && ln == 0)
+          ti = ll_load_d(ll + LL_RECB + ((int64_t)par * 256 + gwin) * 8 + 4, fB, abort, LlDiag{status, 6, g, ll});
         const bool any_abort = __ballot(abort) != 0;
         if (ln == 0) {
           bcd[2] = rm;
@@ -656,7 +688,7 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
   }
   __syncthreads();
   const int nM = s_nM, nMa = s_nMa;
-  const int GT = G * 256, gt = g * 256 + t;
+  const int GT = G * NT, gt = g * NT + t;
   const int nt = w - kend;                 // trailing block size
   const int64_t no = r - w;                // rows below the diagonal block
   T* S1 = scr;                                  // [nMa][nt]
@@ -731,21 +763,28 @@ __global__ __launch_bounds__(256) void k_bk_wide(const int* list, int G, int nb,
 }
 
 template <typename T>
-void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
+void launch_bk_wide(const int* list, int G, int nt, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
                     int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                     int* sync_next, uint64_t* ll, uint32_t ebase, int* status, hipStream_t st) {
-  if (nb > WNB || nb < 4 || G < 1 || G > 256) throw std::runtime_error("bk_wide: launch shape out of range");
+  if (nb > wide_panel_nb(nt, sizeof(T)) || nb < 4 || G < 1 || G > 256 || (nt != 256 && nt != WIDE_NT_TALL))
+    throw std::runtime_error("bk_wide: launch shape out of range");
   int n = a.n;
   void* args[] = {(void*)&list, (void*)&G, (void*)&nb, (void*)&sub, (void*)&wcols, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
                   (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
                   (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&sync_next, (void*)&ll, (void*)&ebase, (void*)&n,
                   (void*)&status};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_wide<T>, dim3(G), dim3(256), args, 0, st);
+  // tall supernodes: 448 far rows per workgroup in the same 256-byte LDS rows
+  // (32 FP64 or 64 FP32 columns)
+  constexpr int NBT = sizeof(T) == 8 ? 32 : 64;
+  const void* fn = nt == 256 ? (const void*)k_bk_wide<T, 256, 64> : (const void*)k_bk_wide<T, WIDE_NT_TALL, NBT>;
+  hipError_t e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(nt), args, 0, st);
   if (e != hipSuccess) throw std::runtime_error(std::string("bk_wide: ") + hipGetErrorString(e));
 }
-template void launch_bk_wide<double>(const int*, int, int, int, int, const SnArgs&, double*, double*, int*, double*,
-                                     const int64_t*, int*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
-template void launch_bk_wide<float>(const int*, int, int, int, int, const SnArgs&, float*, float*, int*, float*,
-                                    const int64_t*, int*, int*, int*, int*, uint64_t*, uint32_t, int*, hipStream_t);
+template void launch_bk_wide<double>(const int*, int, int, int, int, int, const SnArgs&, double*, double*, int*,
+                                     double*, const int64_t*, int*, int*, int*, int*, uint64_t*, uint32_t, int*,
+                                     hipStream_t);
+template void launch_bk_wide<float>(const int*, int, int, int, int, int, const SnArgs&, float*, float*, int*,
+                                    float*, const int64_t*, int*, int*, int*, int*, uint64_t*, uint32_t, int*,
+                                    hipStream_t);
 
 }  // namespace lblg

@@ -52,6 +52,7 @@ struct FactorLevel {
   int* d_medium = nullptr;
   int* d_large = nullptr;
   std::vector<int> large_G, large_nb;  // workgroups and panel width per large supernode
+  std::vector<int> large_nt, large_wsub;  // workgroup size and panels per trailing update
 };
 
 // solve pieces: supernodes split into column blocks of at most SOLVE_W

@@ -286,6 +286,9 @@ static void setup(Handle& h) {
   std::vector<std::vector<int>> bylev(nlev);
   for (int s = 0; s < ns; ++s) bylev[lev[s]].push_back(s);
   const int group_w = h.tune.group_w;
+  // LBLG_WIDE_TALL=1: every cooperative panel takes the tall launch shape
+  // (tests run it at small sizes)
+  const bool wide_tall = env_int("LBLG_WIDE_TALL", 0) != 0;
   int64_t ld_max = 0;
   int64_t wmax_total = 0;
   std::vector<int64_t> woff(ns, 0);
@@ -310,16 +313,30 @@ static void setup(Handle& h) {
           wsum += (int64_t)w * PANEL_NB + 2 * (int64_t)w + 16;
         } else {
           // cooperative panel over G workgroups, <= WIDE_ROWS far rows each
-          const int nb = WIDE_NB;
-          // far rows: the diagonal block below the panel and the off-diagonal rows
-          const int G = std::max(1, (int)((r - nb + WIDE_ROWS - 1) / WIDE_ROWS));
-          if (G > 256) throw std::runtime_error("supernode too tall for the cooperative panel kernel");
-          const int64_t wcols = (int64_t)h.tune.wide_sub * nb;
+          // (the diagonal block below the panel and the off-diagonal rows);
+          // beyond 256 workgroups (49,152 rows), 512-thread workgroups with
+          // WIDE_ROWS_TALL far rows each and 256-byte panel rows
+          int nt = 256, nb = WIDE_NB, rows = WIDE_ROWS;
+          int64_t G = std::max<int64_t>(1, (r - nb + rows - 1) / rows);
+          if (G > 256 || wide_tall) {
+            nt = WIDE_NT_TALL;
+            nb = wide_panel_nb(nt, D.prec == 1 ? sizeof(float) : sizeof(double));
+            rows = WIDE_ROWS_TALL;
+            G = std::max<int64_t>(1, (r - nb + rows - 1) / rows);
+          }
+          if (G > 256)
+            throw std::runtime_error("supernode too tall for the cooperative panel kernel (more than 256 x " +
+                                     std::to_string(WIDE_ROWS_TALL) + " rows)");
+          // trailing updates of rank wide_sub x 64 whatever the panel width
+          const int wsub = std::min(16, h.tune.wide_sub * WIDE_NB / nb);
+          const int64_t wcols = (int64_t)wsub * nb;
           wsum += (int64_t)w * wcols + r * nb + (int64_t)WIDE_MOVED * (w + r + wcols) + 64;
           F.large.push_back(s);
           D.has_coop = true;
-          F.large_G.push_back(G);
+          F.large_G.push_back((int)G);
           F.large_nb.push_back(nb);
+          F.large_nt.push_back(nt);
+          F.large_wsub.push_back(wsub);
         }
       }
       if (r > w) {
@@ -665,9 +682,10 @@ static void factor_device_t(Handle& h) {
         const int s = F.large[q];
         const int w = S.super[s + 1] - S.super[s];
         const int64_t r = S.pi[s + 1] - S.pi[s];
-        const int G = F.large_G[q], nb = F.large_nb[q];
+        const int G = F.large_G[q], nb = F.large_nb[q], nt = F.large_nt[q];
+        const int rows = nt - 64;  // far rows per workgroup
         const int rounds = (w + nb - 2) / (nb - 1) + 1;
-        const int wsub = h.tune.wide_sub;
+        const int wsub = F.large_wsub[q];
         const int wcols = wsub * nb;
         for (int p = 0; p < rounds; ++p) {
           // panels run in blocks of wsub: before panel p of a block, its columns
@@ -686,11 +704,11 @@ static void factor_device_t(Handle& h) {
           // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
           // panel eliminates >= nb - 1 columns); the last panel may be narrower
           const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
-          const int Gp = std::min(G, std::max(1, (far + WIDE_ROWS - 1) / WIDE_ROWS));
+          const int Gp = std::min(G, std::max(1, (far + rows - 1) / rows));
           const uint32_t eb = D.ll_epoch;
           D.ll_epoch += 3u * (uint32_t)nb + 8u;
           timed(D, "bk_panel_group", [&] {
-            launch_bk_wide(F.d_large + q, Gp, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
+            launch_bk_wide(F.d_large + q, Gp, nt, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
                            D.kstart, D.sync + D.wide_par, D.sync + (D.wide_par ^ 1), D.ll, eb, D.status, st);
           });
           D.wide_par ^= 1;
@@ -734,7 +752,7 @@ static void factor_device_t(Handle& h) {
   } else {
     run_graph(D, D.gfactor, graph_key({(double)D.prec, reg, reg_factor, (double)h.set.regularization}, {}), seq);
   }
-  int stat[6] = {0, 0, 0, 0, 0, 0};
+  int stat[12] = {0};
   HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(stat), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -743,7 +761,16 @@ static void factor_device_t(Handle& h) {
   h.interchanges = stat[3];
   h.wide_panels = stat[4];
   h.wide_panels_pivoted = stat[5];
-  if (stat[0] & 4) throw std::runtime_error("cross-workgroup exchange timed out");
+  if (stat[0] & 4) {
+    // status[8..11]: the first exchange wait that ran out (bk_wide.hip, LlDiag);
+    // zero when a group barrier ran out instead
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "cross-workgroup exchange timed out (wait site %d at word %d, workgroup %d, waited for flag %u, "
+                  "saw %u, epoch base now %u)",
+                  stat[8] & 15, stat[8] >> 4, stat[9], (unsigned)stat[10], (unsigned)stat[11], D.ll_epoch);
+    throw std::runtime_error(msg);
+  }
   if (stat[0] & 8) throw std::runtime_error("wide panel launch shape does not fit the supernode");
   h.factored = true;
 }

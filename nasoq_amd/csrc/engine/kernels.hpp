@@ -45,11 +45,16 @@ void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL
                      int* status, bool cooperative, hipStream_t st);
 // wide panels with flag-in-word exchange (bk_wide.hip)
 constexpr int WIDE_NB = 64;         // panel width
-constexpr int WIDE_ROWS = 192;      // far rows per workgroup
+constexpr int WIDE_ROWS = 192;      // far rows per workgroup (256-thread workgroups)
+constexpr int WIDE_NT_TALL = 512;   // workgroup size for supernodes beyond 256 x WIDE_ROWS far rows
+constexpr int WIDE_ROWS_TALL = WIDE_NT_TALL - 64;  // their far rows per workgroup
 constexpr int WIDE_MOVED = 128;     // moved indices per panel (scratch rows)
+// widest panel of a launch shape: 64 columns, or for the tall shape the
+// columns of a 256-byte LDS row (32 FP64, 64 FP32)
+inline int wide_panel_nb(int nt, size_t value_size) { return nt == 256 ? 64 : (value_size == 8 ? 32 : 64); }
 constexpr int64_t LL_WORDS = 2 * 256 * 4 + 2 * 256 * 8 + 2 * 256 * 2 * 64;
 template <typename T>
-void launch_bk_wide(const int* list, int G, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
+void launch_bk_wide(const int* list, int G, int nt, int nb, int sub, int wcols, const SnArgs& a, T* valL, T* dval,
                     int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
                     int* sync_next, uint64_t* ll, uint32_t ebase, int* status, hipStream_t st);
 template <typename T>

@@ -96,6 +96,42 @@ def kkt_small(n_h, m, seed=0, density=0.05, nrhs=1, shift=1.0):
     return lower_csc(K) + (B,)
 
 
+def tall_border(n1, m2, chains=2, seed=0, nrhs=2):
+    """Quasi-definite [T B'; B -E] in natural order: T = `chains` independent
+    dense SPD blocks of c = n1 / chains columns (T_ij = 1/2 off the diagonal,
+    c/2 + 1 on it), each one fundamental supernode c wide with c + m2 rows
+    (relaxed amalgamation off); B a dense m2 x n1 N(0,1) block; E =
+    diag(U[1, 2]).  The m2 rows form one dense m2 x m2 clique, the blocks'
+    common parent.  Inertia (n1, m2, 0).  Built directly as lower CSC."""
+    rng = np.random.default_rng(seed)
+    n = n1 + m2
+    c = n1 // chains
+    assert c * chains == n1
+    Bd = rng.standard_normal((m2, n1))
+    jj = np.arange(n1)
+    per = np.r_[(c - jj % c) + m2, np.ones(m2, dtype=np.int64)]
+    cnt = np.r_[0, np.cumsum(per)]
+    nnz = int(cnt[-1])
+    Ai = np.empty(nnz, dtype=np.int32)
+    Ax = np.empty(nnz, dtype=np.float64)
+    rows_b = np.arange(n1, n, dtype=np.int32)
+    for j in range(n1):
+        q = int(cnt[j])
+        e = (j // c + 1) * c  # end of j's block
+        k = e - j
+        Ai[q:q + k] = np.arange(j, e)
+        Ax[q:q + k] = 0.5
+        Ax[q] = c / 2.0 + 1.0
+        q += k
+        Ai[q:q + m2] = rows_b
+        Ax[q:q + m2] = Bd[:, j]
+    q0 = int(cnt[n1])
+    Ai[q0:] = rows_b
+    Ax[q0:] = -rng.uniform(1.0, 2.0, m2)
+    B = rng.uniform(-1.0, 1.0, size=(n, nrhs))
+    return n, cnt.astype(np.int32), Ai, Ax, B
+
+
 def checksum(*arrays):
     """Order-sensitive checksum of generated arrays (for recording inputs)."""
     import hashlib

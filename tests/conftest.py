@@ -4,6 +4,11 @@ import sys
 import numpy as np
 import pytest
 
+# torch's HIP runtime must be loaded before the engine library's when both
+# are used in one process (tests hand torch device tensors to the C-ABI):
+# with the engine's runtime initialised first, torch finds no HIP device
+import torch  # noqa: F401
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
