@@ -167,6 +167,18 @@ int lblg_residual_norms(lblg_handle* h, int nrhs, const double* B, int64_t ldb, 
 int lblg_backward_error_device(lblg_handle* h, int nrhs, const double* dB, int64_t ldb, const double* dX,
                                int64_t ldx, double* d_err);
 
+/* lblg_backward_error_device that allocates nothing (no hipMalloc, hipFree or
+ * allocating library call), for callers that cannot allocate: code under
+ * stream capture, serving loops, harnesses that forbid allocation.  No
+ * counterpart in the reference.  Its partial sums go to a fixed 192 KiB
+ * device array of the library (one per GPU, kept until the process ends),
+ * so it needs no per-handle buffer; limit: 1 <= nrhs <= 8 (LBLG_ERR_ARG
+ * beyond).  Every call on a GPU shares that array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be serialised by the
+ * caller. */
+int lblg_backward_error_device_noalloc(lblg_handle* h, int nrhs, const double* dB, int64_t ldb,
+                                       const double* dX, int64_t ldx, double* d_err);
+
 /* Device schedule of a handle (engine extension, no reference counterpart:
  * the reference's schedule is fixed by its cost model).  Defaults come from
  * LBLG_GROUP_W, LBLG_CHAIN_W, LBLG_SMALL_SOLVE_NNZ and LBLG_SCHUR_BIG_M. */

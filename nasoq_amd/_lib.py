@@ -108,6 +108,8 @@ SIGNATURES = [
     ("lblg_backward_error_device", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64,
                                                    _vp]),
     ("lblg_kkt_handle", _vp, [_vp]),
+    ("lblg_backward_error_device_noalloc", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp,
+                                                           ctypes.c_int64, _vp]),
     ("lblg_synchronize", ctypes.c_int, [_vp]),
     ("lblg_stream", _vp, [_vp]),
     ("lblg_get_factor", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, _i32p]),

@@ -201,3 +201,57 @@ extern "C" int lblg_residual_norms(lblg_handle* hh, int nrhs, const double* B, i
     return LBLG_OK;
   });
 }
+
+// ---------------------------------------------------------------------------
+// The same backward errors with no allocation at all: the partial sums go to
+// a file-scope device array (one copy per GPU, held until the process ends)
+// instead of a buffer the handle allocates.  hipGetSymbolAddress gives that
+// array's address on a device; the host never passes the array's own name to
+// a kernel (on the host it is a stand-in, not a device address).
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace lblg {
+static __device__ __align__(256) double scratch[RES_PART];  // 192 KiB per GPU
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[SCRATCH_DEVICES] = {nullptr};
+
+// the scratch array's address on device dev, looked up once per device with
+// that device current
+static double* scratch_on(int dev) {
+  if (dev < 0 || dev >= SCRATCH_DEVICES) throw std::runtime_error("no-alloc residual: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) throw std::runtime_error("no-alloc residual: hipGetDevice failed");
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) throw std::runtime_error("no-alloc residual: hipSetDevice failed");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("no-alloc residual: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  return g_scratch[dev];
+}
+}  // namespace lblg
+
+extern "C" int lblg_backward_error_device_noalloc(lblg_handle* hh, int nrhs, const double* dB, int64_t ldb,
+                                                  const double* dX, int64_t ldx, double* d_err) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (h && (nrhs < 1 || nrhs > lblg::MAX_RHS)) {
+    lblg::set_last_error("no-alloc residual: 1 <= nrhs <= 8 (the scratch array holds 1024 x 3 x 8 partial sums)");
+    return LBLG_ERR_ARG;
+  }
+  const int rc = lblg::residual_args(h, nrhs, dB, ldb, dX, ldx, d_err);
+  if (rc != LBLG_OK) return rc;
+  return lblg::guard([&] {
+    DeviceFactor& D = *h->dev;
+    int dev = -1;
+    if (hipStreamGetDevice(D.stream, &dev) != hipSuccess) throw std::runtime_error("no-alloc residual: stream device");
+    double* part = lblg::scratch_on(dev);
+    lblg::launch_residual(D, nrhs, dB, ldb, dX, ldx, part, nullptr, d_err, D.stream);
+    return LBLG_OK;
+  });
+}
