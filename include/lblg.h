@@ -55,9 +55,14 @@ typedef struct lblg_settings {
   int n_relax[3];         /* default {4,16,48} */
   double z_relax[3];      /* default {0.8,0.1,0.05} */
   int ordering;           /* LBLG_ORDER_* */
-  int precision;          /* factor storage and MFMA type: LBLG_F64 (plain), LBLG_F32
-                             (FP32 factor and sweeps; residual, GMRES and the
-                             solution update stay FP64); LBLG_BF16 is rejected */
+  int precision;          /* factor storage and MFMA type: LBLG_F64 (plain, the factor
+                             of K~); LBLG_F32 (FP32 factor and sweeps of S K S, S
+                             the diagonal scaling s_i = 1/sqrt(max_j |K_ij|) of
+                             every supernode's rows and columns, diagonals below
+                             2^-12 raised to it; residual, GMRES and the solution
+                             update stay FP64 on K); LBLG_BF16 (as LBLG_F32, with
+                             the large Schur and trailing updates on BF16 MFMA
+                             operands and FP32 accumulation) */
   double target_bwd_err;  /* >0: keep refining until the backward error is <= target
                              (at most max_refine_rounds GMRES cycles); 0: the reference's
                              fixed GMRES budget */
@@ -224,6 +229,11 @@ void* lblg_stream(lblg_handle* h);
  * the pivoted ordering), perm_final (n: Perm composed with the pivot
  * permutation, as L->Perm after reorder_matrix). */
 int lblg_get_factor(lblg_handle* h, double* valL, double* d_val, int* rows, int* perm_final);
+
+/* The diagonal scaling s (n, original ordering) of an FP32 or BF16 factor,
+ * which factors S K S with S = diag(s); all ones for FP64.
+ * Engine extension (the reference factors K~ itself). */
+int lblg_get_scaling(lblg_handle* h, double* s);
 
 /* Inertia (positive, negative, zero eigenvalue counts) of K~ from D.
  * The reference does not compute it; the count follows Sylvester's law. */

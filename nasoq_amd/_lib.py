@@ -114,6 +114,7 @@ SIGNATURES = [
     ("lblg_stream", _vp, [_vp]),
     ("lblg_get_factor", ctypes.c_int, [_vp, _f64p, _f64p, _i32p, _i32p]),
     ("lblg_inertia", ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
+    ("lblg_get_scaling", ctypes.c_int, [_vp, _f64p]),
     ("lblg_factor_info", ctypes.c_int, [_vp, _i32p, _f64p, _f64p]),
     ("lblg_pivot_stats", ctypes.c_int, [_vp, _i64p, _i64p]),
     ("lblg_panel_stats", ctypes.c_int, [_vp, _i64p, _i64p]),

@@ -62,7 +62,8 @@ extern "C" int lblg_factor(lblg_handle* hh) {
   Handle* h = reinterpret_cast<Handle*>(hh);
   if (!h) return LBLG_ERR_ARG;
   if (!h->analyzed) return LBLG_ERR_STATE;
-  if (h->set.precision != LBLG_F64 && h->set.precision != LBLG_F32) return LBLG_ERR_ARG;
+  if (h->set.precision != LBLG_F64 && h->set.precision != LBLG_F32 && h->set.precision != LBLG_BF16)
+    return LBLG_ERR_ARG;
   return guarded([&] {
     auto t0 = std::chrono::steady_clock::now();
     lblg::factor(*h);
@@ -206,6 +207,23 @@ extern "C" int lblg_get_factor(lblg_handle* hh, double* valL, double* d_val, int
     if (rows) hipMemcpyAsync(rows, D.rows, sizeof(int) * S.pi[S.nsuper], hipMemcpyDeviceToHost, st);
     if (perm_final) hipMemcpyAsync(perm_final, D.perm_f, sizeof(int) * h->n, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("copy failed");
+    return LBLG_OK;
+  });
+}
+
+extern "C" int lblg_get_scaling(lblg_handle* hh, double* s) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || !s) return LBLG_ERR_ARG;
+  if (!h->factored) return LBLG_ERR_STATE;
+  return guarded([&] {
+    lblg::DeviceFactor& D = *h->dev;
+    if (D.prec == 0) {
+      for (int i = 0; i < h->n; ++i) s[i] = 1.0;
+      return LBLG_OK;
+    }
+    if (hipMemcpyAsync(s, D.scale, sizeof(double) * h->n, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
+        hipStreamSynchronize(D.stream) != hipSuccess)
+      throw std::runtime_error("copy failed");
     return LBLG_OK;
   });
 }

@@ -82,8 +82,9 @@ struct DeviceFactor {
   std::vector<int> hsuper;      // host copies of super / pi (launch shapes)
   std::vector<int64_t> hpi;
   // numeric: the factor's value type is fixed at setup (prec 0: double,
-  // 1: float); valL, dval, W, LD and the sweep vectors X, Y, Z, acc, O hold
-  // that type
+  // 1: float, 2: float with BF16 operands in the 128-tile updates; both
+  // float forms factor the diagonally scaled S K S); valL, dval, W, LD and
+  // the sweep vectors X, Y, Z, acc, O hold that type
   int prec = 0;
   void* valL = nullptr;
   void* dval = nullptr;
@@ -122,6 +123,13 @@ struct DeviceFactor {
   uint32_t ll_epoch = 0;        // next epoch base (monotone for the handle's lifetime)
   double* red = nullptr;        // reduction scratch
   int* status = nullptr;        // device flags (zero pivots etc.)
+  // FP32 / BF16 factors (prec 1, 2): the symmetric diagonal scaling S and S K S
+  int* Krow = nullptr;          // row and column of every entry of K (lower CSC)
+  int* Kcol = nullptr;
+  double* Kf = nullptr;         // S K S, lower CSC values
+  double* scale = nullptr;      // s_i, original ordering
+  double* scale_ord = nullptr;  // s in the factor's pivoted ordering
+  double* sbuf = nullptr;       // scaled block for the one-workgroup sweeps (n x nrhs_cap)
   // solve
   void* X = nullptr;            // n x nrhs sweep vectors in the factor's type (row-interleaved)
   void* Y = nullptr;

@@ -148,8 +148,12 @@ static T* dupload(DeviceFactor& D, const std::vector<T>& v) {
 
 // an array of the factor's value type
 static void* valloc(DeviceFactor& D, int64_t count) {
-  return D.prec == 1 ? (void*)dalloc<float>(D, count) : (void*)dalloc<double>(D, count);
+  return D.prec != 0 ? (void*)dalloc<float>(D, count) : (void*)dalloc<double>(D, count);
 }
+
+// the device factor's precision code of a setting: 0 FP64, 1 FP32, 2 BF16
+// operands in the 128-tile updates (FP32 storage) with diagonal scaling
+static int prec_code(int precision) { return precision == LBLG_F32 ? 1 : precision == LBLG_BF16 ? 2 : 0; }
 
 static SnArgs snargs(const DeviceFactor& D) {
   return SnArgs{D.super, D.pi, D.px, D.rows, D.col2sup, D.n};
@@ -201,7 +205,7 @@ static void setup(Handle& h) {
   D.rows = dalloc<int>(D, (int64_t)rows0.size());
   D.col2sup = dupload(D, S.col2sup);
   D.perm0 = dupload(D, S.perm);
-  D.prec = h.set.precision == LBLG_F32 ? 1 : 0;
+  D.prec = prec_code(h.set.precision);
   D.valL = valloc(D, S.p[n]);
   D.dval = valloc(D, 2 * (int64_t)n);
   D.perm_piv = dalloc<int>(D, n);
@@ -230,6 +234,16 @@ static void setup(Handle& h) {
     }
   }
   D.amap = dupload(D, amap);
+  if (D.prec != 0) {  // the scaled matrix S K S of the FP32 / BF16 factors
+    std::vector<int> kcol(nnz);
+    for (int j = 0; j < n; ++j)
+      for (int q = h.Ap[j]; q < h.Ap[j + 1]; ++q) kcol[q] = j;
+    D.Krow = dupload(D, h.Ai);
+    D.Kcol = dupload(D, kcol);
+    D.Kf = dalloc<double>(D, nnz);
+    D.scale = dalloc<double>(D, n);
+    D.scale_ord = dalloc<double>(D, n);
+  }
   D.Kx = dalloc<double>(D, nnz);
   D.pert = dalloc<unsigned char>(D, nnz);
   D.isdiag = dupload(D, isdiag);
@@ -320,7 +334,7 @@ static void setup(Handle& h) {
           int64_t G = std::max<int64_t>(1, (r - nb + rows - 1) / rows);
           if (G > 256 || wide_tall) {
             nt = WIDE_NT_TALL;
-            nb = wide_panel_nb(nt, D.prec == 1 ? sizeof(float) : sizeof(double));
+            nb = wide_panel_nb(nt, D.prec != 0 ? sizeof(float) : sizeof(double));
             rows = WIDE_ROWS_TALL;
             G = std::max<int64_t>(1, (r - nb + rows - 1) / rows);
           }
@@ -615,7 +629,7 @@ static void setup(Handle& h) {
 // numeric factorization
 // ---------------------------------------------------------------------------
 void factor(Handle& h) {
-  if (h.dev && h.dev->prec != (h.set.precision == LBLG_F32 ? 1 : 0)) h.dev.reset();  // value type changed
+  if (h.dev && h.dev->prec != prec_code(h.set.precision)) h.dev.reset();  // value type changed
   if (!h.dev) setup(h);
   DeviceFactor& D = *h.dev;
   HIPCHK(hipMemcpyAsync(D.Kx, h.Ax.data(), sizeof(double) * D.nnzK, hipMemcpyHostToDevice, D.stream));
@@ -636,12 +650,20 @@ static void factor_device_t(Handle& h) {
   const int n = h.n;
   hipStream_t st = D.stream;
   const double reg = h.set.reg_diag;
-  // The FP32 factor perturbs tiny diagonals up to the static-pivoting
-  // threshold sqrt(eps_f32) max_ij |K_ij| (2^-12; pivots of reg_diag = 1e-9
-  // would give growth ~1e9, beyond FP32); the refinement still runs on the
-  // reference's K0, so only the preconditioner changes.
+  // The FP32 and BF16 factors take S K S (S: max-norm diagonal scaling) and
+  // perturb its diagonals below the static-pivoting threshold 2^pexp
+  // (sqrt(eps_f32) by default; pivots of reg_diag = 1e-9 would give growth
+  // ~1e9, beyond FP32); the refinement still runs on the reference's K0, so
+  // only the preconditioner changes.
   const int pexp = env_int("LBLG_F32_PIVOT_EXP", -12);
-  const double reg_factor = D.prec == 1 ? std::max(reg, std::ldexp(h.max_abs_entry, pexp)) : reg;
+  // the FP32 / BF16 factors take S K S, whose entries have magnitude <= 1
+  // (max-norm scaling): the threshold is 2^pexp itself
+  const double reg_factor = D.prec != 0 ? std::ldexp(1.0, pexp) : reg;
+  // BF16 operands: split into two BF16 terms (three MFMAs per product) by
+  // default, one rounded term with LBLG_BF16_TERMS=1
+  const int bf16_mode = env_int("LBLG_BF16_TERMS", 3) == 1 ? GEMM_BF16 : GEMM_BF16X3;
+  const int gemm_schur = D.prec == 2 ? bf16_mode : (D.schur_direct ? GEMM_DIRECT : GEMM_STAGED);
+  const int gemm_trailing = D.prec == 2 ? bf16_mode : (D.trailing_direct ? GEMM_DIRECT : GEMM_STAGED);
   SnArgs a = snargs(D);
   a.rows = D.rows0;  // pre-pivot labels during the factorization
   // the whole launch sequence depends only on the symbolic structure, the
@@ -653,8 +675,13 @@ static void factor_device_t(Handle& h) {
     launch_iota(D.perm_piv, n, st);
     launch_fill_int(D.status, 16, 0, st);
     launch_fill_int(D.kpos, D.nsuper, 0, st);
-    launch_pert_flags(D.nnzK, D.isdiag, D.Kx, reg, reg_factor, h.set.regularization, D.pert, st);
-    launch_assemble(D.amap, D.Kx, D.pert, D.nnzK, reg_factor, valL, st);
+    const double* Kfac = D.Kx;  // the matrix the factor takes: K, or S K S for FP32 / BF16
+    if (D.prec != 0) {
+      launch_diag_scaling(n, D.nnzK, D.Krp, D.Ksrc, D.Krow, D.Kcol, D.Kx, D.scale, D.Kf, st);
+      Kfac = D.Kf;
+    }
+    launch_pert_flags(D.nnzK, D.isdiag, D.Kx, Kfac, reg, reg_factor, h.set.regularization, D.pert, st);
+    launch_assemble(D.amap, Kfac, D.pert, D.nnzK, reg_factor, valL, st);
     for (size_t l = 0; l < D.flevels.size(); ++l) {
       FactorLevel& F = D.flevels[l];
       timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, F.small_wmax, a, valL, dval, D.perm_piv, D.status, st); });
@@ -698,7 +725,7 @@ static void factor_device_t(Handle& h) {
             if (Tr > 0)
               timed(D, "trailing", [&] {
                 launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
-                                    D.trailing_direct, st);
+                                    gemm_trailing, st);
               });
           }
           // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
@@ -719,7 +746,7 @@ static void factor_device_t(Handle& h) {
             if (Tj > 0)
               timed(D, "trailing", [&] {
                 launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
-                                    D.trailing_direct, st);
+                                    gemm_trailing, st);
               });
           }
         }
@@ -738,11 +765,12 @@ static void factor_device_t(Handle& h) {
         });
         timed(D, "schur_big", [&] {
           launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, valL, LDb,
-                           F.d_schur_big_ldoff, D.schur_direct, st);
+                           F.d_schur_big_ldoff, gemm_schur, st);
         });
       }
     }
     launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
+    if (D.prec != 0) launch_gather_scale(n, D.perm_f, D.scale, D.scale_ord, st);
     // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
     launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
     launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
@@ -776,7 +804,7 @@ static void factor_device_t(Handle& h) {
 }
 
 void factor_device(Handle& h) {
-  if (h.dev->prec == 1)
+  if (h.dev->prec != 0)
     factor_device_t<float>(h);
   else
     factor_device_t<double>(h);
@@ -807,6 +835,7 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
   D.acc2 = dalloc<double>(D, nn);
   D.stage_b = dalloc<double>(D, nn);
   D.stage_x = dalloc<double>(D, nn);
+  if (D.prec != 0) D.sbuf = dalloc<double>(D, nn);
   D.nrhs_cap = cap;
   D.mr_cap = mrc;
 }
@@ -825,11 +854,19 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   T* Y = static_cast<T*>(D.Y);
   T* Z = static_cast<T*>(D.Z);
   T* acc = static_cast<T*>(D.acc);
+  // the FP32 / BF16 factors are of S K S: M^-1 = S (L D L^T)^-1 S in the pivoted order
+  const bool scaled = D.prec != 0;
   if (D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && sizeof(T) * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS) {
+    const double* sin = in;
+    if (scaled) {
+      launch_row_scale_copy(D.sbuf, in, D.scale_ord, D.n, nrhs, st);
+      sin = D.sbuf;
+    }
     timed(D, "solve_small", [&] {
       launch_solve_small<T>(D.ss_info, D.ss_ipx, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
-                            static_cast<const T*>(D.dval), in, out, nrhs, active, astride, st);
+                            static_cast<const T*>(D.dval), sin, out, nrhs, active, astride, st);
     });
+    if (scaled) launch_row_scale_copy(out, out, D.scale_ord, D.n, nrhs, st);
     return;
   }
   T* O;
@@ -837,7 +874,10 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
     launch_copy(X, in, nn, st);
     O = out;
   } else {
-    launch_convert(X, in, nn, st);
+    if (scaled)
+      launch_row_scale_copy(X, in, D.scale_ord, D.n, nrhs, st);
+    else
+      launch_convert(X, in, nn, st);
     O = static_cast<T*>(D.O);
   }
   for (size_t l = 0; l < D.slevels.size(); ++l) {
@@ -897,13 +937,18 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
       });
     }
   }
-  if constexpr (!std::is_same<T, double>::value) launch_convert(out, (const T*)O, nn, st);
+  if constexpr (!std::is_same<T, double>::value) {
+    if (scaled)
+      launch_row_scale_copy(out, (const T*)O, D.scale_ord, D.n, nrhs, st);
+    else
+      launch_convert(out, (const T*)O, nn, st);
+  }
 }
 
 // active: see launch_solve_small (only the one-workgroup sweeps use it; the
 // level sweeps always run)
 void apply_solve(DeviceFactor& D, const double* in, double* out, int nrhs, const double* active, int64_t astride) {
-  if (D.prec == 1)
+  if (D.prec != 0)
     apply_solve_t<float>(D, in, out, nrhs, active, astride);
   else
     apply_solve_t<double>(D, in, out, nrhs, active, astride);
@@ -998,7 +1043,7 @@ static void backward_errors(Handle& h, const double* b, const double* x, int nrh
 
 // whether a solve of nrhs columns takes the one-workgroup sweeps
 static bool small_sweeps(const DeviceFactor& D, int nrhs) {
-  const size_t vs = D.prec == 1 ? sizeof(float) : sizeof(double);
+  const size_t vs = D.prec != 0 ? sizeof(float) : sizeof(double);
   return D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && vs * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS;
 }
 

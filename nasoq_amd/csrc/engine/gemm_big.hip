@@ -19,11 +19,22 @@
 // the next chunk into registers while the current one is multiplied.  As in the 64 x 64 engine, the second operand is passed first
 // so the accumulator holds the transposed block and lanes 0..15 cover 16
 // consecutive rows in the epilogue.
+//
+// BF16 forms (the FP32 factor of precision LBLG_BF16): chunks of 32 columns
+// of k are loaded as FP32, rounded to BF16 once and staged in LDS row by row
+// ([row][k], 80-byte pitch), so the four waves of a tile share their
+// operands; each wave multiplies them on v_mfma_f32_16x16x16_bf16 with FP32
+// accumulation (big_gemm_bf16).  The next chunk is loaded into registers
+// while the current one is multiplied.  With three terms (GEMM_BF16X3) every
+// operand x is staged as hi = bf16(x) and lo = bf16(x - hi), and the product
+// takes hi hi + hi lo + lo hi: three MFMAs, relative error ~2^-16 per term
+// instead of ~2^-8.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "engine/device_util.hpp"
 #include "engine/kernels.hpp"
@@ -152,6 +163,122 @@ __device__ inline void big_gemm_direct(BigAcc<T>& acc, int K, const T* Ab, int64
   }
 }
 
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_rne(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+#define B16K 32  // k per staged chunk
+#define B16P 40  // LDS row pitch in BF16 values
+#define B16_LDS (2 * BT * B16P)  // BF16 values of the A and B stages (one term)
+
+// acc = sum_k A(x, k) B(y, k) with A(x, k) = Ab[x + k lda], B(y, k) = Bb[y + k ldb],
+// k < K, the operands rounded to BF16 (TERMS = 1) or split into two BF16
+// terms (TERMS = 3: hi hi + hi lo + lo hi) and accumulated in FP32.  Thread
+// t stages row t % 128 and k = (t / 128) 16 .. + 15 of each 32-wide chunk.
+// Rows beyond xa / xb are clamped to the last valid one (their results are
+// discarded by the caller); k beyond K is zero.  S holds TERMS == 3 ? 2 : 1
+// stages of B16_LDS values.
+template <int TERMS>
+__device__ inline void big_gemm_bf16(BigAcc<float>& acc, int K, const float* Ab, int64_t lda, int xa,
+                                     const float* Bb, int64_t ldb, int xb, unsigned short* S) {
+  unsigned short* As = S;
+  unsigned short* Bs = S + BT * B16P;
+  unsigned short* Al = S + B16_LDS;           // lo terms (TERMS == 3)
+  unsigned short* Bl = S + B16_LDS + BT * B16P;
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int r0 = (wv >> 1) * 64, q0 = (wv & 1) * 64;
+  const int li = ln & 15, lk = ln >> 4;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) acc.c[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const int x = t & 127, kq = (t >> 7) * 16;
+  const float* pa = Ab + min(x, xa - 1);
+  const float* pb = Bb + min(x, xb - 1);
+  float ra[16], rb[16];
+  auto load = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = kc + kq + q;
+      ra[q] = k < K ? pa[(int64_t)k * lda] : 0.f;
+      rb[q] = k < K ? pb[(int64_t)k * ldb] : 0.f;
+    }
+  };
+  // 16 FP32 values -> 16 BF16 (hi) and, for three terms, their residuals (lo)
+  auto pack = [&](const float* v, uint4* hi, uint4* lo) {
+    unsigned* uh = reinterpret_cast<unsigned*>(hi);
+    unsigned* ul = reinterpret_cast<unsigned*>(lo);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const unsigned short h0 = bf16_rne(v[2 * q]), h1 = bf16_rne(v[2 * q + 1]);
+      uh[q] = (unsigned)h0 | ((unsigned)h1 << 16);
+      if constexpr (TERMS == 3) {
+        const float r0 = v[2 * q] - __uint_as_float((unsigned)h0 << 16);
+        const float r1 = v[2 * q + 1] - __uint_as_float((unsigned)h1 << 16);
+        ul[q] = (unsigned)bf16_rne(r0) | ((unsigned)bf16_rne(r1) << 16);
+      }
+    }
+  };
+  auto stage = [&]() {
+    uint4 ha[2], hb[2], la[2], lb[2];
+    pack(ra, ha, la);
+    pack(rb, hb, lb);
+    uint4* da = reinterpret_cast<uint4*>(As + x * B16P + kq);
+    uint4* db = reinterpret_cast<uint4*>(Bs + x * B16P + kq);
+    da[0] = ha[0];
+    da[1] = ha[1];
+    db[0] = hb[0];
+    db[1] = hb[1];
+    if constexpr (TERMS == 3) {
+      uint4* ea = reinterpret_cast<uint4*>(Al + x * B16P + kq);
+      uint4* eb = reinterpret_cast<uint4*>(Bl + x * B16P + kq);
+      ea[0] = la[0];
+      ea[1] = la[1];
+      eb[0] = lb[0];
+      eb[1] = lb[1];
+    }
+  };
+  load(0);
+  for (int kc = 0; kc < K; kc += B16K) {
+    __syncthreads();  // the previous chunk's operands have been read
+    stage();
+    __syncthreads();
+    if (kc + B16K < K) load(kc + B16K);
+#pragma unroll
+    for (int kk = 0; kk < B16K; kk += 16) {
+      s4v av[4], bv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        av[q] = *reinterpret_cast<const s4v*>(As + (r0 + 16 * q + li) * B16P + kk + 4 * lk);
+        bv[q] = *reinterpret_cast<const s4v*>(Bs + (q0 + 16 * q + li) * B16P + kk + 4 * lk);
+      }
+      if constexpr (TERMS == 3) {
+        s4v al[4], bl[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          al[q] = *reinterpret_cast<const s4v*>(Al + (r0 + 16 * q + li) * B16P + kk + 4 * lk);
+          bl[q] = *reinterpret_cast<const s4v*>(Bl + (q0 + 16 * q + li) * B16P + kk + 4 * lk);
+        }
+        // the small terms first, then hi hi
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            acc.c[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bl[b], av[a], acc.c[a][b], 0, 0, 0);
+            acc.c[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bv[b], al[a], acc.c[a][b], 0, 0, 0);
+          }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc.c[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bv[b], av[a], acc.c[a][b], 0, 0, 0);
+    }
+  }
+}
+
 template <typename T, typename F>
 __device__ inline void big_foreach(const BigAcc<T>& acc, F f) {
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -180,14 +307,16 @@ __device__ inline int64_t big_row_pos(const int* rows, int64_t b, int64_t e, int
   return lo - b;
 }
 
-// DIRECT: operands loaded by each lane from memory (big_gemm_direct) instead
-// of staged through LDS (big_gemm)
-template <typename T, bool DIRECT>
+// MODE: GEMM_STAGED (operands staged through LDS, big_gemm), GEMM_DIRECT
+// (loaded by each lane from memory, big_gemm_direct) or GEMM_BF16 (float
+// operands rounded to BF16 in LDS, big_gemm_bf16)
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_t* tile_prefix, int count,
                                                    const int* super, const int64_t* pi, const int64_t* px,
                                                    const int* rows, const int* col2sup, T* valL,
                                                    const T* LD, const int64_t* ldoff) {
-  __shared__ T As[DIRECT ? 1 : BKC * BLD], Bs[DIRECT ? 1 : BKC * BLD];
+  __shared__ T As[MODE == GEMM_STAGED ? BKC * BLD : 1], Bs[MODE == GEMM_STAGED ? BKC * BLD : 1];
+  __shared__ __align__(16) unsigned short S16[MODE == GEMM_BF16X3 ? 2 * B16_LDS : MODE == GEMM_BF16 ? B16_LDS : 1];
   __shared__ int rlab[BT], clab[BT], ctg[BT], dtcol[BT];
   __shared__ int tpos[BRUNS][BT];
   __shared__ int64_t cbase[BT];
@@ -243,7 +372,9 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
   __syncthreads();
   const T* LDs = LD + ldoff[lo];
   BigAcc<T> acc;
-  if constexpr (DIRECT) {
+  if constexpr (MODE == GEMM_BF16 || MODE == GEMM_BF16X3) {
+    big_gemm_bf16<MODE == GEMM_BF16X3 ? 3 : 1>(acc, w, P + w + i0, r, m - i0, LDs + j0, (int64_t)m, m - j0, S16);
+  } else if constexpr (MODE == GEMM_DIRECT) {
     big_gemm_direct(acc, w, P + w + i0, r, m - i0, LDs + j0, (int64_t)m, m - j0);
   } else {
     big_gemm(
@@ -278,12 +409,13 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
 //              written to the panel buffer Pp[(j - k1) r + i] (the matrix itself
 //              stays without the block's updates until the block ends); one
 //              column of tiles starting at k1
-template <typename T, bool DIRECT>
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, int strip, int wcols, const int* super,
                                                       const int64_t* pi, const int64_t* px, T* valL,
                                                       T* Wbase, const int64_t* woff,
                                                       const int* kpos, const int* kstart) {
-  __shared__ T As[DIRECT ? 1 : BKC * BLD], Bs[DIRECT ? 1 : BKC * BLD];
+  __shared__ T As[MODE == GEMM_STAGED ? BKC * BLD : 1], Bs[MODE == GEMM_STAGED ? BKC * BLD : 1];
+  __shared__ __align__(16) unsigned short S16[MODE == GEMM_BF16X3 ? 2 * B16_LDS : MODE == GEMM_BF16 ? B16_LDS : 1];
   const int s = list[0];
   const int w = super[s + 1] - super[s];
   const int64_t r = pi[s + 1] - pi[s];
@@ -314,7 +446,10 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
   T* Pp = Wbase + woff[s] + (int64_t)w * wcols;  // r x strip, pitch r
   const int kb = k1 - k0;
   BigAcc<T> acc;
-  if constexpr (DIRECT) {
+  if constexpr (MODE == GEMM_BF16 || MODE == GEMM_BF16X3) {
+    big_gemm_bf16<MODE == GEMM_BF16X3 ? 3 : 1>(acc, kb, A + (int64_t)k0 * r + i0, r, (int)min<int64_t>(r - i0, BT),
+                                              Wp + j0, (int64_t)w, w - j0, S16);
+  } else if constexpr (MODE == GEMM_DIRECT) {
     big_gemm_direct(acc, kb, A + (int64_t)k0 * r + i0, r, (int)min<int64_t>(r - i0, BT), Wp + j0, (int64_t)w,
                     w - j0);
   } else {
@@ -402,29 +537,54 @@ void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncol
 
 template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                      T* valL, const T* LD, const int64_t* ldoff, bool direct, hipStream_t st) {
+                      T* valL, const T* LD, const int64_t* ldoff, int mode, hipStream_t st) {
   if (count <= 0 || ntiles <= 0) return;
   if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur_big: too many tiles");
-  if (direct)
-    k_schur_big<T, true><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+  const dim3 grid((unsigned)ntiles);
+  if (mode == GEMM_BF16 || mode == GEMM_BF16X3) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (mode == GEMM_BF16X3)
+        k_schur_big<T, GEMM_BF16X3><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
                                                           a.col2sup, valL, LD, ldoff);
-  else
-    k_schur_big<T, false><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                                           a.col2sup, valL, LD, ldoff);
+      else
+        k_schur_big<T, GEMM_BF16><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                                        a.col2sup, valL, LD, ldoff);
+    } else {
+      throw std::runtime_error("schur_big: BF16 operands need the FP32 factor");
+    }
+  } else if (mode == GEMM_DIRECT) {
+    k_schur_big<T, GEMM_DIRECT><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                                      a.col2sup, valL, LD, ldoff);
+  } else {
+    k_schur_big<T, GEMM_STAGED><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
+                                                      a.col2sup, valL, LD, ldoff);
+  }
   HIP_LAUNCH_CHECK("schur_big");
 }
 
 template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
-                         T* W, const int64_t* woff, const int* kpos, const int* kstart, bool direct, hipStream_t st) {
+                         T* W, const int64_t* woff, const int* kpos, const int* kstart, int mode, hipStream_t st) {
   if (ntiles <= 0) return;
   if (strip < 0 || strip > BT) throw std::runtime_error("trailing_big: strip wider than a tile");
-  if (direct)
-    k_trailing_big<T, true><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff,
-                                                    kpos, kstart);
-  else
-    k_trailing_big<T, false><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W, woff,
-                                                     kpos, kstart);
+  if (mode == GEMM_BF16 || mode == GEMM_BF16X3) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (mode == GEMM_BF16X3)
+        k_trailing_big<T, GEMM_BF16X3><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL,
+                                                               W, woff, kpos, kstart);
+      else
+        k_trailing_big<T, GEMM_BF16><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W,
+                                                             woff, kpos, kstart);
+    } else {
+      throw std::runtime_error("trailing_big: BF16 operands need the FP32 factor");
+    }
+  } else if (mode == GEMM_DIRECT) {
+    k_trailing_big<T, GEMM_DIRECT><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W,
+                                                           woff, kpos, kstart);
+  } else {
+    k_trailing_big<T, GEMM_STAGED><<<ntiles, 256, 0, st>>>(list, lo, strip, wcols, a.super, a.pi, a.px, valL, W,
+                                                           woff, kpos, kstart);
+  }
   HIP_LAUNCH_CHECK("trailing_big");
 }
 
@@ -432,9 +592,9 @@ void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wco
   template void launch_scale_ld<T>(const int*, int, const int*, int, const SnArgs&, const T*, const T*, T*,  \
                                    const int64_t*, hipStream_t);                                            \
   template void launch_schur_big<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,   \
-                                    const int64_t*, bool, hipStream_t);                                     \
+                                    const int64_t*, int, hipStream_t);                                      \
   template void launch_trailing_big<T>(const int*, int, int, int, int, const SnArgs&, T*, T*, const int64_t*, \
-                                       const int*, const int*, bool, hipStream_t);
+                                       const int*, const int*, int, hipStream_t);
 LBLG_INSTANTIATE(double)
 LBLG_INSTANTIATE(float)
 #undef LBLG_INSTANTIATE

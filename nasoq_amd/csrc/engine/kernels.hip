@@ -1970,21 +1970,71 @@ namespace lblg {
 // (norm_sparse_int, src/common/Norm.cpp:115-185)
 // --------------------------------------------------------------------------
 // bit 0: |K_jj| < reg (the reference's rule; those diagonals are zero in K0),
-// bit 1: |K_jj| < reg_factor (replaced by reg_factor in the factored matrix)
-__global__ void k_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg,
+// bit 1: |Kf_jj| < reg_factor (replaced by reg_factor in the factored matrix;
+// Kf = K, or the diagonally scaled S K S of the BF16 factor)
+__global__ void k_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, const double* Kf, double reg,
                              double reg_factor, unsigned char* pert) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
-    const double a = fabs(Kx[q]);
-    pert[q] = isdiag[q] ? (unsigned char)((a < reg ? 1 : 0) | (a < reg_factor ? 2 : 0)) : (unsigned char)0;
+    const double a = fabs(Kx[q]), af = fabs(Kf[q]);
+    pert[q] = isdiag[q] ? (unsigned char)((a < reg ? 1 : 0) | (af < reg_factor ? 2 : 0)) : (unsigned char)0;
   }
 }
-void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, double reg_factor,
-                       int regularization, unsigned char* pert, hipStream_t st) {
+void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, const double* Kf, double reg,
+                       double reg_factor, int regularization, unsigned char* pert, hipStream_t st) {
   if (nnz <= 0) return;
   const bool on = regularization == 1;
-  k_pert_flags<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, isdiag, Kx, on ? reg : -1.0, on ? reg_factor : -1.0, pert);
+  k_pert_flags<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, isdiag, Kx, Kf, on ? reg : -1.0, on ? reg_factor : -1.0,
+                                                   pert);
   HIP_LAUNCH_CHECK("pert_flags");
 }
+// --------------------------------------------------------------------------
+// symmetric diagonal scaling of the BF16 factor: s_i = 1 / sqrt(max_j |K_ij|)
+// (one max-norm equilibration step; rows without entries keep 1), the scaled
+// entries Kf = s_row K s_col of the lower CSC, and s in the pivoted order of
+// the factor (s_ord[k] = s[perm_f[k]])
+// --------------------------------------------------------------------------
+__global__ void k_row_scale(int n, const int* rp, const int64_t* src, const double* Kx, double* s) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double m = 0.0;
+    for (int q = rp[i]; q < rp[i + 1]; ++q) m = fmax(m, fabs(Kx[src[q]]));
+    s[i] = m > 0.0 ? 1.0 / sqrt(m) : 1.0;
+  }
+}
+__global__ void k_scale_entries(int nnz, const int* row, const int* col, const double* Kx, const double* s,
+                                double* Kf) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x)
+    Kf[q] = s[row[q]] * Kx[q] * s[col[q]];
+}
+__global__ void k_gather_scale(int n, const int* perm_f, const double* s, double* s_ord) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) s_ord[k] = s[perm_f[k]];
+}
+void launch_diag_scaling(int n, int nnz, const int* rp, const int64_t* src, const int* row, const int* col,
+                         const double* Kx, double* s, double* Kf, hipStream_t st) {
+  k_row_scale<<<grid_for(n, 256), 256, 0, st>>>(n, rp, src, Kx, s);
+  HIP_LAUNCH_CHECK("row_scale");
+  if (nnz <= 0) return;
+  k_scale_entries<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, row, col, Kx, s, Kf);
+  HIP_LAUNCH_CHECK("scale_entries");
+}
+void launch_gather_scale(int n, const int* perm_f, const double* s, double* s_ord, hipStream_t st) {
+  k_gather_scale<<<grid_for(n, 256), 256, 0, st>>>(n, perm_f, s, s_ord);
+  HIP_LAUNCH_CHECK("gather_scale");
+}
+// dst[i nrhs + c] = s[i] src[i nrhs + c] (row i of a row-interleaved block), converted to TO
+template <typename TO, typename TI>
+__global__ void k_row_scale_copy(TO* dst, const TI* src, const double* s, int n, int nrhs) {
+  const int64_t nn = (int64_t)n * nrhs;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nn; q += (int64_t)gridDim.x * blockDim.x)
+    dst[q] = (TO)((double)src[q] * s[q / nrhs]);
+}
+template <typename TO, typename TI>
+void launch_row_scale_copy(TO* dst, const TI* src, const double* s, int n, int nrhs, hipStream_t st) {
+  k_row_scale_copy<TO, TI><<<grid_for((int64_t)n * nrhs, 256), 256, 0, st>>>(dst, src, s, n, nrhs);
+  HIP_LAUNCH_CHECK("row_scale_copy");
+}
+template void launch_row_scale_copy<float, double>(float*, const double*, const double*, int, int, hipStream_t);
+template void launch_row_scale_copy<double, float>(double*, const float*, const double*, int, int, hipStream_t);
+template void launch_row_scale_copy<double, double>(double*, const double*, const double*, int, int, hipStream_t);
 __global__ void k_k0_values(int64_t nnzf, const int64_t* src, const double* Kx, const unsigned char* pert,
                             double* Kv) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnzf; e += (int64_t)gridDim.x * blockDim.x) {

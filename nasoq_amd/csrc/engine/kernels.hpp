@@ -61,16 +61,20 @@ template <typename T>
 void launch_trailing(const int* list, int count, int tiles_per_sn, int lo, const SnArgs& a, T* valL,
                      const T* W, const int64_t* woff, const int* kpos, const int* kstart,
                      hipStream_t st);
-// 128 x 128 MFMA tiles (gemm_big.hip)
+// 128 x 128 MFMA tiles (gemm_big.hip); operand modes of the GEMM
+constexpr int GEMM_STAGED = 0;  // chunks of the value type staged in LDS
+constexpr int GEMM_DIRECT = 1;  // every lane loads its own operands from memory
+constexpr int GEMM_BF16 = 2;    // float operands rounded to BF16 in LDS, FP32 accumulation
+constexpr int GEMM_BF16X3 = 3;  // float operands split into BF16 hi + lo, three MFMA terms
 template <typename T>
 void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncols, const SnArgs& a,
                      const T* valL, const T* dval, T* LD, const int64_t* ldoff, hipStream_t st);
 template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
-                      T* valL, const T* LD, const int64_t* ldoff, bool direct, hipStream_t st);
+                      T* valL, const T* LD, const int64_t* ldoff, int mode, hipStream_t st);
 template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
-                         T* W, const int64_t* woff, const int* kpos, const int* kstart, bool direct, hipStream_t st);
+                         T* W, const int64_t* woff, const int* kpos, const int* kstart, int mode, hipStream_t st);
 template <typename T>
 void launch_offdiag(const int* list, int count, int wmax, const int* blk_prefix, const int* blkpos, int nblocks,
                     const SnArgs& a, T* valL, const T* dval, const int* perm_piv, hipStream_t st);
@@ -151,8 +155,14 @@ void launch_seg_abs_sums(int nseg, const int64_t* seg, int nrhs, const double* r
                          double* out, hipStream_t st);
 void launch_seg_rowabs_max(int nseg, const int64_t* seg, const int* rp, const double* Kv, double* out,
                            hipStream_t st);
-void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, double reg, double reg_factor,
-                       int regularization, unsigned char* pert, hipStream_t st);
+void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, const double* Kf, double reg,
+                       double reg_factor, int regularization, unsigned char* pert, hipStream_t st);
+// symmetric diagonal scaling of the BF16 factor (kernels.hip)
+void launch_diag_scaling(int n, int nnz, const int* rp, const int64_t* src, const int* row, const int* col,
+                         const double* Kx, double* s, double* Kf, hipStream_t st);
+void launch_gather_scale(int n, const int* perm_f, const double* s, double* s_ord, hipStream_t st);
+template <typename TO, typename TI>
+void launch_row_scale_copy(TO* dst, const TI* src, const double* s, int n, int nrhs, hipStream_t st);
 void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const double* Kx,
                const unsigned char* pert, double* Kv, double* part, int nblk, double* norm_out, hipStream_t st);
 void launch_mfma_selftest(const double* A, const double* B, double* C, int f32, hipStream_t st);

@@ -278,13 +278,20 @@ def _numeric_methods():
         e = self.residual_norms(rhs, x)[:, 3]
         return float(e[0]) if np.asarray(rhs).ndim == 1 else e
 
+    def scaling(self):
+        """The diagonal scaling s of the factor (S K~ S is factored; ones unless precision is BF16)."""
+        out = np.zeros(self.n, dtype=np.float64)
+        check(self.lib.lblg_get_scaling(self.h, dptr(out)), "lblg_get_scaling")
+        return out
+
     def factor_info(self):
         z, nrm, sec = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         check(self.lib.lblg_factor_info(self.h, ctypes.byref(z), ctypes.byref(nrm), ctypes.byref(sec)), "info")
         return {"zero_pivot": z.value, "k0_norm_inf": nrm.value, "factor_seconds": sec.value}
 
     for f in (numerical_factorization, _push_settings, refactor_resident, solve_only, set_values, graph_stats,
-              tuning, set_tuning, autotune, inertia, get_factor, residual_norms, backward_error, factor_info):
+              tuning, set_tuning, autotune, inertia, get_factor, residual_norms, backward_error, scaling,
+              factor_info):
         setattr(SolverSettings, f.__name__, f)
 
 

@@ -1,14 +1,15 @@
 """FP32 factor with FP64 refinement (precision = LBLG_F32) vs the CPU oracle.
 
-The factor, D and the L / D / L^T sweeps run in FP32 (FP32 MFMA for the
-dense updates); the residual b - K x, GMRES and the solution update stay
-FP64.  The FP32 factor perturbs diagonals below max(reg_diag, 2^-12
-max|K_ij|) (static pivoting at sqrt(eps_f32)); the refinement works on the
-reference's unperturbed K0.  Bar: the same inertia as the FP64 oracle's
-factor of the same perturbed K~; L D L^T reproduces P K~ P^T to an FP32
-rounding bound scaled by the growth; target-driven refinement reaches a
-backward error <= 1e-10 on K (recomputed by the oracle) and the solution
-agrees with the oracle's refined FP64 solution.
+The factor is of S K S, S = diag(s) with s_i = 1 / sqrt(max_j |K_ij|) (one
+max-norm equilibration step); the factor, D and the L / D / L^T sweeps run
+in FP32 (FP32 MFMA for the dense updates); the residual b - K x, GMRES and
+the solution update stay FP64.  Diagonals of S K S below 2^-12 are raised
+to it (static pivoting at sqrt(eps_f32)); the refinement works on the
+reference's unperturbed, unscaled K0.  Bar: the same inertia as the FP64
+oracle's factor of the same scaled, perturbed matrix; L D L^T reproduces
+P S K~ S P^T to an FP32 rounding bound scaled by the growth; target-driven
+refinement reaches a backward error <= 1e-10 on K (recomputed by the
+oracle) and the solution agrees with the oracle's refined FP64 solution.
 """
 import numpy as np
 import pytest
@@ -38,23 +39,27 @@ def oracle_factor(O, s, n, Ap, Ai, Ax, reg):
                     s.sym_ints("s"), s.sym_longs("p"))
 
 
-def factor_reg(Ax, reg):
-    """The diagonal perturbation of the FP32 factor: max(reg_diag, 2^-12 max|K_ij|)."""
-    return max(reg, np.ldexp(np.abs(Ax).max(), -12))
+REG_HAT = 2.0 ** -12  # the static-pivot threshold of the scaled matrix
 
 
 def check_f32(O, s, n, Ap, Ai, Ax, B, reg0):
-    # the oracle factors the same perturbed matrix K~ (threshold of the FP32 factor)
-    reg = factor_reg(Ax, reg0)
-    F = oracle_factor(O, s, n, Ap, Ai, Ax, reg)
+    # the oracle factors the same scaled, perturbed matrix S K~ S (s_i =
+    # 1 / sqrt(max_j |K_ij|), recomputed here from K)
+    sc = s.scaling()
+    Kf = P.lower_to_full(n, Ap, Ai, Ax).tocsr()
+    rmax = np.asarray(abs(Kf).max(axis=1).todense()).ravel()
+    assert np.allclose(sc, np.where(rmax > 0, 1.0 / np.sqrt(np.where(rmax > 0, rmax, 1.0)), 1.0), rtol=1e-14)
+    Axs = sc[Ai] * Ax * sc[np.repeat(np.arange(n), np.diff(Ap))]
+    reg = REG_HAT
+    F = oracle_factor(O, s, n, Ap, Ai, Axs, reg)
     assert s.inertia() == F.inertia()
     valL, dval, rows, permf = s.get_factor()
     sup, pi = s.sym_ints("super"), s.sym_longs("pi")
     px = s.sym_longs("p")[sup[:-1]]
     L, D = dense_from_factor(n, sup, pi, px, valL, rows, dval)
     K = P.lower_to_full(n, Ap, Ai, Ax).toarray()
-    Kt = K.copy()
-    d = np.diag(K)
+    Kt = P.lower_to_full(n, Ap, Ai, Axs).toarray()
+    d = np.diag(Kt).copy()
     Kt[np.arange(n), np.arange(n)] = np.where(np.abs(d) < reg, reg, d)
     Pm = np.eye(n)[:, permf]
     R = Pm @ L @ D @ L.T @ Pm.T
@@ -65,12 +70,13 @@ def check_f32(O, s, n, Ap, Ai, Ax, B, reg0):
     s.target_bwd_err = 1e-10
     s.max_refine_rounds = 30
     X, bwd = s.solve_only(B, return_bwd=True)
+    Fo = oracle_factor(O, s, n, Ap, Ai, Ax, reg0)  # the FP64 factor of K~ itself
+    cond_est = np.linalg.cond(K)
     for c in range(B.shape[1]):
         e = O.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c])
         assert e <= 1e-10, (c, e)
         assert abs(e - bwd[c]) <= max(1e-3 * e, 1e-16)
-        xr, _ = F.refine(B[:, c], F.solve(B[:, c]))
-        cond_est = np.linalg.cond(K)
+        xr, _ = Fo.refine(B[:, c], Fo.solve(B[:, c]))
         assert np.linalg.norm(X[:, c] - xr) <= max(1e-8, 1e-9 * cond_est) * max(1.0, np.linalg.norm(xr))
     return F
 
