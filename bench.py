@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KKT solves/s at 1e-10 residual: mixed/batched/learned vs plain FP64 on GPU"
 CONFIGS = {  # name: (grid, constraint rows, nnz per row)
+    "kkt96": (96, 65536, 8),
     "kkt64": (64, 16384, 8),
     "kkt48": (48, 6912, 8),
     "kkt32": (32, 2048, 8),
@@ -71,43 +72,101 @@ def committed_traffic(kernel):
     return None
 
 
-def cpu_baseline(config, full_fl):
-    """Oracle (FP64 CPU restatement, one thread) on a bounded sample of the same
-    workload: the same KKT construction on a 36^3 grid, scaled to the full
-    configuration by the ratio of sum c_j^2."""
+def cpu_info():
+    """(model name, usable cores) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        cores = min(cores, int(omp))
+    return model, max(cores, 1)
+
+
+def cpu_run(grid, m, reps, threads):
+    """The BLAS-3 CPU restatement (oracle/cpu_lbl.c) on the KKT construction at
+    grid^3: symbolic analysis once, then one warm-up and `reps` timed runs of
+    factor and of the 4-RHS solve with refinement (x += M^-1 (b - K x)) to a
+    backward error <= 1e-10; medians of the timed runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from nasoq_amd import lbl
     from nasoq_amd import problems as P
-    g = 36
-    m = int(round(CONFIGS[config][1] * (g / CONFIGS[config][0]) ** 3))
-    n, Ap, Ai, Ax, B = P.kkt_config(g, m, CONFIGS[config][2], seed=0, nrhs=4)
+    if O.cpu_lib() is None:
+        return None
+    n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 8, seed=0, nrhs=4)
     s = lbl.SolverSettings(n, Ap, Ai, Ax)
     s.reg_diag = 1e-9
+    s.cost_param = threads
+    t = time.perf_counter()
     s.symbolic_analysis()
+    t_sym = time.perf_counter() - t
     fl, _ = s.flops()
-    t0 = time.perf_counter()
-    F = O.Factor(n, Ap, Ai, Ax, 1e-9, s.sym_ints("Perm"), s.sym_ints("super"), s.sym_longs("pi"),
-                 s.sym_ints("s"), s.sym_longs("p"))
-    worst = 0.0
-    for c in range(4):
-        x = F.solve(B[:, c])
-        for _ in range(20):
-            e = O.backward_error(n, Ap, Ai, Ax, x, B[:, c])
-            if e <= 1e-10:
+    args = (n, Ap, Ai, Ax, 1e-9, s.sym_ints("Perm"), s.sym_ints("super"), s.sym_longs("pi"), s.sym_ints("s"),
+            s.sym_longs("p"), s.sym_ints("sParent"), threads)
+    tf, tsv, worst, rounds = [], [], 0.0, []
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        C = O.CpuFactor(*args)
+        t1 = time.perf_counter()
+        X = C.solve(B)
+        k = 0
+        while True:
+            e = max(O.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) for c in range(4))
+            if e <= 1e-10 or k >= 20:
                 break
-            x, _ = F.refine(B[:, c], x, itr_max=1, mr=2)
-        worst = max(worst, e)
-    dt = time.perf_counter() - t0
+            R = np.stack([B[:, c] - O.spmv(n, Ap, Ai, Ax, X[:, c]) for c in range(4)], axis=1)
+            X = X + C.solve(R)
+            k += 1
+        t2 = time.perf_counter()
+        if rep > 0:  # the first run is the warm-up
+            tf.append(t1 - t0)
+            tsv.append(t2 - t1)
+            worst = max(worst, e)
+            rounds.append(k)
+        del C
     s.close()
+    return {"grid": grid, "N": n, "sum_colcount_sq": fl, "symbolic_s": t_sym, "factor_s_median": float(np.median(tf)),
+            "solve_refine_s_median": float(np.median(tsv)), "factor_s_runs": tf, "solve_refine_s_runs": tsv,
+            "worst_backward_error": worst, "refine_rounds": rounds, "blas": O.cpu_lib().cpu_lbl_blas_name().decode()}
+
+
+def cpu_baseline(config, full_fl, grid=48, reps=5):
+    """The reference's CPU numeric path restated on BLAS-3 (oracle/cpu_lbl.c:
+    supernodal left-looking LDL^T with DGEMM updates, DSYTRF, DTRSM, tree
+    levels in parallel) on all cores of the host, on a bounded sample of the
+    same workload (the KKT construction at grid^3, constraint rows scaled
+    with the volume), one warm-up and the median of `reps` runs, scaled to
+    the full configuration by the ratio of sum c_j^2.  A CPU restatement,
+    not the reference binary (METIS and OpenBLAS are not in the image)."""
+    model, cores = cpu_info()
+    m = int(round(CONFIGS[config][1] * (grid / CONFIGS[config][0]) ** 3))
+    r = cpu_run(grid, m, reps, cores)
+    if r is None:
+        return {"value": None, "unit": "KKT solves/s", "cores": cores, "kind": "port",
+                "sample": "no BLAS library found for the CPU restatement"}
+    t_step = r["factor_s_median"] + r["solve_refine_s_median"]
     return {
-        "value": (1.0 / dt) * (fl / full_fl),
+        "value": (1.0 / t_step) * (r["sum_colcount_sq"] / full_fl),
         "unit": "KKT solves/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": (f"oracle (CPU FP64 restatement, 1 thread) on the same KKT construction at grid {g}^3 "
-                   f"(n={n}, m={m}): factor + 4-RHS solve/refine to 1e-10 took {dt:.2f} s "
-                   f"(worst bwd {worst:.1e}); scaled by sum c_j^2 ratio {fl:.3e}/{full_fl:.3e}"),
+        "sample": (f"CPU restatement, not the reference binary: BLAS-3 supernodal LDL^T (DGEMM / DSYTRF / DTRSM, "
+                   f"tree levels in parallel) on {cores} threads of {model}, {r['blas'][:60]}; KKT construction at "
+                   f"grid {grid}^3 (N={r['N']}, sum c^2={r['sum_colcount_sq']:.3e}): symbolic {r['symbolic_s']:.2f} s "
+                   f"once, factor {r['factor_s_median']:.2f} s and 4-RHS solve+refine to 1e-10 "
+                   f"{r['solve_refine_s_median']:.2f} s (medians of {reps} after a warm-up, worst bwd "
+                   f"{r['worst_backward_error']:.1e}); scaled to the full configuration by sum c^2 "
+                   f"{r['sum_colcount_sq']:.3e} -> {full_fl:.3e}"),
+        "measured": r,
+        "cpu_model": model,
     }
 
 
@@ -164,6 +223,82 @@ def mixed_leg(s, lib, step, bwd, dX, x64, args, torch):
             "value": args.steps / dt, "unit": "KKT solves/s", "ms_per_step": dt / args.steps * 1e3,
             "dtype": "f32 factor / f64 refinement", "worst_backward_error": worst, "refine_rounds": rounds,
             "forward_error_vs_fp64_per_rhs": fwd, "kernel_ms_per_step": kt}
+
+
+def precision_leg(args, grid=96, ill=False):
+    """The mixed-precision configuration: the KKT of a grid^3 Laplacian (+1e-3 I)
+    with grid^3 / 13.5 constraint rows of 8 N(0,1) entries (96^3: 65,536
+    rows), 4 right-hand sides, factored in FP64, FP32 and BF16 (FP32 storage,
+    three-term BF16 MFMA updates) on the same inputs and refined in FP64 to a
+    backward error <= 1e-10 on K; ill=True scales H's diagonal by 10^U[-3,3].
+    Per precision: first factor untimed, one untimed step, then
+    args.mixed_steps timed steps; the forward error of every right-hand side
+    against the FP64 solution."""
+    from nasoq_amd import _lib, lbl
+    from nasoq_amd import problems as P
+    import torch
+
+    m = CONFIGS["kkt96"][1] if grid == 96 else int(round(65536 * (grid / 96.0) ** 3))
+    t = time.perf_counter()
+    n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 8, shift=1e-3, seed=0, nrhs=4, ill=ill)
+    s = lbl.SolverSettings(n, Ap, Ai, Ax)
+    s.reg_diag = 1e-9
+    s.cost_param = 16
+    s.target_bwd_err = 1e-10
+    s.max_refine_rounds = 20
+    assert s.symbolic_analysis() == 1
+    fl, _ = s.flops()
+    t_sym = time.perf_counter() - t
+    print(f"[bench] precision leg {grid}^3{' ill' if ill else ''}: N={n}, generated and analysed in {t_sym:.1f}s, "
+          f"xsize={s.sym_scalar('xsize'):.3e} sum c^2={fl:.3e}", file=sys.stderr, flush=True)
+    lib = s.lib
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")
+    dX = torch.zeros_like(dB)
+    torch.cuda.synchronize()
+    bwd = np.zeros(4)
+
+    def step():
+        _lib.check(lib.lblg_refactor_resident(s.h), "refactor")
+        _lib.check(lib.lblg_solve_device(s.h, 4, ctypes.c_void_p(dB.data_ptr()), n,
+                                         ctypes.c_void_p(dX.data_ptr()), n, _lib.dptr(bwd)), "solve")
+
+    res, x64 = {}, None
+    for name, prec in (("fp64", 0), ("fp32", 1), ("bf16", 2)):
+        s.precision = prec
+        s.numerical_factorization()
+        step()
+        _lib.check(lib.lblg_synchronize(s.h), "sync")
+        torch.cuda.synchronize()
+        worst, rounds = 0.0, []
+        t0 = time.perf_counter()
+        for _ in range(args.mixed_steps):
+            step()
+            worst = max(worst, float(bwd.max()))
+            rounds.append(int(lib.lblg_num_ref_iter(s.h)))
+        _lib.check(lib.lblg_synchronize(s.h), "sync")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.mixed_steps
+        assert worst <= 1e-10, f"{name} backward error {worst} above 1e-10"
+        X = dX.cpu().numpy().copy()
+        if x64 is None:
+            x64 = X
+        fwd = [float(np.linalg.norm(X[c] - x64[c]) / max(np.linalg.norm(x64[c]), 1e-300)) for c in range(4)]
+        res[name] = {"value": 1.0 / dt, "unit": "KKT solves/s", "ms_per_step": dt * 1e3, "worst_backward_error": worst,
+                     "refine_rounds": rounds, "inertia": list(s.inertia()),
+                     "forward_error_vs_fp64_per_rhs": fwd}
+        print(f"[bench] {grid}^3{' ill' if ill else ''} {name}: {dt * 1e3:.0f} ms/step rounds={rounds} "
+              f"worst={worst:.1e} fwd={max(fwd):.1e} inertia={res[name]['inertia']}", file=sys.stderr, flush=True)
+    s.close()
+    del dB, dX
+    torch.cuda.empty_cache()
+    base = res["fp64"]["value"]
+    out = {"workload": f"KKT of 3-D Laplacian {grid}^3 + 1e-3 I{' with diagonal x 10^U[-3,3]' if ill else ''}, "
+                       f"{m} constraint rows of 8 N(0,1), 4 RHS, refined to 1e-10 on K",
+           "N": n, "sum_colcount_sq": fl, "steps": args.mixed_steps,
+           "bf16_form": "FP32 storage of S K S (diagonal scaling), BF16 MFMA operands hi+lo (3 terms), FP32 accumulation",
+           **res,
+           "speedup_fp32_vs_fp64": res["fp32"]["value"] / base, "speedup_bf16_vs_fp64": res["bf16"]["value"] / base}
+    return out
 
 
 def bench_batch(args):
@@ -402,13 +537,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS) + ["batch1024", "learned"])
+    ap.add_argument("--config", default="kkt64", choices=sorted(CONFIGS) + ["batch1024", "learned", "cpu64"])
     ap.add_argument("--batch", type=int, default=1024, help="instances of the batch1024 configuration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-factor + FP64-refinement leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-qp", action="store_true", help="skip the plain NASOQ QP configuration leg")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 96^3 precision leg (FP64 / FP32 / BF16 and the ill-conditioned variant)")
+    ap.add_argument("--mixed-steps", type=int, default=2, help="timed steps per precision of the 96^3 leg")
+    ap.add_argument("--precision-grid", type=int, default=96, help="grid of the precision leg")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the device schedule with lblg_autotune before timing (plain and batched legs)")
     ap.add_argument("--learned-grid", type=int, default=12, help="grid side of the learned leg's QP family")
@@ -443,6 +582,13 @@ def main():
         return
     if args.config == "learned":
         print(json.dumps(learned_leg(args)))
+        return
+    if args.config == "cpu64":  # the CPU restatement at the full 64^3 configuration
+        model, cores = cpu_info()
+        r = cpu_run(64, CONFIGS["kkt64"][1], 5, cores)
+        print(json.dumps({"workload": "CPU restatement (not the reference binary) of the 64^3 KKT configuration",
+                          "cpu_model": model, "cores": cores, **(r or {}),
+                          "kkt_solves_per_s": 1.0 / (r["factor_s_median"] + r["solve_refine_s_median"]) if r else None}))
         return
     grid, m, k = CONFIGS[args.config]
     t = time.perf_counter()
@@ -570,6 +716,10 @@ def main():
     s.close()
     del dB, dX
     torch.cuda.empty_cache()
+    if not args.no_precision and args.config == "kkt64":
+        print("[bench] precision leg ...", file=sys.stderr, flush=True)
+        out["precision_96"] = precision_leg(args, grid=args.precision_grid)
+        out["precision_96_ill"] = precision_leg(args, grid=args.precision_grid, ill=True)
     if not args.no_batched and args.config == "kkt64":
         print("[bench] batched leg ...", file=sys.stderr, flush=True)
         out["batched"] = batch_leg(args)[0]

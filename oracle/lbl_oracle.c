@@ -174,7 +174,7 @@ int oracle_sytf2(int n, double* a, int lda, int* ipiv) {
 /* L, split out the 2x2 sub-diagonals into D[lda_D + i], return the composed */
 /* local permutation perm[new] = old.                                         */
 /* ------------------------------------------------------------------------ */
-static int reorder_after_sytrf(int n, double* a, int lda, const int* ipiv, int* perm, double* D,
+int oracle_reorder_after_sytrf(int n, double* a, int lda, const int* ipiv, int* perm, double* D,
                                int64_t lda_D) {
   int* swapv = (int*)malloc(sizeof(int) * (n + 1));
   int* kind = (int*)malloc(sizeof(int) * (n + 1));
@@ -228,7 +228,7 @@ static int reorder_after_sytrf(int n, double* a, int lda, const int* ipiv, int* 
 /* supernodal factorization                                                  */
 /* ------------------------------------------------------------------------ */
 /* Lower CSC of P K P^T (rows sorted) with P given as pinv (old -> new). */
-static void permute_lower(int n, const int* Ap, const int* Ai, const double* Ax, const int* pinv,
+void oracle_permute_pinv(int n, const int* Ap, const int* Ai, const double* Ax, const int* pinv,
                           int** Bp_o, int** Bi_o, double** Bx_o) {
   int nnz = Ap[n];
   int* cnt = (int*)calloc(n + 1, sizeof(int));
@@ -303,7 +303,7 @@ int oracle_factor(int n, const int* Ap, const int* Ai, const double* Ax, double 
   for (int k = 0; k < n; ++k) pinv[perm[k]] = k;
   int *Bp, *Bi;
   double* Bx;
-  permute_lower(n, Ap, Ai, Kx, pinv, &Bp, &Bi, &Bx);
+  oracle_permute_pinv(n, Ap, Ai, Kx, pinv, &Bp, &Bi, &Bx);
   free(Kx);
 
   int* col2sup = (int*)malloc(sizeof(int) * n);
@@ -411,7 +411,7 @@ int oracle_factor(int n, const int* Ap, const int* Ai, const double* Ax, double 
     }
     /* Bunch-Kaufman on the diagonal block */
     oracle_sytf2(w, cur, r, ipiv);
-    int is_perm = reorder_after_sytrf(w, cur, r, ipiv, lperm, d_val + c0, n);
+    int is_perm = oracle_reorder_after_sytrf(w, cur, r, ipiv, lperm, d_val + c0, n);
     const int rows = r - w;
     if (is_perm) {
       /* column permutation of the off-diagonal rows: new col k <- old col lperm[k] */
@@ -693,7 +693,7 @@ void oracle_permute_lower(int n, const int* Ap, const int* Ai, const double* Ax,
   for (int k = 0; k < n; ++k) pinv[perm[k]] = k;
   int *tp, *ti;
   double* tx;
-  permute_lower(n, Ap, Ai, Ax, pinv, &tp, &ti, &tx);
+  oracle_permute_pinv(n, Ap, Ai, Ax, pinv, &tp, &ti, &tx);
   memcpy(Bp, tp, sizeof(int) * (n + 1));
   memcpy(Bi, ti, sizeof(int) * Ap[n]);
   memcpy(Bx, tx, sizeof(double) * Ap[n]);

@@ -201,3 +201,82 @@ def backward_error(n, Ap, Ai, Ax, x, b):
     return lib().oracle_backward_error(n, _ip(Ap), _ip(Ai), _dp(Ax),
                                        _dp(np.ascontiguousarray(x, dtype=np.float64)),
                                        _dp(np.ascontiguousarray(b, dtype=np.float64)))
+
+
+# --------------------------------------------------------------------------
+# BLAS-3 CPU restatement (cpu_lbl.c): the same factorization on DGEMM /
+# DSYTRF / DTRSM of a BLAS library loaded at run time, level-parallel with
+# OpenMP.  The bench's CPU baseline ("CPU restatement, not the reference
+# binary").
+# --------------------------------------------------------------------------
+CPU_SO = os.path.join(HERE, "liboracle_cpu.so")
+BLAS_CANDIDATES = ["/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1"]
+_c = None
+
+
+def cpu_lib():
+    """The BLAS-3 restatement with its BLAS loaded, or None when no BLAS library is found."""
+    global _c
+    if _c is None:
+        if not os.path.exists(CPU_SO):
+            build()
+        c = ctypes.CDLL(CPU_SO)
+        c.cpu_lbl_init.restype = ctypes.c_int
+        c.cpu_lbl_init.argtypes = [ctypes.c_char_p]
+        c.cpu_lbl_blas_name.restype = ctypes.c_char_p
+        c.cpu_lbl_factor.restype = ctypes.c_int
+        c.cpu_lbl_factor.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_double, _i32p, ctypes.c_int, _i32p,
+                                     _i64p, _i32p, _i64p, _i32p, _f64p, _f64p, _i32p, _i32p, ctypes.c_int]
+        c.cpu_lbl_solve.restype = None
+        c.cpu_lbl_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i64p, _i32p, _i64p, _f64p,
+                                    _f64p, _f64p, ctypes.c_int]
+        ok = False
+        for path in BLAS_CANDIDATES:
+            if os.path.exists(path) and c.cpu_lbl_init(path.encode()) == 0:
+                ok = True
+                break
+        if not ok:
+            return None
+        _c = c
+    return _c
+
+
+class CpuFactor:
+    """FP64 LDL^T of K~ by the BLAS-3 restatement on `threads` threads, with
+    the same symbolic structure as Factor; solve() works on several
+    right-hand sides at once in the original ordering."""
+
+    def __init__(self, n, Ap, Ai, Ax, reg_diag, perm, sup, pi, s, p, sparent, threads):
+        c = cpu_lib()
+        self.n, self.threads = n, threads
+        self.sup = np.ascontiguousarray(sup, dtype=np.int32)
+        self.pi = np.ascontiguousarray(pi, dtype=np.int64)
+        self.p = np.ascontiguousarray(p, dtype=np.int64)
+        self.nsuper = len(self.sup) - 1
+        self.valL = np.empty(int(self.p[-1]), dtype=np.float64)
+        self.d_val = np.empty(2 * n, dtype=np.float64)
+        self.s_lab = np.empty(int(self.pi[-1]), dtype=np.int32)
+        self.perm = np.empty(n, dtype=np.int32)
+        args = [np.ascontiguousarray(x, dtype=t) for x, t in
+                ((Ap, np.int32), (Ai, np.int32), (Ax, np.float64), (perm, np.int32), (s, np.int32),
+                 (sparent, np.int32))]
+        self.pivoted = c.cpu_lbl_factor(n, _ip(args[0]), _ip(args[1]), _dp(args[2]), reg_diag, _ip(args[3]),
+                                        self.nsuper, _ip(self.sup), _lp(self.pi), _ip(args[4]), _lp(self.p),
+                                        _ip(args[5]), _dp(self.valL), _dp(self.d_val), _ip(self.s_lab),
+                                        _ip(self.perm), threads)
+
+    def solve(self, B):
+        B = np.asarray(B, dtype=np.float64)
+        one = B.ndim == 1
+        B2 = B.reshape(self.n, -1)
+        X = np.asfortranarray(B2[self.perm, :])
+        cpu_lib().cpu_lbl_solve(self.n, X.shape[1], self.nsuper, _ip(self.sup), _lp(self.pi), _ip(self.s_lab),
+                                _lp(self.p), _dp(self.valL), _dp(self.d_val), _dp(X), self.threads)
+        out = np.empty_like(B2)
+        out[self.perm, :] = X
+        return out[:, 0] if one else out
+
+    def inertia(self):
+        a, b, z = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        lib().oracle_inertia(self.n, _dp(self.d_val), ctypes.byref(a), ctypes.byref(b), ctypes.byref(z))
+        return a.value, b.value, z.value

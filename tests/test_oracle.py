@@ -290,3 +290,40 @@ def test_spmv_and_backward_error_vs_reference_build(oracle, ref):
     er = ref.ref_backward_error(n, oracle._ip(Ap), oracle._ip(Ai), oracle._dp(Ax), oracle._dp(x),
                                 oracle._dp(np.ascontiguousarray(B[:, 0])))
     assert np.isclose(eo, er, rtol=1e-13)
+
+
+def test_blas3_cpu_restatement_matches_oracle(oracle):
+    # the bench's CPU baseline (cpu_lbl.c: DGEMM / DSYTRF / DTRSM, level-parallel)
+    # factors the same matrix as oracle_factor: same D block structure and
+    # inertia, P L D L^T P^T = K~, solutions equal to the oracle's
+    from nasoq_amd import lbl
+    from nasoq_amd import problems as P
+    if oracle.cpu_lib() is None:
+        pytest.skip("no BLAS library for the CPU restatement")
+    for n_, Ap, Ai, Ax, B, perm in [P.kkt_config(8, 60, 6, seed=2, nrhs=3) + (None,),
+                                    P.block_indefinite(4, 60, seed=3, nrhs=3) + ("nat",)]:
+        s = lbl.SolverSettings(n_, Ap, Ai, Ax)
+        s.reg_diag = 1e-9
+        s.symbolic_analysis(perm=np.arange(n_, dtype=np.int32) if perm else None)
+        args = (n_, Ap, Ai, Ax, 1e-9, s.sym_ints("Perm"), s.sym_ints("super"), s.sym_longs("pi"), s.sym_ints("s"),
+                s.sym_longs("p"))
+        F = oracle.Factor(*args)
+        from conftest import dense_from_factor
+        K = P.lower_to_full(n_, Ap, Ai, Ax).toarray()
+        d = np.diag(K).copy()
+        K[np.arange(n_), np.arange(n_)] = np.where(np.abs(d) < 1e-9, 1e-9, d)
+        sup, pi = s.sym_ints("super"), s.sym_longs("pi")
+        px = s.sym_longs("p")[sup[:-1]]
+        for threads in (1, 4):
+            C = oracle.CpuFactor(*args, s.sym_ints("sParent"), threads)
+            assert np.array_equal(C.d_val[n_:] != 0, F.d_val[n_:] != 0)
+            assert C.inertia() == F.inertia()
+            L, D = dense_from_factor(n_, sup, pi, px, C.valL, C.s_lab, C.d_val)
+            Pm = np.eye(n_)[:, C.perm]
+            growth = (np.abs(L) @ np.abs(D) @ np.abs(L).T).max()
+            assert np.abs(Pm @ L @ D @ L.T @ Pm.T - K).max() <= 1e-12 * max(1.0, growth)
+            X = C.solve(B)
+            for c in range(B.shape[1]):  # unrefined solves: rounding amplified by the 1e-9 pivots
+                xo = F.solve(B[:, c])
+                assert np.linalg.norm(X[:, c] - xo) <= 1e-6 * max(1.0, np.linalg.norm(xo))
+        s.close()
