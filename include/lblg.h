@@ -343,6 +343,15 @@ typedef struct lblg_qp_info {
   double complementarity_infn;  /* ||z .* (Bx - b)||_inf */
   int64_t factorizations;       /* device KKT factorizations */
   int64_t kkt_solves;           /* device KKT solves */
+  /* engine extension (no reference counterpart): host wall time of the KKT
+   * factorizations (value upload included) and of the refined KKT solves,
+   * and the supernodes those factorizations refactored, all summed since the
+   * handle was created (as factorizations and kkt_solves are); the number of
+   * supernodes of the super-KKT */
+  double factor_seconds;
+  double solve_seconds;
+  int64_t refactored_supernodes;
+  int64_t supernodes;
 } lblg_qp_info;
 
 void lblg_qp_default_settings(lblg_qp_settings* s);

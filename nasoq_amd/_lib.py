@@ -73,6 +73,10 @@ class QPInfo(ctypes.Structure):
         ("complementarity_infn", ctypes.c_double),
         ("factorizations", ctypes.c_int64),
         ("kkt_solves", ctypes.c_int64),
+        ("factor_seconds", ctypes.c_double),
+        ("solve_seconds", ctypes.c_double),
+        ("refactored_supernodes", ctypes.c_int64),
+        ("supernodes", ctypes.c_int64),
     ]
 
 

@@ -219,6 +219,10 @@ extern "C" int lblg_qp_get_info(const lblg_qp* qp, lblg_qp_info* info) {
   info->complementarity_infn = s->complementarity_infn;
   info->factorizations = s->factorizations;
   info->kkt_solves = s->kkt_solves;
+  info->factor_seconds = s->factor_seconds;
+  info->solve_seconds = s->solve_seconds;
+  info->refactored_supernodes = s->refactored_supernodes;
+  info->supernodes = s->h ? lblg_symbolic_scalar(s->h, 1) : 0;
   return LBLG_OK;
 }
 

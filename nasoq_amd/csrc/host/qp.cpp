@@ -4,6 +4,7 @@
 #include "host/qp.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -233,14 +234,20 @@ void QpSolver::analyze_lbl(const lblg_settings& st) {
 // Nasoq::solve_kkt (nasoq.cpp:918-949): refactor (UPDATE / REFACTOR) or reuse
 // the factor (SOLVE), then the refined solve
 void QpSolver::solve_kkt(bool refactor) {
+  using clock = std::chrono::steady_clock;
   if (refactor) {
+    const auto t0 = clock::now();
     check(lblg_set_values(h, SM.x.data()), "qp: set values");
     check(lblg_factor(h), "qp: factor");
+    factor_seconds += std::chrono::duration<double>(clock::now() - t0).count();
+    refactored_supernodes += lblg_symbolic_scalar(h, 1);
     ++factorizations;
   }
   const int64_t N = SM.ncol;
   double bwd = 0.0;
+  const auto t0 = clock::now();
   check(lblg_solve(h, 1, rhs.data(), N, sol.data(), N, &bwd), "qp: solve");
+  solve_seconds += std::chrono::duration<double>(clock::now() - t0).count();
   ++kkt_solves;
 }
 

@@ -43,6 +43,8 @@ struct QpSolver {
   double primal_obj = 0, dual_obj = 0, objective = 0, lag_res = 0, cons_sat_norm = 0, non_negativity_infn = 0,
          complementarity_infn = 0;
   int64_t factorizations = 0, kkt_solves = 0;
+  double factor_seconds = 0, solve_seconds = 0;  // host wall time in the KKT factorizations / solves
+  int64_t refactored_supernodes = 0;             // supernodes refactored, summed over the factorizations
   // super-KKT [H A' B'; A 0 0; B 0 0] (lower CSC, explicit diagonal slots)
   SparseCsc SM;
   std::vector<int64_t> bpos;                // SM entry of every entry of B (B's CSC order)

@@ -308,29 +308,97 @@ int cpu_lbl_factor(int n, const int* Ap, const int* Ai, const double* Ax, double
   return npiv;
 }
 
+/* forward step of supernode t: X_t := L_tt^-1 X_t, X_rows -= L_rt X_t
+ * (atomic scatter when supernodes of one level run concurrently) */
+static void fwd_one(int t, int n, int nrhs, const int* super, const int64_t* pi, const int* s_lab, const int64_t* p,
+                    const double* valL, double* X, double* Y, int atomic) {
+  const double one = 1.0, zero = 0.0;
+  const int c0 = super[t], w = super[t + 1] - c0, r = (int)(pi[t + 1] - pi[t]), rows = r - w;
+  const double* L = valL + p[c0];
+  p_dtrsm("L", "L", "N", "U", &w, &nrhs, &one, L, &r, X + c0, &n);
+  if (rows <= 0) return;
+  p_dgemm("N", "N", &rows, &nrhs, &w, &one, L + w, &r, X + c0, &n, &zero, Y, &rows);
+  for (int c = 0; c < nrhs; ++c)
+    for (int i = 0; i < rows; ++i) {
+      double* dst = X + (int64_t)c * n + s_lab[pi[t] + w + i];
+      const double v = Y[(int64_t)c * rows + i];
+      if (atomic) {
+#pragma omp atomic
+        *dst -= v;
+      } else {
+        *dst -= v;
+      }
+    }
+}
+
+/* backward step of supernode t: X_t -= L_rt^T X_rows, X_t := L_tt^-T X_t
+ * (reads ancestors' final values only; no conflicts within a level) */
+static void bwd_one(int t, int n, int nrhs, const int* super, const int64_t* pi, const int* s_lab, const int64_t* p,
+                    const double* valL, double* X, double* Y) {
+  const double one = 1.0, mone = -1.0;
+  const int c0 = super[t], w = super[t + 1] - c0, r = (int)(pi[t + 1] - pi[t]), rows = r - w;
+  const double* L = valL + p[c0];
+  if (rows > 0) {
+    for (int c = 0; c < nrhs; ++c)
+      for (int i = 0; i < rows; ++i) Y[(int64_t)c * rows + i] = X[(int64_t)c * n + s_lab[pi[t] + w + i]];
+    p_dgemm("T", "N", &w, &nrhs, &rows, &mone, L + w, &r, Y, &rows, &one, X + c0, &n);
+  }
+  p_dtrsm("L", "L", "T", "U", &w, &nrhs, &one, L, &r, X + c0, &n);
+}
+
 /*
  * X (n x nrhs, column-major, leading dimension n, pivoted ordering) :=
  * L^-T D^-1 L^-1 X with the factor of cpu_lbl_factor: per supernode DTRSM on
  * the diagonal block and DGEMM for the rows below, all columns at once.
+ * Supernodes of one tree level (height above the leaves; the parent of a
+ * supernode is the supernode of its first row below the diagonal block) run
+ * in parallel with single-threaded BLAS, as in the factor; levels with fewer
+ * supernodes than threads run one at a time with threaded BLAS.
  */
 void cpu_lbl_solve(int n, int nrhs, int nsuper, const int* super, const int64_t* pi, const int* s_lab,
                    const int64_t* p, const double* valL, const double* d_val, double* X, int nthreads) {
-  blas_threads(nthreads < 1 ? 1 : nthreads);
+  if (nthreads < 1) nthreads = 1;
   int maxr = 1;
   for (int t = 0; t < nsuper; ++t)
     if (pi[t + 1] - pi[t] > maxr) maxr = (int)(pi[t + 1] - pi[t]);
-  double* Y = (double*)malloc(sizeof(double) * (size_t)maxr * nrhs);
-  const double one = 1.0, zero = 0.0, mone = -1.0;
+  int* col2sup = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int t = 0; t < nsuper; ++t)
+    for (int c = super[t]; c < super[t + 1]; ++c) col2sup[c] = t;
+  int* lev = (int*)calloc((size_t)(nsuper > 0 ? nsuper : 1), sizeof(int));
+  int nlev = 0;
   for (int t = 0; t < nsuper; ++t) {
-    const int c0 = super[t], w = super[t + 1] - c0, r = (int)(pi[t + 1] - pi[t]), rows = r - w;
-    const double* L = valL + p[c0];
-    p_dtrsm("L", "L", "N", "U", &w, &nrhs, &one, L, &r, X + c0, &n);
-    if (rows > 0) {
-      p_dgemm("N", "N", &rows, &nrhs, &w, &one, L + w, &r, X + c0, &n, &zero, Y, &rows);
-      for (int c = 0; c < nrhs; ++c)
-        for (int i = 0; i < rows; ++i) X[(int64_t)c * n + s_lab[pi[t] + w + i]] -= Y[(int64_t)c * rows + i];
+    if (lev[t] + 1 > nlev) nlev = lev[t] + 1;
+    const int w = super[t + 1] - super[t];
+    if (pi[t + 1] - pi[t] > w) {
+      const int par = col2sup[s_lab[pi[t] + w]];
+      if (lev[par] < lev[t] + 1) lev[par] = lev[t] + 1;
     }
   }
+  int* lptr = (int*)calloc(nlev + 1, sizeof(int));
+  for (int t = 0; t < nsuper; ++t) lptr[lev[t] + 1]++;
+  for (int l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
+  int* lset = (int*)malloc(sizeof(int) * (size_t)(nsuper > 0 ? nsuper : 1));
+  int* lfill = (int*)malloc(sizeof(int) * (nlev + 1));
+  memcpy(lfill, lptr, sizeof(int) * (nlev + 1));
+  for (int t = 0; t < nsuper; ++t) lset[lfill[lev[t]]++] = t;
+  free(lfill);
+  double* Yall = (double*)malloc(sizeof(double) * (size_t)maxr * nrhs * nthreads);
+  for (int l = 0; l < nlev; ++l) {
+    const int cnt = lptr[l + 1] - lptr[l];
+    if (cnt >= nthreads && nthreads > 1) {
+#pragma omp parallel num_threads(nthreads)
+      {
+        blas_threads(1);
+        double* Y = Yall + (size_t)maxr * nrhs * omp_get_thread_num();
+#pragma omp for schedule(dynamic, 16)
+        for (int q = lptr[l]; q < lptr[l + 1]; ++q) fwd_one(lset[q], n, nrhs, super, pi, s_lab, p, valL, X, Y, 1);
+      }
+    } else {
+      blas_threads(nthreads);
+      for (int q = lptr[l]; q < lptr[l + 1]; ++q) fwd_one(lset[q], n, nrhs, super, pi, s_lab, p, valL, X, Yall, 0);
+    }
+  }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
   for (int c = 0; c < nrhs; ++c) {
     double* x = X + (int64_t)c * n;
     for (int i = 0; i < n; ++i) {
@@ -346,15 +414,24 @@ void cpu_lbl_solve(int n, int nrhs, int nsuper, const int* super, const int64_t*
       }
     }
   }
-  for (int t = nsuper - 1; t >= 0; --t) {
-    const int c0 = super[t], w = super[t + 1] - c0, r = (int)(pi[t + 1] - pi[t]), rows = r - w;
-    const double* L = valL + p[c0];
-    if (rows > 0) {
-      for (int c = 0; c < nrhs; ++c)
-        for (int i = 0; i < rows; ++i) Y[(int64_t)c * rows + i] = X[(int64_t)c * n + s_lab[pi[t] + w + i]];
-      p_dgemm("T", "N", &w, &nrhs, &rows, &mone, L + w, &r, Y, &rows, &one, X + c0, &n);
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int cnt = lptr[l + 1] - lptr[l];
+    if (cnt >= nthreads && nthreads > 1) {
+#pragma omp parallel num_threads(nthreads)
+      {
+        blas_threads(1);
+        double* Y = Yall + (size_t)maxr * nrhs * omp_get_thread_num();
+#pragma omp for schedule(dynamic, 16)
+        for (int q = lptr[l]; q < lptr[l + 1]; ++q) bwd_one(lset[q], n, nrhs, super, pi, s_lab, p, valL, X, Y);
+      }
+    } else {
+      blas_threads(nthreads);
+      for (int q = lptr[l]; q < lptr[l + 1]; ++q) bwd_one(lset[q], n, nrhs, super, pi, s_lab, p, valL, X, Yall);
     }
-    p_dtrsm("L", "L", "T", "U", &w, &nrhs, &one, L, &r, X + c0, &n);
   }
-  free(Y);
+  free(Yall);
+  free(lset);
+  free(lptr);
+  free(lev);
+  free(col2sup);
 }

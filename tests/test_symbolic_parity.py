@@ -85,6 +85,13 @@ def test_kkt_config_shape_nd(oracle, ref):
     compare(oracle, ref, n, Ap, Ai, Ax, lbl.nd_order(n, Ap, Ai), None, 8)
 
 
+def test_kkt_configuration_64(oracle, ref):
+    # the full plain-LBL configuration (64^3 Laplacian + 16,384 constraint
+    # rows, N = 278,528) with the bench's level-schedule parameter
+    n, Ap, Ai, Ax, _ = P.kkt_config(64, 16384, 8, seed=0)
+    compare(oracle, ref, n, Ap, Ai, Ax, lbl.nd_order(n, Ap, Ai), None, 16)
+
+
 def test_natural_order_tridiagonal_closed_form(oracle, ref):
     # tridiagonal: etree parent(i) = i+1, column counts 2 (last 1), one supernode
     n = 50
