@@ -492,6 +492,46 @@ def qp_leg(args):
     return out
 
 
+def qp_batch_leg(args):
+    """The batched QP configuration: `args.qp_batch` independent box-constrained
+    QPs (H = 2-D Laplacian r x c + 1e-2 I with r, c ~ U{16..160}, 1..64
+    equality rows of 4 N(0,1), |x| <= 1), each solved to NASOQ's own KKT test
+    by its dual active-set loop with the KKT factor in FP32 (FP64
+    refinement) and, beside it, in FP64.  Every iteration's KKT factor is
+    the SoMod update along the tree paths of the edited rows; `args.qp_threads`
+    host threads run the problems concurrently, one engine handle and HIP
+    stream each.  Throughput is whole-batch wall time."""
+    from nasoq_amd import problems as P
+    from nasoq_amd import qp
+    probs = P.box_qp_batch(args.qp_batch, seed=0)
+    sizes = [p[0].shape[0] for p in probs]
+    settings = {"eps_abs": 1e-6, "variant": qp.FIXED}
+    qp.solve_many(probs[:2], settings, threads=2, keep=())  # warm-up: library load, first launches
+    out = {"workload": f"{len(probs)} box-constrained QPs, H = 2-D Laplacian r x c + 1e-2 I (r, c ~ U{{16..160}}), "
+                       f"1..64 equality rows of 4 N(0,1), |x| <= 1, eps 1e-6",
+           "qps": len(probs), "n_total": int(sum(sizes)), "n_range": [int(min(sizes)), int(max(sizes))],
+           "threads": args.qp_threads}
+    objs = {}
+    for prec, name in ((1, "fp32_factor"), (0, "fp64")):
+        t0 = time.perf_counter()
+        res = qp.solve_many(probs, dict(settings, precision=prec), threads=args.qp_threads, keep=())
+        dt = time.perf_counter() - t0
+        fac = sum(r["factorizations"] for r in res)
+        full = sum(r["factorizations"] * r["supernodes"] for r in res)
+        out[name] = {"seconds": dt, "qps_per_s": len(res) / dt, "optimal": sum(r["status"] == qp.OPTIMAL for r in res),
+                     "active_set_iterations": int(sum(r["num_iter"] for r in res)),
+                     "kkt_solves": int(sum(r["kkt_solves"] for r in res)),
+                     "kkt_solves_per_s": sum(r["kkt_solves"] for r in res) / dt, "kkt_factorizations": int(fac),
+                     "refactored_fraction": sum(r["refactored_supernodes"] for r in res) / max(full, 1)}
+        objs[name] = np.array([r["objective"] for r in res])
+        print(f"[bench] qp batch {name}: {len(res)} QPs in {dt:.1f} s, {out[name]['optimal']} optimal, "
+              f"{out[name]['active_set_iterations']} iterations, {out[name]['kkt_solves_per_s']:.0f} KKT solves/s, "
+              f"refactored fraction {out[name]['refactored_fraction']:.3f}", file=sys.stderr, flush=True)
+    rel = np.abs(objs["fp32_factor"] - objs["fp64"]) / np.maximum(1.0, np.abs(objs["fp64"]))
+    out["objective_rel_diff_fp32_vs_fp64_max"] = float(rel.max())
+    return out
+
+
 def learned_leg(args):
     """Learned warm start on a parametric QP family: min 1/2 x'Hx + (U theta)'x
     s.t. Bx <= d with H the 5-point Laplacian of a g x g grid + 1e-2 I,
@@ -557,6 +597,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-qp", action="store_true", help="skip the plain NASOQ QP configuration leg")
+    ap.add_argument("--qp-batch", type=int, default=64, help="box-constrained QPs of the batched QP leg (0: skip)")
+    ap.add_argument("--qp-threads", type=int, default=16, help="host threads of the batched QP leg")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 96^3 precision leg (FP64 / FP32 / BF16 and the ill-conditioned variant)")
     ap.add_argument("--mixed-steps", type=int, default=2, help="timed steps per precision of the 96^3 leg")
@@ -760,6 +802,9 @@ def main():
     if not args.no_qp and args.config == "kkt64":
         print("[bench] plain NASOQ QP configuration ...", file=sys.stderr, flush=True)
         out["qp_configuration"] = qp_leg(args)
+    if args.qp_batch > 0 and args.config == "kkt64":
+        print("[bench] batched box-constrained QPs ...", file=sys.stderr, flush=True)
+        out["qp_batch"] = qp_batch_leg(args)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.config, fl, xs)

@@ -137,6 +137,23 @@ int lblg_refactor_resident(lblg_handle* h);
 /* Replace the values of K (same pattern; the reference edits A->x in place). */
 int lblg_set_values(lblg_handle* h, const double* Ax);
 
+/* Change some values of K in place: Ax[idx[k]] = vals[k], idx being positions
+ * in the lower-CSC arrays given to lblg_create (add_del_matrix_qp,
+ * linear_solver_wrapper.cpp:1020-1137, edits A_ord->x the same way).  The
+ * next lblg_factor is then the SoMod update (update_factorization,
+ * linear_solver_wrapper.cpp:1151-1290, with update_ldl_left_sn_02_v2,
+ * src/ldl/Serial_update_ldl.cpp:17-277): it uploads only the changed values
+ * and refactors only the supernodes on the elimination-tree paths from the
+ * changed entries to the root, as long as the handle holds a factor of the
+ * previous values with the same precision, reg_diag and regularization
+ * (otherwise it factors everything).  The FP32 / BF16 factors then keep the
+ * diagonal scaling of the last full factorization.  Indices out of range:
+ * LBLG_ERR_ARG, nothing changed. */
+int lblg_update_values(lblg_handle* h, int64_t count, const int64_t* idx, const double* vals);
+/* Supernodes the last lblg_factor refactored (all of them after a full
+ * factorization; engine extension, the reference's modified_sns count). */
+int64_t lblg_refactored_supernodes(const lblg_handle* h);
+
 /* SolverSettings::solve_only(n_rhs, rhs) (linear_solver_wrapper.cpp:1300-1404):
  * solve_phase_ldl_blocked_parallel_permuted (src/linear_solver/solve_phase.cpp:113-128)
  * then pmgmres_ldlt_auto refinement (src/gmres/mgmres.cpp:1015-1211) on the
@@ -325,6 +342,9 @@ typedef struct lblg_qp_settings {
   int max_iter_nas;     /* active-set iteration cap, default 4000 */
   int variant;          /* LBLG_QP_*, default LBLG_QP_FIXED */
   int ordering;         /* LBLG_ORDER_ND (default) or LBLG_ORDER_NATURAL */
+  int precision;        /* factor of the KKT systems: LBLG_F64 (default; the
+                           reference's), LBLG_F32 or LBLG_BF16 with FP64
+                           refinement (engine extension) */
 } lblg_qp_settings;
 
 /* Results and KKT checks of the last solve (the reference's public fields). */

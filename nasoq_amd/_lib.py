@@ -51,6 +51,7 @@ class QPSettings(ctypes.Structure):
         ("max_iter_nas", ctypes.c_int),
         ("variant", ctypes.c_int),
         ("ordering", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 
@@ -104,6 +105,8 @@ SIGNATURES = [
     ("lblg_factor", ctypes.c_int, [_vp]),
     ("lblg_refactor_resident", ctypes.c_int, [_vp]),
     ("lblg_set_values", ctypes.c_int, [_vp, _f64p]),
+    ("lblg_update_values", ctypes.c_int, [_vp, ctypes.c_int64, _i64p, _f64p]),
+    ("lblg_refactored_supernodes", ctypes.c_int64, [_vp]),
     ("lblg_solve", ctypes.c_int, [_vp, ctypes.c_int, _f64p, ctypes.c_int64, _f64p, ctypes.c_int64, _f64p]),
     ("lblg_solve_device", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _f64p]),
     ("lblg_num_ref_iter", ctypes.c_int, [_vp]),

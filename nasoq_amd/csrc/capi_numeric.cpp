@@ -87,7 +87,33 @@ extern "C" int lblg_set_values(lblg_handle* hh, const double* Ax) {
   if (!h || !Ax) return LBLG_ERR_ARG;
   std::memcpy(h->Ax.data(), Ax, sizeof(double) * h->Ax.size());
   h->factored = false;
+  h->values_dirty = true;
   return LBLG_OK;
+}
+
+extern "C" int lblg_update_values(lblg_handle* hh, int64_t count, const int64_t* idx, const double* vals) {
+  Handle* h = reinterpret_cast<Handle*>(hh);
+  if (!h || count < 0 || (count > 0 && (!idx || !vals))) return LBLG_ERR_ARG;
+  const int64_t nnz = (int64_t)h->Ax.size();
+  for (int64_t k = 0; k < count; ++k)
+    if (idx[k] < 0 || idx[k] >= nnz) return LBLG_ERR_ARG;
+  if (h->changed_flag.size() != h->Ax.size()) h->changed_flag.assign(h->Ax.size(), 0);
+  for (int64_t k = 0; k < count; ++k) {
+    const int64_t q = idx[k];
+    h->Ax[q] = vals[k];
+    if (!h->changed_flag[q]) {
+      h->changed_flag[q] = 1;
+      h->changed.push_back(q);
+    }
+  }
+  if (count > 0) h->factored = false;
+  return LBLG_OK;
+}
+
+extern "C" int64_t lblg_refactored_supernodes(const lblg_handle* hh) {
+  const Handle* h = reinterpret_cast<const Handle*>(hh);
+  if (!h) return LBLG_ERR_ARG;
+  return h->refactored;
 }
 
 static int do_solve(lblg_handle* hh, int nrhs, const double* B, int64_t ldb, double* X, int64_t ldx, bool dev,

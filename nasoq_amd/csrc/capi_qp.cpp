@@ -57,6 +57,7 @@ extern "C" void lblg_qp_default_settings(lblg_qp_settings* s) {
   s->max_iter_nas = 4000;
   s->variant = LBLG_QP_FIXED;
   s->ordering = LBLG_ORDER_ND;
+  s->precision = LBLG_F64;
 }
 
 extern "C" int lblg_qp_create(lblg_qp** out, int n, const int* Hp, const int* Hi, const double* Hx, const double* q,
@@ -110,7 +111,8 @@ extern "C" int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* st) {
   if (!s || !st) return LBLG_ERR_ARG;
   if (st->variant < LBLG_QP_FIXED || st->variant > LBLG_QP_PREDET || st->max_iter < 0 || st->max_iter_nas < 0 ||
       !(st->diag_perturb >= 0) || !(st->eps_abs > 0) || !(st->stop_tol >= 0) ||
-      (st->ordering != LBLG_ORDER_ND && st->ordering != LBLG_ORDER_NATURAL))
+      (st->ordering != LBLG_ORDER_ND && st->ordering != LBLG_ORDER_NATURAL) ||
+      (st->precision != LBLG_F64 && st->precision != LBLG_F32 && st->precision != LBLG_BF16))
     return LBLG_ERR_ARG;
   s->diag_perturb = st->diag_perturb;
   s->zero_thresh = st->diag_perturb;
@@ -120,6 +122,7 @@ extern "C" int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* st) {
   s->max_iter_nas = st->max_iter_nas;
   s->variant = st->variant;
   s->ordering = st->ordering;
+  s->precision = st->precision;
   return LBLG_OK;
 }
 
@@ -278,11 +281,10 @@ extern "C" int lblg_kkt_factor(lblg_qp* qp) {
   QpSolver* s = reinterpret_cast<QpSolver*>(qp);
   if (!s) return LBLG_ERR_ARG;
   if (!s->analysed) return LBLG_ERR_STATE;
-  int rc = lblg_set_values(s->h, s->SM.x.data());
-  if (rc) return rc;
-  rc = lblg_factor(s->h);
-  if (rc == LBLG_OK) ++s->factorizations;
-  return rc;
+  return guarded([&] {
+    s->factor_kkt();
+    return LBLG_OK;
+  });
 }
 
 extern "C" int lblg_kkt_solve(lblg_qp* qp, const double* rhs, double* x, double* bwd_err) {

@@ -53,6 +53,9 @@ struct FactorLevel {
   int* d_large = nullptr;
   std::vector<int> large_G, large_nb;  // workgroups and panel width per large supernode
   std::vector<int> large_nt, large_wsub;  // workgroup size and panels per trailing update
+  // host copies of the lists (the path-restricted refactorization filters them)
+  std::vector<int> h_small, h_off, h_sch, h_schb;
+  std::vector<int64_t> h_schb_ldoff;  // offset of each schur_big supernode's L D block
 };
 
 // solve pieces: supernodes split into column blocks of at most SOLVE_W
@@ -121,6 +124,23 @@ struct DeviceFactor {
   int wide_par = 0;             // which of sync[0], sync[1] the next wide panel counts on
   uint64_t* ll = nullptr;       // flag-in-word exchange buffer of the wide panel kernel
   uint32_t ll_epoch = 0;        // next epoch base (monotone for the handle's lifetime)
+  // path-restricted refactorization (the SoMod update): the supernodes on
+  // the tree paths from changed entries to the root are refactored; the
+  // others keep their panels, and those with rows in a refactored supernode
+  // send their Schur updates again, to the refactored supernodes only
+  bool path_refactor = true;    // LBLG_PATH_REFACTOR=0: every factorization is a full one
+  bool factor_valid = false;    // the device factor is that of the uploaded values
+  double factor_key[3] = {0, 0, 0};  // reg_diag, regularization, precision of that factor
+  int* asup = nullptr;          // entry of K -> supernode of its slot in valL
+  std::vector<int> hasup;
+  std::vector<int> hlevel;      // factor level of every supernode
+  std::vector<int> upd_ptr, upd_idx;  // per supernode: the supernodes with rows in its columns
+  std::vector<unsigned> smark;  // marks of the current refactorization (epoch stamps)
+  unsigned smark_epoch = 0;
+  unsigned char* tmask = nullptr;  // device: flag of every supernode refactored
+  char* blob_h = nullptr;       // pinned staging of the restricted lists and changed values
+  char* blob_d = nullptr;
+  size_t blob_cap = 0;
   double* red = nullptr;        // reduction scratch
   int* status = nullptr;        // device flags (zero pivots etc.)
   // FP32 / BF16 factors (prec 1, 2): the symmetric diagonal scaling S and S K S
@@ -178,6 +198,7 @@ struct DeviceFactor {
   };
   GraphCache gfactor, gsolve, gcycle;
   bool graphs = true;       // LBLG_GRAPH=0 turns capture off
+  bool panel_lds = true;         // one-workgroup panels of diagonal blocks that fit: staged in LDS
   bool schur_direct = true;      // 128-tile Schur updates with operands loaded lane by lane (no LDS)
   bool trailing_direct = true;   // the same for the wide panels' trailing updates
   bool has_coop = false;    // wide panels (cooperative, epoch-stamped): factor not captured

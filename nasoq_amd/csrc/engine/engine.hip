@@ -37,6 +37,8 @@ DeviceFactor::~DeviceFactor() {
   for (GraphCache* g : {&gfactor, &gsolve, &gcycle})
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
   for (void* p : owned) hipFree(p);
+  if (blob_d) hipFree(blob_d);
+  if (blob_h) hipHostFree(blob_h);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -219,6 +221,7 @@ static void setup(Handle& h) {
   D.nnzK = nnz;
   std::vector<int64_t> amap(nnz);
   std::vector<unsigned char> isdiag(nnz, 0);
+  D.hasup.assign(nnz, 0);
   for (int j = 0; j < n; ++j) {
     for (int q = h.Ap[j]; q < h.Ap[j + 1]; ++q) {
       const int i = h.Ai[q];
@@ -231,9 +234,12 @@ static void setup(Handle& h) {
       const int64_t pos = std::lower_bound(rb, re, rw) - rb;
       const int64_t rt = S.pi[t + 1] - S.pi[t];
       amap[q] = px[t] + (int64_t)(c - S.super[t]) * rt + pos;
+      D.hasup[q] = t;
     }
   }
   D.amap = dupload(D, amap);
+  D.asup = dupload(D, D.hasup);
+  D.tmask = dalloc<unsigned char>(D, std::max(ns, 1));
   if (D.prec != 0) {  // the scaled matrix S K S of the FP32 / BF16 factors
     std::vector<int> kcol(nnz);
     for (int j = 0; j < n; ++j)
@@ -378,6 +384,11 @@ static void setup(Handle& h) {
       }
     }
     wmax_total = std::max(wmax_total, wsum);
+    F.h_small = small;
+    F.h_off = off;
+    F.h_sch = sch;
+    F.h_schb = schb;
+    F.h_schb_ldoff = ldoffb;
     F.small.count = (int)small.size();
     F.small_wmax = 0;
     for (int s : small) F.small_wmax = std::max(F.small_wmax, S.super[s + 1] - S.super[s]);
@@ -419,6 +430,27 @@ static void setup(Handle& h) {
     ld_max = std::max(ld_max, ldsum);
     F.d_medium = dupload(D, F.medium);
     F.d_large = dupload(D, F.large);
+  }
+  D.hlevel = lev;
+  {  // supernodes with rows in each supernode's columns (distinct, ascending)
+    std::vector<int> cnt(ns + 1, 0);
+    auto each_target = [&](int d, auto f) {
+      const int w = S.super[d + 1] - S.super[d];
+      int last = -1;
+      for (int64_t q = S.pi[d] + w; q < S.pi[d + 1]; ++q) {
+        const int t = S.col2sup[S.s[q]];
+        if (t != last) f(t);
+        last = t;
+      }
+    };
+    for (int d = 0; d < ns; ++d) each_target(d, [&](int t) { ++cnt[t + 1]; });
+    for (int t = 0; t < ns; ++t) cnt[t + 1] += cnt[t];
+    D.upd_ptr = cnt;
+    D.upd_idx.resize(cnt[ns]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int d = 0; d < ns; ++d) each_target(d, [&](int t) { D.upd_idx[fill[t]++] = d; });
+    D.smark.assign(ns, 0);
+    D.smark_epoch = 0;
   }
   D.W = valloc(D, wmax_total);
   D.LD = valloc(D, std::max<int64_t>(ld_max, 1));
@@ -578,6 +610,8 @@ static void setup(Handle& h) {
     D.ss_max_nnz = h.tune.small_nnz;
     D.graphs = env_int("LBLG_GRAPH", 1) != 0;
     D.schur_direct = env_int("LBLG_SCHUR_DIRECT", 1) != 0;
+    D.panel_lds = env_int("LBLG_PANEL_LDS", 1) != 0;
+    D.path_refactor = env_int("LBLG_PATH_REFACTOR", 1) != 0;
     D.trailing_direct = env_int("LBLG_TRAILING_DIRECT", 1) != 0;
   }
   {  // one published block of SOLVE_W x MAX_RHS values per piece, two words per value
@@ -628,163 +662,193 @@ static void setup(Handle& h) {
 // ---------------------------------------------------------------------------
 // numeric factorization
 // ---------------------------------------------------------------------------
-void factor(Handle& h) {
-  if (h.dev && h.dev->prec != prec_code(h.set.precision)) h.dev.reset();  // value type changed
-  if (!h.dev) setup(h);
-  DeviceFactor& D = *h.dev;
-  HIPCHK(hipMemcpyAsync(D.Kx, h.Ax.data(), sizeof(double) * D.nnzK, hipMemcpyHostToDevice, D.stream));
-  double md = 0.0;
-  for (double v : h.Ax) md = std::max(md, std::fabs(v));
-  h.max_abs_entry = md;
-  factor_device(h);
+// the lists one factor level launches over: all of the level's supernodes,
+// or the subsets of a path-restricted refactorization
+struct LevelLists {
+  const int* small = nullptr;
+  int nsmall = 0, small_wmax = 0;
+  const int* medium = nullptr;
+  int nmedium = 0, medium_wmax = 0;
+  std::vector<int> large_q;  // positions in FactorLevel::large
+  const int* off = nullptr;
+  int noff = 0, off_wmax = 0, off_blocks = 0;
+  const int* off_blk = nullptr;
+  const int* off_pos = nullptr;
+  const int* sch = nullptr;
+  int nsch = 0;
+  int64_t sch_tiles = 0;
+  const int64_t* sch_tile = nullptr;
+  const int* sch_pos = nullptr;
+  const int* schb = nullptr;
+  int nschb = 0, schb_ncols = 0;
+  int64_t schb_tiles = 0;
+  const int64_t* schb_tile = nullptr;
+  const int* schb_cols = nullptr;
+  const int64_t* schb_ldoff = nullptr;
+};
+
+static LevelLists full_lists(const Symbolic& S, const FactorLevel& F) {
+  LevelLists L;
+  L.small = F.small.d;
+  L.nsmall = F.small.count;
+  L.small_wmax = F.small_wmax;
+  L.medium = F.d_medium;
+  L.nmedium = (int)F.medium.size();
+  for (int s : F.medium) L.medium_wmax = std::max(L.medium_wmax, S.super[s + 1] - S.super[s]);
+  for (size_t q = 0; q < F.large.size(); ++q) L.large_q.push_back((int)q);
+  L.off = F.offdiag.d;
+  L.noff = F.offdiag.count;
+  L.off_wmax = F.offdiag_wmax;
+  L.off_blocks = F.offdiag_blocks;
+  L.off_blk = F.d_offdiag_blk;
+  L.off_pos = F.d_offdiag_pos;
+  L.sch = F.schur.d;
+  L.nsch = F.schur.count;
+  L.sch_tiles = F.schur_tiles;
+  L.sch_tile = F.d_schur_tile;
+  L.sch_pos = F.d_schur_pos;
+  L.schb = F.schur_big.d;
+  L.nschb = F.schur_big.count;
+  L.schb_ncols = F.schur_big_ncols;
+  L.schb_tiles = F.schur_big_tiles;
+  L.schb_tile = F.d_schur_big_tile;
+  L.schb_cols = F.d_schur_big_cols;
+  L.schb_ldoff = F.d_schur_big_ldoff;
+  return L;
 }
 
+// the launches of one factor level: Bunch-Kaufman of the level's diagonal
+// blocks (narrow in LDS, medium panel by panel, wide by cooperative groups),
+// the rows below them, and their Schur updates into the ancestors
 template <typename T>
-static void factor_device_t(Handle& h) {
+static void factor_level(Handle& h, FactorLevel& F, const LevelLists& L, const SnArgs& a, int gemm_schur,
+                         int gemm_trailing) {
   DeviceFactor& D = *h.dev;
+  const Symbolic& S = h.sym;
   T* const valL = static_cast<T*>(D.valL);
   T* const dval = static_cast<T*>(D.dval);
   T* const Wb = static_cast<T*>(D.W);
   T* const LDb = static_cast<T*>(D.LD);
-  const Symbolic& S = h.sym;
-  const int n = h.n;
   hipStream_t st = D.stream;
-  const double reg = h.set.reg_diag;
+  timed(D, "bk_small", [&] { launch_bk_small(L.small, L.nsmall, L.small_wmax, a, valL, dval, D.perm_piv, D.status, st); });
+  // medium supernodes: one workgroup each, panel by panel
+  if (L.nmedium > 0) {
+    const int wmax = L.medium_wmax;
+    const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
+    // diagonal blocks of at most panel_lds_rows rows: the panel in LDS
+    const int lds_rows = (D.panel_lds && wmax <= panel_lds_rows<T>()) ? wmax : 0;
+    for (int p = 0; p < rounds; ++p) {
+      timed(D, "bk_panel", [&] {
+        launch_bk_panel(L.medium, L.nmedium, 1, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos, D.kstart, D.sync,
+                        D.status, false, lds_rows, st);
+      });
+      const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
+      const int nt = (wmax - lo + 63) / 64;
+      if (nt > 0)
+        timed(D, "trailing", [&] {
+          launch_trailing(L.medium, L.nmedium, nt * (nt + 1) / 2, lo, a, valL, Wb, D.woff, D.kpos, D.kstart, st);
+        });
+    }
+  }
+  // large supernodes: a cooperative group of workgroups each, one at a time
+  for (int q : L.large_q) {
+    const int s = F.large[q];
+    const int w = S.super[s + 1] - S.super[s];
+    const int64_t r = S.pi[s + 1] - S.pi[s];
+    const int G = F.large_G[q], nb = F.large_nb[q], nt = F.large_nt[q];
+    const int rows = nt - 64;  // far rows per workgroup
+    const int rounds = (w + nb - 2) / (nb - 1) + 1;
+    const int wsub = F.large_wsub[q];
+    const int wcols = wsub * nb;
+    for (int p = 0; p < rounds; ++p) {
+      // panels run in blocks of wsub: before panel p of a block, its columns
+      // receive the block's earlier panels; the rest of the trailing block is
+      // updated once at the end of the block (rank up to wsub * nb)
+      const int sub = p % wsub;
+      const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
+      if (sub > 0) {
+        const int Tr = (int)((r - lo + 127) / 128);
+        if (Tr > 0)
+          timed(D, "trailing", [&] {
+            launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
+                                gemm_trailing, st);
+          });
+      }
+      // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
+      // panel eliminates >= nb - 1 columns); the last panel may be narrower
+      const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
+      const int Gp = std::min(G, std::max(1, (far + rows - 1) / rows));
+      const uint32_t eb = D.ll_epoch;
+      D.ll_epoch += 3u * (uint32_t)nb + 8u;
+      timed(D, "bk_panel_group", [&] {
+        launch_bk_wide(F.d_large + q, Gp, nt, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
+                       D.kstart, D.sync + D.wide_par, D.sync + (D.wide_par ^ 1), D.ll, eb, D.status, st);
+      });
+      D.wide_par ^= 1;
+      if (sub == wsub - 1 || p == rounds - 1) {
+        const int lo2 = std::min(w, (p + 1) * (nb - 1));
+        const int Tj = (w - lo2 + 127) / 128, Ti = (int)((r - lo2 + 127) / 128);
+        const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
+        if (Tj > 0)
+          timed(D, "trailing", [&] {
+            launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
+                                gemm_trailing, st);
+          });
+      }
+    }
+  }
+  timed(D, "offdiag", [&] {
+    launch_offdiag(L.off, L.noff, L.off_wmax, L.off_blk, L.off_pos, L.off_blocks, a, valL, dval, D.perm_piv, st);
+  });
+  timed(D, "schur", [&] { launch_schur(L.sch, L.nsch, L.sch_tile, L.sch_pos, L.sch_tiles, a, valL, dval, st); });
+  if (L.nschb > 0) {
+    timed(D, "scale_ld", [&] {
+      launch_scale_ld(L.schb, L.nschb, L.schb_cols, L.schb_ncols, a, valL, dval, LDb, L.schb_ldoff, st);
+    });
+    timed(D, "schur_big", [&] {
+      launch_schur_big(L.schb, L.nschb, L.schb_tile, L.schb_tiles, a, valL, LDb, L.schb_ldoff, gemm_schur, st);
+    });
+  }
+}
+
+// static-pivoting threshold and GEMM operand modes of the handle's factor
+struct FactorModes {
+  double reg, reg_factor;
+  int gemm_schur, gemm_trailing;
+};
+static FactorModes factor_modes(const Handle& h) {
+  const DeviceFactor& D = *h.dev;
+  FactorModes m;
+  m.reg = h.set.reg_diag;
   // The FP32 and BF16 factors take S K S (S: max-norm diagonal scaling) and
   // perturb its diagonals below the static-pivoting threshold 2^pexp
   // (sqrt(eps_f32) by default; pivots of reg_diag = 1e-9 would give growth
   // ~1e9, beyond FP32); the refinement still runs on the reference's K0, so
-  // only the preconditioner changes.
+  // only the preconditioner changes.  The entries of S K S have magnitude
+  // <= 1 (max-norm scaling): the threshold is 2^pexp itself.
   const int pexp = env_int("LBLG_F32_PIVOT_EXP", -12);
-  // the FP32 / BF16 factors take S K S, whose entries have magnitude <= 1
-  // (max-norm scaling): the threshold is 2^pexp itself
-  const double reg_factor = D.prec != 0 ? std::ldexp(1.0, pexp) : reg;
+  m.reg_factor = D.prec != 0 ? std::ldexp(1.0, pexp) : m.reg;
   // BF16 operands: split into two BF16 terms (three MFMAs per product) by
   // default, one rounded term with LBLG_BF16_TERMS=1
   const int bf16_mode = env_int("LBLG_BF16_TERMS", 3) == 1 ? GEMM_BF16 : GEMM_BF16X3;
-  const int gemm_schur = D.prec == 2 ? bf16_mode : (D.schur_direct ? GEMM_DIRECT : GEMM_STAGED);
-  const int gemm_trailing = D.prec == 2 ? bf16_mode : (D.trailing_direct ? GEMM_DIRECT : GEMM_STAGED);
-  SnArgs a = snargs(D);
-  a.rows = D.rows0;  // pre-pivot labels during the factorization
-  // the whole launch sequence depends only on the symbolic structure, the
-  // buffers and (reg, reg_factor, regularization): captured once and replayed
-  // when no cooperative wide panel (epoch-stamped) takes part
-  auto seq = [&]() {
-    launch_fill(valL, S.p[n], 0.0, st);
-    launch_fill(dval, 2 * (int64_t)n, 0.0, st);
-    launch_iota(D.perm_piv, n, st);
-    launch_fill_int(D.status, 16, 0, st);
-    launch_fill_int(D.kpos, D.nsuper, 0, st);
-    const double* Kfac = D.Kx;  // the matrix the factor takes: K, or S K S for FP32 / BF16
-    if (D.prec != 0) {
-      launch_diag_scaling(n, D.nnzK, D.Krp, D.Ksrc, D.Krow, D.Kcol, D.Kx, D.scale, D.Kf, st);
-      Kfac = D.Kf;
-    }
-    launch_pert_flags(D.nnzK, D.isdiag, D.Kx, Kfac, reg, reg_factor, h.set.regularization, D.pert, st);
-    launch_assemble(D.amap, Kfac, D.pert, D.nnzK, reg_factor, valL, st);
-    for (size_t l = 0; l < D.flevels.size(); ++l) {
-      FactorLevel& F = D.flevels[l];
-      timed(D, "bk_small", [&] { launch_bk_small(F.small.d, F.small.count, F.small_wmax, a, valL, dval, D.perm_piv, D.status, st); });
-      // medium supernodes: one workgroup each, panel by panel
-      if (!F.medium.empty()) {
-        int wmax = 0;
-        for (int s : F.medium) wmax = std::max(wmax, S.super[s + 1] - S.super[s]);
-        const int rounds = (wmax + PANEL_NB - 2) / (PANEL_NB - 1) + 1;
-        for (int p = 0; p < rounds; ++p) {
-          timed(D, "bk_panel", [&] {
-            launch_bk_panel(F.d_medium, (int)F.medium.size(), 1, a, valL, dval, D.perm_piv, Wb, D.woff,
-                            D.kpos, D.kstart, D.sync, D.status, false, st);
-          });
-          const int lo = std::min(wmax, (p + 1) * (PANEL_NB - 1));
-          const int nt = (wmax - lo + 63) / 64;
-          if (nt > 0)
-            timed(D, "trailing", [&] {
-              launch_trailing(F.d_medium, (int)F.medium.size(), nt * (nt + 1) / 2, lo, a, valL, Wb, D.woff,
-                              D.kpos, D.kstart, st);
-            });
-        }
-      }
-      // large supernodes: a cooperative group of workgroups each, one at a time
-      for (size_t q = 0; q < F.large.size(); ++q) {
-        const int s = F.large[q];
-        const int w = S.super[s + 1] - S.super[s];
-        const int64_t r = S.pi[s + 1] - S.pi[s];
-        const int G = F.large_G[q], nb = F.large_nb[q], nt = F.large_nt[q];
-        const int rows = nt - 64;  // far rows per workgroup
-        const int rounds = (w + nb - 2) / (nb - 1) + 1;
-        const int wsub = F.large_wsub[q];
-        const int wcols = wsub * nb;
-        for (int p = 0; p < rounds; ++p) {
-          // panels run in blocks of wsub: before panel p of a block, its columns
-          // receive the block's earlier panels; the rest of the trailing block is
-          // updated once at the end of the block (rank up to wsub * nb)
-          const int sub = p % wsub;
-          const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
-          if (sub > 0) {
-            const int Tr = (int)((r - lo + 127) / 128);
-            if (Tr > 0)
-              timed(D, "trailing", [&] {
-                launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
-                                    gemm_trailing, st);
-              });
-          }
-          // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
-          // panel eliminates >= nb - 1 columns); the last panel may be narrower
-          const int far = (int)(r - std::min<int64_t>(w, (int64_t)p * (nb - 1) + nb));
-          const int Gp = std::min(G, std::max(1, (far + rows - 1) / rows));
-          const uint32_t eb = D.ll_epoch;
-          D.ll_epoch += 3u * (uint32_t)nb + 8u;
-          timed(D, "bk_panel_group", [&] {
-            launch_bk_wide(F.d_large + q, Gp, nt, nb, sub, wcols, a, valL, dval, D.perm_piv, Wb, D.woff, D.kpos,
-                           D.kstart, D.sync + D.wide_par, D.sync + (D.wide_par ^ 1), D.ll, eb, D.status, st);
-          });
-          D.wide_par ^= 1;
-          if (sub == wsub - 1 || p == rounds - 1) {
-            const int lo2 = std::min(w, (p + 1) * (nb - 1));
-            const int Tj = (w - lo2 + 127) / 128, Ti = (int)((r - lo2 + 127) / 128);
-            const int ntiles = Tj * (Tj + 1) / 2 + (Ti - Tj) * Tj;
-            if (Tj > 0)
-              timed(D, "trailing", [&] {
-                launch_trailing_big(F.d_large + q, ntiles, lo2, 0, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
-                                    gemm_trailing, st);
-              });
-          }
-        }
-      }
-      timed(D, "offdiag", [&] {
-        launch_offdiag(F.offdiag.d, F.offdiag.count, F.offdiag_wmax, F.d_offdiag_blk, F.d_offdiag_pos,
-                       F.offdiag_blocks, a, valL, dval, D.perm_piv, st);
-      });
-      timed(D, "schur", [&] {
-        launch_schur(F.schur.d, F.schur.count, F.d_schur_tile, F.d_schur_pos, F.schur_tiles, a, valL, dval, st);
-      });
-      if (F.schur_big.count > 0) {
-        timed(D, "scale_ld", [&] {
-          launch_scale_ld(F.schur_big.d, F.schur_big.count, F.d_schur_big_cols, F.schur_big_ncols, a, valL, dval,
-                          LDb, F.d_schur_big_ldoff, st);
-        });
-        timed(D, "schur_big", [&] {
-          launch_schur_big(F.schur_big.d, F.schur_big.count, F.d_schur_big_tile, F.schur_big_tiles, a, valL, LDb,
-                           F.d_schur_big_ldoff, gemm_schur, st);
-        });
-      }
-    }
-    launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
-    if (D.prec != 0) launch_gather_scale(n, D.perm_f, D.scale, D.scale_ord, st);
-    // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
-    launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
-    launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
-  };
-  if (D.has_coop) {
-    seq();
-  } else {
-    run_graph(D, D.gfactor, graph_key({(double)D.prec, reg, reg_factor, (double)h.set.regularization}, {}), seq);
-  }
+  m.gemm_schur = D.prec == 2 ? bf16_mode : (D.schur_direct ? GEMM_DIRECT : GEMM_STAGED);
+  m.gemm_trailing = D.prec == 2 ? bf16_mode : (D.trailing_direct ? GEMM_DIRECT : GEMM_STAGED);
+  return m;
+}
+
+// status words of a finished factorization: zero pivots and the wide panels'
+// counters; a bounded wait that ran out or a launch shape that did not fit
+// ends the factorization with an error
+static void finish_factor(Handle& h, bool partial) {
+  DeviceFactor& D = *h.dev;
+  hipStream_t st = D.stream;
   int stat[12] = {0};
   HIPCHK(hipMemcpyAsync(stat, D.status, sizeof(stat), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h.k0_norm_inf, D.k0norm, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  h.zero_pivot = stat[0] & 1;
+  // a restricted refactorization reports the refactored supernodes' zero
+  // pivots on top of those of the factorization it updates
+  h.zero_pivot = (partial ? h.zero_pivot : 0) | (stat[0] & 1);
   h.second_col_steps = stat[2];
   h.interchanges = stat[3];
   h.wide_panels = stat[4];
@@ -801,6 +865,297 @@ static void factor_device_t(Handle& h) {
   }
   if (stat[0] & 8) throw std::runtime_error("wide panel launch shape does not fit the supernode");
   h.factored = true;
+  D.factor_valid = true;
+  D.factor_key[0] = h.set.reg_diag;
+  D.factor_key[1] = h.set.regularization;
+  D.factor_key[2] = h.set.precision;
+}
+
+template <typename T>
+static void factor_device_t(Handle& h) {
+  DeviceFactor& D = *h.dev;
+  T* const valL = static_cast<T*>(D.valL);
+  T* const dval = static_cast<T*>(D.dval);
+  const Symbolic& S = h.sym;
+  const int n = h.n;
+  hipStream_t st = D.stream;
+  const FactorModes fm = factor_modes(h);
+  SnArgs a = snargs(D);
+  a.rows = D.rows0;  // pre-pivot labels during the factorization
+  D.factor_valid = false;
+  // the whole launch sequence depends only on the symbolic structure, the
+  // buffers and (reg, reg_factor, regularization): captured once and replayed
+  // when no cooperative wide panel (epoch-stamped) takes part
+  auto seq = [&]() {
+    launch_fill(valL, S.p[n], 0.0, st);
+    launch_fill(dval, 2 * (int64_t)n, 0.0, st);
+    launch_iota(D.perm_piv, n, st);
+    launch_fill_int(D.status, 16, 0, st);
+    launch_fill_int(D.kpos, D.nsuper, 0, st);
+    const double* Kfac = D.Kx;  // the matrix the factor takes: K, or S K S for FP32 / BF16
+    if (D.prec != 0) {
+      launch_diag_scaling(n, D.nnzK, D.Krp, D.Ksrc, D.Krow, D.Kcol, D.Kx, D.scale, D.Kf, st);
+      Kfac = D.Kf;
+    }
+    launch_pert_flags(D.nnzK, D.isdiag, D.Kx, Kfac, fm.reg, fm.reg_factor, h.set.regularization, D.pert, st);
+    launch_assemble<T>(D.amap, Kfac, D.pert, D.nnzK, fm.reg_factor, valL, nullptr, nullptr, st);
+    for (FactorLevel& F : D.flevels) factor_level<T>(h, F, full_lists(S, F), a, fm.gemm_schur, fm.gemm_trailing);
+    launch_relabel(a, D.nsuper, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
+    if (D.prec != 0) launch_gather_scale(n, D.perm_f, D.scale, D.scale_ord, st);
+    // unperturbed K0 (perturbed diagonals set to zero, set_diags(0)) for the SpMV
+    launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
+    launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
+  };
+  if (D.has_coop) {
+    seq();
+  } else {
+    run_graph(D, D.gfactor,
+              graph_key({(double)D.prec, fm.reg, fm.reg_factor, (double)h.set.regularization}, {}), seq);
+  }
+  finish_factor(h, false);
+  h.refactored = D.nsuper;
+}
+
+// staging of one restricted refactorization: the lists of every level and
+// the changed values, packed host side and sent in one copy
+struct Blob {
+  std::vector<char> bytes;
+  template <typename V>
+  size_t add(const std::vector<V>& v) {
+    const size_t off = (bytes.size() + 15) & ~(size_t)15;
+    bytes.resize(off + sizeof(V) * v.size());
+    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), sizeof(V) * v.size());
+    return off;
+  }
+};
+
+// The SoMod update (update_factorization with update_ldl_left_sn_02_v2,
+// Serial_update_ldl.cpp:17-277; modified supernodes as add_del_matrix_qp
+// collects them, linear_solver_wrapper.cpp:1058-1119): the supernodes on the
+// tree paths from the changed entries to the root are zeroed, re-assembled
+// from K and refactored level by level with the same kernels as a full
+// factorization.  Supernodes off those paths keep their panels, D and
+// pivots; the ones whose rows reach a refactored supernode send their Schur
+// updates again, to the refactored supernodes only (tmask), from their kept
+// factor.  Only the changed values travel to the device.
+template <typename T>
+static void path_factor_t(Handle& h) {
+  DeviceFactor& D = *h.dev;
+  const Symbolic& S = h.sym;
+  const int n = h.n, ns = D.nsuper;
+  hipStream_t st = D.stream;
+  T* const valL = static_cast<T*>(D.valL);
+  T* const dval = static_cast<T*>(D.dval);
+  const FactorModes fm = factor_modes(h);
+  // modified supernodes: the root paths from the supernode of the first
+  // (in the ordering) of each changed entry's row and column
+  if (++D.smark_epoch == 0) {
+    std::fill(D.smark.begin(), D.smark.end(), 0u);
+    D.smark_epoch = 1;
+  }
+  const unsigned ep = D.smark_epoch;
+  std::vector<int> mod;
+  std::vector<double> vals(h.changed.size());
+  for (size_t k = 0; k < h.changed.size(); ++k) {
+    const int64_t q = h.changed[k];
+    vals[k] = h.Ax[q];
+    for (int t = D.hasup[q]; t >= 0 && D.smark[t] != ep; t = S.sparent[t]) {
+      D.smark[t] = ep;
+      mod.push_back(t);
+    }
+  }
+  // supernodes outside the paths with rows in a modified one
+  std::vector<unsigned char> hmask(ns, 0);
+  for (int t : mod) hmask[t] = 1;
+  std::vector<unsigned char> sends(ns, 0);   // off-path supernodes that update a modified one
+  for (int t : mod)
+    for (int e = D.upd_ptr[t]; e < D.upd_ptr[t + 1]; ++e) {
+      const int d = D.upd_idx[e];
+      if (!hmask[d]) sends[d] = 1;
+    }
+  const int L = (int)D.flevels.size();
+  std::vector<std::vector<int>> lsmall(L), lmed(L), lqlarge(L), loff(L), lsch(L), lschb(L);
+  std::vector<std::vector<int64_t>> lldoff(L);
+  for (int l = 0; l < L; ++l) {
+    const FactorLevel& F = D.flevels[l];
+    for (int s : F.h_small)
+      if (hmask[s]) lsmall[l].push_back(s);
+    for (int s : F.medium)
+      if (hmask[s]) lmed[l].push_back(s);
+    for (size_t q = 0; q < F.large.size(); ++q)
+      if (hmask[F.large[q]]) lqlarge[l].push_back((int)q);
+    for (int s : F.h_off)
+      if (hmask[s]) loff[l].push_back(s);
+    for (int s : F.h_sch)
+      if (hmask[s] || sends[s]) lsch[l].push_back(s);
+    for (size_t k = 0; k < F.h_schb.size(); ++k) {
+      const int s = F.h_schb[k];
+      if (hmask[s] || sends[s]) {
+        lschb[l].push_back(s);
+        lldoff[l].push_back(F.h_schb_ldoff[k]);
+      }
+    }
+  }
+  // pack: reset list, mask, changed entries, then per level the lists and
+  // their prefixes (the same forms setup builds for full levels)
+  Blob B;
+  const size_t o_mod = B.add(mod), o_mask = B.add(hmask), o_idx = B.add(h.changed), o_val = B.add(vals);
+  struct Offs {
+    size_t small, med, off, offblk, offpos, sch, schtile, schpos, schb, schbtile, schbcols, schbld;
+    int off_blocks = 0, schb_ncols = 0, small_wmax = 0, med_wmax = 0, off_wmax = 0;
+    int64_t sch_tiles = 0, schb_tiles = 0;
+  };
+  std::vector<Offs> O(L);
+  auto width = [&](int s) { return S.super[s + 1] - S.super[s]; };
+  auto offrows = [&](int s) { return S.pi[s + 1] - S.pi[s] - width(s); };
+  for (int l = 0; l < L; ++l) {
+    Offs& o = O[l];
+    for (int s : lsmall[l]) o.small_wmax = std::max(o.small_wmax, width(s));
+    for (int s : lmed[l]) o.med_wmax = std::max(o.med_wmax, width(s));
+    o.small = B.add(lsmall[l]);
+    o.med = B.add(lmed[l]);
+    std::vector<int> blk, pos;
+    for (size_t k = 0; k < loff[l].size(); ++k) {
+      const int s = loff[l][k];
+      o.off_wmax = std::max(o.off_wmax, width(s));
+      blk.push_back(o.off_blocks);
+      const int nb = (int)((offrows(s) + 63) / 64);
+      for (int b = 0; b < nb; ++b) pos.push_back((int)k);
+      o.off_blocks += nb;
+    }
+    o.off = B.add(loff[l]);
+    o.offblk = B.add(blk);
+    o.offpos = B.add(pos);
+    std::vector<int64_t> tiles;
+    std::vector<int> tpos;
+    for (size_t k = 0; k < lsch[l].size(); ++k) {
+      tiles.push_back(o.sch_tiles);
+      const int64_t T64 = (offrows(lsch[l][k]) + 63) / 64, nt = T64 * (T64 + 1) / 2;
+      for (int64_t b = 0; b < nt; ++b) tpos.push_back((int)k);
+      o.sch_tiles += nt;
+    }
+    o.sch = B.add(lsch[l]);
+    o.schtile = B.add(tiles);
+    o.schpos = B.add(tpos);
+    std::vector<int64_t> tilesb;
+    std::vector<int> colsb;
+    for (int s : lschb[l]) {
+      tilesb.push_back(o.schb_tiles);
+      colsb.push_back(o.schb_ncols);
+      o.schb_ncols += width(s);
+      const int64_t T128 = (offrows(s) + 127) / 128;
+      o.schb_tiles += T128 * (T128 + 1) / 2;
+    }
+    o.schb = B.add(lschb[l]);
+    o.schbtile = B.add(tilesb);
+    o.schbcols = B.add(colsb);
+    o.schbld = B.add(lldoff[l]);
+  }
+  const size_t bytes = B.bytes.size();
+  if (bytes > D.blob_cap) {
+    if (D.blob_d) HIPCHK(hipFree(D.blob_d));
+    if (D.blob_h) HIPCHK(hipHostFree(D.blob_h));
+    D.blob_d = nullptr;
+    D.blob_h = nullptr;
+    const size_t cap = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+    HIPCHK(hipMalloc((void**)&D.blob_d, cap));
+    HIPCHK(hipHostMalloc((void**)&D.blob_h, cap, hipHostMallocDefault));
+    D.blob_cap = cap;
+  }
+  std::memcpy(D.blob_h, B.bytes.data(), bytes);
+  char* const bd = D.blob_d;
+  auto dptr = [&](size_t off) { return bd + off; };
+  HIPCHK(hipMemcpyAsync(bd, D.blob_h, bytes, hipMemcpyHostToDevice, st));
+  D.factor_valid = false;
+  SnArgs a = snargs(D);
+  a.rows = D.rows0;
+  launch_fill_int(D.status, 16, 0, st);
+  launch_fill_int(D.kpos, ns, 0, st);
+  launch_scatter_values((int64_t)h.changed.size(), reinterpret_cast<const int64_t*>(dptr(o_idx)),
+                        reinterpret_cast<const double*>(dptr(o_val)), D.Kx, st);
+  const int* dmod = reinterpret_cast<const int*>(dptr(o_mod));
+  const unsigned char* dmask = reinterpret_cast<const unsigned char*>(dptr(o_mask));
+  launch_reset_supernodes<T>(dmod, (int)mod.size(), a, valL, dval, D.perm_piv, st);
+  const double* Kfac = D.Kx;
+  if (D.prec != 0) {  // S K S with the scaling of the last full factorization
+    launch_scale_entries(D.nnzK, D.Krow, D.Kcol, D.Kx, D.scale, D.Kf, st);
+    Kfac = D.Kf;
+  }
+  launch_pert_flags(D.nnzK, D.isdiag, D.Kx, Kfac, fm.reg, fm.reg_factor, h.set.regularization, D.pert, st);
+  launch_assemble<T>(D.amap, Kfac, D.pert, D.nnzK, fm.reg_factor, valL, D.asup, dmask, st);
+  a.tmask = dmask;
+  for (int l = 0; l < L; ++l) {
+    const Offs& o = O[l];
+    LevelLists LL;
+    LL.small = reinterpret_cast<const int*>(dptr(o.small));
+    LL.nsmall = (int)lsmall[l].size();
+    LL.small_wmax = o.small_wmax;
+    LL.medium = reinterpret_cast<const int*>(dptr(o.med));
+    LL.nmedium = (int)lmed[l].size();
+    LL.medium_wmax = o.med_wmax;
+    LL.large_q = lqlarge[l];
+    LL.off = reinterpret_cast<const int*>(dptr(o.off));
+    LL.noff = (int)loff[l].size();
+    LL.off_wmax = o.off_wmax;
+    LL.off_blocks = o.off_blocks;
+    LL.off_blk = reinterpret_cast<const int*>(dptr(o.offblk));
+    LL.off_pos = reinterpret_cast<const int*>(dptr(o.offpos));
+    LL.sch = reinterpret_cast<const int*>(dptr(o.sch));
+    LL.nsch = (int)lsch[l].size();
+    LL.sch_tiles = o.sch_tiles;
+    LL.sch_tile = reinterpret_cast<const int64_t*>(dptr(o.schtile));
+    LL.sch_pos = reinterpret_cast<const int*>(dptr(o.schpos));
+    LL.schb = reinterpret_cast<const int*>(dptr(o.schb));
+    LL.nschb = (int)lschb[l].size();
+    LL.schb_ncols = o.schb_ncols;
+    LL.schb_tiles = o.schb_tiles;
+    LL.schb_tile = reinterpret_cast<const int64_t*>(dptr(o.schbtile));
+    LL.schb_cols = reinterpret_cast<const int*>(dptr(o.schbcols));
+    LL.schb_ldoff = reinterpret_cast<const int64_t*>(dptr(o.schbld));
+    if (LL.nsmall + LL.nmedium + (int)LL.large_q.size() + LL.nsch + LL.nschb == 0) continue;
+    factor_level<T>(h, D.flevels[l], LL, a, fm.gemm_schur, fm.gemm_trailing);
+  }
+  a.tmask = nullptr;
+  launch_relabel(a, ns, D.rows0, D.rows, D.perm_piv, D.newpos, D.perm0, D.perm_f, D.iperm_f, st);
+  if (D.prec != 0) launch_gather_scale(n, D.perm_f, D.scale, D.scale_ord, st);
+  launch_k0(n, D.nnzf, D.Krp, D.Ksrc, D.Kx, D.pert, D.Kv, D.red, 512, D.k0norm, st);
+  launch_seg_rowabs_max(D.nseg, D.seg, D.Krp, D.Kv, D.segnorm, st);
+  finish_factor(h, true);  // synchronizes: the staging buffers are free again
+  h.refactored = (int64_t)mod.size();
+}
+
+void factor(Handle& h) {
+  if (h.dev && h.dev->prec != prec_code(h.set.precision)) h.dev.reset();  // value type changed
+  if (!h.dev) {
+    setup(h);
+    h.values_dirty = true;
+  }
+  DeviceFactor& D = *h.dev;
+  double md = 0.0;
+  for (double v : h.Ax) md = std::max(md, std::fabs(v));
+  h.max_abs_entry = md;
+  // a restricted refactorization updates the device factor of the values
+  // last uploaded (same precision and perturbation settings) after
+  // lblg_update_values changes; the FP32 / BF16 factors keep the diagonal
+  // scaling of the last full factorization (any positive scaling gives a
+  // valid preconditioner; refinement runs on K itself).  Without changes a
+  // factorization recomputes everything, as the reference's
+  // numerical_factorization does.
+  const bool partial = D.path_refactor && !h.values_dirty && !h.changed.empty() && D.factor_valid &&
+                       D.factor_key[0] == h.set.reg_diag && D.factor_key[1] == h.set.regularization &&
+                       D.factor_key[2] == h.set.precision;
+  if (partial) {
+    if (D.prec != 0)
+      path_factor_t<float>(h);
+    else
+      path_factor_t<double>(h);
+  } else {
+    HIPCHK(hipMemcpyAsync(D.Kx, h.Ax.data(), sizeof(double) * D.nnzK, hipMemcpyHostToDevice, D.stream));
+    factor_device(h);
+  }
+  for (int64_t q : h.changed) h.changed_flag[q] = 0;
+  h.changed.clear();
+  h.values_dirty = false;
 }
 
 void factor_device(Handle& h) {

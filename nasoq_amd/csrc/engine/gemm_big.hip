@@ -314,13 +314,13 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_t* tile_prefix, int count,
                                                    const int* super, const int64_t* pi, const int64_t* px,
                                                    const int* rows, const int* col2sup, T* valL,
-                                                   const T* LD, const int64_t* ldoff) {
+                                                   const T* LD, const int64_t* ldoff, const unsigned char* tmask) {
   __shared__ T As[MODE == GEMM_STAGED ? BKC * BLD : 1], Bs[MODE == GEMM_STAGED ? BKC * BLD : 1];
   __shared__ __align__(16) unsigned short S16[MODE == GEMM_BF16X3 ? 2 * B16_LDS : MODE == GEMM_BF16 ? B16_LDS : 1];
   __shared__ int rlab[BT], clab[BT], ctg[BT], dtcol[BT];
   __shared__ int tpos[BRUNS][BT];
   __shared__ int64_t cbase[BT];
-  __shared__ int dtl[BRUNS], ntg;
+  __shared__ int dtl[BRUNS], ntg, reach;
   int lo = 0, hi = count - 1;
   const int64_t bid = blockIdx.x;
   while (lo < hi) {
@@ -340,6 +340,13 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
     rlab[t] = (i0 + t < m) ? rows[pi[s] + w + i0 + t] : -1;
     clab[t] = (j0 + t < m) ? rows[pi[s] + w + j0 + t] : -1;
     ctg[t] = clab[t] >= 0 ? col2sup[clab[t]] : -1;
+  }
+  if (tmask) {  // a restricted refactorization: tiles reaching no flagged supernode are skipped
+    if (t == 0) reach = 0;
+    __syncthreads();
+    if (t < BT && ctg[t] >= 0 && tmask[ctg[t]]) reach = 1;
+    __syncthreads();
+    if (!reach) return;
   }
   __syncthreads();
   // distinct target supernodes of the tile's columns come in runs (labels are sorted)
@@ -387,6 +394,7 @@ __global__ __launch_bounds__(256) void k_schur_big(const int* list, const int64_
   big_foreach(acc, [&](int ri, int ci, T v) {
     const int R = rlab[ri], C = clab[ci];
     if (R < 0 || C < 0 || R < C) return;
+    if (tmask && !tmask[ctg[ci]]) return;
     const int dt = dtcol[ci];
     int64_t pos;
     if (dt >= 0) {
@@ -545,19 +553,19 @@ void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, in
     if constexpr (std::is_same<T, float>::value) {
       if (mode == GEMM_BF16X3)
         k_schur_big<T, GEMM_BF16X3><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                                          a.col2sup, valL, LD, ldoff);
+                                                          a.col2sup, valL, LD, ldoff, a.tmask);
       else
         k_schur_big<T, GEMM_BF16><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                                        a.col2sup, valL, LD, ldoff);
+                                                        a.col2sup, valL, LD, ldoff, a.tmask);
     } else {
       throw std::runtime_error("schur_big: BF16 operands need the FP32 factor");
     }
   } else if (mode == GEMM_DIRECT) {
     k_schur_big<T, GEMM_DIRECT><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                                      a.col2sup, valL, LD, ldoff);
+                                                      a.col2sup, valL, LD, ldoff, a.tmask);
   } else {
     k_schur_big<T, GEMM_STAGED><<<grid, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                                      a.col2sup, valL, LD, ldoff);
+                                                      a.col2sup, valL, LD, ldoff, a.tmask);
   }
   HIP_LAUNCH_CHECK("schur_big");
 }

@@ -32,6 +32,12 @@ struct Handle {
   int64_t wide_panels = 0;       // wide panels factored
   int64_t wide_panels_pivoted = 0;  // wide panels whose 1x1 no-interchange pass stopped early
   double factor_seconds = 0.0;
+  // entries changed by lblg_update_values since the values were last
+  // uploaded (positions in Ax), and whether all of Ax must be uploaded
+  std::vector<int64_t> changed;
+  std::vector<unsigned char> changed_flag;
+  bool values_dirty = true;
+  int64_t refactored = 0;  // supernodes refactored by the last factorization
   std::vector<int64_t> batch_off;  // block-diagonal batch: first column of each system (count + 1)
   // device schedule knobs (-1: the environment's or built-in default at setup)
   struct Tuning {

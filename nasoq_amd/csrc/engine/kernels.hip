@@ -107,18 +107,53 @@ void launch_convert(TO* dst, const TI* src, int64_t count, hipStream_t st) {
 // --------------------------------------------------------------------------
 template <typename T>
 __global__ void k_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz,
-                           double reg, T* valL) {
+                           double reg, T* valL, const int* asup, const unsigned char* tmask) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+    if (tmask && !tmask[asup[q]]) continue;
     const double v = (pert[q] & 2) ? reg : Kx[q];
     valL[amap[q]] = (T)v;
   }
 }
 template <typename T>
 void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
-                     T* valL, hipStream_t st) {
+                     T* valL, const int* asup, const unsigned char* tmask, hipStream_t st) {
   if (nnz <= 0) return;
-  k_assemble<T><<<grid_for(nnz, 256), 256, 0, st>>>(amap, Kx, pert, nnz, reg, valL);
+  k_assemble<T><<<grid_for(nnz, 256), 256, 0, st>>>(amap, Kx, pert, nnz, reg, valL, asup, tmask);
   HIP_LAUNCH_CHECK("assemble");
+}
+
+// one workgroup per listed supernode: zero its panel (r x w at px[s]) and its
+// D entries, identity local pivots over its columns
+template <typename T>
+__global__ void k_reset_supernodes(const int* list, const int* super, const int64_t* pi, const int64_t* px,
+                                   T* valL, T* dval, int* perm_piv, int n) {
+  const int s = list[blockIdx.x];
+  const int c0 = super[s], w = super[s + 1] - c0;
+  const int64_t len = (pi[s + 1] - pi[s]) * (int64_t)w;
+  T* P = valL + px[s];
+  for (int64_t e = threadIdx.x; e < len; e += blockDim.x) P[e] = (T)0;
+  for (int c = threadIdx.x; c < w; c += blockDim.x) {
+    dval[c0 + c] = (T)0;
+    dval[n + c0 + c] = (T)0;
+    perm_piv[c0 + c] = c0 + c;
+  }
+}
+template <typename T>
+void launch_reset_supernodes(const int* list, int count, const SnArgs& a, T* valL, T* dval, int* perm_piv,
+                             hipStream_t st) {
+  if (count <= 0) return;
+  k_reset_supernodes<T><<<count, 256, 0, st>>>(list, a.super, a.pi, a.px, valL, dval, perm_piv, a.n);
+  HIP_LAUNCH_CHECK("reset_supernodes");
+}
+
+__global__ void k_scatter_values(int64_t count, const int64_t* idx, const double* vals, double* Kx) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x)
+    Kx[idx[k]] = vals[k];
+}
+void launch_scatter_values(int64_t count, const int64_t* idx, const double* vals, double* Kx, hipStream_t st) {
+  if (count <= 0) return;
+  k_scatter_values<<<grid_for(count, 256), 256, 0, st>>>(count, idx, vals, Kx);
+  HIP_LAUNCH_CHECK("scatter_values");
 }
 
 // --------------------------------------------------------------------------
@@ -344,41 +379,103 @@ void launch_bk_small(const int* list, int count, int wmax, const SnArgs& a, T* v
 //
 // Per-supernode workspace at W + woff[s]:  Wp (w x NB, ld w) | t0 (w) | t1 (w)
 //                                         | part (2 G doubles) | parti (2 G ints)
+//
+// LDSP (one workgroup per supernode, diagonal block of at most ldsm rows):
+// the panel's columns [k0, k0 + NB) of the diagonal block's rows [k0, w),
+// its W columns and t0 / t1 live in LDS for the whole panel, so each column
+// step reads and writes LDS only; the panel and W (rows the trailing update
+// reads) are written back at the end.  Entries outside the staged block
+// (earlier panels, later columns, the rows below the diagonal block) stay
+// in memory.
 // --------------------------------------------------------------------------
-template <typename T>
+constexpr int PANEL_UF = 8;  // left-looking terms whose loads are issued together
+
+template <typename T, bool LDSP>
 __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const int* super,
                                                   const int64_t* pi, const int64_t* px, T* valL,
                                                   T* dval, int* perm_piv, T* Wbase,
                                                   const int64_t* woff, int* kpos, int* kstart,
-                                                  int* sync, int n, int* status) {
+                                                  int* sync, int n, int* status, int ldsm) {
   __shared__ double sv[8];
   __shared__ int si[8];
+  extern __shared__ double panel_smem[];
   const int sidx = blockIdx.x / G, g = blockIdx.x % G;
   const int s = list[sidx];
   const int c0 = super[s], w = super[s + 1] - c0;
   const int64_t r = pi[s + 1] - pi[s];
   T* A = valL + px[s];
   T* Wp = Wbase + woff[s];
-  T* t0 = Wp + (int64_t)w * PANEL_NB;
-  T* t1 = t0 + w;
-  T* part = t1 + w;  // column maxima (exact in T: magnitudes of T values)
+  T* part = Wp + (int64_t)w * PANEL_NB + 2 * (int64_t)w;  // column maxima (exact in T: magnitudes of T values)
   int* parti = reinterpret_cast<int*>(part + 2 * G);
   int* ctr = sync + sidx;
   const int gt = g * 256 + threadIdx.x, GT = G * 256;
   const int k0 = kpos[s];
   if (k0 >= w) return;  // whole group returns together (k0 is uniform)
   const bool last = (w - k0) <= PANEL_NB;
+  const int pc1 = min(w, k0 + PANEL_NB);  // end of the panel's columns
+  // LDS blocks (LDSP): As(i, j) = L(k0 + i, k0 + j), Ws likewise, ld ldsm
+  T* As = reinterpret_cast<T*>(panel_smem);
+  T* Ws = As + (int64_t)PANEL_NB * ldsm;
+  T* t0 = LDSP ? Ws + (int64_t)PANEL_NB * ldsm - k0 : Wp + (int64_t)w * PANEL_NB;  // indexed by row i >= k0
+  T* t1 = LDSP ? Ws + (int64_t)PANEL_NB * ldsm + ldsm - k0 : Wp + (int64_t)w * PANEL_NB + w;
+  // element (i, j) of the diagonal block / rows below, wherever it lives
+  auto AA = [&](int64_t i, int64_t j) -> T& {
+    if (LDSP && j >= k0 && j < pc1 && i >= k0 && i < w) return As[(j - k0) * ldsm + (i - k0)];
+    return A[j * r + i];
+  };
+  auto WW = [&](int64_t i, int64_t j) -> T& {
+    if (LDSP) return Ws[(j - k0) * ldsm + (i - k0)];
+    return Wp[(j - k0) * w + i];
+  };
+  if (LDSP) {
+    for (int64_t e = threadIdx.x; e < (int64_t)(pc1 - k0) * (w - k0); e += 256) {
+      const int64_t j = e / (w - k0), i = e % (w - k0);
+      As[j * ldsm + i] = A[(k0 + j) * r + (k0 + i)];
+    }
+    __syncthreads();
+  }
+  // v - sum_{j in [k0, k)} L(i, j) W(q, j) for a row i >= k and a row q of W
+  auto left_update = [&](T v, int i, int q, int k) -> T {
+    if (LDSP) {
+      const T* ar = As + (i - k0);
+      const T* wr = Ws + (q - k0);
+      const int nk = k - k0;
+      int j = 0;
+      for (; j + PANEL_UF <= nk; j += PANEL_UF) {
+        T av[PANEL_UF], wv[PANEL_UF];
+#pragma unroll
+        for (int u = 0; u < PANEL_UF; ++u) {
+          av[u] = ar[(int64_t)(j + u) * ldsm];
+          wv[u] = wr[(int64_t)(j + u) * ldsm];
+        }
+#pragma unroll
+        for (int u = 0; u < PANEL_UF; ++u) v -= av[u] * wv[u];
+      }
+      for (; j < nk; ++j) v -= ar[(int64_t)j * ldsm] * wr[(int64_t)j * ldsm];
+    } else {
+      int j = k0;
+      for (; j + PANEL_UF <= k; j += PANEL_UF) {
+        T av[PANEL_UF], wv[PANEL_UF];
+#pragma unroll
+        for (int u = 0; u < PANEL_UF; ++u) {
+          av[u] = A[(int64_t)(j + u) * r + i];
+          wv[u] = Wp[(int64_t)(j + u - k0) * w + q];
+        }
+#pragma unroll
+        for (int u = 0; u < PANEL_UF; ++u) v -= av[u] * wv[u];
+      }
+      for (; j < k; ++j) v -= A[(int64_t)j * r + i] * Wp[(int64_t)(j - k0) * w + q];
+    }
+    return v;
+  };
   int epoch = 0;
   int k = k0;
-#define AA(i, j) A[(int64_t)(j) * r + (i)]
-#define WW(i, j) Wp[(int64_t)((j) - k0) * w + (i)]
   while (k < w && (last || k - k0 < PANEL_NB - 1)) {
     // ---- phase A: updated column k for own rows, local argmax
     double bv = -1.0;
     int bi = 0x7fffffff;
     for (int i = k + gt; i < w; i += GT) {
-      T v = AA(i, k);
-      for (int j = k0; j < k; ++j) v -= AA(i, j) * WW(k, j);
+      const T v = left_update(AA(i, k), i, k, k);
       t0[i] = v;
       if (i > k) {
         double av = fabs(v);
@@ -387,14 +484,19 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       }
     }
     block_argmax(bv, bi, sv, si);
-    if (threadIdx.x == 0) {
+    if (G > 1 && threadIdx.x == 0) {
       part[g] = (T)bv;
       parti[g] = bi;
     }
     group_barrier(ctr, G, epoch, status);
     double colmax = -1.0;
     int imax = 0x7fffffff;
-    for (int q = 0; q < G; ++q) better(colmax, imax, part[q], parti[q]);
+    if (G > 1) {
+      for (int q = 0; q < G; ++q) better(colmax, imax, part[q], parti[q]);
+    } else {  // the block maximum in T, as the cross-group exchange stores it
+      colmax = (double)(T)bv;
+      imax = bi;
+    }
     if (k == w - 1 || colmax < 0) {
       colmax = 0.0;
       imax = k;
@@ -410,8 +512,7 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
       double rv = -1.0;
       int ri = 0x7fffffff;
       for (int i = k + gt; i < w; i += GT) {
-        T v = (i < imax) ? AA(imax, i) : AA(i, imax);
-        for (int j = k0; j < k; ++j) v -= AA(i, j) * WW(imax, j);
+        const T v = left_update((i < imax) ? AA(imax, i) : AA(i, imax), i, imax, k);
         t1[i] = v;
         if (i != imax) {
           double av = fabs(v);
@@ -420,14 +521,19 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
         }
       }
       block_argmax(rv, ri, sv, si);
-      if (threadIdx.x == 0) {
+      if (G > 1 && threadIdx.x == 0) {
         part[G + g] = (T)rv;
         parti[G + g] = ri;
       }
       group_barrier(ctr, G, epoch, status);
       double rowmax = -1.0;
       int rj = 0x7fffffff;
-      for (int q = 0; q < G; ++q) better(rowmax, rj, part[G + q], parti[G + q]);
+      if (G > 1) {
+        for (int q = 0; q < G; ++q) better(rowmax, rj, part[G + q], parti[G + q]);
+      } else {
+        rowmax = (double)(T)rv;
+        rj = ri;
+      }
       if (rowmax < 0) rowmax = 0.0;
       if (absakk >= BK_ALPHA * colmax * (colmax / rowmax)) {
         kp = k;
@@ -524,27 +630,63 @@ __global__ __launch_bounds__(256) void k_bk_panel(const int* list, int G, const 
     group_barrier(ctr, G, epoch, status);
     k += kstep;
   }
+  if (LDSP) {
+    // the staged columns back to the factor, and the W rows the trailing
+    // update reads (rows >= k of the eliminated columns)
+    for (int64_t e = threadIdx.x; e < (int64_t)(pc1 - k0) * (w - k0); e += 256) {
+      const int64_t j = e / (w - k0), i = e % (w - k0);
+      A[(k0 + j) * r + (k0 + i)] = As[j * ldsm + i];
+    }
+    for (int64_t e = threadIdx.x; e < (int64_t)(k - k0) * (w - k); e += 256) {
+      const int64_t j = e / (w - k), i = k + e % (w - k);
+      Wp[j * w + i] = Ws[j * ldsm + (i - k0)];
+    }
+  }
   if (gt == 0) {
     kstart[s] = k0;
     kpos[s] = k;
   }
-#undef AA
-#undef WW
 }
+
+// LDS of the staged panel of a diagonal block with up to `rows` rows
+template <typename T>
+size_t panel_lds(int rows) {
+  return sizeof(T) * (2 * (size_t)PANEL_NB + 2) * (size_t)rows;
+}
+template <typename T>
+int panel_lds_rows() {
+  return (int)(PANEL_LDS_BYTES / (sizeof(T) * (2 * (size_t)PANEL_NB + 2)));
+}
+template int panel_lds_rows<double>();
+template int panel_lds_rows<float>();
+
 template <typename T>
 void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
-                     int* status, bool cooperative, hipStream_t st) {
+                     int* status, bool cooperative, int lds_rows, hipStream_t st) {
   if (count <= 0) return;
-  if (!cooperative || G == 1) {
-    k_bk_panel<T><<<count * G, 256, 0, st>>>(list, G, a.super, a.pi, a.px, valL, dval, perm_piv, W, woff,
-                                          kpos, kstart, sync, a.n, status);
+  if (lds_rows > 0) {
+    if (G != 1 || cooperative || lds_rows > panel_lds_rows<T>())
+      throw std::invalid_argument("bk_panel: LDS panel needs one workgroup per supernode and fitting rows");
+    static bool attr = false;
+    if (!attr) {
+      if (hipFuncSetAttribute((const void*)k_bk_panel<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)PANEL_LDS_BYTES) != hipSuccess)
+        throw std::runtime_error("bk_panel: LDS attribute");
+      attr = true;
+    }
+    k_bk_panel<T, true><<<count, 256, panel_lds<T>(lds_rows), st>>>(list, 1, a.super, a.pi, a.px, valL, dval,
+                                                                     perm_piv, W, woff, kpos, kstart, sync, a.n,
+                                                                     status, lds_rows);
+  } else if (!cooperative || G == 1) {
+    k_bk_panel<T, false><<<count * G, 256, 0, st>>>(list, G, a.super, a.pi, a.px, valL, dval, perm_piv, W, woff,
+                                                    kpos, kstart, sync, a.n, status, 0);
   } else {
-    int n = a.n;
+    int n = a.n, z = 0;
     void* args[] = {(void*)&list, (void*)&G, (void*)&a.super, (void*)&a.pi, (void*)&a.px,
                     (void*)&valL, (void*)&dval, (void*)&perm_piv, (void*)&W, (void*)&woff,
-                    (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&n, (void*)&status};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel<T>, dim3(count * G), dim3(256), args,
+                    (void*)&kpos, (void*)&kstart, (void*)&sync, (void*)&n, (void*)&status, (void*)&z};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_bk_panel<T, false>, dim3(count * G), dim3(256), args,
                                               0, st);
     if (e != hipSuccess) throw std::runtime_error(std::string("bk_panel coop: ") + hipGetErrorString(e));
   }
@@ -914,14 +1056,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* tile_prefix, int count,
                                                const int* super, const int64_t* pi, const int64_t* px,
                                                const int* rows, const int* col2sup, T* valL,
-                                               const T* dval, int n, const int* tilepos) {
+                                               const T* dval, int n, const int* tilepos,
+                                               const unsigned char* tmask) {
   __shared__ T As[KC * LDT], Bs[KC * LDT];
   __shared__ T halo[2][64];
   __shared__ T dd[KC + 2], oo[KC + 2];
   __shared__ int rlab[64], clab[64], ctg[64], dtcol[64];
   __shared__ int tpos[4][64];
   __shared__ int64_t cbase[64];
-  __shared__ int dtl[4], ntg;
+  __shared__ int dtl[4], ntg, reach;
   const int64_t bid = blockIdx.x;
   const int lo = tilepos[bid];  // the tile's supernode: its position in the list
   const int s = list[lo];
@@ -940,6 +1083,13 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
     rlab[t] = (i0 + t < m) ? rows[pi[s] + w + i0 + t] : -1;
     clab[t] = (j0 + t < m) ? rows[pi[s] + w + j0 + t] : -1;
     ctg[t] = clab[t] >= 0 ? col2sup[clab[t]] : -1;
+  }
+  if (tmask) {  // a restricted refactorization: tiles reaching no flagged supernode are skipped
+    if (t == 0) reach = 0;
+    __syncthreads();
+    if (t < 64 && ctg[t] >= 0 && tmask[ctg[t]]) reach = 1;
+    __syncthreads();
+    if (!reach) return;
   }
   __syncthreads();
   // distinct target supernodes of the tile's columns: labels are sorted, so
@@ -1034,6 +1184,7 @@ __global__ __launch_bounds__(256) void k_schur(const int* list, const int64_t* t
   tile_foreach(acc, [&](int ri, int ci, T v) {
     const int R = rlab[ri], C = clab[ci];
     if (R < 0 || C < 0 || R < C) return;
+    if (tmask && !tmask[ctg[ci]]) return;
     const int dt = dtcol[ci];
     int64_t pos;
     if (dt >= 0) {
@@ -1051,7 +1202,7 @@ void launch_schur(const int* list, int count, const int64_t* tile_prefix, const 
   if (count <= 0 || ntiles <= 0) return;
   if (ntiles > 0x7fffffffLL) throw std::runtime_error("schur: too many tiles");
   k_schur<T><<<(unsigned)ntiles, 256, 0, st>>>(list, tile_prefix, count, a.super, a.pi, a.px, a.rows,
-                                            a.col2sup, valL, dval, a.n, tilepos);
+                                            a.col2sup, valL, dval, a.n, tilepos, a.tmask);
   HIP_LAUNCH_CHECK("schur");
 }
 
@@ -2016,6 +2167,12 @@ void launch_diag_scaling(int n, int nnz, const int* rp, const int64_t* src, cons
   k_scale_entries<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, row, col, Kx, s, Kf);
   HIP_LAUNCH_CHECK("scale_entries");
 }
+void launch_scale_entries(int nnz, const int* row, const int* col, const double* Kx, const double* s, double* Kf,
+                          hipStream_t st) {
+  if (nnz <= 0) return;
+  k_scale_entries<<<grid_for(nnz, 256), 256, 0, st>>>(nnz, row, col, Kx, s, Kf);
+  HIP_LAUNCH_CHECK("scale_entries");
+}
 void launch_gather_scale(int n, const int* perm_f, const double* s, double* s_ord, hipStream_t st) {
   k_gather_scale<<<grid_for(n, 256), 256, 0, st>>>(n, perm_f, s, s_ord);
   HIP_LAUNCH_CHECK("gather_scale");
@@ -2055,12 +2212,12 @@ __global__ void k_rowabs_max(int n, const int* rp, const double* Kv, double* par
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
+// one wave: lanes over the partial maxima, then a wave maximum
 __global__ void k_max_final(const double* part, int nblk, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double m = 0.0;
-    for (int b = 0; b < nblk; ++b) m = fmax(m, part[b]);
-    out[0] = m;
-  }
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64) m = fmax(m, part[b]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if (threadIdx.x == 0) out[0] = m;
 }
 void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const double* Kx,
                const unsigned char* pert, double* Kv, double* part, int nblk, double* norm_out, hipStream_t st) {
@@ -2077,10 +2234,11 @@ void launch_k0(int n, int64_t nnzf, const int* rp, const int64_t* src, const dou
 #define LBLG_INSTANTIATE(T)                                                                                      \
   template void launch_fill<T>(T*, int64_t, double, hipStream_t);                                              \
   template void launch_assemble<T>(const int64_t*, const double*, const unsigned char*, int, double, T*,       \
-                                   hipStream_t);                                                               \
+                                   const int*, const unsigned char*, hipStream_t);                             \
+  template void launch_reset_supernodes<T>(const int*, int, const SnArgs&, T*, T*, int*, hipStream_t);          \
   template void launch_bk_small<T>(const int*, int, int, const SnArgs&, T*, T*, int*, int*, hipStream_t);      \
   template void launch_bk_panel<T>(const int*, int, int, const SnArgs&, T*, T*, int*, T*, const int64_t*, int*, \
-                                   int*, int*, int*, bool, hipStream_t);                                       \
+                                   int*, int*, int*, bool, int, hipStream_t);                                       \
   template void launch_trailing<T>(const int*, int, int, int, const SnArgs&, T*, const T*, const int64_t*,      \
                                    const int*, const int*, hipStream_t);                                       \
   template void launch_offdiag<T>(const int*, int, int, const int*, const int*, int, const SnArgs&, T*,      \

@@ -20,6 +20,9 @@ struct SnArgs {  // symbolic arrays on the device
   const int* rows;
   const int* col2sup;
   int n;
+  // path-restricted refactorization: Schur updates reach only supernodes
+  // whose flag is set (nullptr: all)
+  const unsigned char* tmask = nullptr;
 };
 
 template <typename T>
@@ -33,16 +36,31 @@ void launch_convert(TO* dst, const TI* src, int64_t count, hipStream_t st);
 // The value kernels are templates over the factor's type T (double: plain
 // FP64 factor, float: FP32 factor), instantiated for both in kernels.hip,
 // bk_wide.hip and gemm_big.hip.
+// asup / tmask (both or neither): only entries whose slot lies in a supernode
+// with its tmask flag set are written
 template <typename T>
 void launch_assemble(const int64_t* amap, const double* Kx, const unsigned char* pert, int nnz, double reg,
-                     T* valL, hipStream_t st);
+                     T* valL, const int* asup, const unsigned char* tmask, hipStream_t st);
+// the panels, D entries and local pivots of the listed supernodes back to
+// zero / identity before they are refactored
+template <typename T>
+void launch_reset_supernodes(const int* list, int count, const SnArgs& a, T* valL, T* dval, int* perm_piv,
+                             hipStream_t st);
+// Kx[idx[k]] = vals[k]
+void launch_scatter_values(int64_t count, const int64_t* idx, const double* vals, double* Kx, hipStream_t st);
 template <typename T>
 void launch_bk_small(const int* list, int count, int wmax, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, int* status, hipStream_t st);
+// lds_rows > 0: one workgroup per supernode with the panel staged in LDS
+// (every supernode of the list has at most lds_rows <= panel_lds_rows<T>()
+// columns); 0: the panel stays in memory
+constexpr size_t PANEL_LDS_BYTES = 150 * 1024;
+template <typename T>
+int panel_lds_rows();
 template <typename T>
 void launch_bk_panel(const int* list, int count, int G, const SnArgs& a, T* valL, T* dval,
                      int* perm_piv, T* W, const int64_t* woff, int* kpos, int* kstart, int* sync,
-                     int* status, bool cooperative, hipStream_t st);
+                     int* status, bool cooperative, int lds_rows, hipStream_t st);
 // wide panels with flag-in-word exchange (bk_wide.hip)
 constexpr int WIDE_NB = 64;         // panel width
 constexpr int WIDE_ROWS = 192;      // far rows per workgroup (256-thread workgroups)
@@ -160,6 +178,10 @@ void launch_pert_flags(int nnz, const unsigned char* isdiag, const double* Kx, c
 // symmetric diagonal scaling of the BF16 factor (kernels.hip)
 void launch_diag_scaling(int n, int nnz, const int* rp, const int64_t* src, const int* row, const int* col,
                          const double* Kx, double* s, double* Kf, hipStream_t st);
+// Kf = S K S with the scaling s as it is (the path-restricted refactorization
+// of the FP32 / BF16 factors keeps the scaling of the last full one)
+void launch_scale_entries(int nnz, const int* row, const int* col, const double* Kx, const double* s, double* Kf,
+                          hipStream_t st);
 void launch_gather_scale(int n, const int* perm_f, const double* s, double* s_ord, hipStream_t st);
 template <typename TO, typename TI>
 void launch_row_scale_copy(TO* dst, const TI* src, const double* s, int n, int nrhs, hipStream_t st);

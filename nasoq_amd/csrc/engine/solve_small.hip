@@ -48,10 +48,20 @@ __device__ inline T ss_wave_sum(T v) {
   return v;
 }
 
+// Loads of the factor are issued several at a time before their values are
+// used (the loops below would otherwise wait a full memory latency per
+// column), and a lane's gathered right-hand-side rows are read once per
+// 64-column chunk rather than once per column.  The number in flight
+// shrinks with the right-hand sides a lane carries, so that the kernel keeps
+// within the 128 registers of a 1024-thread workgroup.
+template <int NR>
+constexpr int ss_uf() { return NR <= 2 ? 4 : 2; }
+
 // forward sweep of supernode s by one wave: x(s) = L_ss^{-1} x(s), then
 // x(label q) -= L(q, s) x(s) for the rows below
 template <typename T, int NR>
 __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, T* x) {
+  constexpr int SS_UF = ss_uf<NR>();
   const int c0 = I.x, w = I.y;
   const int64_t r = I.z, pis = I.w;
   const T* P = valL + px;
@@ -60,10 +70,22 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
     T v[NR];
 #pragma unroll
     for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
-    for (int j = 0; j + 1 < cw; ++j) {
-      const T l = (ln > j && ln < cw) ? P[(cb + ln) + (int64_t)(cb + j) * r] : (T)0;
+    const T* Pl = P + (cb + ln) + (int64_t)cb * r;  // L(cb + ln, cb + j) = Pl[j r]
+    for (int j0 = 0; j0 + 1 < cw; j0 += SS_UF) {
+      T l[SS_UF];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) v[c] -= l * ss_lane(v[c], j);
+      for (int u = 0; u < SS_UF; ++u) {
+        const int j = j0 + u;
+        l[u] = (j + 1 < cw && ln > j && ln < cw) ? Pl[(int64_t)j * r] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < SS_UF; ++u) {
+        const int j = j0 + u;
+        if (j + 1 < cw) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) v[c] -= l[u] * ss_lane(v[c], j);
+        }
+      }
     }
     if (ln < cw) {
 #pragma unroll
@@ -72,20 +94,28 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
     for (int64_t q0 = cb + cw; q0 < r; q0 += 64) {
       const int64_t q = q0 + ln;
       const bool ok = q < r;
+      const int lab = (ok && q >= w) ? a.rows[pis + q] : 0;
+      const T* Pq = P + q + (int64_t)cb * r;  // L(q, cb + j) = Pq[j r]
       T acc[NR];
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = (T)0;
-      for (int j = 0; j < cw; ++j) {
-        const T l = ok ? P[q + (int64_t)(cb + j) * r] : (T)0;
+      for (int j0 = 0; j0 < cw; j0 += SS_UF) {
+        T l[SS_UF];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] += l * ss_lane(v[c], j);
+        for (int u = 0; u < SS_UF; ++u) l[u] = (ok && j0 + u < cw) ? Pq[(int64_t)(j0 + u) * r] : (T)0;
+#pragma unroll
+        for (int u = 0; u < SS_UF; ++u) {
+          if (j0 + u < cw) {
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[c] += l[u] * ss_lane(v[c], j0 + u);
+          }
+        }
       }
       if (ok) {
         if (q < w) {  // a later chunk of this supernode: its rows are this wave's alone
 #pragma unroll
           for (int c = 0; c < NR; ++c) x[(c0 + (int)q) * NR + c] -= acc[c];
         } else {      // an ancestor's row, shared with other supernodes of the level
-          const int lab = a.rows[pis + q];
 #pragma unroll
           for (int c = 0; c < NR; ++c) atomicAdd(&x[lab * NR + c], -acc[c]);
         }
@@ -97,9 +127,12 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
 // backward sweep of supernode s by one wave: x(s) = L_ss^{-T} (x(s) - L(q, s)^T x(label q))
 template <typename T, int NR>
 __device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, T* x) {
+  constexpr int SS_UF = ss_uf<NR>();
   const int c0 = I.x, w = I.y;
   const int64_t r = I.z, pis = I.w;
   const T* P = valL + px;
+  // row blocks of 64 whose gathered x values a lane holds at once
+  constexpr int RB = NR <= 2 ? 4 : 2;
   for (int cb = ((w - 1) / 64) * 64; cb >= 0; cb -= 64) {
     const int cw = min(64, w - cb);
     const int64_t qb = cb + cw, rr = r - qb;  // rows below the chunk
@@ -108,39 +141,78 @@ __device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, c
     for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
     if (rr > 0) {
       if ((int64_t)cw * ((rr + 63) / 64 + 6 * NR) < rr) {
-        // narrow chunk, many rows: lanes over rows, one wave sum per column
-        for (int j = 0; j < cw; ++j) {
-          const T* col = P + (int64_t)(cb + j) * r;
-          T acc[NR];
+        // narrow chunk, many rows: lanes over rows, one wave sum per column;
+        // the lane's rows and their x values are loaded once for all columns
+        for (int64_t qs = qb; qs < r; qs += 64 * RB) {
+          T xs[RB][NR];
+          bool okb[RB];
 #pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] = (T)0;
-          for (int64_t q = qb + ln; q < r; q += 64) {
-            const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
-            const T l = col[q];
+          for (int b = 0; b < RB; ++b) {
+            const int64_t q = qs + 64 * b + ln;
+            okb[b] = q < r;
+            const int lab = !okb[b] ? 0 : (q < w ? c0 + (int)q : a.rows[pis + q]);
 #pragma unroll
-            for (int c = 0; c < NR; ++c) acc[c] += l * x[lab * NR + c];
+            for (int c = 0; c < NR; ++c) xs[b][c] = okb[b] ? x[lab * NR + c] : (T)0;
           }
+          const T* Pq = P + qs + ln + (int64_t)cb * r;  // L(qs + ln + 64 b, cb + j) = Pq[64 b + j r]
+          for (int j0 = 0; j0 < cw; j0 += SS_UF / 2) {
+            T l[SS_UF / 2][RB];
 #pragma unroll
-          for (int c = 0; c < NR; ++c) {
-            const T tot = ss_wave_sum(acc[c]);
-            if (ln == j) v[c] -= tot;
+            for (int u = 0; u < SS_UF / 2; ++u)
+#pragma unroll
+              for (int b = 0; b < RB; ++b)
+                l[u][b] = (okb[b] && j0 + u < cw) ? Pq[64 * b + (int64_t)(j0 + u) * r] : (T)0;
+#pragma unroll
+            for (int u = 0; u < SS_UF / 2; ++u) {
+              if (j0 + u < cw) {
+#pragma unroll
+                for (int c = 0; c < NR; ++c) {
+                  T part = (T)0;
+#pragma unroll
+                  for (int b = 0; b < RB; ++b) part += l[u][b] * xs[b][c];
+                  const T tot = ss_wave_sum(part);
+                  if (ln == j0 + u) v[c] -= tot;
+                }
+              }
+            }
           }
         }
       } else if (ln < cw) {
-        // lanes over columns, rows in order
+        // lanes over columns, rows in order (SS_UF rows per round of loads)
         const T* col = P + (int64_t)(cb + ln) * r;
-        for (int64_t q = qb; q < r; ++q) {
-          const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
-          const T l = col[q];
+        for (int64_t q0 = qb; q0 < r; q0 += SS_UF) {
+          int lab[SS_UF];
+          T l[SS_UF];
 #pragma unroll
-          for (int c = 0; c < NR; ++c) v[c] -= l * x[lab * NR + c];
+          for (int u = 0; u < SS_UF; ++u) {
+            const int64_t q = q0 + u;
+            lab[u] = q < r ? (q < w ? c0 + (int)q : a.rows[pis + q]) : 0;
+            l[u] = q < r ? col[q] : (T)0;
+          }
+#pragma unroll
+          for (int u = 0; u < SS_UF; ++u) {
+#pragma unroll
+            for (int c = 0; c < NR; ++c) v[c] -= l[u] * x[lab[u] * NR + c];
+          }
         }
       }
     }
-    for (int i = cw - 1; i > 0; --i) {
-      const T l = ln < i ? P[(cb + i) + (int64_t)(cb + ln) * r] : (T)0;
+    const T* Pt = P + (int64_t)(cb + ln) * r + cb;  // L(cb + i, cb + ln) = Pt[i]
+    for (int i0 = cw - 1; i0 > 0; i0 -= SS_UF) {
+      T l[SS_UF];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) v[c] -= l * ss_lane(v[c], i);
+      for (int u = 0; u < SS_UF; ++u) {
+        const int i = i0 - u;
+        l[u] = (i > 0 && ln < i) ? Pt[i] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < SS_UF; ++u) {
+        const int i = i0 - u;
+        if (i > 0) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) v[c] -= l[u] * ss_lane(v[c], i);
+        }
+      }
     }
     if (ln < cw) {
 #pragma unroll
@@ -248,6 +320,7 @@ __device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T
 // backward sweep of supernode s by the whole workgroup (uniform control flow)
 template <typename T, int NR>
 __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x, T* tile, T* red) {
+  constexpr int SS_UF = ss_uf<NR>();
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   const int c0 = I.x, w = I.y;
   const int64_t r = I.z, pis = I.w;
@@ -267,11 +340,20 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
       T acc[NR];
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = (T)0;
-      for (int64_t q = qs; q < qe; ++q) {
-        const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
-        const T l = col[q];
+      for (int64_t q0 = qs; q0 < qe; q0 += SS_UF) {
+        int lab[SS_UF];
+        T l[SS_UF];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] += l * x[lab * NR + c];
+        for (int u = 0; u < SS_UF; ++u) {
+          const int64_t q = q0 + u;
+          lab[u] = q < qe ? (q < w ? c0 + (int)q : a.rows[pis + q]) : 0;
+          l[u] = q < qe ? col[q] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < SS_UF; ++u) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += l[u] * x[lab[u] * NR + c];
+        }
       }
       if (qs < qe) {
 #pragma unroll

@@ -120,6 +120,8 @@ void QpSolver::build_super_matrix() {
   }
   rhs.assign(N, 0.0);
   sol.assign(N, 0.0);
+  kkt_changed.clear();
+  kkt_uploaded = false;
 }
 
 double QpSolver::hessian_frobenius() const {  // norm_sparse_int(n, H, -1, 2)
@@ -192,6 +194,7 @@ void QpSolver::symbolic() {
   st.max_inner_iter = inner_iter_ref;
   st.tol_abs = st.tol_rel = stop_tol;
   st.ordering = ordering;
+  st.precision = precision;
   // the super-KKT pattern depends on the problem only: a later solve of the
   // same problem (new q or b, or the next call) keeps the analysis and the
   // device factor, and the KKT values are refreshed at the next factorization
@@ -231,18 +234,33 @@ void QpSolver::analyze_lbl(const lblg_settings& st) {
   analysed = true;
 }
 
+// SolverSettings::update_factorization (linear_solver_wrapper.cpp:1151-1290)
+// after add_del_matrix_qp edits, numerical_factorization for a new super-KKT:
+// the device receives only the edited entries and refactors the supernodes
+// on their tree paths (lblg_update_values), or all of SM.x the first time
+void QpSolver::factor_kkt() {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  if (!kkt_uploaded) {
+    check(lblg_set_values(h, SM.x.data()), "qp: set values");
+  } else if (!kkt_changed.empty()) {
+    std::vector<double> v(kkt_changed.size());
+    for (size_t k = 0; k < v.size(); ++k) v[k] = SM.x[kkt_changed[k]];
+    check(lblg_update_values(h, (int64_t)v.size(), kkt_changed.data(), v.data()), "qp: update values");
+  }
+  check(lblg_factor(h), "qp: factor");
+  kkt_uploaded = true;
+  kkt_changed.clear();
+  factor_seconds += std::chrono::duration<double>(clock::now() - t0).count();
+  refactored_supernodes += lblg_refactored_supernodes(h);
+  ++factorizations;
+}
+
 // Nasoq::solve_kkt (nasoq.cpp:918-949): refactor (UPDATE / REFACTOR) or reuse
 // the factor (SOLVE), then the refined solve
 void QpSolver::solve_kkt(bool refactor) {
   using clock = std::chrono::steady_clock;
-  if (refactor) {
-    const auto t0 = clock::now();
-    check(lblg_set_values(h, SM.x.data()), "qp: set values");
-    check(lblg_factor(h), "qp: factor");
-    factor_seconds += std::chrono::duration<double>(clock::now() - t0).count();
-    refactored_supernodes += lblg_symbolic_scalar(h, 1);
-    ++factorizations;
-  }
+  if (refactor) factor_kkt();
   const int64_t N = SM.ncol;
   double bwd = 0.0;
   const auto t0 = clock::now();
@@ -257,6 +275,7 @@ void QpSolver::edit_kkt(bool add, int c) {
   for (int e = brow_ptr[c]; e < brow_ptr[c + 1]; ++e) {
     const int k = brow_ent[e];
     SM.x[bpos[k]] = add ? B.x[k] : 0.0;
+    kkt_changed.push_back(bpos[k]);
   }
 }
 

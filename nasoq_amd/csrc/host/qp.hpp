@@ -34,6 +34,7 @@ struct QpSolver {
   // settings (the reference's public fields, Nasoq::default_setting)
   double diag_perturb = 1e-9, zero_thresh = 1e-9, eps_abs = 1e-3, stop_tol = 1e-15;
   int max_iter = 0, max_iter_nas = 4000, variant = QP_FIXED, ordering = LBLG_ORDER_ND;
+  int precision = LBLG_F64;  // value type of the KKT factor (refinement is FP64 either way)
   int inner_iter_ref = 0, outer_iter_ref = 0, auto_reg_en = 0;
   // state and results
   std::vector<double> primal, dual, dual_eq;
@@ -50,6 +51,10 @@ struct QpSolver {
   std::vector<int64_t> bpos;                // SM entry of every entry of B (B's CSC order)
   std::vector<int> brow_ptr, brow_ent;      // entries of B grouped by constraint
   std::vector<double> rhs, sol;
+  // super-KKT entries edited since the device last received SM.x, and
+  // whether it holds SM.x at all (a new super-KKT is uploaded whole)
+  std::vector<int64_t> kkt_changed;
+  bool kkt_uploaded = false;
   lblg_handle* h = nullptr;
   bool analysed = false;
   lblg_settings analysed_settings{};  // settings of the current analysis of h
@@ -70,6 +75,7 @@ struct QpSolver {
   int predicted_prunes = 0;  // KKT solves spent making a predicted active set dual feasible
   int check_status(int converged);
   // pieces of the active-set iteration
+  void factor_kkt();  // the device factor of SM.x (the SoMod update after edits)
   void solve_kkt(bool refactor);
   void edit_kkt(bool add, int c);
   int primal_feasibility(int& nxt_add);

@@ -214,6 +214,22 @@ def _numeric_methods():
         self.Ax = np.ascontiguousarray(Ax, dtype=np.float64).copy()
         check(self.lib.lblg_set_values(self.h, dptr(self.Ax)), "lblg_set_values")
 
+    def update_values(self, idx, vals):
+        """Change entries Ax[idx] = vals in place; the next numerical_factorization
+        refactors only the supernodes on their tree paths to the root (the
+        SoMod update) when the handle holds an FP64 factor of the old values."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        assert idx.shape == vals.shape
+        check(self.lib.lblg_update_values(self.h, len(idx), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                          dptr(vals)), "lblg_update_values")
+        self.Ax = np.array(self.Ax, dtype=np.float64)
+        self.Ax[idx] = vals
+
+    def refactored_supernodes(self):
+        """Supernodes the last numerical_factorization refactored."""
+        return int(self.lib.lblg_refactored_supernodes(self.h))
+
     def tuning(self):
         """The handle's device schedule knobs (dict)."""
         t = _lib.Tuning()
@@ -279,7 +295,7 @@ def _numeric_methods():
         return float(e[0]) if np.asarray(rhs).ndim == 1 else e
 
     def scaling(self):
-        """The diagonal scaling s of the factor (S K~ S is factored; ones unless precision is BF16)."""
+        """The diagonal scaling s of the factor (S K~ S is factored; ones for the FP64 factor)."""
         out = np.zeros(self.n, dtype=np.float64)
         check(self.lib.lblg_get_scaling(self.h, dptr(out)), "lblg_get_scaling")
         return out
@@ -289,9 +305,9 @@ def _numeric_methods():
         check(self.lib.lblg_factor_info(self.h, ctypes.byref(z), ctypes.byref(nrm), ctypes.byref(sec)), "info")
         return {"zero_pivot": z.value, "k0_norm_inf": nrm.value, "factor_seconds": sec.value}
 
-    for f in (numerical_factorization, _push_settings, refactor_resident, solve_only, set_values, graph_stats,
-              tuning, set_tuning, autotune, inertia, get_factor, residual_norms, backward_error, scaling,
-              factor_info):
+    for f in (numerical_factorization, _push_settings, refactor_resident, solve_only, set_values, update_values,
+              refactored_supernodes, graph_stats, tuning, set_tuning, autotune, inertia, get_factor, residual_norms,
+              backward_error, scaling, factor_info):
         setattr(SolverSettings, f.__name__, f)
 
 

@@ -184,6 +184,35 @@ def batch_qp_kkts(count=1024, seed=0, nmin=16, nmax=160, meq_max=64, nnz_row=4):
     return out
 
 
+def box_qp(rng, r, c, meq, nnz_row=4):
+    """One member of the batched QP configuration: H the 5-point Laplacian of
+    an r x c mesh + 1e-2 I, q ~ N(0,1), meq equality rows of nnz_row N(0,1)
+    entries with b = A x0, x0 ~ U[-1,1], and the box |x| <= 1 as the 2n
+    inequality rows [I; -I] x <= 1.  Returns (H (full), q, A, b, C, d)."""
+    H = laplacian_2d(r, c, 1e-2)
+    n = H.shape[0]
+    q = rng.standard_normal(n)
+    A = random_rows(rng, meq, n, nnz_row).tocsc()
+    x0 = rng.uniform(-1.0, 1.0, n)
+    b = A @ x0
+    C = sp.vstack([sp.identity(n), -sp.identity(n)], format="csc")
+    d = np.ones(2 * n)
+    return H, q, A, b, C, d
+
+
+def box_qp_batch(count, seed=0, lo=16, hi=160, meq_max=64):
+    """The batched QP configuration: `count` independent box-constrained QPs
+    (box_qp) with r, c ~ U{lo..hi} and meq ~ U{1..meq_max}, all from one
+    seeded generator (the first k members do not depend on count)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        r, c = (int(v) for v in rng.integers(lo, hi + 1, size=2))
+        meq = int(rng.integers(1, meq_max + 1))
+        out.append(box_qp(rng, r, c, meq))
+    return out
+
+
 def qp_config(grid=40, meq=100, nnz_row=5, seed=0):
     """The convex QP min 1/2 x'Hx + q'x s.t. Ax = b, Cx <= d of the plain
     NASOQ configuration: H the 5-point Laplacian of a grid x grid mesh

@@ -158,6 +158,42 @@ class Nasoq:
         return N, p, i[:nnz], x[:nnz]
 
 
+def solve_many(problems, settings=None, threads=8, keep=("primal_vars",)):
+    """Solve independent QPs (tuples (H, q, A, b, C, d), A / C may be None)
+    concurrently: `threads` host threads each run NASOQ's active-set loop on
+    one problem at a time, every problem with its own engine handle and HIP
+    stream, so the KKT factorizations and solves of different problems
+    overlap on the GPU (ctypes releases the interpreter lock during the
+    library calls).  settings: dict of public Nasoq fields.  Returns one dict
+    per problem, in order: status, num_iter, objective, the KKT checks, the
+    factorization counters and the fields named in `keep`."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(prob):
+        H, q, A, b, C, d = prob
+        m = Nasoq(sp.tril(H, format="csc"), q, A, b, C, d)
+        try:
+            for k, v in (settings or {}).items():
+                setattr(m, k, v)
+            st = m.solve()
+            out = {"status": st, "num_iter": m.num_iter, "objective": m.objective, "lag_res": m.lag_res,
+                   "cons_sat_norm": m.cons_sat_norm, "n_active": len(m.active_set),
+                   "factorizations": int(m.factorizations), "kkt_solves": int(m.kkt_solves),
+                   "refactored_supernodes": int(m.refactored_supernodes),
+                   "supernodes": int(m.supernodes), "factor_seconds": m.factor_seconds,
+                   "solve_seconds": m.solve_seconds}
+            for k in keep:
+                out[k] = np.array(getattr(m, k), copy=True)
+            return out
+        finally:
+            m.close()
+
+    if threads <= 1:
+        return [one(p) for p in problems]
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, problems))
+
+
 def quadprog(H, q, A, b, C, d, settings=None):
     """The reference's Eigen quadprog (eigen_interface/include/nasoq/nasoq_eigen.h:13-73):
     returns (status, x, y, z).  settings: dict of QPSettings fields (eps,

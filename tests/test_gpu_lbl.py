@@ -389,11 +389,16 @@ def test_supernodes_beyond_256_workgroups_of_192_rows(oracle, prec, monkeypatch)
     assert s.lib.lblg_panel_stats(s.h, ctypes.byref(panels), ctypes.byref(piv)) == 0
     assert panels.value > 50100 // (32 if prec == 0 else 64)  # tall-shape panel widths: 32 FP64, 64 FP32
     assert s.inertia() == (n1, m2, 0)
-    # the narrow supernode's panel against dense arithmetic
+    # the narrow supernode's panel against dense arithmetic (the FP32 factor
+    # is that of S K S, S the max-norm diagonal scaling the handle reports)
     c = 100
     T1 = np.full((c, c), 0.5) + np.eye(c) * (c / 2.0 + 0.5)
     K = P.lower_to_full(n, Ap, Ai, Ax)
     B1 = K[n1:, :c].toarray()
+    if prec != 0:
+        sc = s.scaling()
+        T1 = sc[:c, None] * T1 * sc[None, :c]
+        B1 = sc[n1:, None] * B1 * sc[None, :c]
     Lc = np.linalg.cholesky(T1)
     d1 = np.diag(Lc) ** 2
     L11 = Lc / np.diag(Lc)
