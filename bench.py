@@ -549,7 +549,7 @@ def learned_leg(args):
     train = fam.thetas(args.learned_train, 1)
     log = lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True)
     log(f"family n={fam.n} m={fam.mi}, labelling {len(train)} members with plain NASOQ")
-    Y, iters, t_label = learned.generate(fam, train, solver=S, log=log)
+    Y, iters, t_label = learned.generate(fam, train, solver=S, log=log, threads=args.learned_threads)
     net = learned.ActiveSetPredictor(fam.n_theta, fam.mi, hidden=(1024, 1024), seed=0, max_batch=128)
     t0 = time.perf_counter()
     loss = net.fit(train, Y, epochs=args.learned_epochs, batch=64, lr=1e-3)
@@ -558,14 +558,15 @@ def learned_leg(args):
     t0 = time.perf_counter()
     net.predict_proba(test)
     t_pred = time.perf_counter() - t0
-    res = learned.evaluate(fam, net, test, solver=S)
+    res = learned.evaluate(fam, net, test, solver=S, threads=args.learned_threads)
     net.close()
     S.close()
     ci, wi = res["cold_iter"].sum(), res["warm_iter"].sum()
     out = {"workload": f"parametric QP family, H = 2-D Laplacian {args.learned_grid}x{args.learned_grid} + 1e-2 I "
                        f"(n={fam.n}), {fam.mi} inequality rows of 6 N(0,1), d ~ U[0,0.5], q = U theta (32 params)",
            "mlp": f"32 -> 1024 -> 1024 -> {fam.mi}, BF16 weights, FP32 accumulation",
-           "train_solves": len(train), "label_seconds": t_label, "train_epochs": args.learned_epochs,
+           "train_solves": len(train), "label_seconds": t_label, "label_threads": args.learned_threads,
+           "train_iterations_mean": float(iters.mean()), "train_epochs": args.learned_epochs,
            "train_seconds": t_train, "final_train_loss": loss,
            "train_samples_per_s": len(train) * args.learned_epochs / t_train,
            "predict_ms_per_member": t_pred / len(test) * 1e3,
@@ -609,6 +610,8 @@ def main():
     ap.add_argument("--learned-train", type=int, default=400, help="training members (plain NASOQ solves)")
     ap.add_argument("--learned-test", type=int, default=40, help="held-out members solved cold and warm")
     ap.add_argument("--learned-epochs", type=int, default=60)
+    ap.add_argument("--learned-threads", type=int, default=1,
+                    help="host threads labelling and evaluating members concurrently (one solver each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference traffic for HBM counters)")
     args = ap.parse_args()

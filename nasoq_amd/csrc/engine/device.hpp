@@ -176,6 +176,7 @@ struct DeviceFactor {
   int* ss_levptr = nullptr;
   int* ss_mode = nullptr;  // per level: 1 = whole-workgroup sweeps
   int ss_nlev = 0;
+  bool ss_global = false;       // keep the block in memory even where it fits LDS (tests)
   int64_t ss_nnzL = 0, ss_max_nnz = 0;
   double *stage_b = nullptr, *stage_x = nullptr;  // host-pointer solves: staged blocks (n x nrhs_cap)
   // residual norms of caller-given solutions (residual.hip), allocated at first use

@@ -606,6 +606,7 @@ static void setup(Handle& h) {
     D.ss_mode = dupload(D, mode);
     if (!fits) nl = 0;  // too many levels for the LDS copy: the level sweeps only
     D.ss_nlev = nl;
+    D.ss_global = env_int("LBLG_SMALL_SOLVE_GLOBAL", 0) != 0;  // 1: the block in memory even where it fits LDS
     D.ss_nnzL = S.p[n];
     D.ss_max_nnz = h.tune.small_nnz;
     D.graphs = env_int("LBLG_GRAPH", 1) != 0;
@@ -1197,6 +1198,15 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
 
 // sweeps in the factor's type T; in / out are the FP64 ordered blocks of the
 // refinement (converted at entry and exit when T is float)
+// whether a solve of nrhs columns takes the one-workgroup sweeps
+// (a block beyond the LDS takes the level sweeps unless LBLG_SMALL_SOLVE_GLOBAL
+// asks for the one-workgroup sweeps with the block in memory: measured on a
+// 21k-row QP KKT, the level sweeps' many workgroups were faster, 1.5 vs 2.5 ms)
+static bool small_sweeps(const DeviceFactor& D, int nrhs) {
+  const size_t vs = D.prec != 0 ? sizeof(float) : sizeof(double);
+  return D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && (D.ss_global || vs * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS);
+}
+
 template <typename T>
 static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nrhs, const double* active,
                           int64_t astride) {
@@ -1211,15 +1221,18 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   T* acc = static_cast<T*>(D.acc);
   // the FP32 / BF16 factors are of S K S: M^-1 = S (L D L^T)^-1 S in the pivoted order
   const bool scaled = D.prec != 0;
-  if (D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && sizeof(T) * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS) {
+  if (small_sweeps(D, nrhs)) {
     const double* sin = in;
     if (scaled) {
       launch_row_scale_copy(D.sbuf, in, D.scale_ord, D.n, nrhs, st);
       sin = D.sbuf;
     }
+    // the ordered block in LDS, or in memory (Z) when asked for or when it
+    // does not fit (small_sweeps admits that only when asked for)
+    T* xg = (D.ss_global || sizeof(T) * (size_t)D.n * nrhs > SOLVE_SMALL_LDS) ? Z : nullptr;
     timed(D, "solve_small", [&] {
       launch_solve_small<T>(D.ss_info, D.ss_ipx, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
-                            static_cast<const T*>(D.dval), sin, out, nrhs, active, astride, st);
+                            static_cast<const T*>(D.dval), sin, out, nrhs, active, astride, xg, st);
     });
     if (scaled) launch_row_scale_copy(out, out, D.scale_ord, D.n, nrhs, st);
     return;
@@ -1396,11 +1409,6 @@ static void backward_errors(Handle& h, const double* b, const double* x, int nrh
   }
 }
 
-// whether a solve of nrhs columns takes the one-workgroup sweeps
-static bool small_sweeps(const DeviceFactor& D, int nrhs) {
-  const size_t vs = D.prec != 0 ? sizeof(float) : sizeof(double);
-  return D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && vs * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS;
-}
 
 // refinement of the ordered solution xo of the ordered block D.B to a target
 // backward error: GMRES cycles until every column meets it (host decision

@@ -136,8 +136,9 @@ void launch_scatter_sol(int n, int nrhs, const double* Xo, const int* perm_f, do
 // L, D and L^T sweeps of a small factor in one workgroup with the ordered
 // block (n x nrhs, rows interleaved) resident in LDS; supernodes level by
 // level in order[levptr[l] .. levptr[l + 1]), mode[l] = 1 where the whole
-// workgroup takes the level's supernodes one at a time.  Requires n * nrhs *
-// sizeof(T) <= SOLVE_SMALL_LDS.
+// workgroup takes the level's supernodes one at a time.  The block is held in
+// LDS when n * nrhs * sizeof(T) <= SOLVE_SMALL_LDS, otherwise in xg (n x nrhs
+// values of T in memory; nullptr: LDS).
 constexpr size_t SOLVE_SMALL_LDS = 104 * 1024;
 constexpr int SOLVE_SMALL_LEVELS = 2048;  // level pointers and modes kept in LDS
 constexpr int SS_THIN_ROWS = 32;          // one-column supernodes up to this many rows: one lane each,
@@ -150,7 +151,7 @@ template <typename T>
 // participation flags; when none is set the launch returns at once
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                         const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
-                        const double* active, int64_t astride, hipStream_t st);
+                        const double* active, int64_t astride, T* xg, hipStream_t st);
 // per-column dot products out[c] = sum_i a(i, c) b(i, c) (rows interleaved by
 // column), reduced in a fixed order through at most nblk partials
 void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,

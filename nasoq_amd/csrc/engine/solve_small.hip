@@ -42,6 +42,25 @@ __device__ inline double ss_lane(double v, int j) {
 __device__ inline float ss_lane(float v, int j) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
+// The ordered block x: in LDS (XG = false), or in memory when it does not
+// fit (XG = true).  In memory it is read with agent-scope loads, which go to
+// the L2, so a wave never reads a stale vector-L1 line after another wave's
+// atomic update (the atomics are performed in the L2); the workgroup
+// barriers between levels order the writes.
+template <typename T, bool XG>
+struct XBlock {
+  T* p;
+  __device__ T ld(int64_t i) const {
+    if constexpr (XG) return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return p[i];
+  }
+  __device__ void st(int64_t i, T v) const {
+    if constexpr (XG) __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else p[i] = v;
+  }
+  __device__ void add(int64_t i, T v) const { atomicAdd(p + i, v); }
+};
+
 template <typename T>
 __device__ inline T ss_wave_sum(T v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -59,8 +78,8 @@ constexpr int ss_uf() { return NR <= 2 ? 4 : 2; }
 
 // forward sweep of supernode s by one wave: x(s) = L_ss^{-1} x(s), then
 // x(label q) -= L(q, s) x(s) for the rows below
-template <typename T, int NR>
-__device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, T* x) {
+template <typename T, int NR, typename XA>
+__device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, XA x) {
   constexpr int SS_UF = ss_uf<NR>();
   const int c0 = I.x, w = I.y;
   const int64_t r = I.z, pis = I.w;
@@ -69,7 +88,7 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
     const int cw = min(64, w - cb);
     T v[NR];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
+    for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x.ld((c0 + cb + ln) * NR + c) : (T)0;
     const T* Pl = P + (cb + ln) + (int64_t)cb * r;  // L(cb + ln, cb + j) = Pl[j r]
     for (int j0 = 0; j0 + 1 < cw; j0 += SS_UF) {
       T l[SS_UF];
@@ -89,7 +108,7 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
     }
     if (ln < cw) {
 #pragma unroll
-      for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+      for (int c = 0; c < NR; ++c) x.st((c0 + cb + ln) * NR + c, v[c]);
     }
     for (int64_t q0 = cb + cw; q0 < r; q0 += 64) {
       const int64_t q = q0 + ln;
@@ -114,10 +133,10 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
       if (ok) {
         if (q < w) {  // a later chunk of this supernode: its rows are this wave's alone
 #pragma unroll
-          for (int c = 0; c < NR; ++c) x[(c0 + (int)q) * NR + c] -= acc[c];
+          for (int c = 0; c < NR; ++c) x.st((c0 + (int)q) * NR + c, x.ld((c0 + (int)q) * NR + c) - acc[c]);
         } else {      // an ancestor's row, shared with other supernodes of the level
 #pragma unroll
-          for (int c = 0; c < NR; ++c) atomicAdd(&x[lab * NR + c], -acc[c]);
+          for (int c = 0; c < NR; ++c) x.add(lab * NR + c, -acc[c]);
         }
       }
     }
@@ -125,20 +144,20 @@ __device__ void ss_forward(const int4 I, int64_t px, int ln, const SnArgs& a, co
 }
 
 // backward sweep of supernode s by one wave: x(s) = L_ss^{-T} (x(s) - L(q, s)^T x(label q))
-template <typename T, int NR>
-__device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, T* x) {
+template <typename T, int NR, typename XA>
+__device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, const T* valL, XA x) {
   constexpr int SS_UF = ss_uf<NR>();
   const int c0 = I.x, w = I.y;
   const int64_t r = I.z, pis = I.w;
   const T* P = valL + px;
   // row blocks of 64 whose gathered x values a lane holds at once
-  constexpr int RB = NR <= 2 ? 4 : 2;
+  constexpr int RB = NR == 1 ? 4 : 2;
   for (int cb = ((w - 1) / 64) * 64; cb >= 0; cb -= 64) {
     const int cw = min(64, w - cb);
     const int64_t qb = cb + cw, rr = r - qb;  // rows below the chunk
     T v[NR];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
+    for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x.ld((c0 + cb + ln) * NR + c) : (T)0;
     if (rr > 0) {
       if ((int64_t)cw * ((rr + 63) / 64 + 6 * NR) < rr) {
         // narrow chunk, many rows: lanes over rows, one wave sum per column;
@@ -152,7 +171,7 @@ __device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, c
             okb[b] = q < r;
             const int lab = !okb[b] ? 0 : (q < w ? c0 + (int)q : a.rows[pis + q]);
 #pragma unroll
-            for (int c = 0; c < NR; ++c) xs[b][c] = okb[b] ? x[lab * NR + c] : (T)0;
+            for (int c = 0; c < NR; ++c) xs[b][c] = okb[b] ? x.ld(lab * NR + c) : (T)0;
           }
           const T* Pq = P + qs + ln + (int64_t)cb * r;  // L(qs + ln + 64 b, cb + j) = Pq[64 b + j r]
           for (int j0 = 0; j0 < cw; j0 += SS_UF / 2) {
@@ -192,7 +211,7 @@ __device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, c
 #pragma unroll
           for (int u = 0; u < SS_UF; ++u) {
 #pragma unroll
-            for (int c = 0; c < NR; ++c) v[c] -= l[u] * x[lab[u] * NR + c];
+            for (int c = 0; c < NR; ++c) v[c] -= l[u] * x.ld(lab[u] * NR + c);
           }
         }
       }
@@ -216,45 +235,45 @@ __device__ void ss_backward(const int4 I, int64_t px, int ln, const SnArgs& a, c
     }
     if (ln < cw) {
 #pragma unroll
-      for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+      for (int c = 0; c < NR; ++c) x.st((c0 + cb + ln) * NR + c, v[c]);
     }
   }
 }
 
 // thin supernode (one column, a few rows) by one lane: forward
 // x(label q) -= L(q) x(c0) for its rows below the diagonal
-template <typename T, int NR>
-__device__ void ss_thin_forward(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x) {
+template <typename T, int NR, typename XA>
+__device__ void ss_thin_forward(const int4 I, int64_t px, const SnArgs& a, const T* valL, XA x) {
   const int c0 = I.x, r = I.z;
   const int64_t pis = I.w;
   const T* P = valL + px;
   T y[NR];
 #pragma unroll
-  for (int c = 0; c < NR; ++c) y[c] = x[c0 * NR + c];
+  for (int c = 0; c < NR; ++c) y[c] = x.ld(c0 * NR + c);
   for (int q = 1; q < r; ++q) {
     const int lab = a.rows[pis + q];
     const T l = P[q];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) atomicAdd(&x[lab * NR + c], -l * y[c]);
+    for (int c = 0; c < NR; ++c) x.add(lab * NR + c, -l * y[c]);
   }
 }
 // and backward: x(c0) -= sum_q L(q) x(label q)
-template <typename T, int NR>
-__device__ void ss_thin_backward(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x) {
+template <typename T, int NR, typename XA>
+__device__ void ss_thin_backward(const int4 I, int64_t px, const SnArgs& a, const T* valL, XA x) {
   const int c0 = I.x, r = I.z;
   const int64_t pis = I.w;
   const T* P = valL + px;
   T v[NR];
 #pragma unroll
-  for (int c = 0; c < NR; ++c) v[c] = x[c0 * NR + c];
+  for (int c = 0; c < NR; ++c) v[c] = x.ld(c0 * NR + c);
   for (int q = 1; q < r; ++q) {
     const int lab = a.rows[pis + q];
     const T l = P[q];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) v[c] -= l * x[lab * NR + c];
+    for (int c = 0; c < NR; ++c) v[c] -= l * x.ld(lab * NR + c);
   }
 #pragma unroll
-  for (int c = 0; c < NR; ++c) x[c0 * NR + c] = v[c];
+  for (int c = 0; c < NR; ++c) x.st(c0 * NR + c, v[c]);
 }
 
 constexpr int SS_TP = 65;  // pitch of the staged 64 x 64 triangle
@@ -269,8 +288,8 @@ __device__ void ss_stage(const T* P, int64_t r, int cb, int cw, T* tile) {
 }
 
 // forward sweep of supernode s by the whole workgroup (uniform control flow)
-template <typename T, int NR>
-__device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x, T* tile) {
+template <typename T, int NR, typename XA>
+__device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, XA x, T* tile) {
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   const int c0 = I.x, w = I.y;
   const int64_t r = I.z, pis = I.w;
@@ -282,7 +301,7 @@ __device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T
     if (wv == 0) {
       T v[NR];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] : (T)0;
+      for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x.ld((c0 + cb + ln) * NR + c) : (T)0;
       for (int j = 0; j + 1 < cw; ++j) {
         const T l = (ln > j && ln < cw) ? tile[j * SS_TP + ln] : (T)0;
 #pragma unroll
@@ -290,7 +309,7 @@ __device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T
       }
       if (ln < cw) {
 #pragma unroll
-        for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+        for (int c = 0; c < NR; ++c) x.st((c0 + cb + ln) * NR + c, v[c]);
       }
     }
     __syncthreads();
@@ -306,11 +325,11 @@ __device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T
         for (int j = 0; j < cw; ++j) {
           const T l = P[q + (int64_t)(cb + j) * r];
 #pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] += l * x[(c0 + cb + j) * NR + c];
+          for (int c = 0; c < NR; ++c) acc[c] += l * x.ld((c0 + cb + j) * NR + c);
         }
         const int lab = q < w ? c0 + (int)q : a.rows[pis + q];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) x[lab * NR + c] -= acc[c];
+        for (int c = 0; c < NR; ++c) x.st(lab * NR + c, x.ld(lab * NR + c) - acc[c]);
       }
     }
     __syncthreads();
@@ -318,8 +337,8 @@ __device__ void ss_forward_wg(const int4 I, int64_t px, const SnArgs& a, const T
 }
 
 // backward sweep of supernode s by the whole workgroup (uniform control flow)
-template <typename T, int NR>
-__device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, T* x, T* tile, T* red) {
+template <typename T, int NR, typename XA>
+__device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const T* valL, XA x, T* tile, T* red) {
   constexpr int SS_UF = ss_uf<NR>();
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   const int c0 = I.x, w = I.y;
@@ -352,7 +371,7 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
 #pragma unroll
         for (int u = 0; u < SS_UF; ++u) {
 #pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] += l[u] * x[lab[u] * NR + c];
+          for (int c = 0; c < NR; ++c) acc[c] += l[u] * x.ld(lab[u] * NR + c);
         }
       }
       if (qs < qe) {
@@ -364,7 +383,7 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
     if (wv == 0) {
       T v[NR];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x[(c0 + cb + ln) * NR + c] - red[ln * NR + c] : (T)0;
+      for (int c = 0; c < NR; ++c) v[c] = ln < cw ? x.ld((c0 + cb + ln) * NR + c) - red[ln * NR + c] : (T)0;
       for (int i = cw - 1; i > 0; --i) {
         const T l = ln < i ? tile[ln * SS_TP + i] : (T)0;  // L(cb + i, cb + ln)
 #pragma unroll
@@ -372,7 +391,7 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
       }
       if (ln < cw) {
 #pragma unroll
-        for (int c = 0; c < NR; ++c) x[(c0 + cb + ln) * NR + c] = v[c];
+        for (int c = 0; c < NR; ++c) x.st((c0 + cb + ln) * NR + c, v[c]);
       }
     }
     __syncthreads();
@@ -382,16 +401,16 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
 // out = (L D L^T)^{-1} in for the ordered, row-interleaved blocks in / out
 // (n x NR).  Supernodes are listed level by level: info[k] = (first column,
 // width, rows, first row index) and ipx[k] = first value of supernode k of the
-// order, level l holding k in [levptr[l], levptr[l+1]); mode[l] = 1 for
+// order, level l holding k in [levptr[l], levptr[l+1])); mode[l] = 1 for
 // levels swept by the whole workgroup.  levptr and mode are copied to LDS;
 // each wave loads the description of its first supernode of the next level
 // before working on the current one, so a level costs one dependent round of
 // value loads instead of four.
-template <typename T, int NR>
+template <typename T, int NR, bool XG>
 __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, const int64_t* ipx, const int* levptr,
                                                             const int* mode, int nlev, SnArgs a, const T* valL,
                                                             const T* dval, const double* in, double* out,
-                                                            const double* active, int64_t astride) {
+                                                            const double* active, int64_t astride, T* xg) {
   if (active) {  // inside a GMRES step: when no column takes part, no sweep is needed
     bool any = false;
 #pragma unroll
@@ -399,13 +418,14 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
     if (!any) return;
   }
   extern __shared__ double smem_d[];
-  T* x = reinterpret_cast<T*>(smem_d);
+  T* lds = reinterpret_cast<T*>(smem_d);
   const int n = a.n, t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  T* tile = x + ((n * NR + 1) & ~1);  // 64 x SS_TP staged triangle
+  const XBlock<T, XG> x{XG ? xg : lds};
+  T* tile = XG ? lds : lds + ((n * NR + 1) & ~1);  // 64 x SS_TP staged triangle
   T* red = tile + 64 * SS_TP;         // 64 x NR partial sums
   int* lp = reinterpret_cast<int*>(red + 64 * NR);  // nlev + 1 level pointers
   int* lm = lp + nlev + 1;                          // nlev modes
-  for (int i = t; i < n * NR; i += SS_THREADS) x[i] = (T)in[i];
+  for (int i = t; i < n * NR; i += SS_THREADS) x.st(i, (T)in[i]);
   for (int i = t; i <= nlev; i += SS_THREADS) lp[i] = levptr[i];
   for (int i = t; i < nlev; i += SS_THREADS) lm[i] = mode[i];
   __syncthreads();
@@ -451,14 +471,14 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
       const T inv = (T)1 / (d1 * d2 - od * od);
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
-        const T x1 = x[i * NR + c], x2 = x[(i + 1) * NR + c];
-        x[i * NR + c] = (x1 * d2 - x2 * od) * inv;
-        x[(i + 1) * NR + c] = (x2 * d1 - x1 * od) * inv;
+        const T x1 = x.ld(i * NR + c), x2 = x.ld((i + 1) * NR + c);
+        x.st(i * NR + c, (x1 * d2 - x2 * od) * inv);
+        x.st((i + 1) * NR + c, (x2 * d1 - x1 * od) * inv);
       }
     } else if (i == 0 || dval[n + i - 1] == (T)0) {
       const T inv = (T)1 / dval[i];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) x[i * NR + c] *= inv;
+      for (int c = 0; c < NR; ++c) x.st(i * NR + c, x.ld(i * NR + c) * inv);
     }
   }
   nok = first(nlev - 1) < lp[nlev];
@@ -491,7 +511,7 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
       __syncthreads();
     }
   }
-  for (int i = t; i < n * NR; i += SS_THREADS) out[i] = (double)x[i];
+  for (int i = t; i < n * NR; i += SS_THREADS) out[i] = (double)x.ld(i);
 }
 
 // LDS of the kernel: the block (n x NR, padded to an even count), the staged
@@ -505,16 +525,21 @@ size_t ss_lds(int n, int nr, int nlev) {
 template <typename T, int NR>
 void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                     const SnArgs& a, const T* valL, const T* dval, const double* in, double* out,
-                    const double* active, int64_t astride, hipStream_t st) {
+                    const double* active, int64_t astride, T* xg, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute((const void*)k_solve_small<T, NR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (hipFuncSetAttribute((const void*)k_solve_small<T, NR, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(SOLVE_SMALL_LDS + ss_lds<T>(0, MAX_RHS, SOLVE_SMALL_LEVELS))) != hipSuccess)
       throw std::runtime_error("solve_small: LDS attribute");
     attr = true;
   }
-  k_solve_small<T, NR><<<1, SS_THREADS, ss_lds<T>(a.n, NR, nlev), st>>>(info, ipx, levptr, mode, nlev, a, valL, dval,
-                                                                        in, out, active, astride);
+  if (xg) {  // the block in memory
+    k_solve_small<T, NR, true><<<1, SS_THREADS, ss_lds<T>(0, NR, nlev), st>>>(info, ipx, levptr, mode, nlev, a, valL,
+                                                                              dval, in, out, active, astride, xg);
+  } else {
+    k_solve_small<T, NR, false><<<1, SS_THREADS, ss_lds<T>(a.n, NR, nlev), st>>>(
+        info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, nullptr);
+  }
   HIP_LAUNCH_CHECK("solve_small");
 }
 
@@ -523,27 +548,27 @@ void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, con
 template <typename T>
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                         const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
-                        const double* active, int64_t astride, hipStream_t st) {
-  if (nrhs < 1 || nrhs > MAX_RHS || sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS || nlev < 1 ||
+                        const double* active, int64_t astride, T* xg, hipStream_t st) {
+  if (nrhs < 1 || nrhs > MAX_RHS || (!xg && sizeof(T) * (size_t)a.n * nrhs > SOLVE_SMALL_LDS) || nlev < 1 ||
       nlev > SOLVE_SMALL_LEVELS)
     throw std::invalid_argument("solve_small: block or level schedule does not fit the workgroup's LDS");
   switch (nrhs) {
-    case 1: solve_small_nr<T, 1>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    case 2: solve_small_nr<T, 2>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    case 3: solve_small_nr<T, 3>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    case 4: solve_small_nr<T, 4>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    case 5: solve_small_nr<T, 5>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    case 6: solve_small_nr<T, 6>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    case 7: solve_small_nr<T, 7>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
-    default: solve_small_nr<T, 8>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, st); break;
+    case 1: solve_small_nr<T, 1>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    case 2: solve_small_nr<T, 2>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    case 3: solve_small_nr<T, 3>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    case 4: solve_small_nr<T, 4>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    case 5: solve_small_nr<T, 5>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    case 6: solve_small_nr<T, 6>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    case 7: solve_small_nr<T, 7>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
+    default: solve_small_nr<T, 8>(info, ipx, levptr, mode, nlev, a, valL, dval, in, out, active, astride, xg, st); break;
   }
 }
 
 template void launch_solve_small<double>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
                                          const double*, const double*, const double*, double*, int, const double*,
-                                         int64_t, hipStream_t);
+                                         int64_t, double*, hipStream_t);
 template void launch_solve_small<float>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
                                         const float*, const float*, const double*, double*, int, const double*,
-                                        int64_t, hipStream_t);
+                                        int64_t, float*, hipStream_t);
 
 }  // namespace lblg

@@ -102,26 +102,64 @@ def solve_warm(solver, q, predicted):
             iters, _factorizations(solver) - f0, fell_back)
 
 
-def generate(family, thetas, solver=None, log=None):
+def _in_threads(family, count, threads, solver, work):
+    """Run work(solver, k) for k < count: on `solver` alone (threads <= 1) or
+    on `threads` host threads, each with its own solver of the family (its
+    own engine handle and HIP stream; ctypes releases the interpreter lock
+    in the library, so the members' KKT work overlaps on the GPU)."""
+    if threads <= 1:
+        own = solver is None
+        solver = solver or family.solver()
+        try:
+            for k in range(count):
+                work(solver, k)
+        finally:
+            if own:
+                solver.close()
+        return
+    from concurrent.futures import ThreadPoolExecutor
+    import threading
+    nxt = iter(range(count))
+    lock = threading.Lock()
+
+    def run():
+        S = family.solver()
+        try:
+            while True:
+                with lock:
+                    k = next(nxt, None)
+                if k is None:
+                    return
+                work(S, k)
+        finally:
+            S.close()
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for f in [ex.submit(run) for _ in range(threads)]:
+            f.result()
+
+
+def generate(family, thetas, solver=None, log=None, threads=1):
     """Label parameters with plain NASOQ's active sets: returns (Y (count x mi)
-    0/1 float32, iterations per solve, seconds)."""
-    own = solver is None
-    solver = solver or family.solver()
+    0/1 float32, iterations per solve, seconds).  threads > 1: members are
+    solved concurrently, one solver per thread."""
     Y = np.zeros((len(thetas), family.mi), dtype=np.float32)
     iters = np.zeros(len(thetas), dtype=np.int64)
-    t0 = time.perf_counter()
-    for k, th in enumerate(thetas):
-        st, _, _, act, it, _ = solve_cold(solver, family.q(th))
+    done = [0]
+
+    def work(S, k):
+        st, _, _, act, it, _ = solve_cold(S, family.q(thetas[k]))
         if st != qp.OPTIMAL:
             raise RuntimeError(f"plain NASOQ returned status {st} on training member {k}")
         Y[k, act] = 1.0
         iters[k] = it
-        if log and (k + 1) % max(1, len(thetas) // 10) == 0:
-            log(f"labelled {k + 1}/{len(thetas)}")
-    secs = time.perf_counter() - t0
-    if own:
-        solver.close()
-    return Y, iters, secs
+        done[0] += 1
+        if log and (threads > 1 or done[0] % max(1, len(thetas) // 10) == 0):
+            log(f"labelled {done[0]}/{len(thetas)} ({it} active-set iterations)")
+
+    t0 = time.perf_counter()
+    _in_threads(family, len(thetas), threads, solver, work)
+    return Y, iters, time.perf_counter() - t0
 
 
 class ActiveSetPredictor:
@@ -203,26 +241,33 @@ class ActiveSetPredictor:
         return [np.flatnonzero(p > threshold).astype(np.int32) for p in P]
 
 
-def evaluate(family, predictor, thetas, solver=None, threshold=0.5, log=None):
+def evaluate(family, predictor, thetas, solver=None, threshold=0.5, log=None, threads=1):
     """Cold vs warm on held-out parameters.  Returns a dict with per-member
     iterations and KKT factorizations of both, fallbacks, the largest relative
     difference of the two solutions, the largest KKT residual of the warm
-    solutions, and the wall time of each method."""
-    own = solver is None
-    solver = solver or family.solver()
+    solutions, and the wall time of each method (all members cold, then all
+    warm; threads > 1 solves members concurrently in both phases)."""
     preds = predictor.predict_active(thetas, threshold)
+    cnt = len(thetas)
+    cold = [None] * cnt
+    warm = [None] * cnt
+
+    def work_cold(S, k):
+        cold[k] = solve_cold(S, family.q(thetas[k]))
+
+    def work_warm(S, k):
+        warm[k] = solve_warm(S, family.q(thetas[k]), preds[k])
+
+    t0 = time.perf_counter()
+    _in_threads(family, cnt, threads, solver, work_cold)
+    t1 = time.perf_counter()
+    _in_threads(family, cnt, threads, solver, work_warm)
+    t2 = time.perf_counter()
     out = {k: [] for k in ("cold_iter", "warm_iter", "cold_fact", "warm_fact", "fallback", "dx", "kkt",
                            "pred_size", "true_size", "hits")}
-    tc = tw = 0.0
-    for k, th in enumerate(thetas):
-        q = family.q(th)
-        t0 = time.perf_counter()
-        stc, xc, zc, actc, itc, svc = solve_cold(solver, q)
-        t1 = time.perf_counter()
-        stw, xw, zw, actw, itw, svw, fb = solve_warm(solver, q, preds[k])
-        t2 = time.perf_counter()
-        tc += t1 - t0
-        tw += t2 - t1
+    for k in range(cnt):
+        stc, xc, zc, actc, itc, svc = cold[k]
+        stw, xw, zw, actw, itw, svw, fb = warm[k]
         if stc != qp.OPTIMAL or stw != qp.OPTIMAL:
             raise RuntimeError(f"member {k}: cold status {stc}, warm status {stw}")
         out["cold_iter"].append(itc)
@@ -231,15 +276,13 @@ def evaluate(family, predictor, thetas, solver=None, threshold=0.5, log=None):
         out["warm_fact"].append(svw)
         out["fallback"].append(fb)
         out["dx"].append(float(np.linalg.norm(xw - xc) / max(1.0, np.linalg.norm(xc))))
-        out["kkt"].append(max(family.kkt_residuals(xw, zw, th)))
+        out["kkt"].append(max(family.kkt_residuals(xw, zw, thetas[k])))
         out["pred_size"].append(len(preds[k]))
         out["true_size"].append(len(actc))
         out["hits"].append(len(np.intersect1d(preds[k], actc)))
-        if log and (k + 1) % max(1, len(thetas) // 5) == 0:
-            log(f"evaluated {k + 1}/{len(thetas)}")
-    if own:
-        solver.close()
+    if log:
+        log(f"evaluated {cnt} members cold and warm")
     res = {k: np.array(v) for k, v in out.items()}
-    res["cold_seconds"] = tc
-    res["warm_seconds"] = tw
+    res["cold_seconds"] = t1 - t0
+    res["warm_seconds"] = t2 - t1
     return res

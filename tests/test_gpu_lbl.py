@@ -105,14 +105,22 @@ def test_reference_example_lbl_main():
 
 
 # sweeps: "small" = one workgroup with the block in LDS (the default for these
-# sizes), "levels" = the level-scheduled launches
-SWEEPS = {"small": str(1 << 30), "levels": "0"}
+# sizes), "global" = one workgroup with the block in memory (the form for
+# blocks beyond the LDS), "levels" = the level-scheduled launches
+SWEEPS = {"small": {"LBLG_SMALL_SOLVE_NNZ": str(1 << 30), "LBLG_SMALL_SOLVE_GLOBAL": "0"},
+          "global": {"LBLG_SMALL_SOLVE_NNZ": str(1 << 30), "LBLG_SMALL_SOLVE_GLOBAL": "1"},
+          "levels": {"LBLG_SMALL_SOLVE_NNZ": "0", "LBLG_SMALL_SOLVE_GLOBAL": "0"}}
 
 
-@pytest.mark.parametrize("sweep", ["small", "levels"])
+def set_sweep(monkeypatch, sweep):
+    for k, v in SWEEPS[sweep].items():
+        monkeypatch.setenv(k, v)
+
+
+@pytest.mark.parametrize("sweep", ["small", "global", "levels"])
 @pytest.mark.parametrize("seed", range(6))
 def test_small_kkt_parity(oracle, seed, sweep, monkeypatch):
-    monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
+    set_sweep(monkeypatch, sweep)
     n, Ap, Ai, Ax, B = P.kkt_small(50 + 30 * seed, 12 + 6 * seed, seed=seed, nrhs=2)
     print(f"case small_kkt seed={seed} n={n} sweep={sweep}", flush=True)
     reg = 1e-9
@@ -166,7 +174,7 @@ def test_two_by_two_pivots(oracle, nb, bs, order, group_w, tall, sweep, monkeypa
     # w > group_w (cooperative multi-workgroup panel, in both launch shapes)
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
     monkeypatch.setenv("LBLG_WIDE_TALL", str(tall))
-    monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
+    set_sweep(monkeypatch, sweep)
     n, Ap, Ai, Ax, B = P.block_indefinite(nb, bs, seed=1, nrhs=2)
     perm = np.arange(n, dtype=np.int32) if order == "nat" else None
     print(f"case 2x2 n={n} nb={nb} bs={bs} order={order} group_w={group_w}", flush=True)
@@ -251,7 +259,7 @@ def test_chain_sweeps(oracle, chain_w, group_w, grid, m, prec, monkeypatch):
     # (units of 128 rows chained by per-piece flags), the others piece by piece
     monkeypatch.setenv("LBLG_CHAIN_W", str(chain_w))
     monkeypatch.setenv("LBLG_GROUP_W", str(group_w))
-    monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS["levels"])
+    set_sweep(monkeypatch, "levels")
     n, Ap, Ai, Ax, B = P.kkt_config(grid, m, 6, seed=3, nrhs=3)
     print(f"case chain n={n} chain_w={chain_w} group_w={group_w} prec={prec}", flush=True)
     reg = 1e-9
@@ -302,8 +310,8 @@ def test_small_sweeps_match_level_sweeps(oracle, prec, ss_mode, monkeypatch):
         K[np.arange(n), np.arange(n)] = np.where(np.abs(d) < reg, reg, d)
         knorm = np.abs(K).sum(1).max()
         eta = {}
-        for sweep in ("small", "levels"):
-            monkeypatch.setenv("LBLG_SMALL_SOLVE_NNZ", SWEEPS[sweep])
+        for sweep in ("small", "global", "levels"):
+            set_sweep(monkeypatch, sweep)
             print(f"case sweeps n={n} prec={prec} sweep={sweep}", flush=True)
             s = engine(n, Ap, Ai, Ax, reg=1e-9, perm=perm, precision=prec, extra_cols=extra)
             assert s.lib.lblg_set_kernel_timing(s.h, 1) == 0
@@ -315,14 +323,42 @@ def test_small_sweeps_match_level_sweeps(oracle, prec, ss_mode, monkeypatch):
                     x, b = X[:, c], B[:, c]
                     e.append(np.abs(b - K @ x).max() / (knorm * np.abs(x).max() + np.abs(b).max()))
             eta[sweep] = np.array(e)
-            assert (_launches(s, "solve_small") > 0) == (sweep == "small")
+            assert (_launches(s, "solve_small") > 0) == (sweep != "levels")
             s.target_bwd_err = 1e-10
             s.max_refine_rounds = 30
             X = s.solve_only(B)
             for c in range(B.shape[1]):
                 assert oracle.backward_error(n, Ap, Ai, Ax, X[:, c], B[:, c]) <= 1e-10
-        print(f"eta small {eta['small'].max():.2e} levels {eta['levels'].max():.2e}", flush=True)
+        print(f"eta small {eta['small'].max():.2e} global {eta['global'].max():.2e} "
+              f"levels {eta['levels'].max():.2e}", flush=True)
         assert eta["small"].max() <= 4 * eta["levels"].max() + 1e-15
+        assert eta["global"].max() <= 4 * eta["levels"].max() + 1e-15
+
+
+def test_small_sweeps_block_beyond_lds(oracle, monkeypatch):
+    # a factor whose ordered block (n doubles) does not fit the 104 KB of LDS,
+    # swept by one workgroup with the block in memory (LBLG_SMALL_SOLVE_GLOBAL):
+    # refinement reaches 1e-10 and the unrefined solution matches the level
+    # sweeps' to the same backward error
+    n, Ap, Ai, Ax, B = P.kkt_config(24, 800, 6, seed=4, nrhs=1)
+    assert 8 * n > 104 * 1024
+    print(f"case sweeps beyond lds n={n}", flush=True)
+    eta = {}
+    for sweep in ("global", "levels"):
+        set_sweep(monkeypatch, sweep)
+        s = engine(n, Ap, Ai, Ax)
+        assert s.lib.lblg_set_kernel_timing(s.h, 1) == 0
+        s.req_ref_iter = 0
+        x = s.solve_only(B[:, 0])
+        eta[sweep] = oracle.backward_error(n, Ap, Ai, Ax, x, B[:, 0])
+        assert (_launches(s, "solve_small") > 0) == (sweep == "global")
+        s.target_bwd_err = 1e-10
+        s.max_refine_rounds = 30
+        x = s.solve_only(B[:, 0])
+        assert oracle.backward_error(n, Ap, Ai, Ax, x, B[:, 0]) <= 1e-10
+        s.close()
+    print(f"unrefined backward error global {eta['global']:.2e} levels {eta['levels']:.2e}", flush=True)
+    assert eta["global"] <= 4 * eta["levels"] + 1e-15
 
 
 def test_residual_norms_recomputed(oracle):
