@@ -512,7 +512,8 @@ def qp_batch_leg(args):
            "qps": len(probs), "n_total": int(sum(sizes)), "n_range": [int(min(sizes)), int(max(sizes))],
            "threads": args.qp_threads}
     objs = {}
-    for prec, name in ((1, "fp32_factor"), (0, "fp64")):
+    legs = ((1, "fp32_factor"), (0, "fp64")) if args.qp_batch_fp64 else ((1, "fp32_factor"),)
+    for prec, name in legs:
         t0 = time.perf_counter()
         res = qp.solve_many(probs, dict(settings, precision=prec), threads=args.qp_threads, keep=())
         dt = time.perf_counter() - t0
@@ -527,8 +528,9 @@ def qp_batch_leg(args):
         print(f"[bench] qp batch {name}: {len(res)} QPs in {dt:.1f} s, {out[name]['optimal']} optimal, "
               f"{out[name]['active_set_iterations']} iterations, {out[name]['kkt_solves_per_s']:.0f} KKT solves/s, "
               f"refactored fraction {out[name]['refactored_fraction']:.3f}", file=sys.stderr, flush=True)
-    rel = np.abs(objs["fp32_factor"] - objs["fp64"]) / np.maximum(1.0, np.abs(objs["fp64"]))
-    out["objective_rel_diff_fp32_vs_fp64_max"] = float(rel.max())
+    if "fp64" in objs:
+        rel = np.abs(objs["fp32_factor"] - objs["fp64"]) / np.maximum(1.0, np.abs(objs["fp64"]))
+        out["objective_rel_diff_fp32_vs_fp64_max"] = float(rel.max())
     return out
 
 
@@ -545,6 +547,7 @@ def learned_leg(args):
     from nasoq_amd import learned
 
     fam = learned.ParametricQpFamily(grid=args.learned_grid, seed=0)
+    fam.separate_rows = args.learned_separate_rows
     S = fam.solver()
     train = fam.thetas(args.learned_train, 1)
     log = lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True)
@@ -564,6 +567,7 @@ def learned_leg(args):
     ci, wi = res["cold_iter"].sum(), res["warm_iter"].sum()
     out = {"workload": f"parametric QP family, H = 2-D Laplacian {args.learned_grid}x{args.learned_grid} + 1e-2 I "
                        f"(n={fam.n}), {fam.mi} inequality rows of 6 N(0,1), d ~ U[0,0.5], q = U theta (32 params)",
+           "separate_rows": args.learned_separate_rows,
            "mlp": f"32 -> 1024 -> 1024 -> {fam.mi}, BF16 weights, FP32 accumulation",
            "train_solves": len(train), "label_seconds": t_label, "label_threads": args.learned_threads,
            "train_iterations_mean": float(iters.mean()), "train_epochs": args.learned_epochs,
@@ -598,7 +602,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-qp", action="store_true", help="skip the plain NASOQ QP configuration leg")
-    ap.add_argument("--qp-batch", type=int, default=64, help="box-constrained QPs of the batched QP leg (0: skip)")
+    ap.add_argument("--qp-batch", type=int, default=32, help="box-constrained QPs of the batched QP leg (0: skip)")
+    ap.add_argument("--qp-batch-fp64", action="store_true", help="the batched QP leg also with the FP64 KKT factor")
     ap.add_argument("--qp-threads", type=int, default=16, help="host threads of the batched QP leg")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 96^3 precision leg (FP64 / FP32 / BF16 and the ill-conditioned variant)")
@@ -610,6 +615,8 @@ def main():
     ap.add_argument("--learned-train", type=int, default=400, help="training members (plain NASOQ solves)")
     ap.add_argument("--learned-test", type=int, default=40, help="held-out members solved cold and warm")
     ap.add_argument("--learned-epochs", type=int, default=60)
+    ap.add_argument("--learned-separate-rows", type=int, default=1, choices=[0, 1],
+                    help="the family's KKT inequality rows as one-column supernodes (1, the reference's) or not (0)")
     ap.add_argument("--learned-threads", type=int, default=1,
                     help="host threads labelling and evaluating members concurrently (one solver each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -345,6 +345,14 @@ typedef struct lblg_qp_settings {
   int precision;        /* factor of the KKT systems: LBLG_F64 (default; the
                            reference's), LBLG_F32 or LBLG_BF16 with FP64
                            refinement (engine extension) */
+  int separate_rows;    /* 1 (default, the reference's): every inequality row
+                           of the super-KKT is its own supernode (extra_cols,
+                           supernode_detection.cpp:221-226); 0 (engine
+                           extension): rows join supernodes like any column.
+                           Inactive rows are decoupled either way, so both
+                           factor the same matrix; with 0 the separators full
+                           of coupled rows become dense supernodes instead of
+                           chains of one-column supernodes (fewer levels) */
 } lblg_qp_settings;
 
 /* Results and KKT checks of the last solve (the reference's public fields). */

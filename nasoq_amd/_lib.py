@@ -52,6 +52,7 @@ class QPSettings(ctypes.Structure):
         ("variant", ctypes.c_int),
         ("ordering", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("separate_rows", ctypes.c_int),
     ]
 
 

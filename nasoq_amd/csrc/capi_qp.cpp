@@ -58,6 +58,7 @@ extern "C" void lblg_qp_default_settings(lblg_qp_settings* s) {
   s->variant = LBLG_QP_FIXED;
   s->ordering = LBLG_ORDER_ND;
   s->precision = LBLG_F64;
+  s->separate_rows = 1;
 }
 
 extern "C" int lblg_qp_create(lblg_qp** out, int n, const int* Hp, const int* Hi, const double* Hx, const double* q,
@@ -112,7 +113,8 @@ extern "C" int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* st) {
   if (st->variant < LBLG_QP_FIXED || st->variant > LBLG_QP_PREDET || st->max_iter < 0 || st->max_iter_nas < 0 ||
       !(st->diag_perturb >= 0) || !(st->eps_abs > 0) || !(st->stop_tol >= 0) ||
       (st->ordering != LBLG_ORDER_ND && st->ordering != LBLG_ORDER_NATURAL) ||
-      (st->precision != LBLG_F64 && st->precision != LBLG_F32 && st->precision != LBLG_BF16))
+      (st->precision != LBLG_F64 && st->precision != LBLG_F32 && st->precision != LBLG_BF16) ||
+      (st->separate_rows != 0 && st->separate_rows != 1))
     return LBLG_ERR_ARG;
   s->diag_perturb = st->diag_perturb;
   s->zero_thresh = st->diag_perturb;
@@ -123,6 +125,7 @@ extern "C" int lblg_qp_set_settings(lblg_qp* qp, const lblg_qp_settings* st) {
   s->variant = st->variant;
   s->ordering = st->ordering;
   s->precision = st->precision;
+  s->separate_rows = st->separate_rows;
   return LBLG_OK;
 }
 

@@ -134,6 +134,14 @@ struct DeviceFactor {
   int* asup = nullptr;          // entry of K -> supernode of its slot in valL
   std::vector<int> hasup;
   std::vector<int> hlevel;      // factor level of every supernode
+  // per supernode: panel family (0 LDS, 1 one-workgroup panel, 2 cooperative:
+  // its position in the level's large list is s_large_q), whether it has
+  // off-diagonal rows for k_offdiag, its Schur form (0 none, 1 64-row tiles,
+  // 2 128-row tiles) and the offset of its L D block in LD
+  std::vector<unsigned char> s_kind, s_off, s_sch;
+  std::vector<int> s_large_q;
+  std::vector<int64_t> s_ldoff;
+  std::vector<unsigned char> s_mark, s_send;  // per refactorization: modified / sending supernodes
   std::vector<int> upd_ptr, upd_idx;  // per supernode: the supernodes with rows in its columns
   std::vector<unsigned> smark;  // marks of the current refactorization (epoch stamps)
   unsigned smark_epoch = 0;

@@ -313,6 +313,13 @@ static void setup(Handle& h) {
   int64_t wmax_total = 0;
   std::vector<int64_t> woff(ns, 0);
   D.flevels.resize(nlev);
+  D.s_kind.assign(ns, 0);
+  D.s_off.assign(ns, 0);
+  D.s_sch.assign(ns, 0);
+  D.s_large_q.assign(ns, -1);
+  D.s_ldoff.assign(ns, 0);
+  D.s_mark.assign(ns, 0);
+  D.s_send.assign(ns, 0);
   for (int l = 0; l < nlev; ++l) {
     FactorLevel& F = D.flevels[l];
     std::vector<int> small, off, sch, schb, offblk, big;
@@ -330,6 +337,7 @@ static void setup(Handle& h) {
         woff[s] = wsum;
         if (w <= group_w) {
           F.medium.push_back(s);
+          D.s_kind[s] = 1;
           wsum += (int64_t)w * PANEL_NB + 2 * (int64_t)w + 16;
         } else {
           // cooperative panel over G workgroups, <= WIDE_ROWS far rows each
@@ -353,6 +361,8 @@ static void setup(Handle& h) {
           wsum += (int64_t)w * wcols + r * nb + (int64_t)WIDE_MOVED * (w + r + wcols) + 64;
           F.large.push_back(s);
           D.has_coop = true;
+          D.s_kind[s] = 2;
+          D.s_large_q[s] = (int)F.large_G.size();
           F.large_G.push_back((int)G);
           F.large_nb.push_back(nb);
           F.large_nt.push_back(nt);
@@ -363,11 +373,14 @@ static void setup(Handle& h) {
         // the cooperative panel computes the off-diagonal rows of wide supernodes itself
         if (w <= group_w) {
           off.push_back(s);
+          D.s_off[s] = 1;
           offblk.push_back(blk);
           blk += (int)((r - w + 63) / 64);
         }
         if (r - w >= h.tune.big_m) {
           schb.push_back(s);
+          D.s_sch[s] = 2;
+          D.s_ldoff[s] = ldsum;
           tilesb.push_back(tlb);
           colsb.push_back(ncolb);
           ldoffb.push_back(ldsum);
@@ -377,6 +390,7 @@ static void setup(Handle& h) {
           tlb += T * (T + 1) / 2;
         } else {
           sch.push_back(s);
+          D.s_sch[s] = 1;
           tiles.push_back(tl);
           const int64_t T = (r - w + 63) / 64;
           tl += T * (T + 1) / 2;
@@ -965,42 +979,66 @@ static void path_factor_t(Handle& h) {
       mod.push_back(t);
     }
   }
-  // supernodes outside the paths with rows in a modified one
-  std::vector<unsigned char> hmask(ns, 0);
+  // supernodes outside the paths with rows in a modified one: they keep
+  // their factor and send their Schur updates again, to the modified
+  // supernodes only.  Those updates depend on nothing this refactorization
+  // computes, so the ones in 64-row tiles go out in one launch before the
+  // first level; 128-row-tile senders stay in their level (their L D blocks
+  // share the level's LD space).
+  std::vector<unsigned char>& hmask = D.s_mark;
+  std::vector<unsigned char>& sends = D.s_send;
   for (int t : mod) hmask[t] = 1;
-  std::vector<unsigned char> sends(ns, 0);   // off-path supernodes that update a modified one
+  std::vector<int> senders;
   for (int t : mod)
     for (int e = D.upd_ptr[t]; e < D.upd_ptr[t + 1]; ++e) {
       const int d = D.upd_idx[e];
-      if (!hmask[d]) sends[d] = 1;
+      if (!hmask[d] && !sends[d]) {
+        sends[d] = 1;
+        senders.push_back(d);
+      }
     }
   const int L = (int)D.flevels.size();
   std::vector<std::vector<int>> lsmall(L), lmed(L), lqlarge(L), loff(L), lsch(L), lschb(L);
   std::vector<std::vector<int64_t>> lldoff(L);
-  for (int l = 0; l < L; ++l) {
-    const FactorLevel& F = D.flevels[l];
-    for (int s : F.h_small)
-      if (hmask[s]) lsmall[l].push_back(s);
-    for (int s : F.medium)
-      if (hmask[s]) lmed[l].push_back(s);
-    for (size_t q = 0; q < F.large.size(); ++q)
-      if (hmask[F.large[q]]) lqlarge[l].push_back((int)q);
-    for (int s : F.h_off)
-      if (hmask[s]) loff[l].push_back(s);
-    for (int s : F.h_sch)
-      if (hmask[s] || sends[s]) lsch[l].push_back(s);
-    for (size_t k = 0; k < F.h_schb.size(); ++k) {
-      const int s = F.h_schb[k];
-      if (hmask[s] || sends[s]) {
-        lschb[l].push_back(s);
-        lldoff[l].push_back(F.h_schb_ldoff[k]);
-      }
+  std::vector<int> lsch_send;
+  std::vector<int> touched;
+  auto touch = [&](int l) {
+    if (lsmall[l].empty() && lmed[l].empty() && lqlarge[l].empty() && loff[l].empty() && lsch[l].empty() &&
+        lschb[l].empty())
+      touched.push_back(l);
+  };
+  for (int t : mod) {
+    const int l = D.hlevel[t];
+    touch(l);
+    if (D.s_kind[t] == 0) lsmall[l].push_back(t);
+    else if (D.s_kind[t] == 1) lmed[l].push_back(t);
+    else lqlarge[l].push_back(D.s_large_q[t]);
+    if (D.s_off[t]) loff[l].push_back(t);
+    if (D.s_sch[t] == 1) lsch[l].push_back(t);
+    if (D.s_sch[t] == 2) {
+      lschb[l].push_back(t);
+      lldoff[l].push_back(D.s_ldoff[t]);
     }
   }
+  for (int d : senders) {
+    if (D.s_sch[d] == 1) {
+      lsch_send.push_back(d);
+    } else if (D.s_sch[d] == 2) {
+      const int l = D.hlevel[d];
+      touch(l);
+      lschb[l].push_back(d);
+      lldoff[l].push_back(D.s_ldoff[d]);
+    }
+  }
+  std::sort(touched.begin(), touched.end());
+  std::vector<unsigned char> mask_bytes(ns, 0);
+  for (int t : mod) mask_bytes[t] = 1;
+  for (int t : mod) hmask[t] = 0;  // the marks are per refactorization
+  for (int d : senders) sends[d] = 0;
   // pack: reset list, mask, changed entries, then per level the lists and
   // their prefixes (the same forms setup builds for full levels)
   Blob B;
-  const size_t o_mod = B.add(mod), o_mask = B.add(hmask), o_idx = B.add(h.changed), o_val = B.add(vals);
+  const size_t o_mod = B.add(mod), o_mask = B.add(mask_bytes), o_idx = B.add(h.changed), o_val = B.add(vals);
   struct Offs {
     size_t small, med, off, offblk, offpos, sch, schtile, schpos, schb, schbtile, schbcols, schbld;
     int off_blocks = 0, schb_ncols = 0, small_wmax = 0, med_wmax = 0, off_wmax = 0;
@@ -1009,7 +1047,23 @@ static void path_factor_t(Handle& h) {
   std::vector<Offs> O(L);
   auto width = [&](int s) { return S.super[s + 1] - S.super[s]; };
   auto offrows = [&](int s) { return S.pi[s + 1] - S.pi[s] - width(s); };
-  for (int l = 0; l < L; ++l) {
+  // the senders' 64-row-tile updates: one list, tile prefix and tile map
+  int64_t send_tiles = 0;
+  size_t o_send, o_sendtile, o_sendpos;
+  {
+    std::vector<int64_t> tiles;
+    std::vector<int> tpos;
+    for (size_t k = 0; k < lsch_send.size(); ++k) {
+      tiles.push_back(send_tiles);
+      const int64_t T64 = (offrows(lsch_send[k]) + 63) / 64, nt = T64 * (T64 + 1) / 2;
+      for (int64_t b = 0; b < nt; ++b) tpos.push_back((int)k);
+      send_tiles += nt;
+    }
+    o_send = B.add(lsch_send);
+    o_sendtile = B.add(tiles);
+    o_sendpos = B.add(tpos);
+  }
+  for (int l : touched) {
     Offs& o = O[l];
     for (int s : lsmall[l]) o.small_wmax = std::max(o.small_wmax, width(s));
     for (int s : lmed[l]) o.med_wmax = std::max(o.med_wmax, width(s));
@@ -1085,7 +1139,12 @@ static void path_factor_t(Handle& h) {
   launch_pert_flags(D.nnzK, D.isdiag, D.Kx, Kfac, fm.reg, fm.reg_factor, h.set.regularization, D.pert, st);
   launch_assemble<T>(D.amap, Kfac, D.pert, D.nnzK, fm.reg_factor, valL, D.asup, dmask, st);
   a.tmask = dmask;
-  for (int l = 0; l < L; ++l) {
+  timed(D, "schur", [&] {
+    launch_schur(reinterpret_cast<const int*>(dptr(o_send)), (int)lsch_send.size(),
+                 reinterpret_cast<const int64_t*>(dptr(o_sendtile)), reinterpret_cast<const int*>(dptr(o_sendpos)),
+                 send_tiles, a, valL, dval, st);
+  });
+  for (int l : touched) {
     const Offs& o = O[l];
     LevelLists LL;
     LL.small = reinterpret_cast<const int*>(dptr(o.small));
@@ -1113,7 +1172,6 @@ static void path_factor_t(Handle& h) {
     LL.schb_tile = reinterpret_cast<const int64_t*>(dptr(o.schbtile));
     LL.schb_cols = reinterpret_cast<const int*>(dptr(o.schbcols));
     LL.schb_ldoff = reinterpret_cast<const int64_t*>(dptr(o.schbld));
-    if (LL.nsmall + LL.nmedium + (int)LL.large_q.size() + LL.nsch + LL.nschb == 0) continue;
     factor_level<T>(h, D.flevels[l], LL, a, fm.gemm_schur, fm.gemm_trailing);
   }
   a.tmask = nullptr;

@@ -198,16 +198,21 @@ void QpSolver::symbolic() {
   // the super-KKT pattern depends on the problem only: a later solve of the
   // same problem (new q or b, or the next call) keeps the analysis and the
   // device factor, and the KKT values are refreshed at the next factorization
-  if (h && analysed && std::memcmp(&st, &analysed_settings, sizeof(st)) == 0) return;
+  if (h && analysed && std::memcmp(&st, &analysed_settings, sizeof(st)) == 0 &&
+      analysed_separate == separate_rows)
+    return;
   if (h) lblg_destroy(h);
   h = nullptr;
   check(lblg_create(&h, N, SM.p.data(), SM.i.data(), SM.x.data()), "qp: create");
-  // inequality rows are kept as single-column supernodes (extra_cols, SYM_REMOV)
+  // inequality rows are kept as single-column supernodes (extra_cols,
+  // SYM_REMOV), or join supernodes (separate_rows = 0: an inactive row is
+  // decoupled, so it needs no supernode of its own here)
   std::vector<int> extra(N, 0);
   for (int j = n + me; j < N; ++j) extra[j] = 1;
-  check(lblg_analyze(h, &st, nullptr, extra.data()), "qp: analyze");
+  check(lblg_analyze(h, &st, nullptr, separate_rows ? extra.data() : nullptr), "qp: analyze");
   analysed = true;
   analysed_settings = st;
+  analysed_separate = separate_rows;
 }
 
 // SolverSettings::symbolic_analysis in mode 1 (linear_solver_wrapper.cpp:396-536)

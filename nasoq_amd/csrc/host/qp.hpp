@@ -35,6 +35,8 @@ struct QpSolver {
   double diag_perturb = 1e-9, zero_thresh = 1e-9, eps_abs = 1e-3, stop_tol = 1e-15;
   int max_iter = 0, max_iter_nas = 4000, variant = QP_FIXED, ordering = LBLG_ORDER_ND;
   int precision = LBLG_F64;  // value type of the KKT factor (refinement is FP64 either way)
+  int separate_rows = 1;     // inequality rows as one-column supernodes (the reference's extra_cols)
+  int analysed_separate = -1;  // separate_rows of the current analysis of h
   int inner_iter_ref = 0, outer_iter_ref = 0, auto_reg_en = 0;
   // state and results
   std::vector<double> primal, dual, dual_eq;

@@ -40,6 +40,7 @@ class ParametricQpFamily:
         self.d = rng.uniform(0.0, 0.5, self.mi)
         self.U = rng.standard_normal((self.n, n_theta)) / np.sqrt(self.n)
         self.n_theta = n_theta
+        self.separate_rows = None  # solver(): the Nasoq default unless set
 
     def q(self, theta):
         return self.U @ np.asarray(theta, dtype=np.float64)
@@ -47,13 +48,18 @@ class ParametricQpFamily:
     def thetas(self, count, seed):
         return np.random.default_rng(seed).standard_normal((count, self.n_theta))
 
-    def solver(self, eps_abs=1e-8, autotune=False):
+    def solver(self, eps_abs=1e-8, autotune=False, separate_rows=None):
         """A Nasoq instance of the family (analysis and device buffers kept
         across members; set_linear swaps q); autotune: pick the KKT's device
-        schedule once for the whole family."""
+        schedule once for the whole family; separate_rows: the KKT's
+        inequality rows as one-column supernodes (1, the reference's) or in
+        supernodes with other columns (0); None keeps the family's setting."""
         m = qp.Nasoq(self.H, np.zeros(self.n), B=self.B, b=self.d)
         m.eps_abs = eps_abs
         m.variant = qp.FIXED
+        sep = self.separate_rows if separate_rows is None else separate_rows
+        if sep is not None:
+            m.separate_rows = sep
         if autotune:
             m.autotune()
         return m

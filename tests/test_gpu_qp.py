@@ -48,12 +48,16 @@ def _smp_solver(name, variant, max_iter, eps):
     return m, d
 
 
+@pytest.mark.parametrize("separate_rows", [1, 0])
 @pytest.mark.parametrize("name", ["matt_qp1", "test05_0"])
 @pytest.mark.parametrize("tool,variant,max_iter", [("NASOQ-Fixed", qp.FIXED, 0), ("NASOQ", qp.PREDET, 2)])
-def test_smp_out_correct(name, tool, variant, max_iter):
+def test_smp_out_correct(name, tool, variant, max_iter, separate_rows):
+    # separate_rows = 1: every inequality row its own supernode, as in the
+    # reference; 0: rows join supernodes (same matrix, other blocking)
     row = out_correct()[(tool, name)]
-    print(f"case smp {name} {tool}", flush=True)
+    print(f"case smp {name} {tool} separate_rows={separate_rows}", flush=True)
     m, d = _smp_solver(name, variant, max_iter, float(row["eps_abs"]))
+    m.separate_rows = separate_rows
     st = m.solve()
     assert st == int(row["Status"])
     assert m.num_iter == int(row["# of Iterations"])
