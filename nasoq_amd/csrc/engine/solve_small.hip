@@ -457,9 +457,21 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
       for (int k = lt; k < lp[l + 1]; ++k) ss_forward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile);
       __syncthreads();  // also where the level holds thin supernodes only
     } else {
-      if (ok) ss_forward<T, NR>(I, px, ln, a, valL, x);
-      for (int k = first(l) + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
-        ss_forward<T, NR>(info[k], ipx[k], ln, a, valL, x);
+      // each supernode's description is loaded while the one before is swept
+      int4 cI = I;
+      int64_t cpx = px;
+      for (int k = first(l); ok && k < lp[l + 1]; k += SS_WAVES) {
+        const bool more = k + SS_WAVES < lp[l + 1];
+        int4 fI = cI;
+        int64_t fpx = cpx;
+        if (more) {
+          fI = info[k + SS_WAVES];
+          fpx = ipx[k + SS_WAVES];
+        }
+        ss_forward<T, NR>(cI, cpx, ln, a, valL, x);
+        cI = fI;
+        cpx = fpx;
+      }
       __syncthreads();
     }
   }
@@ -505,9 +517,20 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
       for (int k = lp[l + 1] - 1; k >= lt; --k) ss_backward_wg<T, NR>(info[k], ipx[k], a, valL, x, tile, red);
       __syncthreads();
     } else {
-      if (ok) ss_backward<T, NR>(I, px, ln, a, valL, x);
-      for (int k = first(l) + SS_WAVES; k < lp[l + 1]; k += SS_WAVES)
-        ss_backward<T, NR>(info[k], ipx[k], ln, a, valL, x);
+      int4 cI = I;
+      int64_t cpx = px;
+      for (int k = first(l); ok && k < lp[l + 1]; k += SS_WAVES) {
+        const bool more = k + SS_WAVES < lp[l + 1];
+        int4 fI = cI;
+        int64_t fpx = cpx;
+        if (more) {
+          fI = info[k + SS_WAVES];
+          fpx = ipx[k + SS_WAVES];
+        }
+        ss_backward<T, NR>(cI, cpx, ln, a, valL, x);
+        cI = fI;
+        cpx = fpx;
+      }
       __syncthreads();
     }
   }

@@ -260,9 +260,13 @@ def evaluate(family, predictor, thetas, solver=None, threshold=0.5, log=None, th
 
     def work_cold(S, k):
         cold[k] = solve_cold(S, family.q(thetas[k]))
+        if log and threads > 1:
+            log(f"member {k} cold: {cold[k][4]} iterations")
 
     def work_warm(S, k):
         warm[k] = solve_warm(S, family.q(thetas[k]), preds[k])
+        if log and threads > 1:
+            log(f"member {k} warm: {warm[k][4]} iterations")
 
     t0 = time.perf_counter()
     _in_threads(family, cnt, threads, solver, work_cold)
@@ -292,3 +296,96 @@ def evaluate(family, predictor, thetas, solver=None, threshold=0.5, log=None, th
     res["cold_seconds"] = t1 - t0
     res["warm_seconds"] = t2 - t1
     return res
+
+
+def label_for(family, thetas, budget_s, threads):
+    """Label members with plain NASOQ on `threads` concurrent solvers until
+    `budget_s` seconds have passed (members started by then finish).
+    Returns (thetas, Y, iterations, seconds) of the labelled members."""
+    import threading
+    count = len(thetas)
+    Y = np.zeros((count, family.mi), np.float32)
+    iters = np.full(count, -1, np.int64)
+    t0 = time.perf_counter()
+    lock = threading.Lock()
+    nxt = [0]
+
+    def run():
+        S = family.solver()
+        try:
+            while True:
+                with lock:
+                    k = nxt[0]
+                    if k >= count or time.perf_counter() - t0 > budget_s:
+                        return
+                    nxt[0] += 1
+                st, _, _, act, it, _ = solve_cold(S, family.q(thetas[k]))
+                if st != qp.OPTIMAL:
+                    raise RuntimeError(f"plain NASOQ returned status {st} on member {k}")
+                Y[k, act] = 1.0
+                iters[k] = it
+                print(f"labelled member {k}: {it} iterations, {len(act)} active, "
+                      f"t={time.perf_counter() - t0:.0f}s", flush=True)
+        finally:
+            S.close()
+
+    ths = [threading.Thread(target=run) for _ in range(max(1, threads))]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    done = iters >= 0
+    return thetas[done], Y[done], iters[done], time.perf_counter() - t0
+
+
+def _main(argv=None):
+    """Labelling and evaluation in separate runs (a large family's labels can
+    take longer than one run may):
+      python -m nasoq_amd.learned label --grid 100 --count 200 --budget 850 --out labels.npz
+      python -m nasoq_amd.learned evaluate --grid 100 --labels labels.npz --test 12
+    Both take --threads and --separate-rows; evaluate prints one JSON line."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m nasoq_amd.learned")
+    ap.add_argument("mode", choices=["label", "evaluate"])
+    ap.add_argument("--grid", type=int, default=100)
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--separate-rows", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--count", type=int, default=200, help="label: members to label at most")
+    ap.add_argument("--budget", type=float, default=850.0, help="label: seconds after which no member is started")
+    ap.add_argument("--out", default="labels.npz")
+    ap.add_argument("--labels", default="labels.npz", help="evaluate: labels written by the label mode")
+    ap.add_argument("--test", type=int, default=12, help="evaluate: held-out members")
+    ap.add_argument("--epochs", type=int, default=60)
+    a = ap.parse_args(argv)
+    fam = ParametricQpFamily(grid=a.grid, seed=0)
+    fam.separate_rows = a.separate_rows
+    if a.mode == "label":
+        th, Y, it, secs = label_for(fam, fam.thetas(a.count, 1), a.budget, a.threads)
+        np.savez(a.out, thetas=th, Y=Y, iters=it, seconds=secs)
+        print(f"saved {len(th)} labelled members in {secs:.0f}s", flush=True)
+        return
+    d = np.load(a.labels)
+    net = ActiveSetPredictor(fam.n_theta, fam.mi, hidden=(1024, 1024), seed=0, max_batch=128)
+    t0 = time.perf_counter()
+    loss = net.fit(d["thetas"], d["Y"], epochs=a.epochs, batch=16, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    res = evaluate(fam, net, fam.thetas(a.test, 2), threads=a.threads, log=lambda m: print(m, flush=True))
+    net.close()
+    out = {"grid": a.grid, "n": fam.n, "m": fam.mi, "separate_rows": a.separate_rows,
+           "train_solves": int(len(d["thetas"])), "label_seconds": float(d["seconds"]),
+           "train_iterations_mean": float(d["iters"].mean()), "train_epochs": a.epochs, "train_seconds": t_train,
+           "final_train_loss": float(loss), "test_members": a.test,
+           "cold_iterations_mean": float(res["cold_iter"].mean()), "warm_iterations_mean": float(res["warm_iter"].mean()),
+           "iteration_ratio": float(res["cold_iter"].sum() / max(res["warm_iter"].sum(), 1)),
+           "cold_seconds": res["cold_seconds"], "warm_seconds": res["warm_seconds"],
+           "fallbacks": int(res["fallback"].sum()), "max_rel_solution_difference": float(res["dx"].max()),
+           "max_kkt_residual_warm": float(res["kkt"].max()), "predicted_active_mean": float(res["pred_size"].mean()),
+           "true_active_mean": float(res["true_size"].mean()), "correctly_predicted_mean": float(res["hits"].mean()),
+           "cold_iter": res["cold_iter"].tolist(), "warm_iter": res["warm_iter"].tolist()}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    import torch  # noqa: F401  (the HIP runtime of torch first, as in every entry point)
+    _main()
