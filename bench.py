@@ -515,7 +515,8 @@ def qp_batch_leg(args):
     legs = ((1, "fp32_factor"), (0, "fp64")) if args.qp_batch_fp64 else ((1, "fp32_factor"),)
     for prec, name in legs:
         t0 = time.perf_counter()
-        res = qp.solve_many(probs, dict(settings, precision=prec), threads=args.qp_threads, keep=())
+        log = (lambda m: print(f"[bench] qp batch {name}: {m}", file=sys.stderr, flush=True)) if len(probs) > 64 else None
+        res = qp.solve_many(probs, dict(settings, precision=prec), threads=args.qp_threads, keep=(), log=log)
         dt = time.perf_counter() - t0
         fac = sum(r["factorizations"] for r in res)
         full = sum(r["factorizations"] * r["supernodes"] for r in res)

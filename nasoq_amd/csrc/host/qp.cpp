@@ -195,6 +195,13 @@ void QpSolver::symbolic() {
   st.tol_abs = st.tol_rel = stop_tol;
   st.ordering = ordering;
   st.precision = precision;
+  if (precision != LBLG_F64) {
+    // an FP32 / BF16 factor is a preconditioner: the variant's refinement
+    // budget fits an FP64 factor (one GMRES(1) cycle for larger QPs), so the
+    // KKT solves refine to an FP64-level backward error instead
+    st.target_bwd_err = 1e-13;
+    st.max_refine_rounds = 40;
+  }
   // the super-KKT pattern depends on the problem only: a later solve of the
   // same problem (new q or b, or the next call) keeps the analysis and the
   // device factor, and the KKT values are refreshed at the next factorization

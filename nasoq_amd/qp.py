@@ -158,16 +158,18 @@ class Nasoq:
         return N, p, i[:nnz], x[:nnz]
 
 
-def solve_many(problems, settings=None, threads=8, keep=("primal_vars",)):
+def solve_many(problems, settings=None, threads=8, keep=("primal_vars",), log=None):
     """Solve independent QPs (tuples (H, q, A, b, C, d), A / C may be None)
     concurrently: `threads` host threads each run NASOQ's active-set loop on
     one problem at a time, every problem with its own engine handle and HIP
     stream, so the KKT factorizations and solves of different problems
     overlap on the GPU (ctypes releases the interpreter lock during the
-    library calls).  settings: dict of public Nasoq fields.  Returns one dict
-    per problem, in order: status, num_iter, objective, the KKT checks, the
-    factorization counters and the fields named in `keep`."""
+    library calls).  settings: dict of public Nasoq fields; log(message) is
+    called after every problem.  Returns one dict per problem, in order:
+    status, num_iter, objective, the KKT checks, the factorization counters
+    and the fields named in `keep`."""
     from concurrent.futures import ThreadPoolExecutor
+    done = [0]
 
     def one(prob):
         H, q, A, b, C, d = prob
@@ -184,6 +186,9 @@ def solve_many(problems, settings=None, threads=8, keep=("primal_vars",)):
                    "solve_seconds": m.solve_seconds}
             for k in keep:
                 out[k] = np.array(getattr(m, k), copy=True)
+            done[0] += 1
+            if log:
+                log(f"solved {done[0]}/{len(problems)}: n={m.n} status {st} {m.num_iter} iterations")
             return out
         finally:
             m.close()

@@ -44,7 +44,10 @@ def test_concurrent_matches_sequential():
 
 
 def test_fp32_factor_same_optimum():
-    probs = P.box_qp_batch(4, seed=12, lo=16, hi=24, meq_max=8)
+    # small QPs (the variant refines three GMRES(3) cycles) and larger ones
+    # (one GMRES(1) cycle for an FP64 factor: the FP32 factor's solves refine
+    # to a backward error target instead)
+    probs = P.box_qp_batch(3, seed=12, lo=16, hi=24, meq_max=8) + P.box_qp_batch(2, seed=13, lo=60, hi=72, meq_max=8)
     print("case box qps fp32", [p[0].shape[0] for p in probs], flush=True)
     f64 = qp.solve_many(probs, SETTINGS, threads=2)
     f32 = qp.solve_many(probs, dict(SETTINGS, precision=1), threads=2)
