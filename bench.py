@@ -603,7 +603,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (1024 QP KKT systems)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-qp", action="store_true", help="skip the plain NASOQ QP configuration leg")
-    ap.add_argument("--qp-batch", type=int, default=32, help="box-constrained QPs of the batched QP leg (0: skip)")
+    ap.add_argument("--qp-batch", type=int, default=16, help="box-constrained QPs of the batched QP leg (0: skip)")
     ap.add_argument("--qp-batch-fp64", action="store_true", help="the batched QP leg also with the FP64 KKT factor")
     ap.add_argument("--qp-threads", type=int, default=16, help="host threads of the batched QP leg")
     ap.add_argument("--no-precision", action="store_true",

@@ -352,20 +352,24 @@ def _main(argv=None):
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--separate-rows", type=int, default=1, choices=[0, 1])
     ap.add_argument("--count", type=int, default=200, help="label: members to label at most")
+    ap.add_argument("--skip", type=int, default=0, help="label: start at this training member (continue a run)")
     ap.add_argument("--budget", type=float, default=850.0, help="label: seconds after which no member is started")
     ap.add_argument("--out", default="labels.npz")
-    ap.add_argument("--labels", default="labels.npz", help="evaluate: labels written by the label mode")
+    ap.add_argument("--labels", default="labels.npz",
+                    help="evaluate: labels written by the label mode (several files: comma-separated)")
     ap.add_argument("--test", type=int, default=12, help="evaluate: held-out members")
     ap.add_argument("--epochs", type=int, default=60)
     a = ap.parse_args(argv)
     fam = ParametricQpFamily(grid=a.grid, seed=0)
     fam.separate_rows = a.separate_rows
     if a.mode == "label":
-        th, Y, it, secs = label_for(fam, fam.thetas(a.count, 1), a.budget, a.threads)
+        th, Y, it, secs = label_for(fam, fam.thetas(a.skip + a.count, 1)[a.skip:], a.budget, a.threads)
         np.savez(a.out, thetas=th, Y=Y, iters=it, seconds=secs)
         print(f"saved {len(th)} labelled members in {secs:.0f}s", flush=True)
         return
-    d = np.load(a.labels)
+    parts = [np.load(f) for f in a.labels.split(",")]
+    d = {"thetas": np.concatenate([q["thetas"] for q in parts]), "Y": np.concatenate([q["Y"] for q in parts]),
+         "iters": np.concatenate([q["iters"] for q in parts]), "seconds": sum(float(q["seconds"]) for q in parts)}
     net = ActiveSetPredictor(fam.n_theta, fam.mi, hidden=(1024, 1024), seed=0, max_batch=128)
     t0 = time.perf_counter()
     loss = net.fit(d["thetas"], d["Y"], epochs=a.epochs, batch=16, lr=1e-3)
