@@ -176,7 +176,8 @@ def cpu_baseline(config, full_fl, full_xsize, grid=48, reps=5):
 
 
 KERNEL_FAMILIES = ["bk_small", "bk_panel", "bk_panel_group", "trailing", "offdiag", "schur", "scale_ld", "schur_big",
-                   "fwd_tri", "fwd_gemv", "fwd_chain", "bwd_chain", "bwd_gemv", "bwd_tri", "spmv"]
+                   "fwd_tri", "fwd_gemv", "fwd_chain", "bwd_chain", "bwd_gemv", "bwd_tri", "solve_small",
+                   "solve_batch", "spmv"]
 
 
 def kernel_time(lib, h, name, steps):
@@ -351,6 +352,7 @@ def batch_leg(args):
     bwd = np.zeros(1)
 
     schedules = {}
+    kts = {}
 
     def run(prec):
         b.precision = prec
@@ -378,6 +380,7 @@ def batch_leg(args):
         _lib.check(lib.lblg_synchronize(b.h), "sync")
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        kts[prec] = kernel_breakdown(lib, b.h, step)
         X = dX.cpu().numpy()
         err = b.backward_errors(rhs, X)[:, 0]
         assert err.max() <= 1e-10, f"batched (precision {prec}) worst per-system backward error {err.max()}"
@@ -441,6 +444,11 @@ def batch_leg(args):
         "one_by_one_plain_fp64": {"value": nb / t_one, "ms_for_batch": t_one * 1e3,
                                   "worst_backward_error": worst_one},
         "schedule": dict(schedules[0], autotuned=bool(args.autotune)),
+        "kernel_ms_per_step": {k: v[0] for k, v in kts[0].items()},
+        "kernel_launches_per_step": {k: v[1] for k, v in kts[0].items()},
+        "kernel_ms_note": "one extra untimed FP64 step after the timed ones with every launch bracketed by HIP "
+                          "events (families of the factor and the sweeps; the rest of a step is assembly, "
+                          "refinement and launch gaps)",
         "speedup_batched_vs_one_by_one": t_one / t64,
         "speedup_batched_fp32_vs_one_by_one": t_one / t32,
     }

@@ -185,6 +185,16 @@ struct DeviceFactor {
   int* ss_mode = nullptr;  // per level: 1 = whole-workgroup sweeps
   int ss_nlev = 0;
   bool ss_global = false;       // keep the block in memory even where it fits LDS (tests)
+  // one-workgroup sweeps of each system of a block-diagonal batch (one workgroup per system)
+  int ssb_nsys = 0;             // systems with a schedule (0: the level sweeps)
+  int ssb_maxlev = 0;           // most levels of a system
+  int ssb_lds_kb = 104;         // LDS for a system's block (larger blocks stay in memory)
+  int* ssb_sysd = nullptr;      // 6 ints per system: level-pointer and mode offsets, levels, first column, columns
+  int4* ssb_info = nullptr;     // supernodes system by system, each system level by level
+  int64_t* ssb_ipx = nullptr;
+  int* ssb_levptr = nullptr;    // per system: levels + 1 pointers into ssb_info
+  int* ssb_mode = nullptr;
+  std::vector<int> ssb_cols;    // columns of each system, ascending (the LDS size of a launch)
   int64_t ss_nnzL = 0, ss_max_nnz = 0;
   double *stage_b = nullptr, *stage_x = nullptr;  // host-pointer solves: staged blocks (n x nrhs_cap)
   // residual norms of caller-given solutions (residual.hip), allocated at first use

@@ -166,6 +166,72 @@ static int env_int(const char* name, int dflt) {
   return v ? std::atoi(v) : dflt;
 }
 
+// Level schedule of the one-workgroup sweeps over the supernodes [s0, s1)
+// (the whole factor, or one system of a block-diagonal batch): order lists
+// them level by level (level l in [ptr[l], ptr[l+1])), mode[l] is the level's
+// sweep mode (bit 0) and thin count (bits 1..).
+static void ss_schedule(const Symbolic& S, const std::vector<int>& lev, int s0, int s1, std::vector<int>& order,
+                        std::vector<int>& ptr, std::vector<int>& mode) {
+  int nl = 0;
+  for (int s = s0; s < s1; ++s) nl = std::max(nl, lev[s] + 1);
+  ptr.assign(nl + 1, 0);
+  order.assign(s1 - s0, 0);
+  mode.assign(nl, 0);
+  for (int s = s0; s < s1; ++s) ++ptr[lev[s] + 1];
+  for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  // "thin" supernodes (one column, at most SS_THIN_ROWS rows: the singleton
+  // constraint rows of the QP's KKT) are swept one per lane; they come first
+  // within their level
+  auto thin = [&](int sn) {
+    return S.super[sn + 1] - S.super[sn] == 1 && S.pi[sn + 1] - S.pi[sn] <= SS_THIN_ROWS;
+  };
+  std::vector<int> nthin(nl, 0);
+  for (int s = s0; s < s1; ++s)
+    if (thin(s)) {
+      order[pos[lev[s]]++] = s;
+      ++nthin[lev[s]];
+    }
+  for (int s = s0; s < s1; ++s)
+    if (!thin(s)) order[pos[lev[s]]++] = s;
+  // mode of each level: per-wave supernodes, or the whole workgroup on one
+  // supernode at a time.  Cost model in dependent global-load steps: a
+  // per-wave triangle costs a step per column, a 64-row block below it a
+  // quarter step per column (independent loads); the workgroup mode stages
+  // the triangle (one step), substitutes from LDS (a fifth of a step per
+  // column), splits the row blocks over 16 waves and pays three barriers
+  // per 64-column chunk.
+  // one lane per thin supernode pays where a level holds many of them; a few
+  // (chain tails) stay with the prefetched per-wave sweep
+  const bool thin_on = env_int("LBLG_SMALL_SOLVE_THIN", 1) != 0;
+  std::vector<int> lthin(nl, 0);
+  for (int l = 0; l < nl; ++l) lthin[l] = (thin_on && nthin[l] >= SS_THIN_MIN) ? nthin[l] : 0;
+  for (int l = 0; l < nl; ++l) {
+    double wave_max = 0.0, wave_sum = 0.0, wg = 0.0;
+    for (int k = ptr[l] + lthin[l]; k < ptr[l + 1]; ++k) {
+      const int sn = order[k];
+      const int w = S.super[sn + 1] - S.super[sn];
+      const int64_t r = S.pi[sn + 1] - S.pi[sn];
+      double cw_cost = 0.0, cg_cost = 0.0;
+      for (int cb = 0; cb < w; cb += 64) {
+        const int cw = std::min(64, w - cb);
+        const int64_t rb = r - cb - cw;
+        cw_cost += (cw - 1) + 0.25 * cw * (double)((rb + 63) / 64);
+        cg_cost += 1.0 + 0.2 * (cw - 1) + 0.25 * cw * (double)((rb + 1023) / 1024) + 1.5;
+      }
+      wave_max = std::max(wave_max, cw_cost);
+      wave_sum += cw_cost;
+      wg += cg_cost;
+    }
+    const double per_wave = std::max(wave_max, wave_sum / 16.0) + 0.5;
+    mode[l] = (ptr[l + 1] - ptr[l] > lthin[l] && wg < per_wave) ? 1 : 0;
+  }
+  const int force = env_int("LBLG_SMALL_SOLVE_MODE", -1);  // 0 / 1: every level in that mode
+  if (force == 0 || force == 1) std::fill(mode.begin(), mode.end(), force);
+  for (int l = 0; l < nl; ++l) mode[l] |= lthin[l] << 1;  // bit 0: mode, bits 1..: thin count
+}
+
+
 // ---------------------------------------------------------------------------
 // one-time device setup for an analysed handle
 // ---------------------------------------------------------------------------
@@ -548,60 +614,8 @@ static void setup(Handle& h) {
         lev[par] = std::max(lev[par], lev[s] + 1);
       }
     }
-    std::vector<int> ptr(nl + 1, 0), order(ns);
-    for (int s = 0; s < ns; ++s) ++ptr[lev[s] + 1];
-    for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
-    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-    // "thin" supernodes (one column, at most SS_THIN_ROWS rows: the singleton
-    // constraint rows of the QP's KKT) are swept one per lane; they come first
-    // within their level
-    auto thin = [&](int sn) {
-      return S.super[sn + 1] - S.super[sn] == 1 && S.pi[sn + 1] - S.pi[sn] <= SS_THIN_ROWS;
-    };
-    std::vector<int> nthin(nl, 0);
-    for (int s = 0; s < ns; ++s)
-      if (thin(s)) {
-        order[pos[lev[s]]++] = s;
-        ++nthin[lev[s]];
-      }
-    for (int s = 0; s < ns; ++s)
-      if (!thin(s)) order[pos[lev[s]]++] = s;
-    // mode of each level: per-wave supernodes, or the whole workgroup on one
-    // supernode at a time.  Cost model in dependent global-load steps: a
-    // per-wave triangle costs a step per column, a 64-row block below it a
-    // quarter step per column (independent loads); the workgroup mode stages
-    // the triangle (one step), substitutes from LDS (a fifth of a step per
-    // column), splits the row blocks over 16 waves and pays three barriers
-    // per 64-column chunk.
-    // one lane per thin supernode pays where a level holds many of them; a few
-    // (chain tails) stay with the prefetched per-wave sweep
-    const bool thin_on = env_int("LBLG_SMALL_SOLVE_THIN", 1) != 0;
-    std::vector<int> lthin(nl, 0);
-    for (int l = 0; l < nl; ++l) lthin[l] = (thin_on && nthin[l] >= SS_THIN_MIN) ? nthin[l] : 0;
-    std::vector<int> mode(nl, 0);
-    for (int l = 0; l < nl; ++l) {
-      double wave_max = 0.0, wave_sum = 0.0, wg = 0.0;
-      for (int k = ptr[l] + lthin[l]; k < ptr[l + 1]; ++k) {
-        const int sn = order[k];
-        const int w = S.super[sn + 1] - S.super[sn];
-        const int64_t r = S.pi[sn + 1] - S.pi[sn];
-        double cw_cost = 0.0, cg_cost = 0.0;
-        for (int cb = 0; cb < w; cb += 64) {
-          const int cw = std::min(64, w - cb);
-          const int64_t rb = r - cb - cw;
-          cw_cost += (cw - 1) + 0.25 * cw * (double)((rb + 63) / 64);
-          cg_cost += 1.0 + 0.2 * (cw - 1) + 0.25 * cw * (double)((rb + 1023) / 1024) + 1.5;
-        }
-        wave_max = std::max(wave_max, cw_cost);
-        wave_sum += cw_cost;
-        wg += cg_cost;
-      }
-      const double per_wave = std::max(wave_max, wave_sum / 16.0) + 0.5;
-      mode[l] = (ptr[l + 1] - ptr[l] > lthin[l] && wg < per_wave) ? 1 : 0;
-    }
-    const int force = env_int("LBLG_SMALL_SOLVE_MODE", -1);  // 0 / 1: every level in that mode
-    if (force == 0 || force == 1) std::fill(mode.begin(), mode.end(), force);
-    for (int l = 0; l < nl; ++l) mode[l] |= lthin[l] << 1;  // bit 0: mode, bits 1..: thin count
+    std::vector<int> ptr, order, mode;
+    ss_schedule(S, lev, 0, ns, order, ptr, mode);
     std::vector<int> info(4 * (size_t)ns);
     std::vector<int64_t> ipx(ns);
     bool fits = nl <= SOLVE_SMALL_LEVELS && S.pi[ns] < (int64_t)INT32_MAX;
@@ -621,6 +635,73 @@ static void setup(Handle& h) {
     if (!fits) nl = 0;  // too many levels for the LDS copy: the level sweeps only
     D.ss_nlev = nl;
     D.ss_global = env_int("LBLG_SMALL_SOLVE_GLOBAL", 0) != 0;  // 1: the block in memory even where it fits LDS
+    // A block-diagonal batch: one schedule per system, swept by a workgroup
+    // of its own.  Each system's supernodes must form a contiguous range
+    // whose rows stay within the system's columns (its levels are then the
+    // global ones); otherwise the batch keeps the level sweeps.
+    const int nsys = h.batch_off.size() > 2 ? (int)h.batch_off.size() - 1 : 0;
+    D.ssb_nsys = 0;
+    if (nsys > 0 && env_int("LBLG_BATCH_SWEEPS", 1) != 0 && S.pi[ns] < (int64_t)INT32_MAX) {
+      std::vector<int> sb(nsys + 1, ns);
+      bool ok = true;
+      int g = 0;
+      for (int s = 0; s < ns && ok; ++s) {
+        while (g < nsys && S.super[s] >= h.batch_off[g + 1]) ++g;
+        if (g >= nsys || S.super[s] < h.batch_off[g] || S.super[s + 1] > h.batch_off[g + 1] ||
+            S.s[S.pi[s + 1] - 1] >= h.batch_off[g + 1])
+          ok = false;
+        else if (sb[g] == ns) sb[g] = s;
+      }
+      for (int q = 0; q < nsys && ok; ++q) ok = sb[q] < ns && (q == 0 || sb[q] > sb[q - 1]);
+      std::vector<int> binfo, blp, bmd, bsys;
+      std::vector<int64_t> bipx;
+      std::vector<double> work(nsys, 0.0);  // stored entries of each system's factor
+      int maxlev = 0;
+      for (int q = 0; q < nsys && ok; ++q) {
+        const int s0 = sb[q], s1 = q + 1 < nsys ? sb[q + 1] : ns;
+        std::vector<int> o, lptr, md;
+        ss_schedule(S, lev, s0, s1, o, lptr, md);
+        const int nlq = (int)md.size();
+        if (nlq < 1 || nlq > SOLVE_SMALL_LEVELS) {
+          ok = false;
+          break;
+        }
+        maxlev = std::max(maxlev, nlq);
+        const int base = (int)(binfo.size() / 4);
+        bsys.insert(bsys.end(), {(int)blp.size(), (int)bmd.size(), nlq, (int)h.batch_off[q],
+                                 (int)(h.batch_off[q + 1] - h.batch_off[q]), 0});
+        for (int v : lptr) blp.push_back(base + v);
+        bmd.insert(bmd.end(), md.begin(), md.end());
+        for (int sn : o) {
+          binfo.insert(binfo.end(), {S.super[sn], S.super[sn + 1] - S.super[sn], (int)(S.pi[sn + 1] - S.pi[sn]),
+                                     (int)S.pi[sn]});
+          bipx.push_back(px[sn]);
+          work[q] += (double)(S.super[sn + 1] - S.super[sn]) * (double)(S.pi[sn + 1] - S.pi[sn]);
+        }
+      }
+      if (ok) {
+        // workgroups start in system order: the largest systems first, so that
+        // the small ones fill in behind them instead of a large one starting last
+        std::vector<int> by(nsys);
+        for (int q = 0; q < nsys; ++q) by[q] = q;
+        std::stable_sort(by.begin(), by.end(), [&](int x, int y) { return work[x] > work[y]; });
+        std::vector<int> sorted(bsys.size());
+        for (int q = 0; q < nsys; ++q)
+          for (int k = 0; k < 6; ++k) sorted[6 * (size_t)q + k] = bsys[6 * (size_t)by[q] + k];
+        bsys.swap(sorted);
+        D.ssb_sysd = dupload(D, bsys);
+        D.ssb_info = reinterpret_cast<int4*>(dupload(D, binfo));
+        D.ssb_ipx = dupload(D, bipx);
+        D.ssb_levptr = dupload(D, blp);
+        D.ssb_mode = dupload(D, bmd);
+        D.ssb_maxlev = maxlev;
+        D.ssb_lds_kb = env_int("LBLG_BATCH_SWEEP_LDS_KB", (int)(SOLVE_SMALL_LDS / 1024));
+        D.ssb_cols.resize(nsys);
+        for (int q = 0; q < nsys; ++q) D.ssb_cols[q] = (int)(h.batch_off[q + 1] - h.batch_off[q]);
+        std::sort(D.ssb_cols.begin(), D.ssb_cols.end());
+        D.ssb_nsys = nsys;
+      }
+    }
     D.ss_nnzL = S.p[n];
     D.ss_max_nnz = h.tune.small_nnz;
     D.graphs = env_int("LBLG_GRAPH", 1) != 0;
@@ -1279,6 +1360,24 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
   T* acc = static_cast<T*>(D.acc);
   // the FP32 / BF16 factors are of S K S: M^-1 = S (L D L^T)^-1 S in the pivoted order
   const bool scaled = D.prec != 0;
+  if (D.ssb_nsys > 0 && nrhs <= MAX_RHS) {  // a batch: one workgroup per system
+    const double* sin = in;
+    if (scaled) {
+      launch_row_scale_copy(D.sbuf, in, D.scale_ord, D.n, nrhs, st);
+      sin = D.sbuf;
+    }
+    // the largest systems whose blocks fit the LDS set the launch's LDS; larger ones use Z
+    const size_t cap = std::min<size_t>(SOLVE_SMALL_LDS, (size_t)std::max(1, D.ssb_lds_kb) * 1024);
+    const size_t lim = cap / (sizeof(T) * (size_t)nrhs);
+    auto it = std::upper_bound(D.ssb_cols.begin(), D.ssb_cols.end(), (int)std::min<size_t>(lim, INT32_MAX));
+    const int fit_cols = it == D.ssb_cols.begin() ? 0 : *(it - 1);
+    timed(D, "solve_batch", [&] {
+      launch_solve_batch<T>(D.ssb_sysd, D.ssb_nsys, D.ssb_info, D.ssb_ipx, D.ssb_levptr, D.ssb_mode, fit_cols,
+                            D.ssb_maxlev, a, valL, dval, sin, out, nrhs, active, astride, Z, st);
+    });
+    if (scaled) launch_row_scale_copy(out, out, D.scale_ord, D.n, nrhs, st);
+    return;
+  }
   if (small_sweeps(D, nrhs)) {
     const double* sin = in;
     if (scaled) {

@@ -152,6 +152,15 @@ template <typename T>
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
                         const SnArgs& a, const T* valL, const T* dval, const double* in, double* out, int nrhs,
                         const double* active, int64_t astride, T* xg, hipStream_t st);
+// the same sweeps for every system of a block-diagonal batch, one workgroup
+// per system (sysd: 6 ints per system, see solve_small.hip; its level
+// pointers index info / ipx); systems of at most fit_cols rows keep their
+// block in LDS, the others in xg (n x nrhs values of T in memory)
+template <typename T>
+void launch_solve_batch(const int* sysd, int nsys, const int4* info, const int64_t* ipx, const int* levptr,
+                        const int* mode, int fit_cols, int maxlev, const SnArgs& a, const T* valL, const T* dval,
+                        const double* in, double* out, int nrhs, const double* active, int64_t astride, T* xg,
+                        hipStream_t st);
 // per-column dot products out[c] = sum_i a(i, c) b(i, c) (rows interleaved by
 // column), reduced in a fixed order through at most nblk partials
 void launch_dot(int n, int nrhs, const double* a, const double* b, double* part, int nblk, double* out,

@@ -47,18 +47,22 @@ __device__ inline float ss_lane(float v, int j) {
 // the L2, so a wave never reads a stale vector-L1 line after another wave's
 // atomic update (the atomics are performed in the L2); the workgroup
 // barriers between levels order the writes.
+// Element i of the block (row i / NR of the ordered matrix) is p[i - off]:
+// off = 0 for the whole block, the first element of a system when the LDS
+// holds one system of a batch.
 template <typename T, bool XG>
 struct XBlock {
   T* p;
+  int64_t off;
   __device__ T ld(int64_t i) const {
-    if constexpr (XG) return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else return p[i];
+    if constexpr (XG) return __hip_atomic_load(p + (i - off), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return p[i - off];
   }
   __device__ void st(int64_t i, T v) const {
-    if constexpr (XG) __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else p[i] = v;
+    if constexpr (XG) __hip_atomic_store(p + (i - off), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else p[i - off] = v;
   }
-  __device__ void add(int64_t i, T v) const { atomicAdd(p + i, v); }
+  __device__ void add(int64_t i, T v) const { atomicAdd(p + (i - off), v); }
 };
 
 template <typename T>
@@ -398,36 +402,23 @@ __device__ void ss_backward_wg(const int4 I, int64_t px, const SnArgs& a, const 
   }
 }
 
-// out = (L D L^T)^{-1} in for the ordered, row-interleaved blocks in / out
-// (n x NR).  Supernodes are listed level by level: info[k] = (first column,
-// width, rows, first row index) and ipx[k] = first value of supernode k of the
-// order, level l holding k in [levptr[l], levptr[l+1])); mode[l] = 1 for
-// levels swept by the whole workgroup.  levptr and mode are copied to LDS;
-// each wave loads the description of its first supernode of the next level
-// before working on the current one, so a level costs one dependent round of
-// value loads instead of four.
+// out = (L D L^T)^{-1} in for the rows [col0, col0 + ncol) of the ordered,
+// row-interleaved blocks in / out (n x NR; the whole matrix, or one system of
+// a block-diagonal batch, whose supernodes touch no other rows).  Supernodes
+// are listed level by level: info[k] = (first column, width, rows, first row
+// index) and ipx[k] = first value of supernode k of the order, level l
+// holding k in [lp[l], lp[l+1]); lm[l] & 1 = 1 for levels swept by the whole
+// workgroup, lm[l] >> 1 = the level's thin supernodes.  lp and lm are LDS
+// copies; each wave loads the description of its first supernode of the
+// next level before working on the current one, so a level costs one
+// dependent round of value loads instead of four.
 template <typename T, int NR, bool XG>
-__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, const int64_t* ipx, const int* levptr,
-                                                            const int* mode, int nlev, SnArgs a, const T* valL,
-                                                            const T* dval, const double* in, double* out,
-                                                            const double* active, int64_t astride, T* xg) {
-  if (active) {  // inside a GMRES step: when no column takes part, no sweep is needed
-    bool any = false;
-#pragma unroll
-    for (int c = 0; c < NR; ++c) any |= active[(int64_t)c * astride] != 0.0;
-    if (!any) return;
-  }
-  extern __shared__ double smem_d[];
-  T* lds = reinterpret_cast<T*>(smem_d);
+__device__ inline void ss_sweeps(const int4* info, const int64_t* ipx, const int* lp, const int* lm, int nlev,
+                                 int col0, int ncol, const SnArgs& a, const T* valL, const T* dval, const double* in,
+                                 double* out, const XBlock<T, XG>& x, T* tile, T* red) {
   const int n = a.n, t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  const XBlock<T, XG> x{XG ? xg : lds};
-  T* tile = XG ? lds : lds + ((n * NR + 1) & ~1);  // 64 x SS_TP staged triangle
-  T* red = tile + 64 * SS_TP;         // 64 x NR partial sums
-  int* lp = reinterpret_cast<int*>(red + 64 * NR);  // nlev + 1 level pointers
-  int* lm = lp + nlev + 1;                          // nlev modes
-  for (int i = t; i < n * NR; i += SS_THREADS) x.st(i, (T)in[i]);
-  for (int i = t; i <= nlev; i += SS_THREADS) lp[i] = levptr[i];
-  for (int i = t; i < nlev; i += SS_THREADS) lm[i] = mode[i];
+  const int64_t e0 = (int64_t)col0 * NR, e1 = (int64_t)(col0 + ncol) * NR;
+  for (int64_t i = e0 + t; i < e1; i += SS_THREADS) x.st(i, (T)in[i]);
   __syncthreads();
   // first wave-swept item (after the level's thin supernodes) of this wave
   auto first = [&](int l) { return lp[l] + (lm[l] >> 1) + wv; };
@@ -476,7 +467,7 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
     }
   }
   // D: 1x1 pivots, and 2x2 pivots (off-diagonal in dval[n + i]) by their first row
-  for (int i = t; i < n; i += SS_THREADS) {
+  for (int i = col0 + t; i < col0 + ncol; i += SS_THREADS) {
     const T od = dval[n + i];
     if (od != (T)0) {
       const T d1 = dval[i], d2 = dval[i + 1];
@@ -534,7 +525,69 @@ __global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, co
       __syncthreads();
     }
   }
-  for (int i = t; i < n * NR; i += SS_THREADS) out[i] = (double)x.ld(i);
+  for (int64_t i = e0 + t; i < e1; i += SS_THREADS) out[i] = (double)x.ld(i);
+}
+
+
+template <typename T, int NR, bool XG>
+__global__ __launch_bounds__(SS_THREADS) void k_solve_small(const int4* info, const int64_t* ipx, const int* levptr,
+                                                            const int* mode, int nlev, SnArgs a, const T* valL,
+                                                            const T* dval, const double* in, double* out,
+                                                            const double* active, int64_t astride, T* xg) {
+  if (active) {  // inside a GMRES step: when no column takes part, no sweep is needed
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) any |= active[(int64_t)c * astride] != 0.0;
+    if (!any) return;
+  }
+  extern __shared__ double smem_d[];
+  T* lds = reinterpret_cast<T*>(smem_d);
+  const int n = a.n, t = threadIdx.x;
+  const XBlock<T, XG> x{XG ? xg : lds, 0};
+  T* tile = XG ? lds : lds + ((n * NR + 1) & ~1);  // 64 x SS_TP staged triangle
+  T* red = tile + 64 * SS_TP;         // 64 x NR partial sums
+  int* lp = reinterpret_cast<int*>(red + 64 * NR);  // nlev + 1 level pointers
+  int* lm = lp + nlev + 1;                          // nlev modes
+  for (int i = t; i <= nlev; i += SS_THREADS) lp[i] = levptr[i];
+  for (int i = t; i < nlev; i += SS_THREADS) lm[i] = mode[i];
+  ss_sweeps<T, NR, XG>(info, ipx, lp, lm, nlev, 0, n, a, valL, dval, in, out, x, tile, red);
+}
+
+// The systems of a block-diagonal batch, one workgroup each: system g has
+// sysd[6g ..] = (first level pointer, first mode, levels, first column,
+// columns, -); its level pointers index info / ipx directly.  A system whose
+// block has at most fit_elems values is kept in LDS, a larger one in xg (the
+// same rows of an n x NR block in memory).
+template <typename T, int NR>
+__global__ __launch_bounds__(SS_THREADS) void k_solve_batch(const int* sysd, const int4* info, const int64_t* ipx,
+                                                            const int* levptr, const int* mode, int fit_elems,
+                                                            int maxlev, SnArgs a, const T* valL, const T* dval,
+                                                            const double* in, double* out, const double* active,
+                                                            int64_t astride, T* xg) {
+  if (active) {
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) any |= active[(int64_t)c * astride] != 0.0;
+    if (!any) return;
+  }
+  const int* d = sysd + 6 * (int64_t)blockIdx.x;
+  const int lpo = d[0], mdo = d[1], nlev = d[2], col0 = d[3], ncol = d[4];
+  extern __shared__ double smem_d[];
+  T* lds = reinterpret_cast<T*>(smem_d);
+  const int t = threadIdx.x;
+  T* tile = lds + ((fit_elems + 1) & ~1);
+  T* red = tile + 64 * SS_TP;
+  int* lp = reinterpret_cast<int*>(red + 64 * NR);
+  int* lm = lp + maxlev + 1;
+  for (int i = t; i <= nlev; i += SS_THREADS) lp[i] = levptr[lpo + i];
+  for (int i = t; i < nlev; i += SS_THREADS) lm[i] = mode[mdo + i];
+  if ((int64_t)ncol * NR <= fit_elems) {
+    const XBlock<T, false> x{lds, (int64_t)col0 * NR};
+    ss_sweeps<T, NR, false>(info, ipx, lp, lm, nlev, col0, ncol, a, valL, dval, in, out, x, tile, red);
+  } else {
+    const XBlock<T, true> x{xg, 0};
+    ss_sweeps<T, NR, true>(info, ipx, lp, lm, nlev, col0, ncol, a, valL, dval, in, out, x, tile, red);
+  }
 }
 
 // LDS of the kernel: the block (n x NR, padded to an even count), the staged
@@ -566,7 +619,49 @@ void solve_small_nr(const int4* info, const int64_t* ipx, const int* levptr, con
   HIP_LAUNCH_CHECK("solve_small");
 }
 
+template <typename T, int NR>
+void solve_batch_nr(const int* sysd, int nsys, const int4* info, const int64_t* ipx, const int* levptr,
+                    const int* mode, int fit_elems, int maxlev, const SnArgs& a, const T* valL, const T* dval,
+                    const double* in, double* out, const double* active, int64_t astride, T* xg, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void*)k_solve_batch<T, NR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(SOLVE_SMALL_LDS + ss_lds<T>(0, MAX_RHS, SOLVE_SMALL_LEVELS))) != hipSuccess)
+      throw std::runtime_error("solve_batch: LDS attribute");
+    attr = true;
+  }
+  const size_t lds = sizeof(T) * (((size_t)fit_elems + 1) & ~(size_t)1) + ss_lds<T>(0, NR, maxlev);
+  k_solve_batch<T, NR><<<nsys, SS_THREADS, lds, st>>>(sysd, info, ipx, levptr, mode, fit_elems, maxlev, a, valL,
+                                                      dval, in, out, active, astride, xg);
+  HIP_LAUNCH_CHECK("solve_batch");
+}
+
 }  // namespace
+
+template <typename T>
+void launch_solve_batch(const int* sysd, int nsys, const int4* info, const int64_t* ipx, const int* levptr,
+                        const int* mode, int fit_cols, int maxlev, const SnArgs& a, const T* valL, const T* dval,
+                        const double* in, double* out, int nrhs, const double* active, int64_t astride, T* xg,
+                        hipStream_t st) {
+  const int64_t fit = (int64_t)fit_cols * nrhs;
+  if (nsys < 1 || nrhs < 1 || nrhs > MAX_RHS || maxlev < 1 || maxlev > SOLVE_SMALL_LEVELS ||
+      sizeof(T) * (size_t)fit > SOLVE_SMALL_LDS || !xg)
+    throw std::invalid_argument("solve_batch: schedule or block does not fit the workgroup's LDS");
+  const int fe = (int)fit;
+#define SB(NR) solve_batch_nr<T, NR>(sysd, nsys, info, ipx, levptr, mode, fe, maxlev, a, valL, dval, in, out, active, \
+                                     astride, xg, st)
+  switch (nrhs) {
+    case 1: SB(1); break;
+    case 2: SB(2); break;
+    case 3: SB(3); break;
+    case 4: SB(4); break;
+    case 5: SB(5); break;
+    case 6: SB(6); break;
+    case 7: SB(7); break;
+    default: SB(8); break;
+  }
+#undef SB
+}
 
 template <typename T>
 void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr, const int* mode, int nlev,
@@ -587,6 +682,12 @@ void launch_solve_small(const int4* info, const int64_t* ipx, const int* levptr,
   }
 }
 
+template void launch_solve_batch<double>(const int*, int, const int4*, const int64_t*, const int*, const int*, int,
+                                         int, const SnArgs&, const double*, const double*, const double*, double*, int,
+                                         const double*, int64_t, double*, hipStream_t);
+template void launch_solve_batch<float>(const int*, int, const int4*, const int64_t*, const int*, const int*, int, int,
+                                        const SnArgs&, const float*, const float*, const double*, double*, int,
+                                        const double*, int64_t, float*, hipStream_t);
 template void launch_solve_small<double>(const int4*, const int64_t*, const int*, const int*, int, const SnArgs&,
                                          const double*, const double*, const double*, double*, int, const double*,
                                          int64_t, double*, hipStream_t);

@@ -11,8 +11,10 @@ src/QP/linear_solver_wrapper.cpp:1058-1119).  Checked here:
   recomputed from the symbolic tree (sParent, Perm, col2sup);
 - the updated factor equals a full factorization of the same values on a
   fresh handle: the same pivots, 2x2 blocks, permutation and row labels,
-  values to 8 u rho max(1, |entry|) (the scatter-adds are atomic, so sums
-  may round differently; rho = max|D| / max|K| is the growth);
+  values to the larger of 8 u rho max(1, |entry|) and 16 times the spread
+  between two fresh full factorizations (the scatter-adds are atomic, so
+  sums round differently from run to run; rho = max|D| / max|K| is the
+  growth);
 - the refined solve reaches 1e-10 (backward error recomputed by the oracle);
 - chains of add / drop edits (constraint rows switched on and off, as the
   active-set loop does), with cooperative panels and 128-row Schur tiles on
@@ -58,21 +60,37 @@ def path_set(s, n, Ap, Ai, idx):
 
 
 def same_factor(s, n, Ap, Ai, Ax):
-    """The factor of s against a full factorization of Ax on a new handle."""
+    """The factor of s against a full factorization of Ax on a new handle.
+
+    The Schur updates are scatter-added with atomics, so two full
+    factorizations of the same values differ by rounding in an order that
+    changes from run to run; with pivots of reg_diag = 1e-9 beside O(1)
+    couplings (growth rho ~ 1e8 here) unit-size entries are sums of terms
+    near rho, and that spread reaches 8 u rho.  The bar is therefore the
+    larger of 8 u rho and 16 times the spread measured between two fresh
+    full factorizations (same pivots), relative to max(1, |entry|)."""
     f = analysed(n, Ap, Ai, Ax, reg=s.reg_diag)
     assert f.numerical_factorization() == 1
     assert f.refactored_supernodes() == len(f.sym_ints("super")) - 1
+    f2 = analysed(n, Ap, Ai, Ax, reg=s.reg_diag)
+    assert f2.numerical_factorization() == 1
     valL, dval, rows, permf = s.get_factor()
     rvalL, rd, rrows, rperm = f.get_factor()
+    v2, d2, rows2, perm2 = f2.get_factor()
     f.close()
+    f2.close()
     assert np.array_equal(dval[n:] != 0, rd[n:] != 0)
     assert np.array_equal(permf, rperm)
     assert np.array_equal(rows, rrows)
     rho = np.abs(rd).max() / np.abs(Ax).max()
-    bar = 8 * U * max(rho, 1.0)
-    assert np.all(np.abs(dval - rd) <= bar * np.maximum(1.0, np.abs(rd)))
-    dl = np.abs(valL - rvalL) / np.maximum(1.0, np.abs(rvalL))
-    assert dl.max() <= bar, (dl.max(), bar)
+    sd, sl = np.maximum(1.0, np.abs(rd)), np.maximum(1.0, np.abs(rvalL))
+    spread = 0.0
+    if np.array_equal(d2[n:] != 0, rd[n:] != 0) and np.array_equal(perm2, rperm):
+        spread = max((np.abs(d2 - rd) / sd).max(), (np.abs(v2 - rvalL) / sl).max())
+    bar = max(8 * U * max(rho, 1.0), 16 * spread)
+    dd = (np.abs(dval - rd) / sd).max()
+    dl = (np.abs(valL - rvalL) / sl).max()
+    assert dd <= bar and dl <= bar, (dd, dl, bar, spread, 8 * U * rho)
 
 
 def coupling_positions(n, Ap, Ai, row):
