@@ -37,6 +37,31 @@ def test_batch_symbolic_is_union_of_blocks():
     assert len(b.sym_ints("super")) - 1 == nsup
 
 
+def test_batch_supernodes_stay_within_their_system():
+    # the per-system sweeps (one workgroup per system) rely on each system's
+    # supernodes forming one contiguous range, in system order, whose row
+    # indices stay within the system's columns; their levels (leaves 0, a
+    # parent one above its highest child) are then the system's own
+    inst = _blocks(count=9, seed=7)
+    b = lbl.BatchSolverSettings([(n, Ap, Ai, Ax) for n, Ap, Ai, Ax, _ in inst])
+    b.reg_diag = 1e-9
+    assert b.symbolic_analysis() == 1
+    sup = np.array(b.sym_ints("super"))
+    pi = np.array(b.sym_longs("pi"))
+    rows = np.array(b.sym_ints("s"))
+    par = np.array(b.sym_ints("sParent"))
+    offs = np.array(b.offsets)  # first column of each system, then n
+    sysof = np.searchsorted(offs, sup[:-1], side="right") - 1
+    assert np.all(np.diff(sysof) >= 0) and sysof[0] == 0 and sysof[-1] == len(inst) - 1
+    for s in range(len(sup) - 1):
+        g = sysof[s]
+        assert offs[g] <= sup[s] and sup[s + 1] <= offs[g + 1]
+        r = rows[pi[s]:pi[s + 1]]
+        assert r.min() >= offs[g] and r.max() < offs[g + 1]
+        if par[s] >= 0:
+            assert sysof[par[s]] == g and par[s] > s
+
+
 def test_batch_backward_errors_match_definition(oracle):
     inst = _blocks(4, seed=5)
     b = lbl.BatchSolverSettings([(n, Ap, Ai, Ax) for n, Ap, Ai, Ax, _ in inst])
