@@ -184,7 +184,7 @@ struct DeviceFactor {
   int* ss_levptr = nullptr;
   int* ss_mode = nullptr;  // per level: 1 = whole-workgroup sweeps
   int ss_nlev = 0;
-  bool ss_global = false;       // keep the block in memory even where it fits LDS (tests)
+  int ss_global = 0;            // 1: the block in memory even where it fits LDS (tests); 2: beyond the LDS only
   // one-workgroup sweeps of each system of a block-diagonal batch (one workgroup per system)
   int ssb_nsys = 0;             // systems with a schedule (0: the level sweeps)
   int ssb_maxlev = 0;           // most levels of a system

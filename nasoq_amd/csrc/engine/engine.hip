@@ -634,7 +634,8 @@ static void setup(Handle& h) {
     D.ss_mode = dupload(D, mode);
     if (!fits) nl = 0;  // too many levels for the LDS copy: the level sweeps only
     D.ss_nlev = nl;
-    D.ss_global = env_int("LBLG_SMALL_SOLVE_GLOBAL", 0) != 0;  // 1: the block in memory even where it fits LDS
+    // 1: the block in memory even where it fits LDS; 2: in memory only where it does not
+    D.ss_global = env_int("LBLG_SMALL_SOLVE_GLOBAL", 0);
     // A block-diagonal batch: one schedule per system, swept by a workgroup
     // of its own.  Each system's supernodes must form a contiguous range
     // whose rows stay within the system's columns (its levels are then the
@@ -1343,7 +1344,8 @@ static void ensure_solve_buffers(DeviceFactor& D, int nrhs, int mr) {
 // 21k-row QP KKT, the level sweeps' many workgroups were faster, 1.5 vs 2.5 ms)
 static bool small_sweeps(const DeviceFactor& D, int nrhs) {
   const size_t vs = D.prec != 0 ? sizeof(float) : sizeof(double);
-  return D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz && (D.ss_global || vs * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS);
+  return D.ss_nlev > 0 && D.ss_nnzL <= D.ss_max_nnz &&
+         (D.ss_global != 0 || vs * (size_t)D.n * nrhs <= SOLVE_SMALL_LDS);
 }
 
 template <typename T>
@@ -1386,7 +1388,7 @@ static void apply_solve_t(DeviceFactor& D, const double* in, double* out, int nr
     }
     // the ordered block in LDS, or in memory (Z) when asked for or when it
     // does not fit (small_sweeps admits that only when asked for)
-    T* xg = (D.ss_global || sizeof(T) * (size_t)D.n * nrhs > SOLVE_SMALL_LDS) ? Z : nullptr;
+    T* xg = (D.ss_global == 1 || sizeof(T) * (size_t)D.n * nrhs > SOLVE_SMALL_LDS) ? Z : nullptr;
     timed(D, "solve_small", [&] {
       launch_solve_small<T>(D.ss_info, D.ss_ipx, D.ss_levptr, D.ss_mode, D.ss_nlev, a, static_cast<const T*>(D.valL),
                             static_cast<const T*>(D.dval), sin, out, nrhs, active, astride, xg, st);

@@ -164,10 +164,12 @@ def solve_many(problems, settings=None, threads=8, keep=("primal_vars",), log=No
     one problem at a time, every problem with its own engine handle and HIP
     stream, so the KKT factorizations and solves of different problems
     overlap on the GPU (ctypes releases the interpreter lock during the
-    library calls).  settings: dict of public Nasoq fields; log(message) is
-    called after every problem.  Returns one dict per problem, in order:
-    status, num_iter, objective, the KKT checks, the factorization counters
-    and the fields named in `keep`."""
+    library calls).  The threads take the problems largest first (by
+    variables plus constraints), so that a large problem does not start
+    last and run alone at the end.  settings: dict of public Nasoq fields;
+    log(message) is called after every problem.  Returns one dict per
+    problem, in the order given: status, num_iter, objective, the KKT
+    checks, the factorization counters and the fields named in `keep`."""
     from concurrent.futures import ThreadPoolExecutor
     done = [0]
 
@@ -195,8 +197,17 @@ def solve_many(problems, settings=None, threads=8, keep=("primal_vars",), log=No
 
     if threads <= 1:
         return [one(p) for p in problems]
+
+    def size(prob):
+        H, q, A, b, C, d = prob
+        return H.shape[0] + sum(M.shape[0] for M in (A, C) if M is not None)
+
+    order = sorted(range(len(problems)), key=lambda k: -size(problems[k]))
+    out = [None] * len(problems)
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        return list(ex.map(one, problems))
+        for k, r in zip(order, ex.map(one, [problems[k] for k in order])):
+            out[k] = r
+    return out
 
 
 def quadprog(H, q, A, b, C, d, settings=None):
