@@ -220,6 +220,7 @@ struct DeviceFactor {
   bool panel_lds = true;         // one-workgroup panels of diagonal blocks that fit: staged in LDS
   bool schur_direct = true;      // 128-tile Schur updates with operands loaded lane by lane (no LDS)
   bool trailing_direct = true;   // the same for the wide panels' trailing updates
+  bool strip64 = true;           // the wide panels' strip pre-updates in 64 x 64 tiles
   bool has_coop = false;    // wide panels (cooperative, epoch-stamped): factor not captured
   bool has_chains = false;  // chain sweeps (epoch-stamped): level sweeps not captured
   int64_t graph_launches = 0, graph_captures = 0;

@@ -710,6 +710,7 @@ static void setup(Handle& h) {
     D.panel_lds = env_int("LBLG_PANEL_LDS", 1) != 0;
     D.path_refactor = env_int("LBLG_PATH_REFACTOR", 1) != 0;
     D.trailing_direct = env_int("LBLG_TRAILING_DIRECT", 1) != 0;
+    D.strip64 = env_int("LBLG_STRIP_TILE64", 1) != 0;
   }
   {  // one published block of SOLVE_W x MAX_RHS values per piece, two words per value
     const int64_t words = (int64_t)chain_pmax * SOLVE_W * MAX_RHS * 2;
@@ -864,12 +865,20 @@ static void factor_level(Handle& h, FactorLevel& F, const LevelLists& L, const S
       const int sub = p % wsub;
       const int lo = std::min(w, p * (nb - 1));  // panel p starts at column >= lo
       if (sub > 0) {
-        const int Tr = (int)((r - lo + 127) / 128);
-        if (Tr > 0)
-          timed(D, "trailing", [&] {
-            launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
-                                gemm_trailing, st);
-          });
+        if (D.strip64 && (gemm_trailing == GEMM_DIRECT || gemm_trailing == GEMM_STAGED)) {
+          const int T64 = (int)((r - lo + 63) / 64);
+          if (T64 > 0)
+            timed(D, "trailing", [&] {
+              launch_strip_update(F.d_large + q, T64, nb, wcols, a, (const T*)valL, Wb, D.woff, D.kpos, D.kstart, st);
+            });
+        } else {
+          const int Tr = (int)((r - lo + 127) / 128);
+          if (Tr > 0)
+            timed(D, "trailing", [&] {
+              launch_trailing_big(F.d_large + q, Tr, lo, nb, wcols, a, valL, Wb, D.woff, D.kpos, D.kstart,
+                                  gemm_trailing, st);
+            });
+        }
       }
       // far rows of this panel: r - min(w, k0 + nb) with k0 >= p (nb - 1) (each
       // panel eliminates >= nb - 1 columns); the last panel may be narrower

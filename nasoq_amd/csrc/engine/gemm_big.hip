@@ -506,6 +506,109 @@ __global__ __launch_bounds__(256) void k_trailing_big(const int* list, int lo, i
   }
 }
 
+// The strip pre-update of the next wide panel (the k_trailing_big strip
+// form) in 64 x 64 tiles: A(i, j) for the panel's columns k1 <= j < k1 +
+// strip and rows j <= i < r, minus sum_{l in [k0, k1)} L(i, l) W(j, l),
+// written to the panel buffer Pp[(j - k1) r + i].  A strip is one column of
+// tiles and its launches hold few of them, so a launch takes as long as one
+// tile: each wave here takes a 32 x 32 quarter (2 x 2 MFMA blocks) instead of
+// a 64 x 64 one, with its operands four k-steps ahead.
+#define SPF 4
+template <typename T>
+__global__ __launch_bounds__(256) void k_strip_update(const int* list, int strip, int wcols, const int* super,
+                                                      const int64_t* pi, const int64_t* px, const T* valL, T* Wbase,
+                                                      const int64_t* woff, const int* kpos, const int* kstart) {
+  const int s = list[0];
+  const int w = super[s + 1] - super[s];
+  const int64_t r = pi[s + 1] - pi[s];
+  const int k0 = kstart[s], k1 = kpos[s];
+  if (k1 >= w || k1 <= k0) return;
+  const int64_t i0 = k1 + 64 * (int64_t)blockIdx.x;
+  const int j0 = k1 + 64 * (int)blockIdx.y;
+  const int jend = min(w, k1 + strip);
+  if (i0 >= r || j0 >= jend) return;
+  const T* A = valL + px[s];
+  const T* Wp = Wbase + woff[s];
+  T* Pp = Wbase + woff[s] + (int64_t)w * wcols;  // r x strip, pitch r
+  const int kb = k1 - k0;
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int r0 = (wv >> 1) * 32, q0 = (wv & 1) * 32;
+  const int li = ln & 15, lk = ln >> 4;
+  const int xa = (int)min<int64_t>(r - i0, 64), xb = w - j0;
+  const T* pa[2];
+  const T* pb[2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {  // rows beyond the operands read their last row (results discarded)
+    pa[x] = A + (int64_t)k0 * r + i0 + min(r0 + 16 * x + li, xa - 1) + (int64_t)lk * r;
+    pb[x] = Wp + j0 + min(q0 + 16 * x + li, xb - 1) + (int64_t)lk * w;
+  }
+  const int64_t sa = 4 * r, sb = 4 * (int64_t)w;  // one k-step
+  typename Mfma<T>::acc acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = typename Mfma<T>::acc{0, 0, 0, 0};
+  const int nfull = kb / 4;
+  T ra[SPF][2], rb[SPF][2];
+  auto load = [&](T* a, T* b, int ks) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      a[x] = pa[x][ks * sa];
+      b[x] = pb[x][ks * sb];
+    }
+  };
+  auto mul = [&](const T* a, const T* b) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[x][y] = Mfma<T>::mma(b[y], a[x], acc[x][y]);
+  };
+#pragma unroll
+  for (int p = 0; p < SPF; ++p)
+    if (p < nfull) load(ra[p], rb[p], p);
+  int ks = 0;
+  for (; ks + SPF <= nfull; ks += SPF) {
+#pragma unroll
+    for (int p = 0; p < SPF; ++p) {
+      mul(ra[p], rb[p]);
+      if (ks + SPF + p < nfull) load(ra[p], rb[p], ks + SPF + p);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < SPF; ++p)
+    if (ks + p < nfull) mul(ra[p], rb[p]);
+  if (4 * nfull < kb) {  // last partial k-step: k beyond kb is zero
+    const bool kok = 4 * nfull + lk < kb;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      ra[0][x] = kok ? pa[x][nfull * sa] : (T)0;
+      rb[0][x] = kok ? pb[x][nfull * sb] : (T)0;
+    }
+    mul(ra[0], rb[0]);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t i = i0 + r0 + 16 * x + (ln & 15);
+        const int j = j0 + q0 + 16 * y + Mfma<T>::orow(e, ln);
+        if (i < r && j < jend && j >= k1 && i >= j)
+          Pp[(int64_t)(j - k1) * r + i] = A[(int64_t)j * r + i] - acc[x][y][e];
+      }
+}
+
+template <typename T>
+void launch_strip_update(const int* list, int row_tiles, int strip, int wcols, const SnArgs& a, const T* valL, T* W,
+                         const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st) {
+  if (row_tiles <= 0 || strip <= 0) return;
+  if (strip > BT) throw std::runtime_error("strip_update: strip wider than a tile");
+  const dim3 grid((unsigned)row_tiles, (unsigned)((strip + 63) / 64));
+  k_strip_update<T><<<grid, 256, 0, st>>>(list, strip, wcols, a.super, a.pi, a.px, valL, W, woff, kpos, kstart);
+  HIP_LAUNCH_CHECK("strip_update");
+}
+
 // LD(:, k) = (L_off D)(:, k) of each listed supernode, column-major m x w at
 // LD + ldoff[q] (the D scaling of blocked_2by2_mult, Sym_BLAS.cpp:573-599)
 template <typename T>
@@ -597,6 +700,8 @@ void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wco
 }
 
 #define LBLG_INSTANTIATE(T)                                                                                   \
+  template void launch_strip_update<T>(const int*, int, int, int, const SnArgs&, const T*, T*, const int64_t*,  \
+                                       const int*, const int*, hipStream_t);                                    \
   template void launch_scale_ld<T>(const int*, int, const int*, int, const SnArgs&, const T*, const T*, T*,  \
                                    const int64_t*, hipStream_t);                                            \
   template void launch_schur_big<T>(const int*, int, const int64_t*, int64_t, const SnArgs&, T*, const T*,   \

@@ -90,6 +90,11 @@ void launch_scale_ld(const int* list, int count, const int* col_prefix, int ncol
 template <typename T>
 void launch_schur_big(const int* list, int count, const int64_t* tile_prefix, int64_t ntiles, const SnArgs& a,
                       T* valL, const T* LD, const int64_t* ldoff, int mode, hipStream_t st);
+// the strip pre-update of a wide panel (launch_trailing_big's strip form) in
+// 64 x 64 tiles: row_tiles >= ceil((r - first panel column) / 64)
+template <typename T>
+void launch_strip_update(const int* list, int row_tiles, int strip, int wcols, const SnArgs& a, const T* valL, T* W,
+                         const int64_t* woff, const int* kpos, const int* kstart, hipStream_t st);
 template <typename T>
 void launch_trailing_big(const int* list, int ntiles, int lo, int strip, int wcols, const SnArgs& a, T* valL,
                          T* W, const int64_t* woff, const int* kpos, const int* kstart, int mode, hipStream_t st);
