@@ -11,7 +11,7 @@ src/QP/linear_solver_wrapper.cpp:1058-1119).  Checked here:
   recomputed from the symbolic tree (sParent, Perm, col2sup);
 - the updated factor equals a full factorization of the same values on a
   fresh handle: the same pivots, 2x2 blocks, permutation and row labels,
-  values to the larger of 8 u rho max(1, |entry|) and 16 times the spread
+  values to the larger of 64 u rho max(1, |entry|) and 16 times the spread
   between two fresh full factorizations (the scatter-adds are atomic, so
   sums round differently from run to run; rho = max|D| / max|K| is the
   growth);
@@ -67,8 +67,11 @@ def same_factor(s, n, Ap, Ai, Ax):
     changes from run to run; with pivots of reg_diag = 1e-9 beside O(1)
     couplings (growth rho ~ 1e8 here) unit-size entries are sums of terms
     near rho, and that spread reaches 8 u rho.  The bar is therefore the
-    larger of 8 u rho and 16 times the spread measured between two fresh
-    full factorizations (same pivots), relative to max(1, |entry|)."""
+    larger of 64 u rho and 16 times the spread measured between two fresh
+    full factorizations (same pivots), relative to max(1, |entry|): two
+    full factorizations can happen to sum in one order (spread 0) while the
+    update groups its sums differently.  An error of the update itself (a
+    missed sender, a stale panel) is many orders larger."""
     f = analysed(n, Ap, Ai, Ax, reg=s.reg_diag)
     assert f.numerical_factorization() == 1
     assert f.refactored_supernodes() == len(f.sym_ints("super")) - 1
@@ -87,10 +90,10 @@ def same_factor(s, n, Ap, Ai, Ax):
     spread = 0.0
     if np.array_equal(d2[n:] != 0, rd[n:] != 0) and np.array_equal(perm2, rperm):
         spread = max((np.abs(d2 - rd) / sd).max(), (np.abs(v2 - rvalL) / sl).max())
-    bar = max(8 * U * max(rho, 1.0), 16 * spread)
+    bar = max(64 * U * max(rho, 1.0), 16 * spread)
     dd = (np.abs(dval - rd) / sd).max()
     dl = (np.abs(valL - rvalL) / sl).max()
-    assert dd <= bar and dl <= bar, (dd, dl, bar, spread, 8 * U * rho)
+    assert dd <= bar and dl <= bar, (dd, dl, bar, spread, U * rho)
 
 
 def coupling_positions(n, Ap, Ai, row):
